@@ -1,0 +1,163 @@
+/*
+ * vdbgpu: C ABI of the MI355X (gfx950) occupancy-integration path of
+ * vdb_mapping. Plain pointers and sizes only; no C++ or HIP types cross this
+ * boundary. Every call on one map handle is serialised by the handle's own
+ * mutex and runs on the handle's own HIP stream; different handles are
+ * independent and may be used from different threads.
+ *
+ * Reference interface each entry point replaces (file:line in the
+ * vdb_mapping repository; VM = include/vdb_mapping/VDBMapping.hpp,
+ * OCC = include/vdb_mapping/OccupancyVDBMapping.hpp) is cited per function.
+ * The drop-in C++ classes in include/vdb_mapping/ call only these functions.
+ */
+#ifndef VDBGPU_H
+#define VDBGPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. 0 = success. */
+enum {
+  VDBGPU_OK = 0,
+  VDBGPU_NOT_CONFIGURED = 1, /* VM:478-482: raycast before setConfig */
+  VDBGPU_UNKNOWN_SOURCE = 2, /* VM:320-326: source id never added */
+  VDBGPU_BAD_MAX_RANGE = 3,  /* VM:1458-1463: max_range < 0 (rest of setConfig still applied) */
+  VDBGPU_BAD_PROB_MISS = 4,  /* OCC:64-69: prob_miss > 0.5 */
+  VDBGPU_BAD_PROB_HIT = 5,   /* OCC:70-75: prob_hit < 0.5 */
+  VDBGPU_OUT_OF_RANGE = 6,   /* a voxel index outside [-2^23, 2^23) or non-finite: scan rejected */
+  VDBGPU_UNSUPPORTED = 7,    /* fast_mode raycasting (VM:520-527) is not provided */
+  VDBGPU_DEVICE_ERROR = 8,   /* HIP runtime error or out of device memory */
+  VDBGPU_BAD_ARGUMENT = 9,
+  VDBGPU_NO_INTERSECTOR = 10 /* raytrace before any fast_mode integrate (VM:698, :1438) */
+};
+
+/* BaseConfig (VM:64-71) + Config (OCC:38-44), field for field.
+ * map_directory_path is host-only state kept by the C++ wrapper. */
+typedef struct vdbgpu_config {
+  double max_range;
+  int32_t fast_mode;
+  int32_t _pad;
+  double accumulation_period;
+  double prob_hit;
+  double prob_miss;
+  double prob_thres_min;
+  double prob_thres_max;
+} vdbgpu_config;
+
+/* Cumulative counters of one map handle. Kernel times are measured with HIP
+ * events on the handle's stream when profiling is enabled. */
+typedef struct vdbgpu_stats {
+  uint64_t scans;          /* accumulate calls that cast rays */
+  uint64_t rays;           /* points cast (NaN points excluded) */
+  uint64_t visits;         /* DDA voxel visits, sum of 1 + |dx|+|dy|+|dz| */
+  uint64_t voxel_updates;  /* unique update-set voxels applied to the map */
+  uint64_t update_leaves;  /* update-set leaves applied */
+  uint64_t leaf_count;     /* leaves in the map now */
+  uint64_t device_bytes;   /* device memory held by the handle */
+  double ms_prep, ms_traverse, ms_compact, ms_apply; /* summed kernel times (profiling on) */
+  uint64_t n_prep, n_traverse, n_compact, n_apply;   /* launches timed */
+} vdbgpu_stats;
+
+/* Launch parameters of the integration kernels (see vdbgpu_set_tuning). */
+typedef struct vdbgpu_tuning {
+  int32_t block;          /* threads per workgroup of the traversal kernel (64..1024) */
+  int32_t steps_per_lane; /* DDA visits per lane work item (chunk length) */
+  int32_t waves_per_cu;   /* resident waves per CU requested for the traversal grid */
+  double load_factor;     /* update-set hash target load (0.3..0.9) */
+} vdbgpu_tuning;
+
+typedef struct vdbgpu_map vdbgpu_map;
+
+/* Library version (major*10000 + minor*100 + patch). */
+int32_t vdbgpu_version(void);
+/* Human-readable name of a status code. */
+const char* vdbgpu_status_string(int32_t status);
+/* Number of gfx950 devices visible; 0 when no GPU is present. */
+int32_t vdbgpu_device_count(void);
+
+/* OccupancyVDBMapping(resolution) OCC:49-52 -> VDBMapping(resolution)
+ * VM:115-134 (createVDBMap VM:163-169): empty map of 8^3 float leaves,
+ * background 0, voxel size `resolution`. Uses the current HIP device. */
+int32_t vdbgpu_create(double resolution, vdbgpu_map** out);
+/* ~VDBMapping VM:139-154: frees every device allocation of the handle. */
+void vdbgpu_destroy(vdbgpu_map* map);
+
+/* OccupancyVDBMapping::setConfig OCC:59-89 calling VDBMapping::setConfig
+ * VM:1456-1469, including the reference's partial-application behaviour. */
+int32_t vdbgpu_set_config(vdbgpu_map* map, const vdbgpu_config* config);
+/* Log-odds constants now in force: hit, miss, thres_min, thres_max, clamp max,
+ * clamp min (OCC:79-87). */
+int32_t vdbgpu_logodds(vdbgpu_map* map, float out[6]);
+
+/* addInputSource VM:1352-1375: max_range 0 selects the config's max_range.
+ * max_rate only matters to the reference's background threads. */
+int32_t vdbgpu_add_source(vdbgpu_map* map, const char* source_id, double max_range,
+                          double max_rate);
+
+/* accumulateUpdate VM:316-346 -> raycastPointCloud VM:466-539: cast every
+ * point of the cloud from `origin` into the source's update set. `xyz` holds
+ * n points of 3 float32 at `stride_bytes` (16 for pcl::PointXYZ); it is a host
+ * pointer, or a device pointer when `points_on_device` is non-zero. */
+int32_t vdbgpu_accumulate(vdbgpu_map* map, const char* source_id, const float* xyz, int64_t n,
+                          int64_t stride_bytes, const double origin[3], int32_t points_on_device);
+/* integrateUpdate VM:375-387 -> updateMap VM:731-792 with updateOccupiedNode /
+ * updateFreeNode OCC:92-117: apply and clear every source's update set, in
+ * source-id order. */
+int32_t vdbgpu_integrate(vdbgpu_map* map);
+/* insertPointCloud VM:399-406: accumulate then integrate. */
+int32_t vdbgpu_insert(vdbgpu_map* map, const char* source_id, const float* xyz, int64_t n,
+                      int64_t stride_bytes, const double origin[3], int32_t points_on_device);
+
+/* resetMap VM:174-186: empty map and empty update sets. */
+int32_t vdbgpu_reset(vdbgpu_map* map);
+/* GridT::empty() as used at VM:494 and VM:1438: 1 if no voxel was ever modified. */
+int32_t vdbgpu_empty(vdbgpu_map* map);
+
+/* getGrid VM:799, as a leaf dump in openvdb::FloatTree leaf layout: for leaf
+ * i, origins[3i..3i+2] (multiples of 8), values[512i..] float32 indexed by
+ * ((x&7)<<6)|((y&7)<<3)|(z&7), active_masks[8i..] (bit n&63 of word n>>6).
+ * Leaves are sorted by origin (x, then y, then z). Returns the leaf count, or
+ * minus the needed count when cap is too small (nothing written). Any pointer
+ * may be NULL to query the count only. */
+int64_t vdbgpu_leaf_count(vdbgpu_map* map);
+int64_t vdbgpu_export_leaves(vdbgpu_map* map, int32_t* origins, float* values,
+                             uint64_t* active_masks, int64_t cap);
+/* Leaves changed by the integrations since the previous call of this
+ * function (same layout; used to keep a host grid live without a full dump). */
+int64_t vdbgpu_export_dirty_leaves(vdbgpu_map* map, int32_t* origins, float* values,
+                                   uint64_t* active_masks, int64_t cap);
+
+/* The source's pending update set (UpdateGridT VM:89, filled by
+ * raycastPointCloud VM:466-539): per leaf, visited (= active) and hit
+ * (= value true) masks in the same bit layout, sorted by origin. */
+int64_t vdbgpu_update_leaf_count(vdbgpu_map* map, const char* source_id);
+int64_t vdbgpu_export_update(vdbgpu_map* map, const char* source_id, int32_t* origins,
+                             uint64_t* visited, uint64_t* hit, int64_t cap);
+
+/* raytrace VM:675-721 (batch; the single-ray form VM:643-666 is n = 1):
+ * origins, directions (n x 3 double, world frame), max lengths (n); writes
+ * success[n] and end_points (n x 3 double). Uses the topology snapshot taken
+ * by the last integrate in fast mode (updateVolumeRayIntersectors
+ * VM:1436-1449); VDBGPU_NO_INTERSECTOR when there is none. Host pointers. */
+int32_t vdbgpu_raytrace(vdbgpu_map* map, int64_t n, const double* origins,
+                        const double* directions, const double* max_lengths, uint8_t* success,
+                        double* end_points);
+
+/* Wait for all work of the handle. */
+int32_t vdbgpu_synchronize(vdbgpu_map* map);
+/* Counters; profiling adds HIP-event timing of every kernel launch. */
+int32_t vdbgpu_get_stats(vdbgpu_map* map, vdbgpu_stats* out);
+int32_t vdbgpu_set_profiling(vdbgpu_map* map, int32_t enabled);
+int32_t vdbgpu_set_tuning(vdbgpu_map* map, const vdbgpu_tuning* tuning);
+int32_t vdbgpu_get_tuning(vdbgpu_map* map, vdbgpu_tuning* out);
+/* Text of the last error on this handle ("" when none). */
+const char* vdbgpu_last_error(vdbgpu_map* map);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VDBGPU_H */

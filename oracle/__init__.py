@@ -1,0 +1,198 @@
+"""ctypes wrapper of the CPU oracle (oracle/oracle.cpp).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() and
+by the CPU-baseline leg of bench.py, always as the checker or the baseline,
+never as the measured or shipped path.  The product package
+(vdb_mapping_amd) does not import it.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liborc.so")
+
+OK, NOT_CONFIGURED, UNKNOWN_SOURCE, BAD_MAX_RANGE, BAD_PROB_MISS, BAD_PROB_HIT = 0, 1, 2, 3, 4, 5
+OUT_OF_RANGE, UNSUPPORTED, NO_INTERSECTOR = 6, 7, 10
+
+BUILD_CMD = [
+    "g++", "-O2", "-std=c++17", "-ffp-contract=off", "-fno-fast-math", "-fPIC", "-shared",
+    os.path.join(HERE, "oracle.cpp"), "-o", LIB_PATH,
+]
+
+
+def build() -> None:
+    """Compile liborc.so from oracle.cpp (g++, no FMA contraction)."""
+    subprocess.run(BUILD_CMD, check=True)
+
+
+class Config(ctypes.Structure):
+    """BaseConfig (VDBMapping.hpp:64-71) + Config (OccupancyVDBMapping.hpp:38-44)."""
+
+    _fields_ = [
+        ("max_range", ctypes.c_double),
+        ("fast_mode", ctypes.c_int32),
+        ("_pad", ctypes.c_int32),
+        ("accumulation_period", ctypes.c_double),
+        ("prob_hit", ctypes.c_double),
+        ("prob_miss", ctypes.c_double),
+        ("prob_thres_min", ctypes.c_double),
+        ("prob_thres_max", ctypes.c_double),
+    ]
+
+
+def make_config(max_range=10.0, fast_mode=False, prob_hit=0.9, prob_miss=0.1,
+                prob_thres_min=0.49, prob_thres_max=0.51, accumulation_period=0.0) -> Config:
+    return Config(max_range, int(fast_mode), 0, accumulation_period, prob_hit, prob_miss,
+                  prob_thres_min, prob_thres_max)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        P, D, I64, I32 = ctypes.c_void_p, ctypes.c_double, ctypes.c_int64, ctypes.c_int32
+        L.orc_create.restype = P
+        L.orc_create.argtypes = [D]
+        L.orc_destroy.argtypes = [P]
+        L.orc_set_config.argtypes = [P, ctypes.POINTER(Config)]
+        L.orc_add_source.argtypes = [P, ctypes.c_char_p, D, D]
+        for f in (L.orc_accumulate, L.orc_insert):
+            f.argtypes = [P, ctypes.c_char_p, P, I64, I64, P]
+        L.orc_integrate.argtypes = [P]
+        L.orc_reset.argtypes = [P]
+        L.orc_empty.argtypes = [P]
+        L.orc_logodds.argtypes = [P, P]
+        L.orc_leaf_count.restype = I64
+        L.orc_leaf_count.argtypes = [P]
+        L.orc_export_leaves.restype = I64
+        L.orc_export_leaves.argtypes = [P, P, P, P, I64]
+        L.orc_update_leaf_count.restype = I64
+        L.orc_update_leaf_count.argtypes = [P, ctypes.c_char_p]
+        L.orc_export_update.restype = I64
+        L.orc_export_update.argtypes = [P, ctypes.c_char_p, P, P, P, I64]
+        L.orc_get_value.restype = ctypes.c_float
+        L.orc_get_value.argtypes = [P, P, P]
+        L.orc_world_to_index.argtypes = [D, P, P]
+        L.orc_cast_ray.restype = I64
+        L.orc_cast_ray.argtypes = [P, P, P, I64]
+        L.orc_raytrace.argtypes = [P, I64, P, P, P, P, P]
+        _lib = L
+    return _lib
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def world_to_index(res: float, w) -> np.ndarray:
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    out = np.zeros(3, np.int32)
+    lib().orc_world_to_index(res, _ptr(w), _ptr(out))
+    return out
+
+
+def cast_ray(s, e) -> np.ndarray:
+    s = np.ascontiguousarray(s, dtype=np.int32)
+    e = np.ascontiguousarray(e, dtype=np.int32)
+    n = lib().orc_cast_ray(_ptr(s), _ptr(e), None, 0)
+    out = np.zeros((max(n, 1), 3), np.int32)
+    lib().orc_cast_ray(_ptr(s), _ptr(e), _ptr(out), n)
+    return out[:n]
+
+
+class OracleMap:
+    """OccupancyVDBMapping restated on the CPU (same method meaning as the reference)."""
+
+    def __init__(self, resolution: float):
+        self.h = lib().orc_create(resolution)
+        self.resolution = resolution
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_destroy(self.h)
+            self.h = None
+
+    def set_config(self, cfg: Config) -> int:
+        return lib().orc_set_config(self.h, ctypes.byref(cfg))
+
+    def add_input_source(self, source_id: str, max_range: float, max_rate: float = 0.0) -> int:
+        return lib().orc_add_source(self.h, source_id.encode(), max_range, max_rate)
+
+    @staticmethod
+    def _pts(points):
+        pts = np.ascontiguousarray(points, dtype=np.float32)
+        if pts.ndim != 2 or pts.shape[1] not in (3, 4):
+            raise ValueError("points must be (n,3) or (n,4) float32")
+        return pts, pts.shape[1] * 4
+
+    def accumulate(self, source_id: str, points, origin) -> int:
+        pts, stride = self._pts(points)
+        o = np.ascontiguousarray(origin, dtype=np.float64)
+        return lib().orc_accumulate(self.h, source_id.encode(), _ptr(pts), pts.shape[0], stride,
+                                    _ptr(o))
+
+    def integrate(self) -> int:
+        return lib().orc_integrate(self.h)
+
+    def insert(self, source_id: str, points, origin) -> int:
+        pts, stride = self._pts(points)
+        o = np.ascontiguousarray(origin, dtype=np.float64)
+        return lib().orc_insert(self.h, source_id.encode(), _ptr(pts), pts.shape[0], stride,
+                                _ptr(o))
+
+    def reset(self) -> None:
+        lib().orc_reset(self.h)
+
+    def empty(self) -> bool:
+        return bool(lib().orc_empty(self.h))
+
+    def logodds(self) -> np.ndarray:
+        out = np.zeros(6, np.float32)
+        lib().orc_logodds(self.h, _ptr(out))
+        return out
+
+    def export_leaves(self):
+        n = lib().orc_leaf_count(self.h)
+        o = np.zeros((n, 3), np.int32)
+        v = np.zeros((n, 512), np.float32)
+        m = np.zeros((n, 8), np.uint64)
+        got = lib().orc_export_leaves(self.h, _ptr(o), _ptr(v), _ptr(m), n)
+        assert got == n
+        return o, v, m
+
+    def export_update(self, source_id: str):
+        n = lib().orc_update_leaf_count(self.h, source_id.encode())
+        if n < 0:
+            raise KeyError(source_id)
+        o = np.zeros((n, 3), np.int32)
+        vis = np.zeros((n, 8), np.uint64)
+        hit = np.zeros((n, 8), np.uint64)
+        got = lib().orc_export_update(self.h, source_id.encode(), _ptr(o), _ptr(vis), _ptr(hit), n)
+        assert got == n
+        return o, vis, hit
+
+    def get_value(self, ijk):
+        c = np.ascontiguousarray(ijk, dtype=np.int32)
+        act = np.zeros(1, np.int32)
+        v = lib().orc_get_value(self.h, _ptr(c), _ptr(act))
+        return np.float32(v), bool(act[0])
+
+    def raytrace(self, origins, dirs, lengths):
+        o = np.ascontiguousarray(origins, dtype=np.float64).reshape(-1, 3)
+        d = np.ascontiguousarray(dirs, dtype=np.float64).reshape(-1, 3)
+        ln = np.ascontiguousarray(lengths, dtype=np.float64).reshape(-1)
+        n = o.shape[0]
+        succ = np.zeros(n, np.uint8)
+        ends = np.zeros((n, 3), np.float64)
+        st = lib().orc_raytrace(self.h, n, _ptr(o), _ptr(d), _ptr(ln), _ptr(succ), _ptr(ends))
+        return st, succ.astype(bool), ends

@@ -1,0 +1,711 @@
+// CPU restatement of vdb_mapping's occupancy-integration path.
+//
+// TEST INFRASTRUCTURE ONLY: the parity checker and the CPU baseline. The
+// product never links, loads or calls this file (see oracle.h).
+//
+// Reference files cited: include/vdb_mapping/VDBMapping.hpp (VM) and
+// include/vdb_mapping/OccupancyVDBMapping.hpp (OCC). The reference delegates
+// its voxel arithmetic to OpenVDB (not part of the reference tree and not
+// installed here). The OpenVDB semantics restated below are marked [U#]; each
+// is a choice recalled from OpenVDB's published headers and kept behind one
+// macro so that a different upstream version can be selected:
+//   U1 ScaleMap::applyInverseMap multiplies by a stored 1/scale.
+//   U2 Vec3::unit() divides each component by length().
+//   U3 Vec3::normalize() multiplies each component by 1/length().
+//   U4 math::MinIndex breaks ties toward the larger axis index.
+//   U5 math::DDA::init/step as restated in struct Dda below.
+//   U6 Ray::clip(CoordBBox) uses the bbox corners as given (no extra +1).
+//   U7 VolumeHDDA descends root-child (4096) -> 128 -> 8 and tests node existence.
+//   U8 a modify on a missing leaf creates it only if the op changes value or state.
+//   U9 evalActiveBoundingBox(bbox, false) expands by the node bbox of each leaf
+//      that holds an active voxel.
+//   U10 Ray::valid() uses eps = Delta<float> (1e-5f); TimeSpan::valid() uses
+//      Delta<double> (1e-9).
+//
+// Compile with -ffp-contract=off (no FMA contraction): the reference is built
+// for baseline x86-64 (Release, no -march flags, VM CMakeLists.txt:8-11),
+// where no fused multiply-add exists, so every double op rounds on its own.
+
+#include "oracle.h"
+
+#include <algorithm>
+#include <array>
+#include <cfloat>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <unordered_map>
+#include <unordered_set>
+#include <vector>
+
+#ifndef ORC_U2_UNIT_DIVIDES
+#define ORC_U2_UNIT_DIVIDES 1
+#endif
+#ifndef ORC_U6_CLIP_PAD
+#define ORC_U6_CLIP_PAD 0
+#endif
+
+namespace {
+
+constexpr int kCoordLimit = 1 << 23;  // |voxel coord| must stay below this (21-bit leaf keys)
+
+inline bool in_range(const int32_t c[3]) {
+  for (int a = 0; a < 3; ++a)
+    if (c[a] < -kCoordLimit || c[a] >= kCoordLimit) return false;
+  return true;
+}
+
+inline uint64_t leaf_key(int32_t x, int32_t y, int32_t z) {
+  const uint64_t m = (1ull << 21) - 1;
+  return ((uint64_t)(uint32_t)(x >> 3) & m) << 42 | ((uint64_t)(uint32_t)(y >> 3) & m) << 21 |
+         ((uint64_t)(uint32_t)(z >> 3) & m);
+}
+
+inline int32_t sext21(uint64_t v) { return (int32_t)((int64_t)(v << 43) >> 43); }
+
+inline void key_origin(uint64_t k, int32_t o[3]) {
+  const uint64_t m = (1ull << 21) - 1;
+  o[0] = sext21((k >> 42) & m) * 8;
+  o[1] = sext21((k >> 21) & m) * 8;
+  o[2] = sext21(k & m) * 8;
+}
+
+// Voxel offset inside an 8^3 leaf, OpenVDB LeafNode::coordToOffset layout.
+inline int leaf_offset(int32_t x, int32_t y, int32_t z) {
+  return ((x & 7) << 6) | ((y & 7) << 3) | (z & 7);
+}
+
+// Rounded world->index (VM:612-631). fmod != 0 (NaN counts as non-zero)
+// shifts by res/2, then Coord::floor(ScaleMap::applyInverseMap) [U1].
+inline void world_to_index(double res, double inv, const double w[3], int32_t out[3]) {
+  for (int a = 0; a < 3; ++a) {
+    double c = w[a];
+    if (std::fmod(c, res) != 0.0) c = c + (res / 2.0);
+    const double f = std::floor(c * inv);
+    out[a] = (f >= -2147483648.0 && f < 2147483648.0) ? (int32_t)f : INT32_MIN;
+  }
+}
+
+// math::MinIndex [U4]: ties go to the larger index.
+inline int min_index(const double v[3]) {
+  int r = 0;
+  if (v[1] <= v[r]) r = 1;
+  if (v[2] <= v[r]) r = 2;
+  return r;
+}
+
+// math::DDA<Ray<double>, Log2Dim> [U5].
+struct Dda {
+  int32_t vox[3];
+  int32_t step_[3];
+  double next_[3], delta_[3];
+  double t0, t1;
+  void init(const double eye[3], const double dir[3], const double inv[3], double start,
+            double max_t, int log2dim) {
+    const int dim = 1 << log2dim;
+    t0 = start;
+    t1 = max_t;
+    double pos[3];
+    for (int a = 0; a < 3; ++a) pos[a] = eye[a] + dir[a] * t0;  // Ray::operator()(t0)
+    for (int a = 0; a < 3; ++a) vox[a] = (int32_t)std::floor(pos[a]) & ~(dim - 1);
+    for (int a = 0; a < 3; ++a) {
+      if (dir[a] == 0.0) {
+        step_[a] = 0;
+        next_[a] = DBL_MAX;
+        delta_[a] = DBL_MAX;
+      } else if (inv[a] > 0) {
+        step_[a] = dim;
+        next_[a] = t0 + ((double)(vox[a] + dim) - pos[a]) * inv[a];
+        delta_[a] = (double)step_[a] * inv[a];
+      } else {
+        step_[a] = -dim;
+        next_[a] = t0 + ((double)vox[a] - pos[a]) * inv[a];
+        delta_[a] = (double)step_[a] * inv[a];
+      }
+    }
+  }
+  bool step() {
+    const int a = min_index(next_);
+    t0 = next_[a];
+    next_[a] += delta_[a];
+    vox[a] += step_[a];
+    return t0 <= t1;
+  }
+  double time() const { return t0; }
+  double max_time() const { return t1; }
+  double next() const { return std::min(t1, std::min(next_[0], std::min(next_[1], next_[2]))); }
+};
+
+// castRayIntoGrid (VM:550-566): Ray(s + 0.5, e - s, 0, 1), DDA<.,0> from t=0,
+// visit, then step while the crossing time stays <= 1.
+template <class Visit>
+inline void cast_ray(const int32_t s[3], const int32_t e[3], Visit&& visit) {
+  if (s[0] == e[0] && s[1] == e[1] && s[2] == e[2]) return;
+  double eye[3], dir[3], inv[3];
+  for (int a = 0; a < 3; ++a) {
+    eye[a] = (double)s[a] + 0.5;
+    dir[a] = (double)e[a] - (double)s[a];
+    inv[a] = 1.0 / dir[a];
+  }
+  Dda d;
+  d.init(eye, dir, inv, 0.0, 1.0, 0);
+  do {
+    visit(d.vox[0], d.vox[1], d.vox[2]);
+  } while (d.step());
+}
+
+struct UpdLeaf {
+  uint64_t vis[8];
+  uint64_t hit[8];
+};
+
+// UpdateGridT (VM:89): active = visited, value = hit (VM:535, :563).
+struct UpdateSet {
+  std::unordered_map<uint64_t, UpdLeaf> leaves;
+  uint64_t last_key = ~0ull;
+  UpdLeaf* last = nullptr;  // one-entry cache, like a ValueAccessor
+  UpdLeaf& get(uint64_t k) {
+    if (k == last_key) return *last;
+    auto it = leaves.find(k);
+    if (it == leaves.end()) {
+      UpdLeaf z;
+      std::memset(&z, 0, sizeof z);
+      it = leaves.emplace(k, z).first;
+    }
+    last_key = k;
+    last = &it->second;
+    return *last;
+  }
+  void clear() {
+    leaves.clear();
+    last_key = ~0ull;
+    last = nullptr;
+  }
+};
+
+struct Leaf {
+  float v[512];
+  uint64_t act[8];
+};
+
+struct Source {
+  std::string id;
+  double max_range = 0;
+  UpdateSet upd;
+};
+
+// Snapshot taken by updateVolumeRayIntersectors (VM:1436-1449): topology of
+// the map plus the padded active bbox [U9].
+struct Intersector {
+  std::unordered_set<uint64_t> leaves, n128, n4096;
+  int32_t bmin[3], bmax[3];
+};
+
+struct Map {
+  double res, inv;
+  // Config state (VM:1456-1469, OCC:59-89). Zero-initialised where the
+  // reference leaves members uninitialised.
+  bool config_set = false;
+  double max_range = 0;
+  bool fast_mode = false;
+  float l_hit = 0, l_miss = 0, t_min = 0, t_max = 0, c_max = 0, c_min = 0;
+  std::map<std::string, Source> sources;  // std::map: integrate order = id order (VM:380)
+  std::unordered_map<uint64_t, Leaf> leaves;
+  std::unordered_set<uint64_t> n4096;  // root children (created even by no-op modifies)
+  std::unique_ptr<Intersector> isect;
+};
+
+// Keys of the 4096^3 (root child) and 128^3 internal nodes holding voxel (x,y,z).
+uint64_t key4096(int32_t x, int32_t y, int32_t z) {
+  return leaf_key((x >> 12) * 8, (y >> 12) * 8, (z >> 12) * 8);
+}
+uint64_t key128(int32_t x, int32_t y, int32_t z) {
+  return leaf_key((x >> 7) * 8, (y >> 7) * 8, (z >> 7) * 8);
+}
+
+// updateOccupiedNode / updateFreeNode (OCC:92-117).
+inline void op_hit(const Map& m, float& x, bool& act) {
+  x += m.l_hit;
+  if (x > m.t_max) {
+    act = true;
+    if (x > m.c_max) x = m.c_max;
+  }
+}
+inline void op_miss(const Map& m, float& x, bool& act) {
+  x += m.l_miss;
+  if (x < m.t_min) {
+    act = false;
+    if (x < m.c_min) x = m.c_min;
+  }
+}
+
+// updateMap (VM:731-792) over one source's update set. Leaves are created
+// only when the op would change a background tile [U8].
+void apply_update(Map& m, UpdateSet& u) {
+  for (auto& kv : u.leaves) {
+    const uint64_t k = kv.first;
+    const UpdLeaf& ul = kv.second;
+    int32_t o[3];
+    key_origin(k, o);
+    m.n4096.insert(key4096(o[0], o[1], o[2]));
+    auto it = m.leaves.find(k);
+    Leaf* leaf = nullptr;
+    if (it != m.leaves.end()) {
+      leaf = &it->second;
+    } else {
+      bool need = false;
+      for (int w = 0; w < 8 && !need; ++w) {
+        const uint64_t hv = ul.vis[w] & ul.hit[w], mv = ul.vis[w] & ~ul.hit[w];
+        float x = 0.0f;
+        bool a = false;
+        if (hv) {
+          op_hit(m, x, a);
+          need |= (x != 0.0f) || a;
+        }
+        x = 0.0f;
+        a = false;
+        if (mv) {
+          op_miss(m, x, a);
+          need |= (x != 0.0f) || a;
+        }
+      }
+      if (!need) continue;
+      Leaf z;
+      std::memset(&z, 0, sizeof z);
+      leaf = &m.leaves.emplace(k, z).first->second;
+    }
+    for (int w = 0; w < 8; ++w) {
+      uint64_t vis = ul.vis[w];
+      while (vis) {
+        const int b = __builtin_ctzll(vis);
+        vis &= vis - 1;
+        const int n = w * 64 + b;
+        bool a = (leaf->act[w] >> b) & 1;
+        if ((ul.hit[w] >> b) & 1)
+          op_hit(m, leaf->v[n], a);
+        else
+          op_miss(m, leaf->v[n], a);
+        leaf->act[w] = (leaf->act[w] & ~(1ull << b)) | ((uint64_t)a << b);
+      }
+    }
+  }
+}
+
+// updateVolumeRayIntersectors (VM:1436-1449) for the state that raytrace needs.
+void build_intersector(Map& m) {
+  auto is = std::make_unique<Intersector>();
+  bool any = false;
+  for (auto& kv : m.leaves) {
+    int32_t o[3];
+    key_origin(kv.first, o);
+    is->leaves.insert(kv.first);
+    is->n128.insert(key128(o[0], o[1], o[2]));
+    bool active = false;
+    for (int w = 0; w < 8; ++w) active |= kv.second.act[w] != 0;
+    if (!active) continue;
+    for (int a = 0; a < 3; ++a) {
+      if (!any || o[a] < is->bmin[a]) is->bmin[a] = o[a];
+      if (!any || o[a] + 7 > is->bmax[a]) is->bmax[a] = o[a] + 7;
+    }
+    any = true;
+  }
+  is->n4096 = m.n4096;
+  if (!any) {
+    // CoordBBox default: min = INT_MAX, max = INT_MIN (an empty box).
+    for (int a = 0; a < 3; ++a) {
+      is->bmin[a] = INT32_MAX;
+      is->bmax[a] = INT32_MIN;
+    }
+  }
+  for (int a = 0; a < 3; ++a) is->bmax[a] += 1;  // mBBox.max().offset(1)
+  m.isect = std::move(is);
+}
+
+// raycastPointCloud (VM:466-539) in normal (non-fast) mode.
+int raycast(Map& m, Source& src, const float* xyz, int64_t n, int64_t stride,
+            const double origin[3]) {
+  if (!m.config_set) return ORC_NOT_CONFIGURED;  // VM:478-482
+  if (m.fast_mode) return ORC_UNSUPPORTED;
+  const double R = src.max_range;
+  for (int a = 0; a < 3; ++a)
+    if (std::isnan(origin[a])) return ORC_OK;  // every point is skipped (VM:505-510)
+  int32_t s[3];
+  world_to_index(m.res, m.inv, origin, s);
+  // Pass 1: validate so that an out-of-range scan leaves the update set untouched.
+  std::vector<int32_t> ends((size_t)n * 3);
+  std::vector<uint8_t> flags((size_t)n);
+  const char* base = (const char*)xyz;
+  for (int64_t i = 0; i < n; ++i) {
+    const float* pt = (const float*)(base + i * stride);
+    double p[3] = {(double)pt[0], (double)pt[1], (double)pt[2]};
+    if (std::isnan(p[0]) || std::isnan(p[1]) || std::isnan(p[2])) {
+      flags[i] = 0;
+      continue;
+    }
+    bool trunc = false;
+    const double d[3] = {p[0] - origin[0], p[1] - origin[1], p[2] - origin[2]};
+    const double len = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    if (R > 0.0 && len > R) {  // VM:512-517
+      for (int a = 0; a < 3; ++a) {
+#if ORC_U2_UNIT_DIVIDES
+        const double u = d[a] / len;
+#else
+        const double u = d[a] * (1.0 / len);
+#endif
+        p[a] = origin[a] + u * R;
+      }
+      trunc = true;
+    }
+    int32_t* e = &ends[(size_t)i * 3];
+    world_to_index(m.res, m.inv, p, e);
+    if (!in_range(e) || !in_range(s)) return ORC_OUT_OF_RANGE;
+    flags[i] = trunc ? 1 : 2;
+  }
+  // Pass 2: traverse (VM:529) and mark endpoints (VM:533-536).
+  UpdateSet& u = src.upd;
+  for (int64_t i = 0; i < n; ++i) {
+    if (!flags[i]) continue;
+    const int32_t* e = &ends[(size_t)i * 3];
+    cast_ray(s, e, [&](int32_t x, int32_t y, int32_t z) {
+      UpdLeaf& l = u.get(leaf_key(x, y, z));
+      const int off = leaf_offset(x, y, z);
+      l.vis[off >> 6] |= 1ull << (off & 63);
+    });
+    if (flags[i] == 2) {
+      UpdLeaf& l = u.get(leaf_key(e[0], e[1], e[2]));
+      const int off = leaf_offset(e[0], e[1], e[2]);
+      l.vis[off >> 6] |= 1ull << (off & 63);
+      l.hit[off >> 6] |= 1ull << (off & 63);
+    }
+  }
+  return ORC_OK;
+}
+
+int64_t export_sorted_keys(const std::vector<uint64_t>& keys_in, std::vector<uint64_t>& out) {
+  std::vector<std::pair<std::array<int32_t, 3>, uint64_t>> v;
+  v.reserve(keys_in.size());
+  for (uint64_t k : keys_in) {
+    std::array<int32_t, 3> o;
+    key_origin(k, o.data());
+    v.emplace_back(o, k);
+  }
+  std::sort(v.begin(), v.end());
+  out.clear();
+  for (auto& p : v) out.push_back(p.second);
+  return (int64_t)out.size();
+}
+
+// VolumeHDDA (openvdb/math/DDA.h) over the intersector snapshot [U7].
+struct Span {
+  double t0 = -1, t1 = -1;
+  bool valid() const { return (t1 - t0) > 1e-9; }  // TimeSpan::valid [U10]
+};
+
+struct RayState {
+  double eye[3], dir[3], inv[3];
+  double t0, t1;
+};
+
+bool hdda_leaf(const Intersector& is, RayState& ray, Span& t) {
+  Dda d;
+  d.init(ray.eye, ray.dir, ray.inv, ray.t0, ray.t1, 3);
+  do {
+    if (is.leaves.count(leaf_key(d.vox[0], d.vox[1], d.vox[2]))) {
+      if (t.t0 < 0) t.t0 = d.time();
+    } else if (t.t0 >= 0) {
+      t.t1 = d.time();
+      if (t.valid()) return true;
+      t.t0 = t.t1 = -1;
+    }
+  } while (d.step());
+  if (t.t0 >= 0) t.t1 = d.max_time();
+  return false;
+}
+
+bool hdda_128(const Intersector& is, RayState& ray, Span& t) {
+  Dda d;
+  d.init(ray.eye, ray.dir, ray.inv, ray.t0, ray.t1, 7);
+  do {
+    if (is.n128.count(key128(d.vox[0], d.vox[1], d.vox[2]))) {
+      ray.t0 = d.time();
+      ray.t1 = d.next();
+      if (hdda_leaf(is, ray, t)) return true;
+    } else if (t.t0 >= 0) {
+      t.t1 = d.time();
+      if (t.valid()) return true;
+      t.t0 = t.t1 = -1;
+    }
+  } while (d.step());
+  if (t.t0 >= 0) t.t1 = d.max_time();
+  return false;
+}
+
+bool hdda_4096(const Intersector& is, RayState& ray, Span& t) {
+  Dda d;
+  d.init(ray.eye, ray.dir, ray.inv, ray.t0, ray.t1, 12);
+  do {
+    if (is.n4096.count(key4096(d.vox[0], d.vox[1], d.vox[2]))) {
+      ray.t0 = d.time();
+      ray.t1 = d.next();
+      if (hdda_128(is, ray, t)) return true;
+    } else if (t.t0 >= 0) {
+      t.t1 = d.time();
+      if (t.valid()) return true;
+      t.t0 = t.t1 = -1;
+    }
+  } while (d.step());
+  if (t.t0 >= 0) t.t1 = d.max_time();
+  return false;
+}
+
+bool map_is_on(const Map& m, int32_t x, int32_t y, int32_t z) {
+  if (x < -kCoordLimit || x >= kCoordLimit || y < -kCoordLimit || y >= kCoordLimit ||
+      z < -kCoordLimit || z >= kCoordLimit)
+    return false;
+  auto it = m.leaves.find(leaf_key(x, y, z));
+  if (it == m.leaves.end()) return false;
+  const int off = leaf_offset(x, y, z);
+  return (it->second.act[off >> 6] >> (off & 63)) & 1;
+}
+
+}  // namespace
+
+extern "C" {
+
+void* orc_create(double resolution) {
+  Map* m = new Map();
+  m->res = resolution;
+  m->inv = 1.0 / resolution;  // UniformScaleMap stores 1/scale [U1]
+  return m;
+}
+
+void orc_destroy(void* h) { delete (Map*)h; }
+
+// OccupancyVDBMapping::setConfig (OCC:59-89) calling VDBMapping::setConfig
+// (VM:1456-1469). The base rejects max_range < 0 but the derived part still
+// runs; a failed probability check returns after the base has already marked
+// the map configured.
+int orc_set_config(void* h, const orc_config* c) {
+  Map& m = *(Map*)h;
+  int status = ORC_OK;
+  if (c->max_range < 0.0) {
+    status = ORC_BAD_MAX_RANGE;
+  } else {
+    m.max_range = c->max_range;
+    m.fast_mode = c->fast_mode != 0;
+    m.config_set = true;
+  }
+  if (c->prob_miss > 0.5) return ORC_BAD_PROB_MISS;
+  if (c->prob_hit < 0.5) return ORC_BAD_PROB_HIT;
+  m.l_miss = (float)(std::log(c->prob_miss) - std::log(1 - c->prob_miss));
+  m.l_hit = (float)(std::log(c->prob_hit) - std::log(1 - c->prob_hit));
+  m.t_min = (float)(std::log(c->prob_thres_min) - std::log(1 - c->prob_thres_min));
+  m.t_max = (float)(std::log(c->prob_thres_max) - std::log(1 - c->prob_thres_max));
+  m.c_max = (float)(std::log(0.99) - std::log(0.01));
+  m.c_min = (float)(std::log(0.01) - std::log(0.99));
+  m.config_set = true;
+  return status;
+}
+
+// addInputSource (VM:1352-1375): max_range 0 means the config's max_range.
+int orc_add_source(void* h, const char* id, double max_range, double max_rate) {
+  (void)max_rate;
+  Map& m = *(Map*)h;
+  Source& s = m.sources[id];
+  s.id = id;
+  s.max_range = (max_range == 0) ? m.max_range : max_range;
+  s.upd.clear();
+  return ORC_OK;
+}
+
+// accumulateUpdate (VM:316-346).
+int orc_accumulate(void* h, const char* id, const float* xyz, int64_t n, int64_t stride,
+                   const double origin[3]) {
+  Map& m = *(Map*)h;
+  auto it = m.sources.find(id);
+  if (it == m.sources.end()) return ORC_UNKNOWN_SOURCE;
+  if (!(it->second.max_range > 0)) return ORC_OK;
+  return raycast(m, it->second, xyz, n, stride, origin);
+}
+
+// integrateUpdate (VM:375-387).
+int orc_integrate(void* h) {
+  Map& m = *(Map*)h;
+  for (auto& kv : m.sources) {
+    apply_update(m, kv.second.upd);
+    kv.second.upd.clear();
+  }
+  if (m.fast_mode && !m.n4096.empty()) build_intersector(m);
+  return ORC_OK;
+}
+
+// insertPointCloud (VM:399-406).
+int orc_insert(void* h, const char* id, const float* xyz, int64_t n, int64_t stride,
+               const double origin[3]) {
+  const int s = orc_accumulate(h, id, xyz, n, stride, origin);
+  orc_integrate(h);
+  return s;
+}
+
+// resetMap (VM:174-186).
+void orc_reset(void* h) {
+  Map& m = *(Map*)h;
+  m.leaves.clear();
+  m.n4096.clear();
+  for (auto& kv : m.sources) kv.second.upd.clear();
+}
+
+int orc_empty(void* h) { return ((Map*)h)->n4096.empty() ? 1 : 0; }
+
+void orc_logodds(void* h, float out[6]) {
+  Map& m = *(Map*)h;
+  out[0] = m.l_hit;
+  out[1] = m.l_miss;
+  out[2] = m.t_min;
+  out[3] = m.t_max;
+  out[4] = m.c_max;
+  out[5] = m.c_min;
+}
+
+int64_t orc_leaf_count(void* h) { return (int64_t)((Map*)h)->leaves.size(); }
+
+int64_t orc_export_leaves(void* h, int32_t* origins, float* values, uint64_t* masks,
+                          int64_t cap) {
+  Map& m = *(Map*)h;
+  std::vector<uint64_t> keys, sorted;
+  for (auto& kv : m.leaves) keys.push_back(kv.first);
+  const int64_t n = export_sorted_keys(keys, sorted);
+  if (n > cap) return -n;
+  for (int64_t i = 0; i < n; ++i) {
+    const Leaf& l = m.leaves.at(sorted[i]);
+    key_origin(sorted[i], origins + 3 * i);
+    std::memcpy(values + 512 * i, l.v, sizeof l.v);
+    std::memcpy(masks + 8 * i, l.act, sizeof l.act);
+  }
+  return n;
+}
+
+int64_t orc_update_leaf_count(void* h, const char* id) {
+  Map& m = *(Map*)h;
+  auto it = m.sources.find(id);
+  return it == m.sources.end() ? -1 : (int64_t)it->second.upd.leaves.size();
+}
+
+int64_t orc_export_update(void* h, const char* id, int32_t* origins, uint64_t* vis,
+                          uint64_t* hit, int64_t cap) {
+  Map& m = *(Map*)h;
+  auto it = m.sources.find(id);
+  if (it == m.sources.end()) return -1;
+  std::vector<uint64_t> keys, sorted;
+  for (auto& kv : it->second.upd.leaves) keys.push_back(kv.first);
+  const int64_t n = export_sorted_keys(keys, sorted);
+  if (n > cap) return -n;
+  for (int64_t i = 0; i < n; ++i) {
+    const UpdLeaf& l = it->second.upd.leaves.at(sorted[i]);
+    key_origin(sorted[i], origins + 3 * i);
+    std::memcpy(vis + 8 * i, l.vis, sizeof l.vis);
+    std::memcpy(hit + 8 * i, l.hit, sizeof l.hit);
+  }
+  return n;
+}
+
+float orc_get_value(void* h, const int32_t ijk[3], int32_t* active) {
+  Map& m = *(Map*)h;
+  *active = 0;
+  if (!in_range(ijk)) return 0.0f;
+  auto it = m.leaves.find(leaf_key(ijk[0], ijk[1], ijk[2]));
+  if (it == m.leaves.end()) return 0.0f;
+  const int off = leaf_offset(ijk[0], ijk[1], ijk[2]);
+  *active = (int32_t)((it->second.act[off >> 6] >> (off & 63)) & 1);
+  return it->second.v[off];
+}
+
+void orc_world_to_index(double res, const double w[3], int32_t out[3]) {
+  world_to_index(res, 1.0 / res, w, out);
+}
+
+int64_t orc_cast_ray(const int32_t s[3], const int32_t e[3], int32_t* out, int64_t cap) {
+  int64_t n = 0;
+  cast_ray(s, e, [&](int32_t x, int32_t y, int32_t z) {
+    if (n < cap) {
+      out[3 * n] = x;
+      out[3 * n + 1] = y;
+      out[3 * n + 2] = z;
+    }
+    ++n;
+  });
+  return n;
+}
+
+// raytrace (VM:675-721).
+int orc_raytrace(void* h, int64_t n, const double* origins, const double* dirs,
+                 const double* lens, uint8_t* success, double* ends) {
+  Map& m = *(Map*)h;
+  if (!m.isect) return ORC_NO_INTERSECTOR;
+  const Intersector& is = *m.isect;
+  for (int64_t i = 0; i < n; ++i) {
+    double d[3] = {dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]};
+    // Vec3::normalize [U3] (eps 1e-7), then *= max length.
+    const double len = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    if (!(std::fabs(0.0 - len) <= 1e-7)) {
+      const double r = 1.0 / len;
+      for (int a = 0; a < 3; ++a) d[a] *= r;
+    }
+    for (int a = 0; a < 3; ++a) d[a] *= lens[i];
+    RayState ray;
+    for (int a = 0; a < 3; ++a) {
+      ray.eye[a] = origins[3 * i + a] * m.inv;  // continuous worldToIndex (VM:692)
+      ray.dir[a] = d[a] * m.inv;
+      ray.inv[a] = 1.0 / ray.dir[a];
+    }
+    ray.t0 = 0.0;
+    ray.t1 = 1.0;
+    // setIndexRay: clip to the padded bbox; the result is ignored (VM:698) [U6].
+    {
+      double t0 = ray.t0, t1 = ray.t1;
+      bool hit = true;
+      for (int a = 0; a < 3 && hit; ++a) {
+        double lo = (double)is.bmin[a], hi = (double)is.bmax[a] + ORC_U6_CLIP_PAD;
+        double ta = (lo - ray.eye[a]) * ray.inv[a];
+        double tb = (hi - ray.eye[a]) * ray.inv[a];
+        if (ta > tb) std::swap(ta, tb);
+        if (ta > t0) t0 = ta;
+        if (tb < t1) t1 = tb;
+        if (t0 > t1) hit = false;
+      }
+      if (hit) {
+        ray.t0 = t0;
+        ray.t1 = t1;
+      }
+    }
+    Span t;
+    if ((ray.t1 - ray.t0) > (double)1e-5f) hdda_4096(is, ray, t);  // Ray::valid [U10]
+    double* out = ends + 3 * i;
+    if (t.valid()) {
+      double eye[3], dir[3], inv[3];
+      for (int a = 0; a < 3; ++a) {
+        eye[a] = origins[3 * i + a] * m.inv;
+        dir[a] = d[a] * m.inv;
+        inv[a] = 1.0 / dir[a];
+      }
+      Dda dd;
+      dd.init(eye, dir, inv, t.t0, t.t1, 0);
+      while (dd.step() && !map_is_on(m, dd.vox[0], dd.vox[1], dd.vox[2])) {
+      }
+      for (int a = 0; a < 3; ++a) out[a] = (double)dd.vox[a] * m.res;
+      success[i] = 1;
+    } else {
+      for (int a = 0; a < 3; ++a) {
+        const double oi = origins[3 * i + a] * m.inv, di = d[a] * m.inv;
+        out[a] = (oi + di) * m.res;
+      }
+      success[i] = 0;
+    }
+  }
+  return ORC_OK;
+}
+
+}  // extern "C"

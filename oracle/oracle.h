@@ -1,0 +1,88 @@
+/*
+ * CPU restatement of vdb_mapping's occupancy-integration path.
+ *
+ * TEST INFRASTRUCTURE ONLY. This library is the parity checker: tests/, the
+ * smoke check and the CPU-baseline leg of bench.py may load it. The product
+ * (libvdbgpu.so, the vdb_mapping_amd package, the drop-in headers under include/vdb_mapping)
+ * never links, loads or calls it.
+ *
+ * Each function cites the reference code it restates
+ * (include/vdb_mapping/VDBMapping.hpp = VM, OccupancyVDBMapping.hpp = OCC).
+ * The voxel arithmetic of the reference lives in OpenVDB, which is not part
+ * of the reference tree; the restated OpenVDB semantics are listed in
+ * oracle.cpp next to the code that uses them.
+ */
+#ifndef VDB_ORACLE_H
+#define VDB_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Mirrors BaseConfig (VM:64-71) + Config (OCC:38-44). */
+typedef struct orc_config {
+  double max_range;
+  int32_t fast_mode;
+  int32_t _pad;
+  double accumulation_period;
+  double prob_hit;
+  double prob_miss;
+  double prob_thres_min;
+  double prob_thres_max;
+} orc_config;
+
+/* Status codes shared with the GPU library (include/vdbgpu.h). */
+enum {
+  ORC_OK = 0,
+  ORC_NOT_CONFIGURED = 1,
+  ORC_UNKNOWN_SOURCE = 2,
+  ORC_BAD_MAX_RANGE = 3,
+  ORC_BAD_PROB_MISS = 4,
+  ORC_BAD_PROB_HIT = 5,
+  ORC_OUT_OF_RANGE = 6,
+  ORC_UNSUPPORTED = 7,
+  ORC_NO_INTERSECTOR = 10
+};
+
+void* orc_create(double resolution);
+void orc_destroy(void* h);
+int orc_set_config(void* h, const orc_config* cfg);
+int orc_add_source(void* h, const char* source_id, double max_range, double max_rate);
+int orc_accumulate(void* h, const char* source_id, const float* xyz, int64_t n,
+                   int64_t stride_bytes, const double origin[3]);
+int orc_integrate(void* h);
+int orc_insert(void* h, const char* source_id, const float* xyz, int64_t n,
+               int64_t stride_bytes, const double origin[3]);
+void orc_reset(void* h);
+int orc_empty(void* h);
+/* Log-odds constants as set by setConfig: hit, miss, thres_min, thres_max, max, min. */
+void orc_logodds(void* h, float out[6]);
+
+/* Map export: leaves sorted by origin (x, then y, then z). */
+int64_t orc_leaf_count(void* h);
+int64_t orc_export_leaves(void* h, int32_t* origins, float* values, uint64_t* active_masks,
+                          int64_t cap);
+/* Pending update set of one source (VM:89 UpdateGridT): visited and hit masks per leaf. */
+int64_t orc_update_leaf_count(void* h, const char* source_id);
+int64_t orc_export_update(void* h, const char* source_id, int32_t* origins, uint64_t* visited,
+                          uint64_t* hit, int64_t cap);
+
+/* Single-voxel read, like an OpenVDB accessor: returns value, writes active flag. */
+float orc_get_value(void* h, const int32_t ijk[3], int32_t* active);
+
+/* Primitives, exposed so tests can pin them on their own. */
+void orc_world_to_index(double resolution, const double world[3], int32_t out[3]);
+int64_t orc_cast_ray(const int32_t s[3], const int32_t e[3], int32_t* out_xyz, int64_t cap);
+
+/* Batch raytrace (VM:675-721). Requires the intersector state the reference
+ * builds in integrateUpdate when fast_mode is set (VM:1436-1449). */
+int orc_raytrace(void* h, int64_t n, const double* origins, const double* dirs,
+                 const double* max_lengths, uint8_t* success, double* end_points);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,301 @@
+"""MI355X (gfx950) occupancy-integration path of vdb_mapping.
+
+The product is libvdbgpu.so (C ABI in include/vdbgpu.h, HIP kernels in
+csrc/); this module is a thin ctypes binding of that ABI with the method
+names of the reference's OccupancyVDBMapping (insertPointCloud ->
+insert_point_cloud, ...). There is no CPU fallback: every method calls the
+library, and loading fails loudly when the library is missing.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libvdbgpu.so")
+INCLUDE_DIR = os.path.join(os.path.dirname(HERE), "include")
+
+OK, NOT_CONFIGURED, UNKNOWN_SOURCE, BAD_MAX_RANGE, BAD_PROB_MISS, BAD_PROB_HIT = 0, 1, 2, 3, 4, 5
+OUT_OF_RANGE, UNSUPPORTED, DEVICE_ERROR, BAD_ARGUMENT, NO_INTERSECTOR = 6, 7, 8, 9, 10
+
+# Every symbol include/vdbgpu.h declares.
+ABI_SYMBOLS = [
+    "vdbgpu_version", "vdbgpu_status_string", "vdbgpu_device_count", "vdbgpu_create",
+    "vdbgpu_destroy", "vdbgpu_set_config", "vdbgpu_logodds", "vdbgpu_add_source",
+    "vdbgpu_accumulate", "vdbgpu_integrate", "vdbgpu_insert", "vdbgpu_reset", "vdbgpu_empty",
+    "vdbgpu_leaf_count", "vdbgpu_export_leaves", "vdbgpu_export_dirty_leaves",
+    "vdbgpu_update_leaf_count", "vdbgpu_export_update", "vdbgpu_raytrace", "vdbgpu_synchronize",
+    "vdbgpu_get_stats", "vdbgpu_set_profiling", "vdbgpu_set_tuning", "vdbgpu_get_tuning",
+    "vdbgpu_last_error",
+]
+
+
+class Config(ctypes.Structure):
+    """vdbgpu_config: BaseConfig (VDBMapping.hpp:64-71) + Config (OccupancyVDBMapping.hpp:38-44)."""
+
+    _fields_ = [
+        ("max_range", ctypes.c_double),
+        ("fast_mode", ctypes.c_int32),
+        ("_pad", ctypes.c_int32),
+        ("accumulation_period", ctypes.c_double),
+        ("prob_hit", ctypes.c_double),
+        ("prob_miss", ctypes.c_double),
+        ("prob_thres_min", ctypes.c_double),
+        ("prob_thres_max", ctypes.c_double),
+    ]
+
+
+def make_config(max_range=10.0, fast_mode=False, prob_hit=0.9, prob_miss=0.1,
+                prob_thres_min=0.49, prob_thres_max=0.51, accumulation_period=0.0) -> Config:
+    return Config(max_range, int(fast_mode), 0, accumulation_period, prob_hit, prob_miss,
+                  prob_thres_min, prob_thres_max)
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint64) for n in
+                ("scans", "rays", "visits", "voxel_updates", "update_leaves", "leaf_count",
+                 "device_bytes")] + \
+               [(n, ctypes.c_double) for n in ("ms_prep", "ms_traverse", "ms_compact", "ms_apply")] + \
+               [(n, ctypes.c_uint64) for n in ("n_prep", "n_traverse", "n_compact", "n_apply")]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+class Tuning(ctypes.Structure):
+    _fields_ = [("block", ctypes.c_int32), ("steps_per_lane", ctypes.c_int32),
+                ("waves_per_cu", ctypes.c_int32), ("load_factor", ctypes.c_double)]
+
+
+class VdbGpuError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    """Load libvdbgpu.so (raises when it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise VdbGpuError(f"{LIB_PATH} is missing: run __graft_entry__.build() first")
+        L = ctypes.CDLL(LIB_PATH)
+        P, D, I32, I64 = ctypes.c_void_p, ctypes.c_double, ctypes.c_int32, ctypes.c_int64
+        L.vdbgpu_version.restype = I32
+        L.vdbgpu_status_string.restype = ctypes.c_char_p
+        L.vdbgpu_status_string.argtypes = [I32]
+        L.vdbgpu_device_count.restype = I32
+        L.vdbgpu_create.restype = I32
+        L.vdbgpu_create.argtypes = [D, ctypes.POINTER(P)]
+        L.vdbgpu_destroy.argtypes = [P]
+        L.vdbgpu_set_config.restype = I32
+        L.vdbgpu_set_config.argtypes = [P, ctypes.POINTER(Config)]
+        L.vdbgpu_logodds.restype = I32
+        L.vdbgpu_logodds.argtypes = [P, P]
+        L.vdbgpu_add_source.restype = I32
+        L.vdbgpu_add_source.argtypes = [P, ctypes.c_char_p, D, D]
+        for f in (L.vdbgpu_accumulate, L.vdbgpu_insert):
+            f.restype = I32
+            f.argtypes = [P, ctypes.c_char_p, P, I64, I64, P, I32]
+        for f in (L.vdbgpu_integrate, L.vdbgpu_reset, L.vdbgpu_empty, L.vdbgpu_synchronize):
+            f.restype = I32
+            f.argtypes = [P]
+        L.vdbgpu_leaf_count.restype = I64
+        L.vdbgpu_leaf_count.argtypes = [P]
+        for f in (L.vdbgpu_export_leaves, L.vdbgpu_export_dirty_leaves):
+            f.restype = I64
+            f.argtypes = [P, P, P, P, I64]
+        L.vdbgpu_update_leaf_count.restype = I64
+        L.vdbgpu_update_leaf_count.argtypes = [P, ctypes.c_char_p]
+        L.vdbgpu_export_update.restype = I64
+        L.vdbgpu_export_update.argtypes = [P, ctypes.c_char_p, P, P, P, I64]
+        L.vdbgpu_raytrace.restype = I32
+        L.vdbgpu_raytrace.argtypes = [P, I64, P, P, P, P, P]
+        L.vdbgpu_get_stats.restype = I32
+        L.vdbgpu_get_stats.argtypes = [P, ctypes.POINTER(Stats)]
+        L.vdbgpu_set_profiling.restype = I32
+        L.vdbgpu_set_profiling.argtypes = [P, I32]
+        L.vdbgpu_set_tuning.restype = I32
+        L.vdbgpu_set_tuning.argtypes = [P, ctypes.POINTER(Tuning)]
+        L.vdbgpu_get_tuning.restype = I32
+        L.vdbgpu_get_tuning.argtypes = [P, ctypes.POINTER(Tuning)]
+        L.vdbgpu_last_error.restype = ctypes.c_char_p
+        L.vdbgpu_last_error.argtypes = [P]
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    if isinstance(a, int):
+        return ctypes.c_void_p(a)
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def status_string(code: int) -> str:
+    return lib().vdbgpu_status_string(code).decode()
+
+
+class GpuMap:
+    """OccupancyVDBMapping on one MI355X: method names follow the reference
+    (OccupancyVDBMapping.hpp / VDBMapping.hpp), arguments are numpy arrays.
+
+    Methods return the library's status code (0 = ok) where the reference
+    returns bool/void; `check=True` raises VdbGpuError on non-zero codes.
+    """
+
+    def __init__(self, resolution: float):
+        self.h = ctypes.c_void_p()
+        rc = lib().vdbgpu_create(resolution, ctypes.byref(self.h))
+        if rc != OK:
+            raise VdbGpuError(f"vdbgpu_create failed: {status_string(rc)}")
+        self.resolution = resolution
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            lib().vdbgpu_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        self.close()
+
+    def last_error(self) -> str:
+        return lib().vdbgpu_last_error(self.h).decode()
+
+    def _check(self, rc: int, check: bool) -> int:
+        if check and rc != OK:
+            raise VdbGpuError(f"{status_string(rc)}: {self.last_error()}")
+        return rc
+
+    def set_config(self, cfg: Config) -> int:
+        return lib().vdbgpu_set_config(self.h, ctypes.byref(cfg))
+
+    def logodds(self) -> np.ndarray:
+        out = np.zeros(6, np.float32)
+        lib().vdbgpu_logodds(self.h, _ptr(out))
+        return out
+
+    def add_input_source(self, source_id: str, max_range: float, max_rate: float = 0.0) -> int:
+        return lib().vdbgpu_add_source(self.h, source_id.encode(), max_range, max_rate)
+
+    @staticmethod
+    def _pts(points):
+        pts = np.ascontiguousarray(points, dtype=np.float32)
+        if pts.ndim != 2 or pts.shape[1] not in (3, 4):
+            raise ValueError("points must be (n,3) or (n,4) float32")
+        return pts, pts.shape[1] * 4
+
+    def accumulate(self, source_id: str, points, origin, check=False) -> int:
+        pts, stride = self._pts(points)
+        o = np.ascontiguousarray(origin, dtype=np.float64)
+        rc = lib().vdbgpu_accumulate(self.h, source_id.encode(), _ptr(pts), pts.shape[0], stride,
+                                     _ptr(o), 0)
+        return self._check(rc, check)
+
+    def accumulate_device(self, source_id: str, dev_ptr: int, n: int, stride_bytes: int, origin,
+                          check=False) -> int:
+        o = np.ascontiguousarray(origin, dtype=np.float64)
+        rc = lib().vdbgpu_accumulate(self.h, source_id.encode(), ctypes.c_void_p(dev_ptr), n,
+                                     stride_bytes, _ptr(o), 1)
+        return self._check(rc, check)
+
+    def integrate(self, check=False) -> int:
+        return self._check(lib().vdbgpu_integrate(self.h), check)
+
+    def insert(self, source_id: str, points, origin, check=False) -> int:
+        pts, stride = self._pts(points)
+        o = np.ascontiguousarray(origin, dtype=np.float64)
+        rc = lib().vdbgpu_insert(self.h, source_id.encode(), _ptr(pts), pts.shape[0], stride,
+                                 _ptr(o), 0)
+        return self._check(rc, check)
+
+    insert_point_cloud = insert
+
+    def insert_device(self, source_id: str, dev_ptr: int, n: int, stride_bytes: int, origin,
+                      check=False) -> int:
+        o = np.ascontiguousarray(origin, dtype=np.float64)
+        rc = lib().vdbgpu_insert(self.h, source_id.encode(), ctypes.c_void_p(dev_ptr), n,
+                                 stride_bytes, _ptr(o), 1)
+        return self._check(rc, check)
+
+    def reset(self) -> int:
+        return lib().vdbgpu_reset(self.h)
+
+    reset_map = reset
+
+    def empty(self) -> bool:
+        return bool(lib().vdbgpu_empty(self.h))
+
+    def synchronize(self) -> int:
+        return lib().vdbgpu_synchronize(self.h)
+
+    def leaf_count(self) -> int:
+        return int(lib().vdbgpu_leaf_count(self.h))
+
+    def export_leaves(self):
+        """Leaf dump of the map (getGrid): origins (n,3) i32, values (n,512) f32, masks (n,8) u64."""
+        n = self.leaf_count()
+        o = np.zeros((n, 3), np.int32)
+        v = np.zeros((n, 512), np.float32)
+        m = np.zeros((n, 8), np.uint64)
+        got = lib().vdbgpu_export_leaves(self.h, _ptr(o), _ptr(v), _ptr(m), n)
+        if got != n:
+            raise VdbGpuError(f"export_leaves returned {got}, expected {n}: {self.last_error()}")
+        return o, v, m
+
+    def export_dirty_leaves(self):
+        cap = max(self.leaf_count(), 1)
+        o = np.zeros((cap, 3), np.int32)
+        v = np.zeros((cap, 512), np.float32)
+        m = np.zeros((cap, 8), np.uint64)
+        got = lib().vdbgpu_export_dirty_leaves(self.h, _ptr(o), _ptr(v), _ptr(m), cap)
+        if got < 0:
+            raise VdbGpuError(f"export_dirty_leaves failed ({got}): {self.last_error()}")
+        return o[:got], v[:got], m[:got]
+
+    def export_update(self, source_id: str):
+        n = lib().vdbgpu_update_leaf_count(self.h, source_id.encode())
+        if n < 0:
+            raise KeyError(source_id)
+        o = np.zeros((n, 3), np.int32)
+        vis = np.zeros((n, 8), np.uint64)
+        hit = np.zeros((n, 8), np.uint64)
+        got = lib().vdbgpu_export_update(self.h, source_id.encode(), _ptr(o), _ptr(vis), _ptr(hit), n)
+        if got != n:
+            raise VdbGpuError(f"export_update returned {got}, expected {n}: {self.last_error()}")
+        return o, vis, hit
+
+    def raytrace(self, origins, dirs, lengths):
+        o = np.ascontiguousarray(origins, dtype=np.float64).reshape(-1, 3)
+        d = np.ascontiguousarray(dirs, dtype=np.float64).reshape(-1, 3)
+        ln = np.ascontiguousarray(lengths, dtype=np.float64).reshape(-1)
+        n = o.shape[0]
+        succ = np.zeros(n, np.uint8)
+        ends = np.zeros((n, 3), np.float64)
+        st = lib().vdbgpu_raytrace(self.h, n, _ptr(o), _ptr(d), _ptr(ln), _ptr(succ), _ptr(ends))
+        return st, succ.astype(bool), ends
+
+    def stats(self) -> dict:
+        s = Stats()
+        lib().vdbgpu_get_stats(self.h, ctypes.byref(s))
+        return s.as_dict()
+
+    def set_profiling(self, on: bool) -> None:
+        lib().vdbgpu_set_profiling(self.h, int(on))
+
+    def set_tuning(self, block=256, steps_per_lane=128, waves_per_cu=8, load_factor=0.5) -> int:
+        t = Tuning(block, steps_per_lane, waves_per_cu, load_factor)
+        return lib().vdbgpu_set_tuning(self.h, ctypes.byref(t))
+
+    def get_tuning(self) -> dict:
+        t = Tuning()
+        lib().vdbgpu_get_tuning(self.h, ctypes.byref(t))
+        return {n: getattr(t, n) for n, _ in t._fields_}
+
+
+def device_count() -> int:
+    return int(lib().vdbgpu_device_count())

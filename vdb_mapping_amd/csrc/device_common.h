@@ -1,0 +1,89 @@
+// Shared device/host definitions of the gfx950 occupancy-integration kernels.
+//
+// Data layout in HBM (one map handle):
+//   map leaf hash   : keys u64[mcap] (open addressing, linear probing, EMPTY = ~0),
+//                     idx u32[mcap] (leaf pool index of the key)
+//   leaf pool       : values f32[lcap][512] (OpenVDB LeafNode<float,3> buffer
+//                     order, offset ((x&7)<<6)|((y&7)<<3)|(z&7)),
+//                     active u64[lcap][8] (value mask), leaf_keys u64[lcap], dirty u8[lcap]
+//   update set (per source): keys u64[ucap], masks u64[ucap][16] (visited words 0..7,
+//                     hit words 8..15, same bit layout as the value mask)
+//   per scan        : rayinfo int4[n] (end voxel xyz, visit count | hit flag),
+//                     pre u32[n] (inclusive visit-count prefix inside each 256-ray block),
+//                     bsum u32[nb], boff u64[nb] (exclusive block offsets)
+//   crossing table  : tbl f64, row n (n = 1..tbl_n) at n(n-1)/2 holding the
+//                     DDA crossing times of an axis with |delta| = n
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace vdbgpu {
+
+constexpr uint64_t kEmpty = ~0ull;
+constexpr uint32_t kNone = 0xFFFFFFFFu;
+constexpr int kCoordLimit = 1 << 23;  // voxel coords in [-2^23, 2^23): 21-bit leaf coords
+constexpr uint32_t kHitBit = 0x80000000u;
+constexpr int kPrepBlock = 256;
+
+// Status bits written by kernels.
+constexpr uint32_t kErrRange = 1u;  // a voxel index out of range / non-finite
+constexpr uint32_t kErrTable = 2u;  // crossing table shorter than a ray axis
+constexpr uint32_t kOvfHard = 1u;   // update-set probe found no free slot
+
+struct Status {
+  unsigned long long total;     // visits of the scan (sum of counts)
+  unsigned long long voxels;    // visited voxels in the update set (compact)
+  unsigned long long rays;      // valid rays of the scan
+  unsigned int error;
+  unsigned int overflow;
+  unsigned int entries;         // update-set leaves (compact)
+  unsigned int leaf_count;      // map leaves (map insert)
+  int s[3];                     // origin voxel of the scan
+  unsigned int pad;
+};
+
+struct UpdView {
+  unsigned long long* keys;
+  unsigned long long* masks;  // 16 words per slot
+  uint32_t cap_mask;
+};
+
+struct MapView {
+  unsigned long long* keys;
+  uint32_t* idx;
+  uint32_t cap_mask;
+  float* values;
+  unsigned long long* active;
+  unsigned long long* leaf_keys;
+  uint8_t* dirty;
+};
+
+__host__ __device__ inline uint64_t leaf_key(int x, int y, int z) {
+  const uint64_t m = (1ull << 21) - 1;
+  return ((uint64_t)(uint32_t)(x >> 3) & m) << 42 | ((uint64_t)(uint32_t)(y >> 3) & m) << 21 |
+         ((uint64_t)(uint32_t)(z >> 3) & m);
+}
+
+__host__ __device__ inline int sext21(uint64_t v) { return (int)((int64_t)(v << 43) >> 43); }
+
+__host__ __device__ inline void key_origin(uint64_t k, int o[3]) {
+  const uint64_t m = (1ull << 21) - 1;
+  o[0] = sext21((k >> 42) & m) * 8;
+  o[1] = sext21((k >> 21) & m) * 8;
+  o[2] = sext21(k & m) * 8;
+}
+
+// MurmurHash3 64-bit finaliser.
+__host__ __device__ inline uint64_t fmix64(uint64_t k) {
+  k ^= k >> 33;
+  k *= 0xff51afd7ed558ccdull;
+  k ^= k >> 33;
+  k *= 0xc4ceb9fe1a85ec53ull;
+  k ^= k >> 33;
+  return k;
+}
+
+__device__ inline bool coord_ok(int c) { return c >= -kCoordLimit && c < kCoordLimit; }
+
+}  // namespace vdbgpu

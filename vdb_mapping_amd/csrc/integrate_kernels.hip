@@ -1,0 +1,733 @@
+// gfx950 kernels of the per-scan integration path.
+//
+//   k_prep        one lane per point: NaN skip, max-range truncation and rounded
+//                 world->index (VDBMapping.hpp:499-537, :612-631), visit count
+//                 1+|dx|+|dy|+|dz| per ray and its block-local prefix
+//   k_scan_blocks exclusive scan of the 256-ray block sums
+//   k_traverse    load-balanced DDA (VDBMapping.hpp:550-566): the global visit
+//                 range is cut into equal chunks, one per lane; a lane enters
+//                 its first ray mid-way at the exact DDA state found from the
+//                 crossing table, then walks; visits are OR-ed into the
+//                 source's update set at leaf-word granularity
+//   k_compact     list of occupied update-set slots + visited-voxel count
+//   k_map_insert  one lane per update leaf: find or create the map leaf
+//   k_apply       one wave per update leaf: clamped log-odds update of the
+//                 visited voxels (OccupancyVDBMapping.hpp:92-117); lane l owns
+//                 bit l of each 64-bit mask word, so the new value mask word is
+//                 one ballot
+//
+// Build with -ffp-contract=off: every double operation must round on its own,
+// exactly like the reference's x86-64 build.
+#include <float.h>
+
+#include "device_common.h"
+#include "kernels.h"
+
+namespace vdbgpu {
+
+// ---------------------------------------------------------------- helpers --
+
+__device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
+
+// Rounded world->index of one component (VDBMapping.hpp:612-631): shift by
+// res/2 unless fmod(c, res) == 0, multiply by the stored 1/res, floor.
+__device__ __forceinline__ int world_to_index1(double c, double res, double inv) {
+  if (fmod(c, res) != 0.0) c = c + res / 2.0;
+  const double f = floor(c * inv);
+  return (f >= -2147483648.0 && f < 2147483648.0) ? (int)f : INT_MIN;
+}
+
+// Find or insert `key` in an update set; returns its slot, or kNone after
+// kMaxProbe probes (the host then grows the table and re-runs the scan, which
+// is exact because the update set is an OR of idempotent marks).
+constexpr uint32_t kMaxProbe = 2048;
+__device__ __forceinline__ uint32_t upd_slot(const UpdView& u, uint64_t key, unsigned int* ovf) {
+  uint32_t h = (uint32_t)fmix64(key) & u.cap_mask;
+  const uint32_t limit = u.cap_mask < kMaxProbe ? u.cap_mask + 1 : kMaxProbe;
+  for (uint32_t probe = 0; probe < limit; ++probe) {
+    const unsigned long long k =
+        __hip_atomic_load(&u.keys[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (k == key) return h;
+    if (k == kEmpty) {
+      const unsigned long long prev = atomicCAS(&u.keys[h], kEmpty, (unsigned long long)key);
+      if (prev == kEmpty || prev == key) return h;
+    }
+    h = (h + 1) & u.cap_mask;
+  }
+  atomicOr(ovf, kOvfHard);
+  return kNone;
+}
+
+// OR `bits` into mask word `word` of a slot; skip the atomic when a plain
+// read already shows every bit (bits are never cleared during traversal, so
+// a stale read can only cause an unneeded atomic, never a lost bit).
+__device__ __forceinline__ void upd_or(const UpdView& u, uint32_t slot, int word, uint64_t bits) {
+  if (slot == kNone || bits == 0) return;
+  unsigned long long* p = &u.masks[(size_t)slot * 16 + word];
+  const unsigned long long cur = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if ((cur & bits) != bits) atomicOr(p, (unsigned long long)bits);
+}
+
+// Per-lane accumulator of visited bits for one (leaf, mask word).
+struct Run {
+  int lx, ly, lz;
+  uint32_t slot;
+  int word;
+  uint64_t bits;
+  __device__ void init() {
+    lx = INT_MIN;
+    ly = lz = 0;
+    slot = kNone;
+    word = 0;
+    bits = 0;
+  }
+  __device__ __forceinline__ void add(const UpdView& u, unsigned int* ovf, int x, int y, int z) {
+    const int w = x & 7;
+    const uint64_t b = 1ull << (((y & 7) << 3) | (z & 7));
+    const int nx = x >> 3, ny = y >> 3, nz = z >> 3;
+    if (nx == lx && ny == ly && nz == lz) {
+      if (w == word) {
+        bits |= b;
+        return;
+      }
+      upd_or(u, slot, word, bits);
+    } else {
+      upd_or(u, slot, word, bits);
+      lx = nx;
+      ly = ny;
+      lz = nz;
+      slot = upd_slot(u, leaf_key(x, y, z), ovf);
+    }
+    word = w;
+    bits = b;
+  }
+  __device__ void flush(const UpdView& u) { upd_or(u, slot, word, bits); bits = 0; }
+};
+
+// Key order of DDA crossing events: earlier time first, ties to the larger
+// axis index (math::MinIndex).
+__device__ __forceinline__ bool kless(double t1, int a1, double t2, int a2) {
+  return t1 < t2 || (t1 == t2 && a1 > a2);
+}
+
+// DDA state after k steps of a ray with per-axis |delta| n[3]: the first k
+// events of the merged crossing sequences (time, -axis). Starts from the
+// rational estimate t = k / N and repairs it with exact table values until
+// every included event precedes every excluded one.
+__device__ void dda_enter(const double* __restrict__ tbl, const uint32_t n[3], uint32_t k,
+                          uint32_t c[3], double q[3]) {
+  const double* row[3];
+  double lin[3], fout[3];
+  const uint32_t N = n[0] + n[1] + n[2];
+  const double t = (double)k / (double)N;
+  int sum = 0;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    row[a] = tbl + (((uint64_t)n[a] * (uint64_t)(n[a] ? n[a] - 1 : 0)) >> 1);
+    const double g = floor(t * (double)n[a] + 0.5);
+    c[a] = n[a] ? (uint32_t)fmin(g, (double)n[a]) : 0u;
+    sum += (int)c[a];
+    lin[a] = c[a] > 0 ? row[a][c[a] - 1] : 0.0;
+    fout[a] = c[a] < n[a] ? row[a][c[a]] : 0.0;
+  }
+  while (sum > (int)k) {
+    int best = -1;
+    double bt = 0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+      if (c[a] > 0 && (best < 0 || kless(bt, best, lin[a], a))) {
+        best = a;
+        bt = lin[a];
+      }
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+      if (a == best) {
+        c[a]--;
+        fout[a] = lin[a];
+        lin[a] = c[a] > 0 ? row[a][c[a] - 1] : 0.0;
+      }
+    --sum;
+  }
+  while (sum < (int)k) {
+    int best = -1;
+    double bt = 0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+      if (c[a] < n[a] && (best < 0 || kless(fout[a], a, bt, best))) {
+        best = a;
+        bt = fout[a];
+      }
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+      if (a == best) {
+        c[a]++;
+        lin[a] = fout[a];
+        fout[a] = c[a] < n[a] ? row[a][c[a]] : 0.0;
+      }
+    ++sum;
+  }
+  for (;;) {
+    int amax = -1, amin = -1;
+    double tmax = 0, tmin = 0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      if (c[a] > 0 && (amax < 0 || kless(tmax, amax, lin[a], a))) {
+        amax = a;
+        tmax = lin[a];
+      }
+      if (c[a] < n[a] && (amin < 0 || kless(fout[a], a, tmin, amin))) {
+        amin = a;
+        tmin = fout[a];
+      }
+    }
+    if (amax < 0 || amin < 0 || !kless(tmin, amin, tmax, amax)) break;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      if (a == amax) {
+        c[a]--;
+        fout[a] = lin[a];
+        lin[a] = c[a] > 0 ? row[a][c[a] - 1] : 0.0;
+      } else if (a == amin) {
+        c[a]++;
+        lin[a] = fout[a];
+        fout[a] = c[a] < n[a] ? row[a][c[a]] : 0.0;
+      }
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < 3; ++a) q[a] = c[a] < n[a] ? fout[a] : DBL_MAX;
+}
+
+// ---------------------------------------------------------------- kernels --
+
+// Row n of the crossing table: next_0 = 0.5 * fl(1/n), next_j = fl(next_{j-1} + fl(1/n)),
+// which is what math::DDA::init/step produce for a ray from a voxel centre
+// with integer direction component +-n (VDBMapping.hpp:555-564).
+__global__ void k_build_table(double* __restrict__ tbl, uint32_t n_lo, uint32_t n_hi) {
+  const uint32_t n = n_lo + blockIdx.x * blockDim.x + threadIdx.x;
+  if (n > n_hi) return;
+  const double d = 1.0 / (double)n;
+  double t = 0.5 * d;
+  double* row = tbl + (((uint64_t)n * (n - 1)) >> 1);
+  for (uint32_t j = 0; j < n; ++j) {
+    row[j] = t;
+    t = t + d;
+  }
+}
+
+template <bool kVec4>
+__global__ __launch_bounds__(kPrepBlock) void k_prep(const float* __restrict__ pts,
+                                                     int64_t stride_f, uint32_t n, double ox,
+                                                     double oy, double oz, double R, double res,
+                                                     double inv, uint32_t tbl_n,
+                                                     int4* __restrict__ rayinfo,
+                                                     uint32_t* __restrict__ pre,
+                                                     uint32_t* __restrict__ bsum, Status* st) {
+  __shared__ uint32_t wsum[kPrepBlock / 64];
+  const uint32_t i = blockIdx.x * kPrepBlock + threadIdx.x;
+  uint32_t cnt = 0;
+  bool valid = false;
+  if (i < n) {
+    double px, py, pz;
+    if (kVec4 && i + 1 < n) {  // the last point may end 4 bytes before a 16-byte boundary
+      const float4 p = reinterpret_cast<const float4*>(pts)[i];
+      px = p.x;
+      py = p.y;
+      pz = p.z;
+    } else {
+      const float* p = pts + (int64_t)i * stride_f;
+      px = p[0];
+      py = p[1];
+      pz = p[2];
+    }
+    int4 ri = make_int4(0, 0, 0, 0);
+    valid = !(isnan(px) || isnan(py) || isnan(pz) || isnan(ox) || isnan(oy) || isnan(oz));
+    if (valid) {
+      bool trunc = false;
+      const double dx = px - ox, dy = py - oy, dz = pz - oz;
+      const double len = sqrt(dx * dx + dy * dy + dz * dz);
+      if (R > 0.0 && len > R) {
+        px = ox + (dx / len) * R;
+        py = oy + (dy / len) * R;
+        pz = oz + (dz / len) * R;
+        trunc = true;
+      }
+      const int sx = world_to_index1(ox, res, inv), sy = world_to_index1(oy, res, inv),
+                sz = world_to_index1(oz, res, inv);
+      const int ex = world_to_index1(px, res, inv), ey = world_to_index1(py, res, inv),
+                ez = world_to_index1(pz, res, inv);
+      if (!(coord_ok(sx) && coord_ok(sy) && coord_ok(sz) && coord_ok(ex) && coord_ok(ey) &&
+            coord_ok(ez))) {
+        atomicOr(&st->error, kErrRange);
+      } else {
+        const uint32_t nx = (uint32_t)abs(ex - sx), ny = (uint32_t)abs(ey - sy),
+                       nz = (uint32_t)abs(ez - sz);
+        if (nx > tbl_n || ny > tbl_n || nz > tbl_n) atomicOr(&st->error, kErrTable);
+        cnt = (nx | ny | nz) ? 1u + nx + ny + nz : 0u;
+        ri = make_int4(ex, ey, ez, (int)(cnt | (trunc ? 0u : kHitBit)));
+      }
+    }
+    rayinfo[i] = ri;
+    if (i == 0) {
+      st->s[0] = world_to_index1(ox, res, inv);
+      st->s[1] = world_to_index1(oy, res, inv);
+      st->s[2] = world_to_index1(oz, res, inv);
+    }
+  }
+  // valid-ray count, one atomic per wave
+  const uint64_t vm = __ballot(valid);
+  if (lane_id() == 0 && vm) atomicAdd(&st->rays, (unsigned long long)__popcll(vm));
+  // block-inclusive prefix of the visit counts
+  uint32_t v = cnt;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t t = __shfl_up(v, o, 64);
+    if (lane_id() >= o) v += t;
+  }
+  const int wv = threadIdx.x >> 6;
+  if (lane_id() == 63) wsum[wv] = v;
+  __syncthreads();
+  uint32_t add = 0;
+  for (int w = 0; w < wv; ++w) add += wsum[w];
+  v += add;
+  if (i < n) pre[i] = v;
+  if (threadIdx.x == kPrepBlock - 1) bsum[blockIdx.x] = v;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_blocks(const uint32_t* __restrict__ bsum,
+                                                      uint64_t* __restrict__ boff, uint32_t nb,
+                                                      Status* st) {
+  __shared__ unsigned long long wsum[16];
+  unsigned long long carry = 0;
+  for (uint32_t base = 0; base < nb; base += 1024) {
+    const uint32_t i = base + threadIdx.x;
+    const unsigned long long x = i < nb ? bsum[i] : 0ull;
+    unsigned long long v = x;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const unsigned long long t = __shfl_up(v, o, 64);
+      if (lane_id() >= o) v += t;
+    }
+    const int wv = threadIdx.x >> 6;
+    if (lane_id() == 63) wsum[wv] = v;
+    __syncthreads();
+    unsigned long long add = 0, tot = 0;
+    for (int w = 0; w < 16; ++w) {
+      if (w < wv) add += wsum[w];
+      tot += wsum[w];
+    }
+    if (i < nb) boff[i] = carry + add + v - x;
+    carry += tot;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) st->total = carry;
+}
+
+__global__ __launch_bounds__(256) void k_traverse(const int4* __restrict__ rayinfo,
+                                                  const uint32_t* __restrict__ pre,
+                                                  const uint64_t* __restrict__ boff, uint32_t nb,
+                                                  uint32_t n, const double* __restrict__ tbl,
+                                                  UpdView u, Status* st, uint32_t chunk) {
+  if (__hip_atomic_load(&st->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
+  const uint64_t S = st->total;
+  const int sx = st->s[0], sy = st->s[1], sz = st->s[2];
+  const uint64_t G = (uint64_t)gridDim.x * blockDim.x;
+  const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned int* ovf = &st->overflow;
+
+  // Endpoints of rays inside max range: setValueOn(end, true) (VDBMapping.hpp:533-536).
+  for (uint64_t i = gid; i < n; i += G) {
+    const int4 r = rayinfo[i];
+    if ((uint32_t)r.w & kHitBit) {
+      const uint32_t slot = upd_slot(u, leaf_key(r.x, r.y, r.z), ovf);
+      const int w = r.x & 7;
+      const uint64_t b = 1ull << (((r.y & 7) << 3) | (r.z & 7));
+      upd_or(u, slot, w, b);
+      upd_or(u, slot, 8 + w, b);
+    }
+  }
+
+  for (uint64_t c = gid; c * chunk < S; c += G) {
+    uint64_t g = c * chunk;
+    const uint64_t gend = min(g + chunk, S);
+    // ray holding visit g: last block with boff <= g, then first ray of the
+    // block whose inclusive prefix exceeds the block-local offset
+    uint32_t lo = 0, hi = nb - 1;
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi + 1) >> 1;
+      if (boff[mid] <= g)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    const uint32_t local = (uint32_t)(g - boff[lo]);
+    uint32_t l2 = lo * kPrepBlock, h2 = min(l2 + kPrepBlock, n) - 1;
+    while (l2 < h2) {
+      const uint32_t mid = (l2 + h2) >> 1;
+      if (pre[mid] > local)
+        h2 = mid;
+      else
+        l2 = mid + 1;
+    }
+    uint32_t i = l2;
+    uint32_t k = local - (pre[i] - ((uint32_t)rayinfo[i].w & ~kHitBit));
+
+    Run run;
+    run.init();
+    while (g < gend && i < n) {
+      const int4 r = rayinfo[i];
+      const uint32_t cnt = (uint32_t)r.w & ~kHitBit;
+      if (k >= cnt) {
+        ++i;
+        k = 0;
+        continue;
+      }
+      const uint32_t m = (uint32_t)min((uint64_t)(cnt - k), gend - g);
+      uint32_t nn[3] = {(uint32_t)abs(r.x - sx), (uint32_t)abs(r.y - sy), (uint32_t)abs(r.z - sz)};
+      const int stx = r.x > sx ? 1 : (r.x < sx ? -1 : 0);
+      const int sty = r.y > sy ? 1 : (r.y < sy ? -1 : 0);
+      const int stz = r.z > sz ? 1 : (r.z < sz ? -1 : 0);
+      const double dx = nn[0] ? 1.0 / (double)nn[0] : 0.0;
+      const double dy = nn[1] ? 1.0 / (double)nn[1] : 0.0;
+      const double dz = nn[2] ? 1.0 / (double)nn[2] : 0.0;
+      uint32_t cc[3] = {0, 0, 0};
+      double q[3];
+      if (k == 0) {
+        q[0] = nn[0] ? 0.5 * dx : DBL_MAX;
+        q[1] = nn[1] ? 0.5 * dy : DBL_MAX;
+        q[2] = nn[2] ? 0.5 * dz : DBL_MAX;
+      } else {
+        dda_enter(tbl, nn, k, cc, q);
+      }
+      int vx = sx + stx * (int)cc[0], vy = sy + sty * (int)cc[1], vz = sz + stz * (int)cc[2];
+      double qx = q[0], qy = q[1], qz = q[2];
+      for (uint32_t v = 0;;) {
+        run.add(u, ovf, vx, vy, vz);
+        if (++v == m) break;
+        const bool yx = qy <= qx;
+        const double mxy = yx ? qy : qx;
+        if (qz <= mxy) {
+          qz += dz;
+          vz += stz;
+        } else if (yx) {
+          qy += dy;
+          vy += sty;
+        } else {
+          qx += dx;
+          vx += stx;
+        }
+      }
+      g += m;
+      ++i;
+      k = 0;
+    }
+    run.flush(u);
+  }
+}
+
+// List the occupied update-set slots; count their visited voxels.
+__global__ __launch_bounds__(256) void k_compact(UpdView u, uint32_t cap,
+                                                 uint32_t* __restrict__ list, Status* st) {
+  const uint32_t G = gridDim.x * blockDim.x;
+  unsigned long long vox = 0;
+  for (uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x; slot < cap; slot += G) {
+    const bool valid = u.keys[slot] != kEmpty;
+    const uint64_t m = __ballot(valid);
+    if (m) {
+      const int leader = __ffsll((long long)m) - 1;
+      uint32_t base = 0;
+      if (lane_id() == leader) base = atomicAdd(&st->entries, (unsigned int)__popcll(m));
+      base = __shfl(base, leader);
+      if (valid) {
+        list[base + __popcll(m & ((1ull << lane_id()) - 1))] = slot;
+        const unsigned long long* w = &u.masks[(size_t)slot * 16];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) vox += __popcll(w[j]);
+      }
+    }
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) vox += __shfl_xor(vox, o, 64);
+  if (lane_id() == 0 && vox) atomicAdd(&st->voxels, vox);
+}
+
+// One lane per update leaf: find its map leaf, or create it when the update
+// changes a background voxel (OpenVDB creates a child node only when the
+// modify op changes the tile's value or active state).
+__global__ __launch_bounds__(256) void k_map_insert(UpdView u, const uint32_t* __restrict__ list,
+                                                    uint32_t count, MapView mp,
+                                                    uint32_t* __restrict__ eidx,
+                                                    uint32_t create_hit, uint32_t create_miss,
+                                                    Status* st) {
+  const uint32_t G = gridDim.x * blockDim.x;
+  const uint32_t e0 = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t rounds = (count + G - 1) / G;
+  for (uint32_t rd = 0; rd < rounds; ++rd) {
+    const uint32_t e = e0 + rd * G;
+    bool created = false;
+    uint32_t h = kNone, idx = kNone;
+    uint64_t key = 0;
+    if (e < count) {
+      const uint32_t slot = list[e];
+      key = u.keys[slot];
+      const unsigned long long* w = &u.masks[(size_t)slot * 16];
+      uint64_t anyhit = 0, anymiss = 0;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        anyhit |= w[j] & w[8 + j];
+        anymiss |= w[j] & ~w[8 + j];
+      }
+      const bool need = (anyhit && create_hit) || (anymiss && create_miss);
+      uint32_t p = (uint32_t)fmix64(key) & mp.cap_mask;
+      for (uint32_t probe = 0; probe <= mp.cap_mask; ++probe) {
+        const unsigned long long k = mp.keys[p];
+        if (k == key) {
+          idx = mp.idx[p];
+          break;
+        }
+        if (k == kEmpty) {
+          if (!need) break;
+          const unsigned long long prev = atomicCAS(&mp.keys[p], kEmpty, (unsigned long long)key);
+          if (prev == kEmpty) {
+            created = true;
+            h = p;
+            break;
+          }
+        }
+        p = (p + 1) & mp.cap_mask;
+      }
+    }
+    const uint64_t cm = __ballot(created);
+    if (cm) {
+      const int leader = __ffsll((long long)cm) - 1;
+      uint32_t base = 0;
+      if (lane_id() == leader) base = atomicAdd(&st->leaf_count, (unsigned int)__popcll(cm));
+      base = __shfl(base, leader);
+      if (created) {
+        idx = base + __popcll(cm & ((1ull << lane_id()) - 1));
+        mp.idx[h] = idx;
+        mp.leaf_keys[idx] = key;
+      }
+    }
+    if (e < count) {
+      eidx[e] = idx;
+      if (idx != kNone) mp.dirty[idx] = 1;
+    }
+  }
+}
+
+// One wave per update leaf (updateMap VDBMapping.hpp:764-783 with
+// updateOccupiedNode / updateFreeNode OccupancyVDBMapping.hpp:92-117):
+// hit -> x += l_hit, above t_max: active, clamp at c_max;
+// miss -> x += l_miss, below t_min: inactive, clamp at c_min.
+// Clears the update-set slot afterwards.
+__global__ __launch_bounds__(256) void k_apply(UpdView u, const uint32_t* __restrict__ list,
+                                               const uint32_t* __restrict__ eidx, uint32_t count,
+                                               MapView mp, float l_hit, float l_miss, float t_min,
+                                               float t_max, float c_max, float c_min) {
+  const uint32_t waves = gridDim.x * (blockDim.x >> 6);
+  const int lane = lane_id();
+  for (uint32_t e = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); e < count; e += waves) {
+    const uint32_t slot = list[e];
+    const uint32_t idx = eidx[e];
+    unsigned long long* w = &u.masks[(size_t)slot * 16];
+    if (idx != kNone) {
+      float* vals = mp.values + (size_t)idx * 512;
+      unsigned long long* act = mp.active + (size_t)idx * 8;
+#pragma unroll 1
+      for (int j = 0; j < 8; ++j) {
+        const uint64_t vis = w[j];
+        if (vis == 0) continue;
+        const uint64_t hit = w[8 + j];
+        bool a = (act[j] >> lane) & 1;
+        if ((vis >> lane) & 1) {
+          float x = vals[j * 64 + lane];
+          if ((hit >> lane) & 1) {
+            x += l_hit;
+            if (x > t_max) {
+              a = true;
+              if (x > c_max) x = c_max;
+            }
+          } else {
+            x += l_miss;
+            if (x < t_min) {
+              a = false;
+              if (x < c_min) x = c_min;
+            }
+          }
+          vals[j * 64 + lane] = x;
+        }
+        const uint64_t na = __ballot(a);
+        if (lane == 0) act[j] = na;
+      }
+    }
+    if (lane < 16) w[lane] = 0;
+    if (lane == 0) u.keys[slot] = kEmpty;
+  }
+}
+
+// Move every entry of an update set into a larger one.
+__global__ __launch_bounds__(256) void k_rehash_update(UpdView from, uint32_t from_cap, UpdView to,
+                                                       Status* st) {
+  const uint32_t G = gridDim.x * blockDim.x;
+  for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < from_cap; s += G) {
+    const uint64_t key = from.keys[s];
+    if (key == kEmpty) continue;
+    const uint32_t d = upd_slot(to, key, &st->overflow);
+    if (d == kNone) continue;
+    for (int j = 0; j < 16; ++j) to.masks[(size_t)d * 16 + j] |= from.masks[(size_t)s * 16 + j];
+  }
+}
+
+// Rebuild the map leaf hash from the leaf pool's key list.
+__global__ __launch_bounds__(256) void k_rehash_map(MapView mp, uint32_t leaf_count) {
+  const uint32_t G = gridDim.x * blockDim.x;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < leaf_count; i += G) {
+    const uint64_t key = mp.leaf_keys[i];
+    uint32_t p = (uint32_t)fmix64(key) & mp.cap_mask;
+    for (uint32_t probe = 0; probe <= mp.cap_mask; ++probe) {
+      if (atomicCAS(&mp.keys[p], kEmpty, (unsigned long long)key) == kEmpty) {
+        mp.idx[p] = i;
+        break;
+      }
+      p = (p + 1) & mp.cap_mask;
+    }
+  }
+}
+
+__global__ void k_fill_u64(unsigned long long* p, uint64_t n, unsigned long long v) {
+  const uint64_t G = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += G) p[i] = v;
+}
+
+// List (and clear) the dirty leaves, then gather their data contiguously.
+__global__ __launch_bounds__(256) void k_dirty_list(uint8_t* dirty, uint32_t leaf_count,
+                                                    uint32_t* list, Status* st) {
+  const uint32_t G = gridDim.x * blockDim.x;
+  const uint32_t rounds = (leaf_count + G - 1) / G;
+  for (uint32_t rd = 0; rd < rounds; ++rd) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x + rd * G;
+    const bool d = i < leaf_count && dirty[i];
+    const uint64_t m = __ballot(d);
+    if (m) {
+      const int leader = __ffsll((long long)m) - 1;
+      uint32_t base = 0;
+      if (lane_id() == leader) base = atomicAdd(&st->entries, (unsigned int)__popcll(m));
+      base = __shfl(base, leader);
+      if (d) {
+        list[base + __popcll(m & ((1ull << lane_id()) - 1))] = i;
+        dirty[i] = 0;
+      }
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_gather_leaves(const uint32_t* __restrict__ list,
+                                                       uint32_t count, MapView mp,
+                                                       unsigned long long* __restrict__ keys_out,
+                                                       float* __restrict__ vals_out,
+                                                       unsigned long long* __restrict__ act_out) {
+  const uint32_t waves = gridDim.x * (blockDim.x >> 6);
+  const int lane = lane_id();
+  for (uint32_t e = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); e < count; e += waves) {
+    const uint32_t i = list[e];
+    const float* src = mp.values + (size_t)i * 512;
+    float* dst = vals_out + (size_t)e * 512;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) dst[j * 64 + lane] = src[j * 64 + lane];
+    if (lane < 8) act_out[(size_t)e * 8 + lane] = mp.active[(size_t)i * 8 + lane];
+    if (lane == 0) keys_out[e] = mp.leaf_keys[i];
+  }
+}
+
+// ------------------------------------------------------------- launchers --
+
+hipError_t launch_build_table(hipStream_t s, double* tbl, uint32_t n_lo, uint32_t n_hi) {
+  const uint32_t cnt = n_hi - n_lo + 1;
+  hipLaunchKernelGGL(k_build_table, dim3((cnt + 255) / 256), dim3(256), 0, s, tbl, n_lo, n_hi);
+  return hipGetLastError();
+}
+
+hipError_t launch_prep(hipStream_t s, const float* pts, int64_t stride_bytes, uint32_t n,
+                       const double o[3], double R, double res, double inv, uint32_t tbl_n,
+                       int4* rayinfo, uint32_t* pre, uint32_t* bsum, Status* st) {
+  const uint32_t nb = (n + kPrepBlock - 1) / kPrepBlock;
+  const bool vec4 = stride_bytes == 16 && (reinterpret_cast<uintptr_t>(pts) & 15) == 0;
+  if (vec4)
+    hipLaunchKernelGGL(k_prep<true>, dim3(nb), dim3(kPrepBlock), 0, s, pts, stride_bytes / 4, n,
+                       o[0], o[1], o[2], R, res, inv, tbl_n, rayinfo, pre, bsum, st);
+  else
+    hipLaunchKernelGGL(k_prep<false>, dim3(nb), dim3(kPrepBlock), 0, s, pts, stride_bytes / 4, n,
+                       o[0], o[1], o[2], R, res, inv, tbl_n, rayinfo, pre, bsum, st);
+  return hipGetLastError();
+}
+
+hipError_t launch_scan_blocks(hipStream_t s, const uint32_t* bsum, uint64_t* boff, uint32_t nb,
+                              Status* st) {
+  hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, s, bsum, boff, nb, st);
+  return hipGetLastError();
+}
+
+hipError_t launch_traverse(hipStream_t s, int grid, int block, const int4* rayinfo,
+                           const uint32_t* pre, const uint64_t* boff, uint32_t nb, uint32_t n,
+                           const double* tbl, UpdView u, Status* st, uint32_t chunk) {
+  hipLaunchKernelGGL(k_traverse, dim3(grid), dim3(block), 0, s, rayinfo, pre, boff, nb, n, tbl, u,
+                     st, chunk);
+  return hipGetLastError();
+}
+
+hipError_t launch_compact(hipStream_t s, int grid, UpdView u, uint32_t cap, uint32_t* list,
+                          Status* st) {
+  hipLaunchKernelGGL(k_compact, dim3(grid), dim3(256), 0, s, u, cap, list, st);
+  return hipGetLastError();
+}
+
+hipError_t launch_map_insert(hipStream_t s, int grid, UpdView u, const uint32_t* list,
+                             uint32_t count, MapView mp, uint32_t* eidx, bool create_hit,
+                             bool create_miss, Status* st) {
+  hipLaunchKernelGGL(k_map_insert, dim3(grid), dim3(256), 0, s, u, list, count, mp, eidx,
+                     (uint32_t)create_hit, (uint32_t)create_miss, st);
+  return hipGetLastError();
+}
+
+hipError_t launch_apply(hipStream_t s, int grid, UpdView u, const uint32_t* list,
+                        const uint32_t* eidx, uint32_t count, MapView mp, const float lo[6]) {
+  hipLaunchKernelGGL(k_apply, dim3(grid), dim3(256), 0, s, u, list, eidx, count, mp, lo[0], lo[1],
+                     lo[2], lo[3], lo[4], lo[5]);
+  return hipGetLastError();
+}
+
+hipError_t launch_rehash_update(hipStream_t s, int grid, UpdView from, uint32_t from_cap,
+                                UpdView to, Status* st) {
+  hipLaunchKernelGGL(k_rehash_update, dim3(grid), dim3(256), 0, s, from, from_cap, to, st);
+  return hipGetLastError();
+}
+
+hipError_t launch_rehash_map(hipStream_t s, int grid, MapView mp, uint32_t leaf_count) {
+  hipLaunchKernelGGL(k_rehash_map, dim3(grid), dim3(256), 0, s, mp, leaf_count);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_u64(hipStream_t s, unsigned long long* p, uint64_t n, uint64_t v) {
+  if (n == 0) return hipSuccess;
+  const uint64_t blocks = (n + 255) / 256;
+  const int grid = (int)(blocks < 4096 ? blocks : 4096);
+  hipLaunchKernelGGL(k_fill_u64, dim3(grid), dim3(256), 0, s, p, n, (unsigned long long)v);
+  return hipGetLastError();
+}
+
+hipError_t launch_dirty_list(hipStream_t s, int grid, uint8_t* dirty, uint32_t leaf_count,
+                             uint32_t* list, Status* st) {
+  hipLaunchKernelGGL(k_dirty_list, dim3(grid), dim3(256), 0, s, dirty, leaf_count, list, st);
+  return hipGetLastError();
+}
+
+hipError_t launch_gather_leaves(hipStream_t s, int grid, const uint32_t* list, uint32_t count,
+                                MapView mp, unsigned long long* keys_out, float* vals_out,
+                                unsigned long long* act_out) {
+  hipLaunchKernelGGL(k_gather_leaves, dim3(grid), dim3(256), 0, s, list, count, mp, keys_out,
+                     vals_out, act_out);
+  return hipGetLastError();
+}
+
+}  // namespace vdbgpu

@@ -1,0 +1,36 @@
+// Host-side launchers of the gfx950 kernels (defined in *_kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "device_common.h"
+
+namespace vdbgpu {
+
+hipError_t launch_build_table(hipStream_t s, double* tbl, uint32_t n_lo, uint32_t n_hi);
+hipError_t launch_prep(hipStream_t s, const float* pts, int64_t stride_bytes, uint32_t n,
+                       const double o[3], double R, double res, double inv, uint32_t tbl_n,
+                       int4* rayinfo, uint32_t* pre, uint32_t* bsum, Status* st);
+hipError_t launch_scan_blocks(hipStream_t s, const uint32_t* bsum, uint64_t* boff, uint32_t nb,
+                              Status* st);
+hipError_t launch_traverse(hipStream_t s, int grid, int block, const int4* rayinfo,
+                           const uint32_t* pre, const uint64_t* boff, uint32_t nb, uint32_t n,
+                           const double* tbl, UpdView u, Status* st, uint32_t chunk);
+hipError_t launch_compact(hipStream_t s, int grid, UpdView u, uint32_t cap, uint32_t* list,
+                          Status* st);
+hipError_t launch_map_insert(hipStream_t s, int grid, UpdView u, const uint32_t* list,
+                             uint32_t count, MapView mp, uint32_t* eidx, bool create_hit,
+                             bool create_miss, Status* st);
+hipError_t launch_apply(hipStream_t s, int grid, UpdView u, const uint32_t* list,
+                        const uint32_t* eidx, uint32_t count, MapView mp, const float lo[6]);
+hipError_t launch_rehash_update(hipStream_t s, int grid, UpdView from, uint32_t from_cap,
+                                UpdView to, Status* st);
+hipError_t launch_rehash_map(hipStream_t s, int grid, MapView mp, uint32_t leaf_count);
+hipError_t launch_fill_u64(hipStream_t s, unsigned long long* p, uint64_t n, uint64_t v);
+hipError_t launch_dirty_list(hipStream_t s, int grid, uint8_t* dirty, uint32_t leaf_count,
+                             uint32_t* list, Status* st);
+hipError_t launch_gather_leaves(hipStream_t s, int grid, const uint32_t* list, uint32_t count,
+                                MapView mp, unsigned long long* keys_out, float* vals_out,
+                                unsigned long long* act_out);
+
+}  // namespace vdbgpu

@@ -1,0 +1,842 @@
+// Host runtime and C ABI of the gfx950 occupancy-integration path
+// (declared in include/vdbgpu.h).
+//
+// One vdbgpu_map owns: a HIP stream (default priority), a mutex, the device
+// leaf store + leaf hash, one update set per input source, per-scan scratch,
+// the DDA crossing table and a pinned status block. The per-scan path is
+//   accumulate: [H2D points] -> k_prep -> k_scan_blocks -> k_traverse -> k_compact
+//               -> one status read (the only host sync of a scan)
+//   integrate : k_map_insert -> k_apply per source, in source-id order (no sync)
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/vdbgpu.h"
+#include "device_common.h"
+#include "kernels.h"
+
+using namespace vdbgpu;
+
+namespace {
+
+constexpr uint32_t kMaxTableN = 8192;       // longest per-axis ray extent in voxels
+constexpr uint32_t kMinUpdateCap = 1u << 12;
+constexpr uint32_t kMaxRays = 1u << 30;
+
+enum TimedKind { kPrep = 0, kTraverse = 1, kCompact = 2, kApply = 3 };
+
+struct TimedPair {
+  int kind;
+  hipEvent_t a, b;
+};
+
+struct Source {
+  std::string id;
+  double max_range = 0;
+  unsigned long long* keys = nullptr;
+  unsigned long long* masks = nullptr;
+  uint32_t* list = nullptr;
+  uint32_t cap = 0;
+  uint32_t pending = 0;         // listed update leaves not yet integrated
+  uint64_t pending_voxels = 0;
+};
+
+uint64_t next_pow2(uint64_t v) {
+  uint64_t p = 1;
+  while (p < v) p <<= 1;
+  return p;
+}
+
+}  // namespace
+
+struct vdbgpu_map {
+  std::mutex mu;
+  int device = 0;
+  int num_cu = 256;
+  hipStream_t stream = nullptr;
+  double res = 0, inv = 0;
+
+  // config (VDBMapping.hpp:1456-1469, OccupancyVDBMapping.hpp:59-89)
+  bool config_set = false;
+  double max_range = 0;
+  bool fast_mode = false;
+  float lo[6] = {0, 0, 0, 0, 0, 0};  // l_hit, l_miss, t_min, t_max, c_max, c_min
+  std::map<std::string, Source> sources;
+
+  // leaf store
+  unsigned long long* mkeys = nullptr;
+  uint32_t* midx = nullptr;
+  uint32_t mcap = 0;
+  float* values = nullptr;
+  unsigned long long* active = nullptr;
+  unsigned long long* leaf_keys = nullptr;
+  uint8_t* dirty = nullptr;
+  uint32_t lcap = 0;
+  uint64_t leaf_count_ub = 0;  // upper bound of the device leaf count
+  bool touched = false;        // any voxel modify since creation/reset
+
+  // per-scan scratch
+  float* d_points = nullptr;
+  size_t points_bytes = 0;
+  int4* rayinfo = nullptr;
+  uint32_t* pre = nullptr;
+  uint32_t* bsum = nullptr;
+  uint64_t* boff = nullptr;
+  uint32_t ray_cap = 0;
+  uint32_t* eidx = nullptr;
+  uint32_t eidx_cap = 0;
+  double* tbl = nullptr;
+  uint32_t tbl_n = 0;
+
+  // status: [0] scan, [1] map (leaf_count), [2] auxiliary
+  Status* d_st = nullptr;
+  Status* h_st = nullptr;
+
+  vdbgpu_tuning tuning{256, 128, 8, 0.5};
+  bool profiling = false;
+  std::vector<TimedPair> timed;
+  std::vector<hipEvent_t> event_pool;
+  vdbgpu_stats stats{};
+  uint64_t device_bytes = 0;
+  std::string last_error;
+};
+
+namespace {
+
+#define CK(expr)                                                                 \
+  do {                                                                           \
+    hipError_t e_ = (expr);                                                      \
+    if (e_ != hipSuccess) {                                                      \
+      m->last_error = std::string(#expr) + ": " + hipGetErrorString(e_);         \
+      return VDBGPU_DEVICE_ERROR;                                                \
+    }                                                                            \
+  } while (0)
+
+template <class T>
+hipError_t dalloc(vdbgpu_map* m, T** p, size_t count) {
+  *p = nullptr;
+  if (count == 0) return hipSuccess;
+  hipError_t e = hipMalloc((void**)p, count * sizeof(T));
+  if (e == hipSuccess) m->device_bytes += count * sizeof(T);
+  return e;
+}
+
+template <class T>
+void dfree(vdbgpu_map* m, T** p, size_t count) {
+  if (*p) {
+    (void)hipFree(*p);
+    m->device_bytes -= count * sizeof(T);
+  }
+  *p = nullptr;
+}
+
+hipEvent_t get_event(vdbgpu_map* m) {
+  if (!m->event_pool.empty()) {
+    hipEvent_t e = m->event_pool.back();
+    m->event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  (void)hipEventCreate(&e);
+  return e;
+}
+
+struct Timer {
+  vdbgpu_map* m;
+  int kind;
+  hipEvent_t a = nullptr;
+  Timer(vdbgpu_map* mm, int k) : m(mm), kind(k) {
+    if (m->profiling) {
+      a = get_event(m);
+      (void)hipEventRecord(a, m->stream);
+    }
+  }
+  ~Timer() {
+    if (a) {
+      hipEvent_t b = get_event(m);
+      (void)hipEventRecord(b, m->stream);
+      m->timed.push_back({kind, a, b});
+    }
+  }
+};
+
+// Fold finished event pairs into the stats (call after a stream sync).
+void collect_timing(vdbgpu_map* m) {
+  for (auto& t : m->timed) {
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, t.a, t.b) == hipSuccess) {
+      switch (t.kind) {
+        case kPrep: m->stats.ms_prep += ms; m->stats.n_prep++; break;
+        case kTraverse: m->stats.ms_traverse += ms; m->stats.n_traverse++; break;
+        case kCompact: m->stats.ms_compact += ms; m->stats.n_compact++; break;
+        default: m->stats.ms_apply += ms; m->stats.n_apply++; break;
+      }
+    }
+    m->event_pool.push_back(t.a);
+    m->event_pool.push_back(t.b);
+  }
+  m->timed.clear();
+}
+
+int read_status(vdbgpu_map* m) {
+  CK(hipMemcpyAsync(m->h_st, m->d_st, 3 * sizeof(Status), hipMemcpyDeviceToHost, m->stream));
+  CK(hipStreamSynchronize(m->stream));
+  collect_timing(m);
+  m->leaf_count_ub = m->h_st[1].leaf_count;
+  return VDBGPU_OK;
+}
+
+int grid_lanes(vdbgpu_map* m, int block) {
+  const int lanes = m->num_cu * m->tuning.waves_per_cu * 64;
+  return std::max(1, lanes / block);
+}
+
+UpdView upd_view(const Source& s) { return UpdView{s.keys, s.masks, s.cap - 1}; }
+
+MapView map_view(vdbgpu_map* m) {
+  return MapView{m->mkeys, m->midx, m->mcap - 1, m->values, m->active, m->leaf_keys, m->dirty};
+}
+
+int alloc_update(vdbgpu_map* m, Source& s, uint32_t cap) {
+  s.cap = cap;
+  CK(dalloc(m, &s.keys, cap));
+  CK(dalloc(m, &s.masks, (size_t)cap * 16));
+  CK(dalloc(m, &s.list, cap));
+  CK(launch_fill_u64(m->stream, s.keys, cap, kEmpty));
+  CK(hipMemsetAsync(s.masks, 0, (size_t)cap * 16 * sizeof(unsigned long long), m->stream));
+  return VDBGPU_OK;
+}
+
+void free_update(vdbgpu_map* m, Source& s) {
+  dfree(m, &s.keys, s.cap);
+  dfree(m, &s.masks, (size_t)s.cap * 16);
+  dfree(m, &s.list, s.cap);
+  s.cap = 0;
+}
+
+// Move a source's update set into a table of `cap` slots.
+int grow_update(vdbgpu_map* m, Source& s, uint32_t cap) {
+  Source t;
+  int r = alloc_update(m, t, cap);
+  if (r) return r;
+  if (s.cap) {
+    CK(launch_rehash_update(m->stream, std::min<int>(4096, (s.cap + 255) / 256), upd_view(s), s.cap,
+                            upd_view(t), &m->d_st[2]));
+  }
+  free_update(m, s);  // hipFree waits for the rehash
+  s.keys = t.keys;
+  s.masks = t.masks;
+  s.list = t.list;
+  s.cap = t.cap;
+  return VDBGPU_OK;
+}
+
+int ensure_rays(vdbgpu_map* m, uint32_t n) {
+  if (n <= m->ray_cap) return VDBGPU_OK;
+  const uint32_t cap = (uint32_t)std::min<uint64_t>(kMaxRays, next_pow2(n));
+  const uint32_t nb_old = (m->ray_cap + kPrepBlock - 1) / kPrepBlock;
+  dfree(m, &m->rayinfo, m->ray_cap);
+  dfree(m, &m->pre, m->ray_cap);
+  dfree(m, &m->bsum, nb_old);
+  dfree(m, &m->boff, nb_old);
+  const uint32_t nb = (cap + kPrepBlock - 1) / kPrepBlock;
+  m->ray_cap = cap;
+  CK(dalloc(m, &m->rayinfo, cap));
+  CK(dalloc(m, &m->pre, cap));
+  CK(dalloc(m, &m->bsum, nb));
+  CK(dalloc(m, &m->boff, nb));
+  return VDBGPU_OK;
+}
+
+int ensure_table(vdbgpu_map* m, uint32_t need) {
+  need = std::min(need, kMaxTableN);
+  if (need <= m->tbl_n) return VDBGPU_OK;
+  const uint32_t n = std::min(kMaxTableN, (need + 255u) & ~255u);
+  const uint64_t old_entries = (uint64_t)m->tbl_n * (m->tbl_n + 1) / 2;
+  dfree(m, &m->tbl, old_entries);
+  CK(dalloc(m, &m->tbl, (uint64_t)n * (n + 1) / 2));
+  CK(launch_build_table(m->stream, m->tbl, 1, n));
+  m->tbl_n = n;
+  return VDBGPU_OK;
+}
+
+// Leaf pool and leaf hash large enough for `need` leaves (hash load <= 0.5).
+int ensure_map(vdbgpu_map* m, uint64_t need) {
+  if (need > 0xF0000000ull) {
+    m->last_error = "leaf pool would exceed 2^32 leaves";
+    return VDBGPU_DEVICE_ERROR;
+  }
+  if (need > m->lcap) {
+    const uint32_t cap = (uint32_t)std::max<uint64_t>(need, std::max<uint64_t>(4096, m->lcap + m->lcap / 2));
+    float* v = nullptr;
+    unsigned long long *a = nullptr, *k = nullptr;
+    uint8_t* d = nullptr;
+    CK(dalloc(m, &v, (size_t)cap * 512));
+    CK(dalloc(m, &a, (size_t)cap * 8));
+    CK(dalloc(m, &k, cap));
+    CK(dalloc(m, &d, cap));
+    CK(hipMemsetAsync(v, 0, (size_t)cap * 512 * sizeof(float), m->stream));
+    CK(hipMemsetAsync(a, 0, (size_t)cap * 8 * sizeof(unsigned long long), m->stream));
+    CK(hipMemsetAsync(d, 0, cap, m->stream));
+    if (m->lcap) {
+      CK(hipMemcpyAsync(v, m->values, (size_t)m->lcap * 512 * sizeof(float), hipMemcpyDeviceToDevice, m->stream));
+      CK(hipMemcpyAsync(a, m->active, (size_t)m->lcap * 8 * sizeof(unsigned long long), hipMemcpyDeviceToDevice, m->stream));
+      CK(hipMemcpyAsync(k, m->leaf_keys, (size_t)m->lcap * sizeof(unsigned long long), hipMemcpyDeviceToDevice, m->stream));
+      CK(hipMemcpyAsync(d, m->dirty, m->lcap, hipMemcpyDeviceToDevice, m->stream));
+    }
+    dfree(m, &m->values, (size_t)m->lcap * 512);
+    dfree(m, &m->active, (size_t)m->lcap * 8);
+    dfree(m, &m->leaf_keys, m->lcap);
+    dfree(m, &m->dirty, m->lcap);
+    m->values = v;
+    m->active = a;
+    m->leaf_keys = k;
+    m->dirty = d;
+    m->lcap = cap;
+  }
+  if (2 * need > m->mcap) {
+    const uint32_t cap = (uint32_t)std::min<uint64_t>(1ull << 31, std::max<uint64_t>(8192, next_pow2(4 * need)));
+    dfree(m, &m->mkeys, m->mcap);
+    dfree(m, &m->midx, m->mcap);
+    m->mcap = cap;
+    CK(dalloc(m, &m->mkeys, cap));
+    CK(dalloc(m, &m->midx, cap));
+    CK(launch_fill_u64(m->stream, m->mkeys, cap, kEmpty));
+    const int rs = read_status(m);  // exact leaf count for the rehash bound
+    if (rs) return rs;
+    CK(launch_rehash_map(m->stream, std::min<int>(4096, (int)((m->h_st[1].leaf_count + 255) / 256) + 1),
+                         map_view(m), m->h_st[1].leaf_count));
+  }
+  return VDBGPU_OK;
+}
+
+int fail(vdbgpu_map* m, int code, const std::string& msg) {
+  m->last_error = msg;
+  return code;
+}
+
+int do_accumulate(vdbgpu_map* m, Source& src, const float* xyz, int64_t n, int64_t stride,
+                  const double origin[3], bool on_device) {
+  if (!m->config_set) return fail(m, VDBGPU_NOT_CONFIGURED, "Map not properly configured. Did you call setConfig method?");
+  if (m->fast_mode) return fail(m, VDBGPU_UNSUPPORTED, "fast_mode raycasting is not provided by this library");
+  if (n < 0 || stride < 12 || (stride & 3) || n > (int64_t)kMaxRays || !origin || (n && !xyz))
+    return fail(m, VDBGPU_BAD_ARGUMENT, "bad point buffer arguments");
+  if (n == 0) return VDBGPU_OK;
+  const double R = src.max_range;
+  const uint32_t nn = (uint32_t)n;
+  int r;
+  if ((r = ensure_rays(m, nn))) return r;
+  const double ext = std::ceil(R * m->inv) + 2.0;
+  if ((r = ensure_table(m, ext < (double)kMaxTableN ? (uint32_t)ext : kMaxTableN))) return r;
+  if (src.cap == 0) {
+    if ((r = alloc_update(m, src, (uint32_t)std::max<uint64_t>(kMinUpdateCap, next_pow2(4 * (uint64_t)nn))))) return r;
+  }
+  const float* pts = xyz;
+  if (!on_device) {
+    const size_t bytes = (size_t)(n - 1) * stride + 12;
+    if (bytes > m->points_bytes) {
+      dfree(m, &m->d_points, m->points_bytes / 4);
+      const size_t cap = std::max(bytes, m->points_bytes * 3 / 2) + 16;
+      CK(dalloc(m, &m->d_points, (cap + 3) / 4));
+      m->points_bytes = ((cap + 3) / 4) * 4;
+    }
+    CK(hipMemcpyAsync(m->d_points, xyz, bytes, hipMemcpyHostToDevice, m->stream));
+    pts = m->d_points;
+  }
+  const uint32_t nb = (nn + kPrepBlock - 1) / kPrepBlock;
+  CK(hipMemsetAsync(&m->d_st[0], 0, sizeof(Status), m->stream));
+  {
+    Timer t(m, kPrep);
+    CK(launch_prep(m->stream, pts, stride, nn, origin, R, m->res, m->inv, m->tbl_n, m->rayinfo,
+                   m->pre, m->bsum, &m->d_st[0]));
+    CK(launch_scan_blocks(m->stream, m->bsum, m->boff, nb, &m->d_st[0]));
+  }
+  const int block = m->tuning.block;
+  for (int attempt = 0;; ++attempt) {
+    {
+      Timer t(m, kTraverse);
+      CK(launch_traverse(m->stream, grid_lanes(m, block), block, m->rayinfo, m->pre, m->boff, nb, nn,
+                         m->tbl, upd_view(src), &m->d_st[0], (uint32_t)m->tuning.steps_per_lane));
+    }
+    {
+      Timer t(m, kCompact);
+      CK(launch_compact(m->stream, std::min<int>(4096, (int)((src.cap + 255) / 256)), upd_view(src),
+                        src.cap, src.list, &m->d_st[0]));
+    }
+    if ((r = read_status(m))) return r;
+    const Status& st = m->h_st[0];
+    if (st.error & kErrRange)
+      return fail(m, VDBGPU_OUT_OF_RANGE, "a voxel index is outside [-2^23, 2^23) or not finite; scan rejected");
+    if (st.error & kErrTable)
+      return fail(m, VDBGPU_UNSUPPORTED, "a ray spans more than 8192 voxels along one axis");
+    if (st.overflow & kOvfHard) {
+      if (attempt > 6 || src.cap >= (1u << 30)) return fail(m, VDBGPU_DEVICE_ERROR, "update set cannot grow");
+      if ((r = grow_update(m, src, src.cap * 4))) return r;
+      CK(hipMemsetAsync(&m->d_st[0].overflow, 0, sizeof(unsigned int), m->stream));
+      CK(hipMemsetAsync(&m->d_st[0].entries, 0, sizeof(unsigned int), m->stream));
+      CK(hipMemsetAsync(&m->d_st[0].voxels, 0, sizeof(unsigned long long), m->stream));
+      continue;
+    }
+    break;
+  }
+  const Status& st = m->h_st[0];
+  src.pending = st.entries;
+  src.pending_voxels = st.voxels;
+  m->stats.scans++;
+  m->stats.rays += st.rays;
+  m->stats.visits += st.total;
+  if ((double)st.entries > m->tuning.load_factor * src.cap && src.cap < (1u << 30)) {
+    if ((r = grow_update(m, src, src.cap * 2))) return r;
+    CK(hipMemsetAsync(&m->d_st[0], 0, sizeof(Status), m->stream));
+    CK(launch_compact(m->stream, std::min<int>(4096, (int)((src.cap + 255) / 256)), upd_view(src),
+                      src.cap, src.list, &m->d_st[0]));
+    if ((r = read_status(m))) return r;
+    src.pending = m->h_st[0].entries;
+  }
+  return VDBGPU_OK;
+}
+
+// Creation rule for a missing leaf: the op must change a background voxel
+// (value 0, inactive).
+void create_flags(const vdbgpu_map* m, bool* ch, bool* cm) {
+  float x = 0.0f + m->lo[0];
+  bool a = false;
+  if (x > m->lo[3]) {
+    a = true;
+    if (x > m->lo[4]) x = m->lo[4];
+  }
+  *ch = (x != 0.0f) || a;
+  x = 0.0f + m->lo[1];
+  if (x < m->lo[2]) {
+    if (x < m->lo[5]) x = m->lo[5];
+  }
+  *cm = x != 0.0f;
+}
+
+int do_integrate(vdbgpu_map* m) {
+  bool ch, cm;
+  create_flags(m, &ch, &cm);
+  for (auto& kv : m->sources) {
+    Source& s = kv.second;
+    if (s.pending == 0) continue;
+    int r;
+    if ((r = ensure_map(m, m->leaf_count_ub + s.pending))) return r;
+    if (s.pending > m->eidx_cap) {
+      dfree(m, &m->eidx, m->eidx_cap);
+      m->eidx_cap = (uint32_t)next_pow2(s.pending);
+      CK(dalloc(m, &m->eidx, m->eidx_cap));
+    }
+    const int g1 = std::min<int>(4096, (int)((s.pending + 255) / 256));
+    const int g2 = std::min<int>(m->num_cu * 16, (int)((s.pending + 3) / 4));
+    {
+      Timer t(m, kApply);
+      CK(launch_map_insert(m->stream, g1, upd_view(s), s.list, s.pending, map_view(m), m->eidx, ch, cm,
+                           &m->d_st[1]));
+      CK(launch_apply(m->stream, g2, upd_view(s), s.list, m->eidx, s.pending, map_view(m), m->lo));
+    }
+    m->leaf_count_ub += s.pending;
+    m->stats.voxel_updates += s.pending_voxels;
+    m->stats.update_leaves += s.pending;
+    m->touched = true;
+    s.pending = 0;
+    s.pending_voxels = 0;
+  }
+  return VDBGPU_OK;
+}
+
+struct Guard {
+  vdbgpu_map* m;
+  std::lock_guard<std::mutex> lk;
+  explicit Guard(vdbgpu_map* mm) : m(mm), lk(mm->mu) { (void)hipSetDevice(m->device); }
+};
+
+int64_t export_sorted(vdbgpu_map* m, std::vector<unsigned long long>& keys, std::vector<float>& vals,
+                      std::vector<unsigned long long>& act, int32_t* origins, float* values,
+                      uint64_t* masks, int64_t cap, bool with_vals) {
+  const int64_t n = (int64_t)keys.size();
+  if (!origins && !values && !masks) return n;
+  if (n > cap) return -n;
+  std::vector<std::pair<std::array<int, 3>, uint32_t>> order(n);
+  for (int64_t i = 0; i < n; ++i) {
+    std::array<int, 3> o;
+    key_origin(keys[i], o.data());
+    order[i] = {o, (uint32_t)i};
+  }
+  std::sort(order.begin(), order.end());
+  for (int64_t j = 0; j < n; ++j) {
+    const uint32_t i = order[j].second;
+    if (origins) std::memcpy(origins + 3 * j, order[j].first.data(), 12);
+    if (values && with_vals) std::memcpy(values + 512 * j, &vals[(size_t)i * 512], 2048);
+    if (masks) std::memcpy(masks + 8 * j, &act[(size_t)i * 8], 64);
+  }
+  (void)m;
+  return n;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t vdbgpu_version(void) { return 100; }
+
+const char* vdbgpu_status_string(int32_t s) {
+  switch (s) {
+    case VDBGPU_OK: return "ok";
+    case VDBGPU_NOT_CONFIGURED: return "not configured";
+    case VDBGPU_UNKNOWN_SOURCE: return "unknown source";
+    case VDBGPU_BAD_MAX_RANGE: return "max_range < 0";
+    case VDBGPU_BAD_PROB_MISS: return "prob_miss > 0.5";
+    case VDBGPU_BAD_PROB_HIT: return "prob_hit < 0.5";
+    case VDBGPU_OUT_OF_RANGE: return "voxel index out of range";
+    case VDBGPU_UNSUPPORTED: return "unsupported";
+    case VDBGPU_DEVICE_ERROR: return "device error";
+    case VDBGPU_BAD_ARGUMENT: return "bad argument";
+    case VDBGPU_NO_INTERSECTOR: return "no intersector";
+    default: return "unknown status";
+  }
+}
+
+int32_t vdbgpu_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+int32_t vdbgpu_create(double resolution, vdbgpu_map** out) {
+  if (!out) return VDBGPU_BAD_ARGUMENT;
+  *out = nullptr;
+  if (!(resolution > 0) || !std::isfinite(resolution)) return VDBGPU_BAD_ARGUMENT;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return VDBGPU_DEVICE_ERROR;
+  vdbgpu_map* m = new vdbgpu_map();
+  m->res = resolution;
+  m->inv = 1.0 / resolution;  // ScaleMap keeps 1/scale and multiplies (worldToIndex)
+  if (hipGetDevice(&m->device) != hipSuccess) {
+    delete m;
+    return VDBGPU_DEVICE_ERROR;
+  }
+  int cu = 0;
+  if (hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, m->device) == hipSuccess && cu > 0)
+    m->num_cu = cu;
+  if (hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipMalloc((void**)&m->d_st, 3 * sizeof(Status)) != hipSuccess ||
+      hipHostMalloc((void**)&m->h_st, 3 * sizeof(Status), hipHostMallocDefault) != hipSuccess ||
+      hipMemsetAsync(m->d_st, 0, 3 * sizeof(Status), m->stream) != hipSuccess) {
+    vdbgpu_destroy(m);
+    return VDBGPU_DEVICE_ERROR;
+  }
+  memset(m->h_st, 0, 3 * sizeof(Status));
+  *out = m;
+  return VDBGPU_OK;
+}
+
+void vdbgpu_destroy(vdbgpu_map* m) {
+  if (!m) return;
+  (void)hipSetDevice(m->device);
+  if (m->stream) (void)hipStreamSynchronize(m->stream);
+  for (auto& kv : m->sources) free_update(m, kv.second);
+  dfree(m, &m->mkeys, m->mcap);
+  dfree(m, &m->midx, m->mcap);
+  dfree(m, &m->values, (size_t)m->lcap * 512);
+  dfree(m, &m->active, (size_t)m->lcap * 8);
+  dfree(m, &m->leaf_keys, m->lcap);
+  dfree(m, &m->dirty, m->lcap);
+  dfree(m, &m->d_points, m->points_bytes / 4);
+  const uint32_t nb = (m->ray_cap + kPrepBlock - 1) / kPrepBlock;
+  dfree(m, &m->rayinfo, m->ray_cap);
+  dfree(m, &m->pre, m->ray_cap);
+  dfree(m, &m->bsum, nb);
+  dfree(m, &m->boff, nb);
+  dfree(m, &m->eidx, m->eidx_cap);
+  dfree(m, &m->tbl, (uint64_t)m->tbl_n * (m->tbl_n + 1) / 2);
+  if (m->d_st) (void)hipFree(m->d_st);
+  if (m->h_st) (void)hipHostFree(m->h_st);
+  for (auto& t : m->timed) {
+    (void)hipEventDestroy(t.a);
+    (void)hipEventDestroy(t.b);
+  }
+  for (auto e : m->event_pool) (void)hipEventDestroy(e);
+  if (m->stream) (void)hipStreamDestroy(m->stream);
+  delete m;
+}
+
+int32_t vdbgpu_set_config(vdbgpu_map* m, const vdbgpu_config* c) {
+  if (!m || !c) return VDBGPU_BAD_ARGUMENT;
+  Guard g(m);
+  int status = VDBGPU_OK;
+  if (c->max_range < 0.0) {
+    status = VDBGPU_BAD_MAX_RANGE;
+  } else {
+    m->max_range = c->max_range;
+    m->fast_mode = c->fast_mode != 0;
+    m->config_set = true;
+  }
+  if (c->prob_miss > 0.5) return VDBGPU_BAD_PROB_MISS;
+  if (c->prob_hit < 0.5) return VDBGPU_BAD_PROB_HIT;
+  m->lo[1] = (float)(std::log(c->prob_miss) - std::log(1 - c->prob_miss));
+  m->lo[0] = (float)(std::log(c->prob_hit) - std::log(1 - c->prob_hit));
+  m->lo[2] = (float)(std::log(c->prob_thres_min) - std::log(1 - c->prob_thres_min));
+  m->lo[3] = (float)(std::log(c->prob_thres_max) - std::log(1 - c->prob_thres_max));
+  m->lo[4] = (float)(std::log(0.99) - std::log(0.01));
+  m->lo[5] = (float)(std::log(0.01) - std::log(0.99));
+  m->config_set = true;
+  return status;
+}
+
+int32_t vdbgpu_logodds(vdbgpu_map* m, float out[6]) {
+  if (!m || !out) return VDBGPU_BAD_ARGUMENT;
+  Guard g(m);
+  std::memcpy(out, m->lo, sizeof m->lo);
+  return VDBGPU_OK;
+}
+
+int32_t vdbgpu_add_source(vdbgpu_map* m, const char* id, double max_range, double max_rate) {
+  (void)max_rate;
+  if (!m || !id) return VDBGPU_BAD_ARGUMENT;
+  Guard g(m);
+  Source& s = m->sources[id];
+  s.id = id;
+  s.max_range = (max_range == 0) ? m->max_range : max_range;
+  if (s.cap) {  // re-adding a source starts it with an empty update grid
+    free_update(m, s);
+    s.pending = 0;
+    s.pending_voxels = 0;
+  }
+  return VDBGPU_OK;
+}
+
+int32_t vdbgpu_accumulate(vdbgpu_map* m, const char* id, const float* xyz, int64_t n, int64_t stride,
+                          const double origin[3], int32_t on_device) {
+  if (!m || !id) return VDBGPU_BAD_ARGUMENT;
+  Guard g(m);
+  auto it = m->sources.find(id);
+  if (it == m->sources.end()) return fail(m, VDBGPU_UNKNOWN_SOURCE, std::string("Tried to accumulate update for ") + id + ". Source not available");
+  if (!(it->second.max_range > 0)) return VDBGPU_OK;
+  return do_accumulate(m, it->second, xyz, n, stride, origin, on_device != 0);
+}
+
+int32_t vdbgpu_integrate(vdbgpu_map* m) {
+  if (!m) return VDBGPU_BAD_ARGUMENT;
+  Guard g(m);
+  return do_integrate(m);
+}
+
+int32_t vdbgpu_insert(vdbgpu_map* m, const char* id, const float* xyz, int64_t n, int64_t stride,
+                      const double origin[3], int32_t on_device) {
+  if (!m || !id) return VDBGPU_BAD_ARGUMENT;
+  Guard g(m);
+  int status = VDBGPU_OK;
+  auto it = m->sources.find(id);
+  if (it == m->sources.end())
+    status = fail(m, VDBGPU_UNKNOWN_SOURCE, std::string("Tried to accumulate update for ") + id + ". Source not available");
+  else if (it->second.max_range > 0)
+    status = do_accumulate(m, it->second, xyz, n, stride, origin, on_device != 0);
+  const int r = do_integrate(m);
+  return status ? status : r;
+}
+
+int32_t vdbgpu_reset(vdbgpu_map* m) {
+  if (!m) return VDBGPU_BAD_ARGUMENT;
+  Guard g(m);
+  if (m->lcap) {
+    CK(hipMemsetAsync(m->values, 0, (size_t)m->lcap * 512 * sizeof(float), m->stream));
+    CK(hipMemsetAsync(m->active, 0, (size_t)m->lcap * 8 * sizeof(unsigned long long), m->stream));
+    CK(hipMemsetAsync(m->dirty, 0, m->lcap, m->stream));
+  }
+  if (m->mcap) CK(launch_fill_u64(m->stream, m->mkeys, m->mcap, kEmpty));
+  CK(hipMemsetAsync(&m->d_st[1], 0, sizeof(Status), m->stream));
+  for (auto& kv : m->sources) {
+    Source& s = kv.second;
+    if (s.cap) {
+      CK(launch_fill_u64(m->stream, s.keys, s.cap, kEmpty));
+      CK(hipMemsetAsync(s.masks, 0, (size_t)s.cap * 16 * sizeof(unsigned long long), m->stream));
+    }
+    s.pending = 0;
+    s.pending_voxels = 0;
+  }
+  m->leaf_count_ub = 0;
+  m->touched = false;
+  CK(hipStreamSynchronize(m->stream));
+  return VDBGPU_OK;
+}
+
+int32_t vdbgpu_empty(vdbgpu_map* m) {
+  if (!m) return VDBGPU_BAD_ARGUMENT;
+  Guard g(m);
+  return m->touched ? 0 : 1;
+}
+
+int64_t vdbgpu_leaf_count(vdbgpu_map* m) {
+  if (!m) return -1;
+  Guard g(m);
+  if (read_status(m)) return -1;
+  return m->h_st[1].leaf_count;
+}
+
+int64_t vdbgpu_export_leaves(vdbgpu_map* m, int32_t* origins, float* values, uint64_t* masks,
+                             int64_t cap) {
+  if (!m) return -1;
+  Guard g(m);
+  if (read_status(m)) return -1;
+  const uint32_t n = m->h_st[1].leaf_count;
+  if (!origins && !values && !masks) return n;
+  if ((int64_t)n > cap) return -(int64_t)n;
+  std::vector<unsigned long long> keys(n), act((size_t)n * 8);
+  std::vector<float> vals(values ? (size_t)n * 512 : 0);
+  if (n) {
+    if (hipMemcpy(keys.data(), m->leaf_keys, n * sizeof(unsigned long long), hipMemcpyDeviceToHost) ||
+        hipMemcpy(act.data(), m->active, (size_t)n * 64, hipMemcpyDeviceToHost) ||
+        (values && hipMemcpy(vals.data(), m->values, (size_t)n * 2048, hipMemcpyDeviceToHost))) {
+      m->last_error = "export copy failed";
+      return -1;
+    }
+  }
+  return export_sorted(m, keys, vals, act, origins, values, masks, cap, values != nullptr);
+}
+
+int64_t vdbgpu_export_dirty_leaves(vdbgpu_map* m, int32_t* origins, float* values, uint64_t* masks,
+                                   int64_t cap) {
+  if (!m) return -1;
+  Guard g(m);
+  if (read_status(m)) return -1;
+  const uint32_t lc = m->h_st[1].leaf_count;
+  if (lc == 0) return 0;
+  if (hipMemsetAsync(&m->d_st[2], 0, sizeof(Status), m->stream) != hipSuccess) return -1;
+  // the dirty list reuses the eidx scratch (>= leaf_count entries needed)
+  if (lc > m->eidx_cap) {
+    dfree(m, &m->eidx, m->eidx_cap);
+    m->eidx_cap = (uint32_t)next_pow2(lc);
+    if (dalloc(m, &m->eidx, m->eidx_cap) != hipSuccess) return -1;
+  }
+  const int grid = std::min<int>(4096, (int)((lc + 255) / 256));
+  if (launch_dirty_list(m->stream, grid, m->dirty, lc, m->eidx, &m->d_st[2]) || read_status(m)) return -1;
+  const uint32_t nd = m->h_st[2].entries;
+  if ((int64_t)nd > cap) {
+    // put the flags back so that a retry with a larger buffer sees the same set
+    std::vector<uint32_t> idx(nd);
+    std::vector<uint8_t> one(1, 1);
+    (void)hipMemcpy(idx.data(), m->eidx, nd * 4, hipMemcpyDeviceToHost);
+    for (uint32_t i : idx) (void)hipMemcpy(m->dirty + i, one.data(), 1, hipMemcpyHostToDevice);
+    return -(int64_t)nd;
+  }
+  if (nd == 0) return 0;
+  unsigned long long *dk = nullptr, *da = nullptr;
+  float* dv = nullptr;
+  if (dalloc(m, &dk, nd) || dalloc(m, &da, (size_t)nd * 8) || dalloc(m, &dv, (size_t)nd * 512)) return -1;
+  std::vector<unsigned long long> keys(nd), act((size_t)nd * 8);
+  std::vector<float> vals((size_t)nd * 512);
+  int64_t r = -1;
+  if (!launch_gather_leaves(m->stream, std::min<int>(4096, (int)((nd + 3) / 4)), m->eidx, nd, map_view(m), dk, dv, da) &&
+      !hipStreamSynchronize(m->stream) &&
+      !hipMemcpy(keys.data(), dk, nd * 8, hipMemcpyDeviceToHost) &&
+      !hipMemcpy(act.data(), da, (size_t)nd * 64, hipMemcpyDeviceToHost) &&
+      !hipMemcpy(vals.data(), dv, (size_t)nd * 2048, hipMemcpyDeviceToHost))
+    r = export_sorted(m, keys, vals, act, origins, values, masks, cap, true);
+  dfree(m, &dk, nd);
+  dfree(m, &da, (size_t)nd * 8);
+  dfree(m, &dv, (size_t)nd * 512);
+  return r;
+}
+
+int64_t vdbgpu_update_leaf_count(vdbgpu_map* m, const char* id) {
+  if (!m || !id) return -1;
+  Guard g(m);
+  auto it = m->sources.find(id);
+  if (it == m->sources.end()) return -1;
+  return it->second.pending;
+}
+
+int64_t vdbgpu_export_update(vdbgpu_map* m, const char* id, int32_t* origins, uint64_t* visited,
+                             uint64_t* hit, int64_t cap) {
+  if (!m || !id) return -1;
+  Guard g(m);
+  auto it = m->sources.find(id);
+  if (it == m->sources.end()) return -1;
+  Source& s = it->second;
+  if (!origins && !visited && !hit) return s.pending;
+  if ((int64_t)s.pending > cap) return -(int64_t)s.pending;
+  if (s.cap == 0 || s.pending == 0) return 0;
+  if (hipStreamSynchronize(m->stream)) return -1;
+  std::vector<unsigned long long> keys(s.cap), masks((size_t)s.cap * 16);
+  if (hipMemcpy(keys.data(), s.keys, (size_t)s.cap * 8, hipMemcpyDeviceToHost) ||
+      hipMemcpy(masks.data(), s.masks, (size_t)s.cap * 128, hipMemcpyDeviceToHost))
+    return -1;
+  std::vector<std::pair<std::array<int, 3>, uint32_t>> order;
+  for (uint32_t i = 0; i < s.cap; ++i) {
+    if (keys[i] == kEmpty) continue;
+    std::array<int, 3> o;
+    key_origin(keys[i], o.data());
+    order.push_back({o, i});
+  }
+  std::sort(order.begin(), order.end());
+  if ((int64_t)order.size() > cap) return -(int64_t)order.size();
+  for (size_t j = 0; j < order.size(); ++j) {
+    const uint32_t i = order[j].second;
+    if (origins) std::memcpy(origins + 3 * j, order[j].first.data(), 12);
+    if (visited) std::memcpy(visited + 8 * j, &masks[(size_t)i * 16], 64);
+    if (hit) std::memcpy(hit + 8 * j, &masks[(size_t)i * 16 + 8], 64);
+  }
+  return (int64_t)order.size();
+}
+
+int32_t vdbgpu_raytrace(vdbgpu_map* m, int64_t n, const double* o, const double* d,
+                        const double* len, uint8_t* succ, double* ends) {
+  (void)n; (void)o; (void)d; (void)len; (void)succ; (void)ends;
+  if (!m) return VDBGPU_BAD_ARGUMENT;
+  Guard g(m);
+  return fail(m, VDBGPU_NO_INTERSECTOR, "raytrace needs an integrate in fast mode first");
+}
+
+int32_t vdbgpu_synchronize(vdbgpu_map* m) {
+  if (!m) return VDBGPU_BAD_ARGUMENT;
+  Guard g(m);
+  return read_status(m);
+}
+
+int32_t vdbgpu_get_stats(vdbgpu_map* m, vdbgpu_stats* out) {
+  if (!m || !out) return VDBGPU_BAD_ARGUMENT;
+  Guard g(m);
+  int r = read_status(m);
+  if (r) return r;
+  m->stats.leaf_count = m->h_st[1].leaf_count;
+  m->stats.device_bytes = m->device_bytes;
+  *out = m->stats;
+  return VDBGPU_OK;
+}
+
+int32_t vdbgpu_set_profiling(vdbgpu_map* m, int32_t on) {
+  if (!m) return VDBGPU_BAD_ARGUMENT;
+  Guard g(m);
+  m->profiling = on != 0;
+  return VDBGPU_OK;
+}
+
+int32_t vdbgpu_set_tuning(vdbgpu_map* m, const vdbgpu_tuning* t) {
+  if (!m || !t) return VDBGPU_BAD_ARGUMENT;
+  if (t->block < 64 || t->block > 1024 || (t->block & 63) || t->steps_per_lane < 1 ||
+      t->steps_per_lane > 1 << 20 || t->waves_per_cu < 1 || t->waves_per_cu > 64 ||
+      !(t->load_factor >= 0.2 && t->load_factor <= 0.95))
+    return VDBGPU_BAD_ARGUMENT;
+  Guard g(m);
+  m->tuning = *t;
+  return VDBGPU_OK;
+}
+
+int32_t vdbgpu_get_tuning(vdbgpu_map* m, vdbgpu_tuning* out) {
+  if (!m || !out) return VDBGPU_BAD_ARGUMENT;
+  Guard g(m);
+  *out = m->tuning;
+  return VDBGPU_OK;
+}
+
+const char* vdbgpu_last_error(vdbgpu_map* m) { return m ? m->last_error.c_str() : ""; }
+
+}  // extern "C"
