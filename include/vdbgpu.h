@@ -63,7 +63,7 @@ typedef struct vdbgpu_stats {
 
 /* Launch parameters of the integration kernels (see vdbgpu_set_tuning). */
 typedef struct vdbgpu_tuning {
-  int32_t block;          /* threads per workgroup of the traversal kernel (64..1024) */
+  int32_t block;          /* threads per workgroup of the traversal kernel (64, 128, 192, 256) */
   int32_t steps_per_lane; /* DDA visits per lane work item (chunk length) */
   int32_t waves_per_cu;   /* resident waves per CU requested for the traversal grid */
   double load_factor;     /* update-set hash target load (0.3..0.9) */
@@ -145,6 +145,15 @@ int64_t vdbgpu_export_update(vdbgpu_map* map, const char* source_id, int32_t* or
 int32_t vdbgpu_raytrace(vdbgpu_map* map, int64_t n, const double* origins,
                         const double* directions, const double* max_lengths, uint8_t* success,
                         double* end_points);
+
+/* Device buffers for callers that keep point clouds resident in HBM (the
+ * points_on_device form of vdbgpu_accumulate / vdbgpu_insert). No reference
+ * counterpart: the reference takes host pcl clouds only (VM:399). Allocation
+ * is on the current HIP device; copies are synchronous. */
+int32_t vdbgpu_device_alloc(int64_t bytes, void** out);
+int32_t vdbgpu_device_free(void* ptr);
+int32_t vdbgpu_device_upload(void* dst, const void* src, int64_t bytes);
+int32_t vdbgpu_device_download(void* dst, const void* src, int64_t bytes);
 
 /* Wait for all work of the handle. */
 int32_t vdbgpu_synchronize(vdbgpu_map* map);

@@ -28,7 +28,8 @@ ABI_SYMBOLS = [
     "vdbgpu_leaf_count", "vdbgpu_export_leaves", "vdbgpu_export_dirty_leaves",
     "vdbgpu_update_leaf_count", "vdbgpu_export_update", "vdbgpu_raytrace", "vdbgpu_synchronize",
     "vdbgpu_get_stats", "vdbgpu_set_profiling", "vdbgpu_set_tuning", "vdbgpu_get_tuning",
-    "vdbgpu_last_error",
+    "vdbgpu_last_error", "vdbgpu_device_alloc", "vdbgpu_device_free", "vdbgpu_device_upload",
+    "vdbgpu_device_download",
 ]
 
 
@@ -124,6 +125,13 @@ def lib():
         L.vdbgpu_get_tuning.argtypes = [P, ctypes.POINTER(Tuning)]
         L.vdbgpu_last_error.restype = ctypes.c_char_p
         L.vdbgpu_last_error.argtypes = [P]
+        L.vdbgpu_device_alloc.restype = I32
+        L.vdbgpu_device_alloc.argtypes = [I64, ctypes.POINTER(P)]
+        L.vdbgpu_device_free.restype = I32
+        L.vdbgpu_device_free.argtypes = [P]
+        for f in (L.vdbgpu_device_upload, L.vdbgpu_device_download):
+            f.restype = I32
+            f.argtypes = [P, P, I64]
         _lib = L
     return _lib
 
@@ -295,6 +303,50 @@ class GpuMap:
         t = Tuning()
         lib().vdbgpu_get_tuning(self.h, ctypes.byref(t))
         return {n: getattr(t, n) for n, _ in t._fields_}
+
+
+class DeviceBuffer:
+    """Device memory from libvdbgpu (same HIP runtime as the kernels)."""
+
+    def __init__(self, nbytes: int):
+        self.ptr = ctypes.c_void_p()
+        self.nbytes = int(nbytes)
+        rc = lib().vdbgpu_device_alloc(self.nbytes, ctypes.byref(self.ptr))
+        if rc != OK:
+            raise VdbGpuError(f"device alloc of {nbytes} bytes failed: {status_string(rc)}")
+
+    @classmethod
+    def from_array(cls, a: np.ndarray) -> "DeviceBuffer":
+        a = np.ascontiguousarray(a)
+        b = cls(max(a.nbytes, 1))
+        b.upload(a)
+        return b
+
+    def upload(self, a: np.ndarray, offset: int = 0) -> None:
+        a = np.ascontiguousarray(a)
+        assert offset + a.nbytes <= self.nbytes
+        rc = lib().vdbgpu_device_upload(ctypes.c_void_p(self.ptr.value + offset), _ptr(a), a.nbytes)
+        if rc != OK:
+            raise VdbGpuError("upload failed")
+
+    def download(self, a: np.ndarray, offset: int = 0) -> np.ndarray:
+        assert a.flags.c_contiguous and offset + a.nbytes <= self.nbytes
+        rc = lib().vdbgpu_device_download(_ptr(a), ctypes.c_void_p(self.ptr.value + offset), a.nbytes)
+        if rc != OK:
+            raise VdbGpuError("download failed")
+        return a
+
+    @property
+    def address(self) -> int:
+        return int(self.ptr.value)
+
+    def free(self) -> None:
+        if self.ptr.value:
+            lib().vdbgpu_device_free(self.ptr)
+            self.ptr = ctypes.c_void_p()
+
+    def __del__(self):
+        self.free()
 
 
 def device_count() -> int:

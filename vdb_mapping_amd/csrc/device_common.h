@@ -28,7 +28,8 @@ constexpr int kPrepBlock = 256;
 
 // Status bits written by kernels.
 constexpr uint32_t kErrRange = 1u;  // a voxel index out of range / non-finite
-constexpr uint32_t kErrTable = 2u;  // crossing table shorter than a ray axis
+constexpr uint32_t kErrTable = 2u;  // crossing table or chunk table too short
+constexpr uint32_t kErrEntry = 4u;  // mid-ray DDA entry did not converge
 constexpr uint32_t kOvfHard = 1u;   // update-set probe found no free slot
 
 struct Status {

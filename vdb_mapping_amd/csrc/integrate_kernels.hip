@@ -40,7 +40,7 @@ __device__ __forceinline__ int world_to_index1(double c, double res, double inv)
 // Find or insert `key` in an update set; returns its slot, or kNone after
 // kMaxProbe probes (the host then grows the table and re-runs the scan, which
 // is exact because the update set is an OR of idempotent marks).
-constexpr uint32_t kMaxProbe = 2048;
+constexpr uint32_t kMaxProbe = 512;
 __device__ __forceinline__ uint32_t upd_slot(const UpdView& u, uint64_t key, unsigned int* ovf) {
   uint32_t h = (uint32_t)fmix64(key) & u.cap_mask;
   const uint32_t limit = u.cap_mask < kMaxProbe ? u.cap_mask + 1 : kMaxProbe;
@@ -104,99 +104,132 @@ struct Run {
   __device__ void flush(const UpdView& u) { upd_or(u, slot, word, bits); bits = 0; }
 };
 
-// Key order of DDA crossing events: earlier time first, ties to the larger
-// axis index (math::MinIndex).
-__device__ __forceinline__ bool kless(double t1, int a1, double t2, int a2) {
-  return t1 < t2 || (t1 == t2 && a1 > a2);
+// Exact order of DDA crossing events without floating point.
+//
+// An axis with |delta| = n (ray from a voxel centre, VDBMapping.hpp:555-564)
+// crosses voxel faces at the times math::DDA accumulates: T(n, 0) = 0.5*fl(1/n),
+// T(n, j) = fl(T(n, j-1) + fl(1/n)); the rational value is (2j+1)/(2n). For
+// n <= 8192, |T(n, j) - (2j+1)/(2n)| <= 2n * 2^-53 < 1.9e-12, while two
+// different rational times differ by at least 1/(2 n_a n_b) >= 7.4e-9. So the
+// double-precision order equals the rational order, which is decided exactly
+// by the integer sign of (2c_b+1) n_a - (2c_a+1) n_b, except at exact rational
+// ties. There the doubles decide (read from the crossing table), and equal
+// doubles go to the larger axis index (math::MinIndex). Axes with equal n have
+// identical double sequences, so their ties need no table read.
+
+// Event (b, cb) precedes event (a, ca) in DDA order.
+__device__ __forceinline__ bool ev_before(const double* __restrict__ tbl, int b, uint32_t nb,
+                                          uint32_t cb, int a, uint32_t na, uint32_t ca) {
+  const int64_t d = (int64_t)(2 * cb + 1) * na - (int64_t)(2 * ca + 1) * nb;
+  if (d != 0) return d < 0;
+  if (nb == na) return b > a;
+  const double fb = tbl[(((uint64_t)nb * (nb - 1)) >> 1) + cb];
+  const double fa = tbl[(((uint64_t)na * (na - 1)) >> 1) + ca];
+  return fb < fa || (fb == fa && b > a);
 }
 
-// DDA state after k steps of a ray with per-axis |delta| n[3]: the first k
-// events of the merged crossing sequences (time, -axis). Starts from the
-// rational estimate t = k / N and repairs it with exact table values until
-// every included event precedes every excluded one.
-__device__ void dda_enter(const double* __restrict__ tbl, const uint32_t n[3], uint32_t k,
-                          uint32_t c[3], double q[3]) {
-  const double* row[3];
-  double lin[3], fout[3];
-  const uint32_t N = n[0] + n[1] + n[2];
-  const double t = (double)k / (double)N;
+// Crossing counts c[3] after k steps of a ray with |delta| n[3] (k <= sum n):
+// the first k events of the merged crossing sequences. Starts from the
+// rational estimate at t = k / N (off by at most one event) and repairs it
+// until every included event precedes every excluded one; each repair moves
+// one misplaced event, so a handful of rounds suffice. Returns false only if
+// the repair did not converge (reported as a kernel error, never silently).
+__device__ bool dda_enter(const double* __restrict__ tbl, const uint32_t n[3], uint32_t k,
+                          uint32_t c[3]) {
+  const uint64_t N = (uint64_t)n[0] + n[1] + n[2];
   int sum = 0;
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    row[a] = tbl + (((uint64_t)n[a] * (uint64_t)(n[a] ? n[a] - 1 : 0)) >> 1);
-    const double g = floor(t * (double)n[a] + 0.5);
-    c[a] = n[a] ? (uint32_t)fmin(g, (double)n[a]) : 0u;
+    // #{j : (2j+1)/(2n) <= k/N} = floor((2 n k + N) / (2N))
+    const uint64_t g = (2ull * n[a] * k + N) / (2ull * N);
+    c[a] = (uint32_t)(g < n[a] ? g : n[a]);
     sum += (int)c[a];
-    lin[a] = c[a] > 0 ? row[a][c[a] - 1] : 0.0;
-    fout[a] = c[a] < n[a] ? row[a][c[a]] : 0.0;
   }
-  while (sum > (int)k) {
-    int best = -1;
-    double bt = 0;
-#pragma unroll
-    for (int a = 0; a < 3; ++a)
-      if (c[a] > 0 && (best < 0 || kless(bt, best, lin[a], a))) {
-        best = a;
-        bt = lin[a];
-      }
-#pragma unroll
-    for (int a = 0; a < 3; ++a)
-      if (a == best) {
-        c[a]--;
-        fout[a] = lin[a];
-        lin[a] = c[a] > 0 ? row[a][c[a] - 1] : 0.0;
-      }
-    --sum;
-  }
-  while (sum < (int)k) {
-    int best = -1;
-    double bt = 0;
-#pragma unroll
-    for (int a = 0; a < 3; ++a)
-      if (c[a] < n[a] && (best < 0 || kless(fout[a], a, bt, best))) {
-        best = a;
-        bt = fout[a];
-      }
-#pragma unroll
-    for (int a = 0; a < 3; ++a)
-      if (a == best) {
-        c[a]++;
-        lin[a] = fout[a];
-        fout[a] = c[a] < n[a] ? row[a][c[a]] : 0.0;
-      }
-    ++sum;
-  }
-  for (;;) {
+  for (int round = 0; round < 64; ++round) {
+    // included event that comes last (amax) and excluded event that comes first (amin)
     int amax = -1, amin = -1;
-    double tmax = 0, tmin = 0;
+    uint32_t nmax = 0, cmax = 0, nmin = 0, cmin = 0;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-      if (c[a] > 0 && (amax < 0 || kless(tmax, amax, lin[a], a))) {
+      if (c[a] > 0 && (amax < 0 || ev_before(tbl, amax, nmax, cmax - 1, a, n[a], c[a] - 1))) {
         amax = a;
-        tmax = lin[a];
+        nmax = n[a];
+        cmax = c[a];
       }
-      if (c[a] < n[a] && (amin < 0 || kless(fout[a], a, tmin, amin))) {
+      if (c[a] < n[a] && (amin < 0 || ev_before(tbl, a, n[a], c[a], amin, nmin, cmin))) {
         amin = a;
-        tmin = fout[a];
+        nmin = n[a];
+        cmin = c[a];
       }
     }
-    if (amax < 0 || amin < 0 || !kless(tmin, amin, tmax, amax)) break;
+    int dec = -1, inc = -1;
+    if (sum > (int)k) {
+      dec = amax;
+    } else if (sum < (int)k) {
+      inc = amin;
+    } else if (amax >= 0 && amin >= 0 && ev_before(tbl, amin, nmin, cmin, amax, nmax, cmax - 1)) {
+      dec = amax;
+      inc = amin;
+    } else {
+      return true;
+    }
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-      if (a == amax) {
-        c[a]--;
-        fout[a] = lin[a];
-        lin[a] = c[a] > 0 ? row[a][c[a] - 1] : 0.0;
-      } else if (a == amin) {
-        c[a]++;
-        lin[a] = fout[a];
-        fout[a] = c[a] < n[a] ? row[a][c[a]] : 0.0;
-      }
+      if (a == dec) c[a]--;
+      if (a == inc) c[a]++;
+    }
+    sum += (inc >= 0) - (dec >= 0);
+  }
+  return false;
+}
+
+// Integer DDA state of one ray: crossing counts and the pairwise order
+// differences D_yx = (2c_y+1) n_x - (2c_x+1) n_y, D_zx, D_zy.
+struct IntDda {
+  uint32_t n0, n1, n2, c0, c1, c2;
+  int dyx, dzx, dzy;
+  __device__ __forceinline__ void init(const uint32_t n[3], const uint32_t c[3]) {
+    n0 = n[0];
+    n1 = n[1];
+    n2 = n[2];
+    c0 = c[0];
+    c1 = c[1];
+    c2 = c[2];
+    dyx = (int)((2 * c1 + 1) * n0) - (int)((2 * c0 + 1) * n1);
+    dzx = (int)((2 * c2 + 1) * n0) - (int)((2 * c0 + 1) * n2);
+    dzy = (int)((2 * c2 + 1) * n1) - (int)((2 * c1 + 1) * n2);
+  }
+  // Float tie-break: does (b, cb) come no later than (a, ca) under MinIndex?
+  __device__ __forceinline__ bool tie_le(const double* __restrict__ tbl, uint32_t nb, uint32_t cb,
+                                         uint32_t na, uint32_t ca) const {
+    if (nb == na) return true;
+    if (cb >= nb) return false;
+    if (ca >= na) return true;
+    return tbl[(((uint64_t)nb * (nb - 1)) >> 1) + cb] <= tbl[(((uint64_t)na * (na - 1)) >> 1) + ca];
+  }
+  // Axis of the next step (0, 1, 2).
+  __device__ __forceinline__ int next_axis(const double* __restrict__ tbl) const {
+    const bool yx = dyx < 0 || (dyx == 0 && tie_le(tbl, n1, c1, n0, c0));
+    const int dzm = yx ? dzy : dzx;
+    const bool zm = dzm < 0 || (dzm == 0 && (yx ? tie_le(tbl, n2, c2, n1, c1) : tie_le(tbl, n2, c2, n0, c0)));
+    return zm ? 2 : (yx ? 1 : 0);
+  }
+  __device__ __forceinline__ void advance(int a) {
+    if (a == 0) {
+      ++c0;
+      dyx -= 2 * (int)n1;
+      dzx -= 2 * (int)n2;
+    } else if (a == 1) {
+      ++c1;
+      dyx += 2 * (int)n0;
+      dzy -= 2 * (int)n2;
+    } else {
+      ++c2;
+      dzx += 2 * (int)n0;
+      dzy += 2 * (int)n1;
     }
   }
-#pragma unroll
-  for (int a = 0; a < 3; ++a) q[a] = c[a] < n[a] ? fout[a] : DBL_MAX;
-}
+};
 
 // ---------------------------------------------------------------- kernels --
 
@@ -323,9 +356,31 @@ __global__ __launch_bounds__(1024) void k_scan_blocks(const uint32_t* __restrict
   if (threadIdx.x == 0) st->total = carry;
 }
 
+// Global visit offset of each ray and, for every chunk of `chunk` visits,
+// the ray holding the chunk's first visit.
+__global__ __launch_bounds__(256) void k_ray_offsets(const int4* __restrict__ rayinfo,
+                                                     const uint32_t* __restrict__ pre,
+                                                     const uint64_t* __restrict__ boff, uint32_t n,
+                                                     uint32_t chunk, uint64_t* __restrict__ roff,
+                                                     uint32_t* __restrict__ first_ray,
+                                                     uint64_t max_chunks, Status* st) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t cnt = (uint32_t)rayinfo[i].w & ~kHitBit;
+  const uint64_t off = boff[i / kPrepBlock] + pre[i] - cnt;
+  roff[i] = off;
+  if (cnt == 0) return;
+  const uint64_t c0 = (off + chunk - 1) / chunk, c1 = (off + cnt + chunk - 1) / chunk;
+  if (c1 > max_chunks) {
+    atomicOr(&st->error, kErrTable);
+    return;
+  }
+  for (uint64_t c = c0; c < c1; ++c) first_ray[c] = i;
+}
+
 __global__ __launch_bounds__(256) void k_traverse(const int4* __restrict__ rayinfo,
-                                                  const uint32_t* __restrict__ pre,
-                                                  const uint64_t* __restrict__ boff, uint32_t nb,
+                                                  const uint64_t* __restrict__ roff,
+                                                  const uint32_t* __restrict__ first_ray,
                                                   uint32_t n, const double* __restrict__ tbl,
                                                   UpdView u, Status* st, uint32_t chunk) {
   if (__hip_atomic_load(&st->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
@@ -348,30 +403,12 @@ __global__ __launch_bounds__(256) void k_traverse(const int4* __restrict__ rayin
   }
 
   for (uint64_t c = gid; c * chunk < S; c += G) {
+    // a full update set aborts the scan early: the host grows it and re-runs
+    if (__hip_atomic_load(ovf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
     uint64_t g = c * chunk;
     const uint64_t gend = min(g + chunk, S);
-    // ray holding visit g: last block with boff <= g, then first ray of the
-    // block whose inclusive prefix exceeds the block-local offset
-    uint32_t lo = 0, hi = nb - 1;
-    while (lo < hi) {
-      const uint32_t mid = (lo + hi + 1) >> 1;
-      if (boff[mid] <= g)
-        lo = mid;
-      else
-        hi = mid - 1;
-    }
-    const uint32_t local = (uint32_t)(g - boff[lo]);
-    uint32_t l2 = lo * kPrepBlock, h2 = min(l2 + kPrepBlock, n) - 1;
-    while (l2 < h2) {
-      const uint32_t mid = (l2 + h2) >> 1;
-      if (pre[mid] > local)
-        h2 = mid;
-      else
-        l2 = mid + 1;
-    }
-    uint32_t i = l2;
-    uint32_t k = local - (pre[i] - ((uint32_t)rayinfo[i].w & ~kHitBit));
-
+    uint32_t i = first_ray[c];
+    uint32_t k = (uint32_t)(g - roff[i]);
     Run run;
     run.init();
     while (g < gend && i < n) {
@@ -383,39 +420,26 @@ __global__ __launch_bounds__(256) void k_traverse(const int4* __restrict__ rayin
         continue;
       }
       const uint32_t m = (uint32_t)min((uint64_t)(cnt - k), gend - g);
-      uint32_t nn[3] = {(uint32_t)abs(r.x - sx), (uint32_t)abs(r.y - sy), (uint32_t)abs(r.z - sz)};
+      const uint32_t nn[3] = {(uint32_t)abs(r.x - sx), (uint32_t)abs(r.y - sy), (uint32_t)abs(r.z - sz)};
       const int stx = r.x > sx ? 1 : (r.x < sx ? -1 : 0);
       const int sty = r.y > sy ? 1 : (r.y < sy ? -1 : 0);
       const int stz = r.z > sz ? 1 : (r.z < sz ? -1 : 0);
-      const double dx = nn[0] ? 1.0 / (double)nn[0] : 0.0;
-      const double dy = nn[1] ? 1.0 / (double)nn[1] : 0.0;
-      const double dz = nn[2] ? 1.0 / (double)nn[2] : 0.0;
       uint32_t cc[3] = {0, 0, 0};
-      double q[3];
-      if (k == 0) {
-        q[0] = nn[0] ? 0.5 * dx : DBL_MAX;
-        q[1] = nn[1] ? 0.5 * dy : DBL_MAX;
-        q[2] = nn[2] ? 0.5 * dz : DBL_MAX;
-      } else {
-        dda_enter(tbl, nn, k, cc, q);
+      if (k > 0 && !dda_enter(tbl, nn, k, cc)) {
+        atomicOr(&st->error, kErrEntry);
+        return;
       }
+      IntDda d;
+      d.init(nn, cc);
       int vx = sx + stx * (int)cc[0], vy = sy + sty * (int)cc[1], vz = sz + stz * (int)cc[2];
-      double qx = q[0], qy = q[1], qz = q[2];
       for (uint32_t v = 0;;) {
         run.add(u, ovf, vx, vy, vz);
         if (++v == m) break;
-        const bool yx = qy <= qx;
-        const double mxy = yx ? qy : qx;
-        if (qz <= mxy) {
-          qz += dz;
-          vz += stz;
-        } else if (yx) {
-          qy += dy;
-          vy += sty;
-        } else {
-          qx += dx;
-          vx += stx;
-        }
+        const int a = d.next_axis(tbl);
+        d.advance(a);
+        vx += a == 0 ? stx : 0;
+        vy += a == 1 ? sty : 0;
+        vz += a == 2 ? stz : 0;
       }
       g += m;
       ++i;
@@ -425,11 +449,10 @@ __global__ __launch_bounds__(256) void k_traverse(const int4* __restrict__ rayin
   }
 }
 
-// List the occupied update-set slots; count their visited voxels.
+// List the occupied update-set slots (keys only: coalesced 8-byte reads).
 __global__ __launch_bounds__(256) void k_compact(UpdView u, uint32_t cap,
                                                  uint32_t* __restrict__ list, Status* st) {
   const uint32_t G = gridDim.x * blockDim.x;
-  unsigned long long vox = 0;
   for (uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x; slot < cap; slot += G) {
     const bool valid = u.keys[slot] != kEmpty;
     const uint64_t m = __ballot(valid);
@@ -438,17 +461,9 @@ __global__ __launch_bounds__(256) void k_compact(UpdView u, uint32_t cap,
       uint32_t base = 0;
       if (lane_id() == leader) base = atomicAdd(&st->entries, (unsigned int)__popcll(m));
       base = __shfl(base, leader);
-      if (valid) {
-        list[base + __popcll(m & ((1ull << lane_id()) - 1))] = slot;
-        const unsigned long long* w = &u.masks[(size_t)slot * 16];
-#pragma unroll
-        for (int j = 0; j < 8; ++j) vox += __popcll(w[j]);
-      }
+      if (valid) list[base + __popcll(m & ((1ull << lane_id()) - 1))] = slot;
     }
   }
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) vox += __shfl_xor(vox, o, 64);
-  if (lane_id() == 0 && vox) atomicAdd(&st->voxels, vox);
 }
 
 // One lane per update leaf: find its map leaf, or create it when the update
@@ -520,42 +535,57 @@ __global__ __launch_bounds__(256) void k_map_insert(UpdView u, const uint32_t* _
 // updateOccupiedNode / updateFreeNode OccupancyVDBMapping.hpp:92-117):
 // hit -> x += l_hit, above t_max: active, clamp at c_max;
 // miss -> x += l_miss, below t_min: inactive, clamp at c_min.
-// Clears the update-set slot afterwards.
+// Lane l owns bit l of the 8 mask words, i.e. voxels l, 64+l, ..., 448+l:
+// all loads of a leaf are issued before the first update. Clears the
+// update-set slot afterwards and counts the updated voxels.
 __global__ __launch_bounds__(256) void k_apply(UpdView u, const uint32_t* __restrict__ list,
                                                const uint32_t* __restrict__ eidx, uint32_t count,
                                                MapView mp, float l_hit, float l_miss, float t_min,
-                                               float t_max, float c_max, float c_min) {
+                                               float t_max, float c_max, float c_min, Status* st) {
   const uint32_t waves = gridDim.x * (blockDim.x >> 6);
   const int lane = lane_id();
+  unsigned long long vox = 0;
   for (uint32_t e = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); e < count; e += waves) {
     const uint32_t slot = list[e];
     const uint32_t idx = eidx[e];
     unsigned long long* w = &u.masks[(size_t)slot * 16];
+    uint64_t vis[8], hit[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      vis[j] = w[j];
+      hit[j] = w[8 + j];
+      vox += __popcll(vis[j]);
+    }
     if (idx != kNone) {
       float* vals = mp.values + (size_t)idx * 512;
       unsigned long long* act = mp.active + (size_t)idx * 8;
-#pragma unroll 1
+      float x[8];
+      uint64_t aw[8];
+#pragma unroll
       for (int j = 0; j < 8; ++j) {
-        const uint64_t vis = w[j];
-        if (vis == 0) continue;
-        const uint64_t hit = w[8 + j];
-        bool a = (act[j] >> lane) & 1;
-        if ((vis >> lane) & 1) {
-          float x = vals[j * 64 + lane];
-          if ((hit >> lane) & 1) {
-            x += l_hit;
-            if (x > t_max) {
+        aw[j] = act[j];
+        x[j] = ((vis[j] >> lane) & 1) ? vals[j * 64 + lane] : 0.0f;
+      }
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        if (vis[j] == 0) continue;
+        bool a = (aw[j] >> lane) & 1;
+        if ((vis[j] >> lane) & 1) {
+          float v = x[j];
+          if ((hit[j] >> lane) & 1) {
+            v += l_hit;
+            if (v > t_max) {
               a = true;
-              if (x > c_max) x = c_max;
+              if (v > c_max) v = c_max;
             }
           } else {
-            x += l_miss;
-            if (x < t_min) {
+            v += l_miss;
+            if (v < t_min) {
               a = false;
-              if (x < c_min) x = c_min;
+              if (v < c_min) v = c_min;
             }
           }
-          vals[j * 64 + lane] = x;
+          vals[j * 64 + lane] = v;
         }
         const uint64_t na = __ballot(a);
         if (lane == 0) act[j] = na;
@@ -564,6 +594,7 @@ __global__ __launch_bounds__(256) void k_apply(UpdView u, const uint32_t* __rest
     if (lane < 16) w[lane] = 0;
     if (lane == 0) u.keys[slot] = kEmpty;
   }
+  if (lane == 0 && vox) atomicAdd(&st->voxels, vox);
 }
 
 // Move every entry of an update set into a larger one.
@@ -668,10 +699,18 @@ hipError_t launch_scan_blocks(hipStream_t s, const uint32_t* bsum, uint64_t* bof
   return hipGetLastError();
 }
 
+hipError_t launch_ray_offsets(hipStream_t s, const int4* rayinfo, const uint32_t* pre,
+                              const uint64_t* boff, uint32_t n, uint32_t chunk, uint64_t* roff,
+                              uint32_t* first_ray, uint64_t max_chunks, Status* st) {
+  hipLaunchKernelGGL(k_ray_offsets, dim3((n + 255) / 256), dim3(256), 0, s, rayinfo, pre, boff, n,
+                     chunk, roff, first_ray, max_chunks, st);
+  return hipGetLastError();
+}
+
 hipError_t launch_traverse(hipStream_t s, int grid, int block, const int4* rayinfo,
-                           const uint32_t* pre, const uint64_t* boff, uint32_t nb, uint32_t n,
+                           const uint64_t* roff, const uint32_t* first_ray, uint32_t n,
                            const double* tbl, UpdView u, Status* st, uint32_t chunk) {
-  hipLaunchKernelGGL(k_traverse, dim3(grid), dim3(block), 0, s, rayinfo, pre, boff, nb, n, tbl, u,
+  hipLaunchKernelGGL(k_traverse, dim3(grid), dim3(block), 0, s, rayinfo, roff, first_ray, n, tbl, u,
                      st, chunk);
   return hipGetLastError();
 }
@@ -691,9 +730,10 @@ hipError_t launch_map_insert(hipStream_t s, int grid, UpdView u, const uint32_t*
 }
 
 hipError_t launch_apply(hipStream_t s, int grid, UpdView u, const uint32_t* list,
-                        const uint32_t* eidx, uint32_t count, MapView mp, const float lo[6]) {
+                        const uint32_t* eidx, uint32_t count, MapView mp, const float lo[6],
+                        Status* st) {
   hipLaunchKernelGGL(k_apply, dim3(grid), dim3(256), 0, s, u, list, eidx, count, mp, lo[0], lo[1],
-                     lo[2], lo[3], lo[4], lo[5]);
+                     lo[2], lo[3], lo[4], lo[5], st);
   return hipGetLastError();
 }
 

@@ -13,8 +13,11 @@ hipError_t launch_prep(hipStream_t s, const float* pts, int64_t stride_bytes, ui
                        int4* rayinfo, uint32_t* pre, uint32_t* bsum, Status* st);
 hipError_t launch_scan_blocks(hipStream_t s, const uint32_t* bsum, uint64_t* boff, uint32_t nb,
                               Status* st);
+hipError_t launch_ray_offsets(hipStream_t s, const int4* rayinfo, const uint32_t* pre,
+                              const uint64_t* boff, uint32_t n, uint32_t chunk, uint64_t* roff,
+                              uint32_t* first_ray, uint64_t max_chunks, Status* st);
 hipError_t launch_traverse(hipStream_t s, int grid, int block, const int4* rayinfo,
-                           const uint32_t* pre, const uint64_t* boff, uint32_t nb, uint32_t n,
+                           const uint64_t* roff, const uint32_t* first_ray, uint32_t n,
                            const double* tbl, UpdView u, Status* st, uint32_t chunk);
 hipError_t launch_compact(hipStream_t s, int grid, UpdView u, uint32_t cap, uint32_t* list,
                           Status* st);
@@ -22,7 +25,8 @@ hipError_t launch_map_insert(hipStream_t s, int grid, UpdView u, const uint32_t*
                              uint32_t count, MapView mp, uint32_t* eidx, bool create_hit,
                              bool create_miss, Status* st);
 hipError_t launch_apply(hipStream_t s, int grid, UpdView u, const uint32_t* list,
-                        const uint32_t* eidx, uint32_t count, MapView mp, const float lo[6]);
+                        const uint32_t* eidx, uint32_t count, MapView mp, const float lo[6],
+                        Status* st);
 hipError_t launch_rehash_update(hipStream_t s, int grid, UpdView from, uint32_t from_cap,
                                 UpdView to, Status* st);
 hipError_t launch_rehash_map(hipStream_t s, int grid, MapView mp, uint32_t leaf_count);

@@ -30,6 +30,7 @@ namespace {
 constexpr uint32_t kMaxTableN = 8192;       // longest per-axis ray extent in voxels
 constexpr uint32_t kMinUpdateCap = 1u << 12;
 constexpr uint32_t kMaxRays = 1u << 30;
+constexpr int kTraverseMaxBlock = 256;  // __launch_bounds__ of k_traverse
 
 enum TimedKind { kPrep = 0, kTraverse = 1, kCompact = 2, kApply = 3 };
 
@@ -46,7 +47,6 @@ struct Source {
   uint32_t* list = nullptr;
   uint32_t cap = 0;
   uint32_t pending = 0;         // listed update leaves not yet integrated
-  uint64_t pending_voxels = 0;
 };
 
 uint64_t next_pow2(uint64_t v) {
@@ -91,16 +91,19 @@ struct vdbgpu_map {
   uint32_t* bsum = nullptr;
   uint64_t* boff = nullptr;
   uint32_t ray_cap = 0;
+  uint64_t* roff = nullptr;      // global visit offset of each ray
+  uint32_t* first_ray = nullptr; // ray holding the first visit of each chunk
+  uint64_t chunk_cap = 0;
   uint32_t* eidx = nullptr;
   uint32_t eidx_cap = 0;
   double* tbl = nullptr;
   uint32_t tbl_n = 0;
 
-  // status: [0] scan, [1] map (leaf_count), [2] auxiliary
+  // status: [0] scan, [1] map (leaf_count), [2] auxiliary, [3] cumulative counters
   Status* d_st = nullptr;
   Status* h_st = nullptr;
 
-  vdbgpu_tuning tuning{256, 128, 8, 0.5};
+  vdbgpu_tuning tuning{256, 128, 16, 0.5};
   bool profiling = false;
   std::vector<TimedPair> timed;
   std::vector<hipEvent_t> event_pool;
@@ -187,7 +190,7 @@ void collect_timing(vdbgpu_map* m) {
 }
 
 int read_status(vdbgpu_map* m) {
-  CK(hipMemcpyAsync(m->h_st, m->d_st, 3 * sizeof(Status), hipMemcpyDeviceToHost, m->stream));
+  CK(hipMemcpyAsync(m->h_st, m->d_st, 4 * sizeof(Status), hipMemcpyDeviceToHost, m->stream));
   CK(hipStreamSynchronize(m->stream));
   collect_timing(m);
   m->leaf_count_ub = m->h_st[1].leaf_count;
@@ -245,14 +248,24 @@ int ensure_rays(vdbgpu_map* m, uint32_t n) {
   const uint32_t nb_old = (m->ray_cap + kPrepBlock - 1) / kPrepBlock;
   dfree(m, &m->rayinfo, m->ray_cap);
   dfree(m, &m->pre, m->ray_cap);
+  dfree(m, &m->roff, m->ray_cap);
   dfree(m, &m->bsum, nb_old);
   dfree(m, &m->boff, nb_old);
   const uint32_t nb = (cap + kPrepBlock - 1) / kPrepBlock;
   m->ray_cap = cap;
   CK(dalloc(m, &m->rayinfo, cap));
   CK(dalloc(m, &m->pre, cap));
+  CK(dalloc(m, &m->roff, cap));
   CK(dalloc(m, &m->bsum, nb));
   CK(dalloc(m, &m->boff, nb));
+  return VDBGPU_OK;
+}
+
+int ensure_chunks(vdbgpu_map* m, uint64_t need) {
+  if (need <= m->chunk_cap) return VDBGPU_OK;
+  dfree(m, &m->first_ray, m->chunk_cap);
+  m->chunk_cap = std::max<uint64_t>(need, m->chunk_cap + m->chunk_cap / 2);
+  CK(dalloc(m, &m->first_ray, m->chunk_cap));
   return VDBGPU_OK;
 }
 
@@ -335,7 +348,14 @@ int do_accumulate(vdbgpu_map* m, Source& src, const float* xyz, int64_t n, int64
   int r;
   if ((r = ensure_rays(m, nn))) return r;
   const double ext = std::ceil(R * m->inv) + 2.0;
-  if ((r = ensure_table(m, ext < (double)kMaxTableN ? (uint32_t)ext : kMaxTableN))) return r;
+  const uint32_t ext_n = ext < (double)kMaxTableN ? (uint32_t)ext : kMaxTableN;
+  if ((r = ensure_table(m, ext_n))) return r;
+  const uint32_t chunk = (uint32_t)m->tuning.steps_per_lane;
+  // visits per ray <= 1 + 3 * (per-axis extent)
+  const uint64_t max_chunks = ((uint64_t)nn * (1 + 3ull * ext_n) + chunk - 1) / chunk + 1;
+  if (max_chunks > (1ull << 29))
+    return fail(m, VDBGPU_BAD_ARGUMENT, "steps_per_lane too small for this scan (chunk table > 2 GiB)");
+  if ((r = ensure_chunks(m, max_chunks))) return r;
   if (src.cap == 0) {
     if ((r = alloc_update(m, src, (uint32_t)std::max<uint64_t>(kMinUpdateCap, next_pow2(4 * (uint64_t)nn))))) return r;
   }
@@ -358,13 +378,15 @@ int do_accumulate(vdbgpu_map* m, Source& src, const float* xyz, int64_t n, int64
     CK(launch_prep(m->stream, pts, stride, nn, origin, R, m->res, m->inv, m->tbl_n, m->rayinfo,
                    m->pre, m->bsum, &m->d_st[0]));
     CK(launch_scan_blocks(m->stream, m->bsum, m->boff, nb, &m->d_st[0]));
+    CK(launch_ray_offsets(m->stream, m->rayinfo, m->pre, m->boff, nn, chunk, m->roff, m->first_ray,
+                          m->chunk_cap, &m->d_st[0]));
   }
   const int block = m->tuning.block;
   for (int attempt = 0;; ++attempt) {
     {
       Timer t(m, kTraverse);
-      CK(launch_traverse(m->stream, grid_lanes(m, block), block, m->rayinfo, m->pre, m->boff, nb, nn,
-                         m->tbl, upd_view(src), &m->d_st[0], (uint32_t)m->tuning.steps_per_lane));
+      CK(launch_traverse(m->stream, grid_lanes(m, block), block, m->rayinfo, m->roff, m->first_ray, nn,
+                         m->tbl, upd_view(src), &m->d_st[0], chunk));
     }
     {
       Timer t(m, kCompact);
@@ -377,6 +399,8 @@ int do_accumulate(vdbgpu_map* m, Source& src, const float* xyz, int64_t n, int64
       return fail(m, VDBGPU_OUT_OF_RANGE, "a voxel index is outside [-2^23, 2^23) or not finite; scan rejected");
     if (st.error & kErrTable)
       return fail(m, VDBGPU_UNSUPPORTED, "a ray spans more than 8192 voxels along one axis");
+    if (st.error & kErrEntry)
+      return fail(m, VDBGPU_DEVICE_ERROR, "mid-ray DDA entry did not converge");
     if (st.overflow & kOvfHard) {
       if (attempt > 6 || src.cap >= (1u << 30)) return fail(m, VDBGPU_DEVICE_ERROR, "update set cannot grow");
       if ((r = grow_update(m, src, src.cap * 4))) return r;
@@ -389,7 +413,6 @@ int do_accumulate(vdbgpu_map* m, Source& src, const float* xyz, int64_t n, int64
   }
   const Status& st = m->h_st[0];
   src.pending = st.entries;
-  src.pending_voxels = st.voxels;
   m->stats.scans++;
   m->stats.rays += st.rays;
   m->stats.visits += st.total;
@@ -440,14 +463,13 @@ int do_integrate(vdbgpu_map* m) {
       Timer t(m, kApply);
       CK(launch_map_insert(m->stream, g1, upd_view(s), s.list, s.pending, map_view(m), m->eidx, ch, cm,
                            &m->d_st[1]));
-      CK(launch_apply(m->stream, g2, upd_view(s), s.list, m->eidx, s.pending, map_view(m), m->lo));
+      CK(launch_apply(m->stream, g2, upd_view(s), s.list, m->eidx, s.pending, map_view(m), m->lo,
+                      &m->d_st[3]));
     }
     m->leaf_count_ub += s.pending;
-    m->stats.voxel_updates += s.pending_voxels;
     m->stats.update_leaves += s.pending;
     m->touched = true;
     s.pending = 0;
-    s.pending_voxels = 0;
   }
   return VDBGPU_OK;
 }
@@ -527,13 +549,13 @@ int32_t vdbgpu_create(double resolution, vdbgpu_map** out) {
   if (hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, m->device) == hipSuccess && cu > 0)
     m->num_cu = cu;
   if (hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) != hipSuccess ||
-      hipMalloc((void**)&m->d_st, 3 * sizeof(Status)) != hipSuccess ||
-      hipHostMalloc((void**)&m->h_st, 3 * sizeof(Status), hipHostMallocDefault) != hipSuccess ||
-      hipMemsetAsync(m->d_st, 0, 3 * sizeof(Status), m->stream) != hipSuccess) {
+      hipMalloc((void**)&m->d_st, 4 * sizeof(Status)) != hipSuccess ||
+      hipHostMalloc((void**)&m->h_st, 4 * sizeof(Status), hipHostMallocDefault) != hipSuccess ||
+      hipMemsetAsync(m->d_st, 0, 4 * sizeof(Status), m->stream) != hipSuccess) {
     vdbgpu_destroy(m);
     return VDBGPU_DEVICE_ERROR;
   }
-  memset(m->h_st, 0, 3 * sizeof(Status));
+  memset(m->h_st, 0, 4 * sizeof(Status));
   *out = m;
   return VDBGPU_OK;
 }
@@ -556,6 +578,8 @@ void vdbgpu_destroy(vdbgpu_map* m) {
   dfree(m, &m->bsum, nb);
   dfree(m, &m->boff, nb);
   dfree(m, &m->eidx, m->eidx_cap);
+  dfree(m, &m->roff, m->ray_cap);
+  dfree(m, &m->first_ray, m->chunk_cap);
   dfree(m, &m->tbl, (uint64_t)m->tbl_n * (m->tbl_n + 1) / 2);
   if (m->d_st) (void)hipFree(m->d_st);
   if (m->h_st) (void)hipHostFree(m->h_st);
@@ -608,7 +632,6 @@ int32_t vdbgpu_add_source(vdbgpu_map* m, const char* id, double max_range, doubl
   if (s.cap) {  // re-adding a source starts it with an empty update grid
     free_update(m, s);
     s.pending = 0;
-    s.pending_voxels = 0;
   }
   return VDBGPU_OK;
 }
@@ -660,7 +683,6 @@ int32_t vdbgpu_reset(vdbgpu_map* m) {
       CK(hipMemsetAsync(s.masks, 0, (size_t)s.cap * 16 * sizeof(unsigned long long), m->stream));
     }
     s.pending = 0;
-    s.pending_voxels = 0;
   }
   m->leaf_count_ub = 0;
   m->touched = false;
@@ -795,6 +817,29 @@ int32_t vdbgpu_raytrace(vdbgpu_map* m, int64_t n, const double* o, const double*
   return fail(m, VDBGPU_NO_INTERSECTOR, "raytrace needs an integrate in fast mode first");
 }
 
+int32_t vdbgpu_device_alloc(int64_t bytes, void** out) {
+  if (!out || bytes <= 0) return VDBGPU_BAD_ARGUMENT;
+  *out = nullptr;
+  return hipMalloc(out, (size_t)bytes) == hipSuccess ? VDBGPU_OK : VDBGPU_DEVICE_ERROR;
+}
+
+int32_t vdbgpu_device_free(void* ptr) {
+  if (!ptr) return VDBGPU_OK;
+  return hipFree(ptr) == hipSuccess ? VDBGPU_OK : VDBGPU_DEVICE_ERROR;
+}
+
+int32_t vdbgpu_device_upload(void* dst, const void* src, int64_t bytes) {
+  if (!dst || !src || bytes < 0) return VDBGPU_BAD_ARGUMENT;
+  return hipMemcpy(dst, src, (size_t)bytes, hipMemcpyHostToDevice) == hipSuccess ? VDBGPU_OK
+                                                                                 : VDBGPU_DEVICE_ERROR;
+}
+
+int32_t vdbgpu_device_download(void* dst, const void* src, int64_t bytes) {
+  if (!dst || !src || bytes < 0) return VDBGPU_BAD_ARGUMENT;
+  return hipMemcpy(dst, src, (size_t)bytes, hipMemcpyDeviceToHost) == hipSuccess ? VDBGPU_OK
+                                                                                 : VDBGPU_DEVICE_ERROR;
+}
+
 int32_t vdbgpu_synchronize(vdbgpu_map* m) {
   if (!m) return VDBGPU_BAD_ARGUMENT;
   Guard g(m);
@@ -807,6 +852,7 @@ int32_t vdbgpu_get_stats(vdbgpu_map* m, vdbgpu_stats* out) {
   int r = read_status(m);
   if (r) return r;
   m->stats.leaf_count = m->h_st[1].leaf_count;
+  m->stats.voxel_updates = m->h_st[3].voxels;
   m->stats.device_bytes = m->device_bytes;
   *out = m->stats;
   return VDBGPU_OK;
@@ -821,7 +867,7 @@ int32_t vdbgpu_set_profiling(vdbgpu_map* m, int32_t on) {
 
 int32_t vdbgpu_set_tuning(vdbgpu_map* m, const vdbgpu_tuning* t) {
   if (!m || !t) return VDBGPU_BAD_ARGUMENT;
-  if (t->block < 64 || t->block > 1024 || (t->block & 63) || t->steps_per_lane < 1 ||
+  if (t->block < 64 || t->block > kTraverseMaxBlock || (t->block & 63) || t->steps_per_lane < 1 ||
       t->steps_per_lane > 1 << 20 || t->waves_per_cu < 1 || t->waves_per_cu > 64 ||
       !(t->load_factor >= 0.2 && t->load_factor <= 0.95))
     return VDBGPU_BAD_ARGUMENT;
