@@ -57,8 +57,8 @@ typedef struct vdbgpu_stats {
   uint64_t update_leaves;  /* update-set leaves applied */
   uint64_t leaf_count;     /* leaves in the map now */
   uint64_t device_bytes;   /* device memory held by the handle */
-  double ms_prep, ms_traverse, ms_compact, ms_apply; /* summed kernel times (profiling on) */
-  uint64_t n_prep, n_traverse, n_compact, n_apply;   /* launches timed */
+  double ms_prep, ms_traverse, ms_compact, ms_apply, ms_raytrace; /* summed kernel times */
+  uint64_t n_prep, n_traverse, n_compact, n_apply, n_raytrace;      /* launches timed */
 } vdbgpu_stats;
 
 /* Launch parameters of the integration kernels (see vdbgpu_set_tuning). */
@@ -96,6 +96,11 @@ int32_t vdbgpu_logodds(vdbgpu_map* map, float out[6]);
  * max_rate only matters to the reference's background threads. */
 int32_t vdbgpu_add_source(vdbgpu_map* map, const char* source_id, double max_range,
                           double max_rate);
+
+/* Remove a source and its pending update set. No reference counterpart (the
+ * reference never removes sources); used by the drop-in raycastPointCloud to
+ * cast into a private scratch source. */
+int32_t vdbgpu_remove_source(vdbgpu_map* map, const char* source_id);
 
 /* accumulateUpdate VM:316-346 -> raycastPointCloud VM:466-539: cast every
  * point of the cloud from `origin` into the source's update set. `xyz` holds
@@ -154,6 +159,11 @@ int32_t vdbgpu_device_alloc(int64_t bytes, void** out);
 int32_t vdbgpu_device_free(void* ptr);
 int32_t vdbgpu_device_upload(void* dst, const void* src, int64_t bytes);
 int32_t vdbgpu_device_download(void* dst, const void* src, int64_t bytes);
+
+/* raytrace with every array in device memory (same layout). */
+int32_t vdbgpu_raytrace_device(vdbgpu_map* map, int64_t n, const double* origins,
+                               const double* directions, const double* max_lengths,
+                               uint8_t* success, double* end_points);
 
 /* Wait for all work of the handle. */
 int32_t vdbgpu_synchronize(vdbgpu_map* map);

@@ -1,0 +1,188 @@
+// The reference's unit tests (tests/mapping.cpp:6-158) re-expressed against
+// the drop-in headers (include/vdb_mapping/), plus checks of the live grid,
+// raycastPointCloud and raytrace. Plain asserts: googletest is not required.
+// Needs a gfx950 GPU; prints one line per test and exits non-zero on failure.
+#include <cmath>
+#include <cstdio>
+#include <vdb_mapping/OccupancyVDBMapping.hpp>
+
+using namespace vdb_mapping;
+
+static int g_failures = 0;
+#define CHECK(cond)                                                         \
+  do {                                                                      \
+    if (!(cond)) {                                                          \
+      std::printf("  FAILED %s:%d: %s\n", __FILE__, __LINE__, #cond);       \
+      ++g_failures;                                                         \
+    }                                                                       \
+  } while (0)
+
+static Config make_config(double max_range)
+{
+  Config conf;
+  conf.max_range           = max_range;
+  conf.fast_mode           = false;
+  conf.accumulation_period = 0;
+  conf.prob_hit            = 0.9;
+  conf.prob_miss           = 0.1;
+  conf.prob_thres_max      = 0.51;
+  conf.prob_thres_min      = 0.49;
+  return conf;
+}
+
+static float logodds(double p) { return static_cast<float>(std::log(p) - std::log(1 - p)); }
+
+static void set_config_test()
+{
+  OccupancyVDBMapping map(1);
+  OccupancyVDBMapping::PointCloudT::Ptr cloud(new OccupancyVDBMapping::PointCloudT);
+  cloud->points.emplace_back(0, 0, 1);
+  Eigen::Matrix<double, 3, 1> origin(0, 0, 0);
+  map.insertPointCloud(cloud, origin, "test");  // before setConfig / addInputSource: no-op
+  OccupancyVDBMapping::GridT::Accessor acc = map.getGrid()->getAccessor();
+  CHECK(acc.getValue(openvdb::Coord(0, 0, 1)) == 0.0);
+  Config conf = make_config(10);
+  map.setConfig(conf);
+  map.addInputSource("test", conf.max_range, 0);
+  map.insertPointCloud(cloud, origin, "test");
+  // the accessor taken before the insert sees the new values (live grid)
+  CHECK(acc.getValue(openvdb::Coord(0, 0, 0)) == logodds(conf.prob_miss));
+  CHECK(acc.getValue(openvdb::Coord(0, 0, 1)) == logodds(conf.prob_hit));
+}
+
+static void insert_point_test(int sign)
+{
+  const double resolution = 0.1;
+  OccupancyVDBMapping map(resolution);
+  Config conf = make_config(10);
+  map.setConfig(conf);
+  map.addInputSource("test", conf.max_range, 0);
+  OccupancyVDBMapping::PointCloudT::Ptr cloud(new OccupancyVDBMapping::PointCloudT);
+  cloud->points.emplace_back(0, 0, sign * 5 * resolution);
+  Eigen::Matrix<double, 3, 1> origin(0, 0, 0);
+  map.insertPointCloud(cloud, origin, "test");
+  OccupancyVDBMapping::GridT::Accessor acc = map.getGrid()->getAccessor();
+  for (int i = 0; i < 5; ++i)
+  {
+    openvdb::Coord coord(0, 0, sign * i);
+    CHECK(acc.getValue(coord) == logodds(conf.prob_miss));
+    CHECK(!acc.isValueOn(coord));
+  }
+  openvdb::Coord coord(0, 0, sign * 5);
+  CHECK(acc.getValue(coord) == logodds(conf.prob_hit));
+  CHECK(acc.isValueOn(coord));
+}
+
+static void insert_max_range_test()
+{
+  const double resolution = 0.1;
+  OccupancyVDBMapping map(resolution);
+  Config conf = make_config(0.5);
+  map.setConfig(conf);
+  map.addInputSource("test", conf.max_range, 0);
+  OccupancyVDBMapping::PointCloudT::Ptr cloud(new OccupancyVDBMapping::PointCloudT);
+  cloud->points.emplace_back(0, 0, 7 * resolution);
+  Eigen::Matrix<double, 3, 1> origin(0, 0, 0);
+  map.insertPointCloud(cloud, origin, "test");
+  OccupancyVDBMapping::GridT::Accessor acc = map.getGrid()->getAccessor();
+  for (int i = 0; i <= (int)(conf.max_range / resolution); ++i)
+  {
+    openvdb::Coord coord(0, 0, i);
+    CHECK(acc.getValue(coord) == logodds(conf.prob_miss));
+    CHECK(!acc.isValueOn(coord));
+  }
+  openvdb::Coord coord(0, 0, (int)(conf.max_range / resolution) + 1);
+  CHECK(acc.getValue(coord) == 0);
+  CHECK(!acc.isValueOn(coord));
+}
+
+static void reset_map_test()
+{
+  OccupancyVDBMapping map(1);
+  Config conf = make_config(10);
+  map.setConfig(conf);
+  map.addInputSource("test", conf.max_range, 0);
+  OccupancyVDBMapping::PointCloudT::Ptr cloud(new OccupancyVDBMapping::PointCloudT);
+  cloud->points.emplace_back(0, 0, 1);
+  Eigen::Matrix<double, 3, 1> origin(0, 0, 0);
+  map.insertPointCloud(cloud, origin, "test");
+  OccupancyVDBMapping::GridT::Accessor acc = map.getGrid()->getAccessor();
+  CHECK(acc.getValue(openvdb::Coord(0, 0, 1)) == logodds(conf.prob_hit));
+  map.resetMap();
+  acc = map.getGrid()->getAccessor();
+  CHECK(acc.getValue(openvdb::Coord(0, 0, 1)) == 0.0);
+  CHECK(map.getGrid()->empty());
+}
+
+static void raycast_point_cloud_test()
+{
+  OccupancyVDBMapping map(1);
+  Config conf = make_config(10);
+  map.setConfig(conf);
+  map.addInputSource("test", conf.max_range, 0);
+  OccupancyVDBMapping::PointCloudT::Ptr cloud(new OccupancyVDBMapping::PointCloudT);
+  cloud->points.emplace_back(0, 0, 3);
+  cloud->points.emplace_back(0, 0, 20);  // beyond the 5 m cast range: misses only
+  Eigen::Matrix<double, 3, 1> origin(0, 0, 0);
+  auto update = OccupancyVDBMapping::UpdateGridT::create(false);
+  auto uacc   = update->getAccessor();
+  CHECK(map.raycastPointCloud(cloud, origin, 5.0, uacc));
+  CHECK(uacc.isValueOn(openvdb::Coord(0, 0, 0)) && !uacc.getValue(openvdb::Coord(0, 0, 0)));
+  CHECK(uacc.isValueOn(openvdb::Coord(0, 0, 3)) && uacc.getValue(openvdb::Coord(0, 0, 3)));
+  CHECK(uacc.isValueOn(openvdb::Coord(0, 0, 5)) && !uacc.getValue(openvdb::Coord(0, 0, 5)));
+  CHECK(!uacc.isValueOn(openvdb::Coord(0, 0, 6)));
+  CHECK(update->activeVoxelCount() == 6);
+  CHECK(map.getGrid()->empty());  // the map itself is untouched
+  map.insertPointCloud(cloud, origin, "test");
+  CHECK(map.getGrid()->getAccessor().isValueOn(openvdb::Coord(0, 0, 3)));
+}
+
+static void raytrace_test()
+{
+  OccupancyVDBMapping map(1);
+  Config conf = make_config(50);
+  map.setConfig(conf);
+  map.addInputSource("test", conf.max_range, 0);
+  OccupancyVDBMapping::PointCloudT::Ptr cloud(new OccupancyVDBMapping::PointCloudT);
+  for (int y = -4; y <= 4; ++y)
+    for (int z = -4; z <= 4; ++z) cloud->points.emplace_back(10, y, z);
+  Eigen::Matrix<double, 3, 1> origin(0, 0, 0);
+  map.insertPointCloud(cloud, origin, "test");
+  conf.fast_mode = true;
+  map.setConfig(conf);
+  map.integrateUpdate();  // builds the raytrace snapshot (updateVolumeRayIntersectors)
+  bool success = false;
+  openvdb::Vec3d end;
+  map.raytrace(openvdb::Vec3d(0.5, 0.5, 0.5), openvdb::Vec3d(1, 0, 0), 20.0, success, end);
+  CHECK(success);
+  CHECK(end[0] == 10.0 && end[1] == 0.0 && end[2] == 0.0);
+  map.raytrace(openvdb::Vec3d(0.5, 30.5, 0.5), openvdb::Vec3d(0, 1, 0), 4.0, success, end);
+  CHECK(!success);
+  CHECK(end[0] == 0.5 && end[1] == 34.5 && end[2] == 0.5);
+}
+
+int main()
+{
+  struct T
+  {
+    const char* name;
+    void (*fn)();
+  } tests[] = {
+    {"SetConfig", set_config_test},
+    {"InsertPositivePoint", [] { insert_point_test(1); }},
+    {"InsertNegativePoint", [] { insert_point_test(-1); }},
+    {"InsertMaxRangePoint", insert_max_range_test},
+    {"ResetMap", reset_map_test},
+    {"RaycastPointCloud", raycast_point_cloud_test},
+    {"Raytrace", raytrace_test},
+  };
+  for (const T& t : tests)
+  {
+    const int before = g_failures;
+    t.fn();
+    std::printf("[%s] %s\n", g_failures == before ? "  OK  " : "FAILED", t.name);
+    std::fflush(stdout);
+  }
+  std::printf("%d failure(s)\n", g_failures);
+  return g_failures == 0 ? 0 : 1;
+}

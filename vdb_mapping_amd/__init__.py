@@ -23,10 +23,12 @@ OUT_OF_RANGE, UNSUPPORTED, DEVICE_ERROR, BAD_ARGUMENT, NO_INTERSECTOR = 6, 7, 8,
 # Every symbol include/vdbgpu.h declares.
 ABI_SYMBOLS = [
     "vdbgpu_version", "vdbgpu_status_string", "vdbgpu_device_count", "vdbgpu_create",
+    "vdbgpu_remove_source",
     "vdbgpu_destroy", "vdbgpu_set_config", "vdbgpu_logodds", "vdbgpu_add_source",
     "vdbgpu_accumulate", "vdbgpu_integrate", "vdbgpu_insert", "vdbgpu_reset", "vdbgpu_empty",
     "vdbgpu_leaf_count", "vdbgpu_export_leaves", "vdbgpu_export_dirty_leaves",
-    "vdbgpu_update_leaf_count", "vdbgpu_export_update", "vdbgpu_raytrace", "vdbgpu_synchronize",
+    "vdbgpu_update_leaf_count", "vdbgpu_export_update", "vdbgpu_raytrace", "vdbgpu_raytrace_device",
+    "vdbgpu_synchronize",
     "vdbgpu_get_stats", "vdbgpu_set_profiling", "vdbgpu_set_tuning", "vdbgpu_get_tuning",
     "vdbgpu_last_error", "vdbgpu_device_alloc", "vdbgpu_device_free", "vdbgpu_device_upload",
     "vdbgpu_device_download",
@@ -58,8 +60,10 @@ class Stats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint64) for n in
                 ("scans", "rays", "visits", "voxel_updates", "update_leaves", "leaf_count",
                  "device_bytes")] + \
-               [(n, ctypes.c_double) for n in ("ms_prep", "ms_traverse", "ms_compact", "ms_apply")] + \
-               [(n, ctypes.c_uint64) for n in ("n_prep", "n_traverse", "n_compact", "n_apply")]
+               [(n, ctypes.c_double) for n in ("ms_prep", "ms_traverse", "ms_compact", "ms_apply",
+                                               "ms_raytrace")] + \
+               [(n, ctypes.c_uint64) for n in ("n_prep", "n_traverse", "n_compact", "n_apply",
+                                               "n_raytrace")]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
@@ -98,6 +102,8 @@ def lib():
         L.vdbgpu_logodds.argtypes = [P, P]
         L.vdbgpu_add_source.restype = I32
         L.vdbgpu_add_source.argtypes = [P, ctypes.c_char_p, D, D]
+        L.vdbgpu_remove_source.restype = I32
+        L.vdbgpu_remove_source.argtypes = [P, ctypes.c_char_p]
         for f in (L.vdbgpu_accumulate, L.vdbgpu_insert):
             f.restype = I32
             f.argtypes = [P, ctypes.c_char_p, P, I64, I64, P, I32]
@@ -113,8 +119,9 @@ def lib():
         L.vdbgpu_update_leaf_count.argtypes = [P, ctypes.c_char_p]
         L.vdbgpu_export_update.restype = I64
         L.vdbgpu_export_update.argtypes = [P, ctypes.c_char_p, P, P, P, I64]
-        L.vdbgpu_raytrace.restype = I32
-        L.vdbgpu_raytrace.argtypes = [P, I64, P, P, P, P, P]
+        for f in (L.vdbgpu_raytrace, L.vdbgpu_raytrace_device):
+            f.restype = I32
+            f.argtypes = [P, I64, P, P, P, P, P]
         L.vdbgpu_get_stats.restype = I32
         L.vdbgpu_get_stats.argtypes = [P, ctypes.POINTER(Stats)]
         L.vdbgpu_set_profiling.restype = I32
@@ -189,6 +196,9 @@ class GpuMap:
 
     def add_input_source(self, source_id: str, max_range: float, max_rate: float = 0.0) -> int:
         return lib().vdbgpu_add_source(self.h, source_id.encode(), max_range, max_rate)
+
+    def remove_input_source(self, source_id: str) -> int:
+        return lib().vdbgpu_remove_source(self.h, source_id.encode())
 
     @staticmethod
     def _pts(points):
@@ -286,6 +296,12 @@ class GpuMap:
         ends = np.zeros((n, 3), np.float64)
         st = lib().vdbgpu_raytrace(self.h, n, _ptr(o), _ptr(d), _ptr(ln), _ptr(succ), _ptr(ends))
         return st, succ.astype(bool), ends
+
+    def raytrace_device(self, n: int, d_origins: int, d_dirs: int, d_lens: int, d_success: int,
+                        d_ends: int) -> int:
+        return lib().vdbgpu_raytrace_device(self.h, n, ctypes.c_void_p(d_origins),
+                                            ctypes.c_void_p(d_dirs), ctypes.c_void_p(d_lens),
+                                            ctypes.c_void_p(d_success), ctypes.c_void_p(d_ends))
 
     def stats(self) -> dict:
         s = Stats()

@@ -25,6 +25,7 @@ constexpr uint32_t kNone = 0xFFFFFFFFu;
 constexpr int kCoordLimit = 1 << 23;  // voxel coords in [-2^23, 2^23): 21-bit leaf coords
 constexpr uint32_t kHitBit = 0x80000000u;
 constexpr int kPrepBlock = 256;
+constexpr int kMaxQuerySteps = 1 << 24;  // per-axis voxel extent bound of a raytrace query
 
 // Status bits written by kernels.
 constexpr uint32_t kErrRange = 1u;  // a voxel index out of range / non-finite
@@ -60,6 +61,16 @@ struct MapView {
   uint8_t* dirty;
 };
 
+// Topology snapshot used by raytrace: hash sets of leaf, 128^3-node and
+// 4096^3-node keys, plus the padded bbox of leaves with an active voxel.
+struct RtView {
+  unsigned long long* leaf;
+  unsigned long long* n128;
+  unsigned long long* n4096;
+  uint32_t leaf_mask, n128_mask, n4096_mask;
+  int bmin[3], bmax[3];
+};
+
 __host__ __device__ inline uint64_t leaf_key(int x, int y, int z) {
   const uint64_t m = (1ull << 21) - 1;
   return ((uint64_t)(uint32_t)(x >> 3) & m) << 42 | ((uint64_t)(uint32_t)(y >> 3) & m) << 21 |
@@ -86,5 +97,33 @@ __host__ __device__ inline uint64_t fmix64(uint64_t k) {
 }
 
 __device__ inline bool coord_ok(int c) { return c >= -kCoordLimit && c < kCoordLimit; }
+
+// Key of the internal node (2^log2dim voxels per side) holding voxel (x,y,z).
+__host__ __device__ inline uint64_t node_key(int x, int y, int z, int log2dim) {
+  return leaf_key((x >> log2dim) * 8, (y >> log2dim) * 8, (z >> log2dim) * 8);
+}
+
+__device__ __forceinline__ bool set_insert(unsigned long long* keys, uint32_t mask, uint64_t key) {
+  uint32_t h = (uint32_t)fmix64(key) & mask;
+  for (uint32_t p = 0; p <= mask; ++p) {
+    const unsigned long long prev = atomicCAS(&keys[h], kEmpty, (unsigned long long)key);
+    if (prev == kEmpty) return true;
+    if (prev == key) return false;
+    h = (h + 1) & mask;
+  }
+  return false;
+}
+
+__device__ __forceinline__ bool set_has(const unsigned long long* __restrict__ keys, uint32_t mask,
+                                        uint64_t key) {
+  uint32_t h = (uint32_t)fmix64(key) & mask;
+  for (uint32_t p = 0; p <= mask; ++p) {
+    const unsigned long long k = keys[h];
+    if (k == key) return true;
+    if (k == kEmpty) return false;
+    h = (h + 1) & mask;
+  }
+  return false;
+}
 
 }  // namespace vdbgpu

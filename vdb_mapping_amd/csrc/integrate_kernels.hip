@@ -68,20 +68,52 @@ __device__ __forceinline__ void upd_or(const UpdView& u, uint32_t slot, int word
   if ((cur & bits) != bits) atomicOr(p, (unsigned long long)bits);
 }
 
-// Per-lane accumulator of visited bits for one (leaf, mask word).
+// Workgroup-local aggregation table in LDS: leaf key -> 8 visited words.
+constexpr int kLdsEntries = 512;  // 36 KiB per workgroup
+constexpr int kLdsProbe = 16;
+
+__device__ __forceinline__ int lds_slot(unsigned long long* lkey, uint64_t key) {
+  uint32_t h = (uint32_t)(fmix64(key) >> 40) & (kLdsEntries - 1);
+  for (int p = 0; p < kLdsProbe; ++p) {
+    const unsigned long long k =
+        __hip_atomic_load(&lkey[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (k == key) return (int)h;
+    if (k == kEmpty) {
+      const unsigned long long prev = atomicCAS(&lkey[h], kEmpty, (unsigned long long)key);
+      if (prev == kEmpty || prev == key) return (int)h;
+    }
+    h = (h + 1) & (kLdsEntries - 1);
+  }
+  return -1;
+}
+
+// Per-lane accumulator of visited bits for one (leaf, mask word); a finished
+// run is OR-ed into the workgroup's LDS entry of the leaf, or straight into
+// the global update set when the LDS table is full.
 struct Run {
   int lx, ly, lz;
-  uint32_t slot;
+  int ls;        // LDS entry of the current leaf, -1 = global
+  uint32_t gs;   // global slot when ls < 0
   int word;
   uint64_t bits;
   __device__ void init() {
     lx = INT_MIN;
     ly = lz = 0;
-    slot = kNone;
+    ls = -1;
+    gs = kNone;
     word = 0;
     bits = 0;
   }
-  __device__ __forceinline__ void add(const UpdView& u, unsigned int* ovf, int x, int y, int z) {
+  __device__ __forceinline__ void flush(const UpdView& u, unsigned long long* lvis) {
+    if (bits == 0) return;
+    if (ls >= 0)
+      atomicOr(&lvis[ls * 8 + word], (unsigned long long)bits);
+    else
+      upd_or(u, gs, word, bits);
+    bits = 0;
+  }
+  __device__ __forceinline__ void add(const UpdView& u, unsigned int* ovf, unsigned long long* lkey,
+                                      unsigned long long* lvis, int x, int y, int z) {
     const int w = x & 7;
     const uint64_t b = 1ull << (((y & 7) << 3) | (z & 7));
     const int nx = x >> 3, ny = y >> 3, nz = z >> 3;
@@ -90,18 +122,19 @@ struct Run {
         bits |= b;
         return;
       }
-      upd_or(u, slot, word, bits);
+      flush(u, lvis);
     } else {
-      upd_or(u, slot, word, bits);
+      flush(u, lvis);
       lx = nx;
       ly = ny;
       lz = nz;
-      slot = upd_slot(u, leaf_key(x, y, z), ovf);
+      const uint64_t key = leaf_key(x, y, z);
+      ls = lds_slot(lkey, key);
+      if (ls < 0) gs = upd_slot(u, key, ovf);
     }
     word = w;
     bits = b;
   }
-  __device__ void flush(const UpdView& u) { upd_or(u, slot, word, bits); bits = 0; }
 };
 
 // Exact order of DDA crossing events without floating point.
@@ -383,12 +416,24 @@ __global__ __launch_bounds__(256) void k_traverse(const int4* __restrict__ rayin
                                                   const uint32_t* __restrict__ first_ray,
                                                   uint32_t n, const double* __restrict__ tbl,
                                                   UpdView u, Status* st, uint32_t chunk) {
-  if (__hip_atomic_load(&st->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
+  __shared__ unsigned long long lkey[kLdsEntries];
+  __shared__ unsigned long long lvis[kLdsEntries * 8];
+  __shared__ unsigned int s_stop;
+  // every exit before the last barrier must be taken by the whole workgroup
+  if (threadIdx.x == 0)
+    s_stop = __hip_atomic_load(&st->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __syncthreads();
+  if (s_stop) return;
   const uint64_t S = st->total;
   const int sx = st->s[0], sy = st->s[1], sz = st->s[2];
   const uint64_t G = (uint64_t)gridDim.x * blockDim.x;
   const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   unsigned int* ovf = &st->overflow;
+  for (int e = threadIdx.x; e < kLdsEntries; e += blockDim.x) {
+    lkey[e] = kEmpty;
+#pragma unroll
+    for (int w = 0; w < 8; ++w) lvis[e * 8 + w] = 0;
+  }
 
   // Endpoints of rays inside max range: setValueOn(end, true) (VDBMapping.hpp:533-536).
   for (uint64_t i = gid; i < n; i += G) {
@@ -401,51 +446,79 @@ __global__ __launch_bounds__(256) void k_traverse(const int4* __restrict__ rayin
       upd_or(u, slot, 8 + w, b);
     }
   }
+  __syncthreads();
 
-  for (uint64_t c = gid; c * chunk < S; c += G) {
-    // a full update set aborts the scan early: the host grows it and re-runs
-    if (__hip_atomic_load(ovf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
-    uint64_t g = c * chunk;
-    const uint64_t gend = min(g + chunk, S);
-    uint32_t i = first_ray[c];
-    uint32_t k = (uint32_t)(g - roff[i]);
-    Run run;
-    run.init();
-    while (g < gend && i < n) {
-      const int4 r = rayinfo[i];
-      const uint32_t cnt = (uint32_t)r.w & ~kHitBit;
-      if (k >= cnt) {
+  // Workgroup iterations: blockDim.x consecutive chunks of `chunk` visits.
+  for (uint64_t it = blockIdx.x;; it += gridDim.x) {
+    const uint64_t cfirst = it * blockDim.x;
+    if (cfirst * chunk >= S) break;  // same for every thread of the workgroup
+    if (threadIdx.x == 0)
+      s_stop = __hip_atomic_load(ovf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    if (s_stop) break;  // full update set: the host grows it and re-runs
+
+    const uint64_t c = cfirst + threadIdx.x;
+    if (c * chunk < S) {
+      uint64_t g = c * chunk;
+      const uint64_t gend = min(g + chunk, S);
+      uint32_t i = first_ray[c];
+      uint32_t k = (uint32_t)(g - roff[i]);
+      Run run;
+      run.init();
+      while (g < gend && i < n) {
+        const int4 r = rayinfo[i];
+        const uint32_t cnt = (uint32_t)r.w & ~kHitBit;
+        if (k >= cnt) {
+          ++i;
+          k = 0;
+          continue;
+        }
+        const uint32_t m = (uint32_t)min((uint64_t)(cnt - k), gend - g);
+        const uint32_t nn[3] = {(uint32_t)abs(r.x - sx), (uint32_t)abs(r.y - sy),
+                                (uint32_t)abs(r.z - sz)};
+        const int stx = r.x > sx ? 1 : (r.x < sx ? -1 : 0);
+        const int sty = r.y > sy ? 1 : (r.y < sy ? -1 : 0);
+        const int stz = r.z > sz ? 1 : (r.z < sz ? -1 : 0);
+        uint32_t cc[3] = {0, 0, 0};
+        if (k > 0 && !dda_enter(tbl, nn, k, cc)) {
+          atomicOr(&st->error, kErrEntry);
+          break;
+        }
+        IntDda d;
+        d.init(nn, cc);
+        int vx = sx + stx * (int)cc[0], vy = sy + sty * (int)cc[1], vz = sz + stz * (int)cc[2];
+        for (uint32_t v = 0;;) {
+          run.add(u, ovf, lkey, lvis, vx, vy, vz);
+          if (++v == m) break;
+          const int a = d.next_axis(tbl);
+          d.advance(a);
+          vx += a == 0 ? stx : 0;
+          vy += a == 1 ? sty : 0;
+          vz += a == 2 ? stz : 0;
+        }
+        g += m;
         ++i;
         k = 0;
-        continue;
       }
-      const uint32_t m = (uint32_t)min((uint64_t)(cnt - k), gend - g);
-      const uint32_t nn[3] = {(uint32_t)abs(r.x - sx), (uint32_t)abs(r.y - sy), (uint32_t)abs(r.z - sz)};
-      const int stx = r.x > sx ? 1 : (r.x < sx ? -1 : 0);
-      const int sty = r.y > sy ? 1 : (r.y < sy ? -1 : 0);
-      const int stz = r.z > sz ? 1 : (r.z < sz ? -1 : 0);
-      uint32_t cc[3] = {0, 0, 0};
-      if (k > 0 && !dda_enter(tbl, nn, k, cc)) {
-        atomicOr(&st->error, kErrEntry);
-        return;
-      }
-      IntDda d;
-      d.init(nn, cc);
-      int vx = sx + stx * (int)cc[0], vy = sy + sty * (int)cc[1], vz = sz + stz * (int)cc[2];
-      for (uint32_t v = 0;;) {
-        run.add(u, ovf, vx, vy, vz);
-        if (++v == m) break;
-        const int a = d.next_axis(tbl);
-        d.advance(a);
-        vx += a == 0 ? stx : 0;
-        vy += a == 1 ? sty : 0;
-        vz += a == 2 ? stz : 0;
-      }
-      g += m;
-      ++i;
-      k = 0;
+      run.flush(u, lvis);
     }
-    run.flush(u);
+    __syncthreads();
+    // one global probe per distinct leaf, one atomic per new visited word
+    for (int e = threadIdx.x; e < kLdsEntries; e += blockDim.x) {
+      const uint64_t key = lkey[e];
+      if (key == kEmpty) continue;
+      const uint32_t slot = upd_slot(u, key, ovf);
+#pragma unroll
+      for (int w = 0; w < 8; ++w) {
+        const uint64_t v = lvis[e * 8 + w];
+        if (v) {
+          upd_or(u, slot, w, v);
+          lvis[e * 8 + w] = 0;
+        }
+      }
+      lkey[e] = kEmpty;
+    }
+    __syncthreads();
   }
 }
 
@@ -473,7 +546,8 @@ __global__ __launch_bounds__(256) void k_map_insert(UpdView u, const uint32_t* _
                                                     uint32_t count, MapView mp,
                                                     uint32_t* __restrict__ eidx,
                                                     uint32_t create_hit, uint32_t create_miss,
-                                                    Status* st) {
+                                                    unsigned long long* noop_set,
+                                                    uint32_t noop_mask, Status* st) {
   const uint32_t G = gridDim.x * blockDim.x;
   const uint32_t e0 = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t rounds = (count + G - 1) / G;
@@ -510,6 +584,12 @@ __global__ __launch_bounds__(256) void k_map_insert(UpdView u, const uint32_t* _
           }
         }
         p = (p + 1) & mp.cap_mask;
+      }
+      if (!need && idx == kNone) {
+        // the root still gains the 4096^3 child even when no leaf is created
+        int o[3];
+        key_origin(key, o);
+        set_insert(noop_set, noop_mask, node_key(o[0], o[1], o[2], 12));
       }
     }
     const uint64_t cm = __ballot(created);
@@ -723,9 +803,10 @@ hipError_t launch_compact(hipStream_t s, int grid, UpdView u, uint32_t cap, uint
 
 hipError_t launch_map_insert(hipStream_t s, int grid, UpdView u, const uint32_t* list,
                              uint32_t count, MapView mp, uint32_t* eidx, bool create_hit,
-                             bool create_miss, Status* st) {
+                             bool create_miss, unsigned long long* noop_set, uint32_t noop_mask,
+                             Status* st) {
   hipLaunchKernelGGL(k_map_insert, dim3(grid), dim3(256), 0, s, u, list, count, mp, eidx,
-                     (uint32_t)create_hit, (uint32_t)create_miss, st);
+                     (uint32_t)create_hit, (uint32_t)create_miss, noop_set, noop_mask, st);
   return hipGetLastError();
 }
 

@@ -23,7 +23,8 @@ hipError_t launch_compact(hipStream_t s, int grid, UpdView u, uint32_t cap, uint
                           Status* st);
 hipError_t launch_map_insert(hipStream_t s, int grid, UpdView u, const uint32_t* list,
                              uint32_t count, MapView mp, uint32_t* eidx, bool create_hit,
-                             bool create_miss, Status* st);
+                             bool create_miss, unsigned long long* noop_set, uint32_t noop_mask,
+                             Status* st);
 hipError_t launch_apply(hipStream_t s, int grid, UpdView u, const uint32_t* list,
                         const uint32_t* eidx, uint32_t count, MapView mp, const float lo[6],
                         Status* st);
@@ -36,5 +37,12 @@ hipError_t launch_dirty_list(hipStream_t s, int grid, uint8_t* dirty, uint32_t l
 hipError_t launch_gather_leaves(hipStream_t s, int grid, const uint32_t* list, uint32_t count,
                                 MapView mp, unsigned long long* keys_out, float* vals_out,
                                 unsigned long long* act_out);
+
+hipError_t launch_snapshot(hipStream_t s, MapView mp, uint32_t leaf_count, RtView rt, int* bbox);
+hipError_t launch_copy_set(hipStream_t s, const unsigned long long* src, uint32_t src_cap,
+                           unsigned long long* dst, uint32_t dst_mask);
+hipError_t launch_raytrace(hipStream_t s, int grid, RtView rt, MapView mp, double inv, double res,
+                           int64_t n, const double* origins, const double* dirs,
+                           const double* lens, uint8_t* success, double* ends, unsigned int* err);
 
 }  // namespace vdbgpu

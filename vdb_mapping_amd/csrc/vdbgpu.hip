@@ -32,7 +32,7 @@ constexpr uint32_t kMinUpdateCap = 1u << 12;
 constexpr uint32_t kMaxRays = 1u << 30;
 constexpr int kTraverseMaxBlock = 256;  // __launch_bounds__ of k_traverse
 
-enum TimedKind { kPrep = 0, kTraverse = 1, kCompact = 2, kApply = 3 };
+enum TimedKind { kPrep = 0, kTraverse = 1, kCompact = 2, kApply = 3, kRaytrace = 4 };
 
 struct TimedPair {
   int kind;
@@ -82,6 +82,19 @@ struct vdbgpu_map {
   uint32_t lcap = 0;
   uint64_t leaf_count_ub = 0;  // upper bound of the device leaf count
   bool touched = false;        // any voxel modify since creation/reset
+  unsigned long long* noop4096 = nullptr;  // root children of updates that created no leaf
+  static constexpr uint32_t kNoopCap = 4096;
+
+  // raytrace topology snapshot (updateVolumeRayIntersectors)
+  bool has_isect = false;
+  RtView rt{};
+  uint32_t rt_leaf_cap = 0, rt_n128_cap = 0, rt_n4096_cap = 0;
+  int* d_bbox = nullptr;
+  // raytrace query staging
+  double* q_in = nullptr;
+  uint8_t* q_succ = nullptr;
+  double* q_end = nullptr;
+  int64_t q_cap = 0;
 
   // per-scan scratch
   float* d_points = nullptr;
@@ -103,7 +116,7 @@ struct vdbgpu_map {
   Status* d_st = nullptr;
   Status* h_st = nullptr;
 
-  vdbgpu_tuning tuning{256, 128, 16, 0.5};
+  vdbgpu_tuning tuning{256, 32, 16, 0.5};
   bool profiling = false;
   std::vector<TimedPair> timed;
   std::vector<hipEvent_t> event_pool;
@@ -180,6 +193,7 @@ void collect_timing(vdbgpu_map* m) {
         case kPrep: m->stats.ms_prep += ms; m->stats.n_prep++; break;
         case kTraverse: m->stats.ms_traverse += ms; m->stats.n_traverse++; break;
         case kCompact: m->stats.ms_compact += ms; m->stats.n_compact++; break;
+        case kRaytrace: m->stats.ms_raytrace += ms; m->stats.n_raytrace++; break;
         default: m->stats.ms_apply += ms; m->stats.n_apply++; break;
       }
     }
@@ -200,6 +214,12 @@ int read_status(vdbgpu_map* m) {
 int grid_lanes(vdbgpu_map* m, int block) {
   const int lanes = m->num_cu * m->tuning.waves_per_cu * 64;
   return std::max(1, lanes / block);
+}
+
+// Traversal workgroups: LDS (36 KiB per workgroup) allows 4 resident per CU.
+int traverse_grid(vdbgpu_map* m, int block) {
+  const int per_cu = std::max(1, std::min(m->tuning.waves_per_cu * 64 / block, 4 * 256 / block));
+  return m->num_cu * per_cu;
 }
 
 UpdView upd_view(const Source& s) { return UpdView{s.keys, s.masks, s.cap - 1}; }
@@ -385,7 +405,7 @@ int do_accumulate(vdbgpu_map* m, Source& src, const float* xyz, int64_t n, int64
   for (int attempt = 0;; ++attempt) {
     {
       Timer t(m, kTraverse);
-      CK(launch_traverse(m->stream, grid_lanes(m, block), block, m->rayinfo, m->roff, m->first_ray, nn,
+      CK(launch_traverse(m->stream, traverse_grid(m, block), block, m->rayinfo, m->roff, m->first_ray, nn,
                          m->tbl, upd_view(src), &m->d_st[0], chunk));
     }
     {
@@ -444,6 +464,47 @@ void create_flags(const vdbgpu_map* m, bool* ch, bool* cm) {
   *cm = x != 0.0f;
 }
 
+// updateVolumeRayIntersectors (VDBMapping.hpp:1436-1449): in fast mode, a
+// non-empty map gets a fresh topology snapshot after every integrate.
+int build_snapshot(vdbgpu_map* m) {
+  int r = read_status(m);
+  if (r) return r;
+  const uint32_t L = m->h_st[1].leaf_count;
+  const uint32_t cap = (uint32_t)std::max<uint64_t>(1024, next_pow2(2ull * L + 2));
+  const uint32_t cap_root = (uint32_t)std::max<uint64_t>(1024, next_pow2(2ull * L + 2 * vdbgpu_map::kNoopCap));
+  if (cap > m->rt_leaf_cap) {
+    dfree(m, &m->rt.leaf, m->rt_leaf_cap);
+    dfree(m, &m->rt.n128, m->rt_n128_cap);
+    m->rt_leaf_cap = m->rt_n128_cap = cap;
+    CK(dalloc(m, &m->rt.leaf, cap));
+    CK(dalloc(m, &m->rt.n128, cap));
+  }
+  if (cap_root > m->rt_n4096_cap) {
+    dfree(m, &m->rt.n4096, m->rt_n4096_cap);
+    m->rt_n4096_cap = cap_root;
+    CK(dalloc(m, &m->rt.n4096, cap_root));
+  }
+  m->rt.leaf_mask = m->rt_leaf_cap - 1;
+  m->rt.n128_mask = m->rt_n128_cap - 1;
+  m->rt.n4096_mask = m->rt_n4096_cap - 1;
+  CK(launch_fill_u64(m->stream, m->rt.leaf, m->rt_leaf_cap, kEmpty));
+  CK(launch_fill_u64(m->stream, m->rt.n128, m->rt_n128_cap, kEmpty));
+  CK(launch_fill_u64(m->stream, m->rt.n4096, m->rt_n4096_cap, kEmpty));
+  if (!m->d_bbox) CK(dalloc(m, &m->d_bbox, 6));
+  int hb[6] = {INT_MAX, INT_MAX, INT_MAX, INT_MIN, INT_MIN, INT_MIN};
+  CK(hipMemcpyAsync(m->d_bbox, hb, sizeof hb, hipMemcpyHostToDevice, m->stream));
+  if (L) CK(launch_snapshot(m->stream, map_view(m), L, m->rt, m->d_bbox));
+  CK(launch_copy_set(m->stream, m->noop4096, vdbgpu_map::kNoopCap, m->rt.n4096, m->rt.n4096_mask));
+  CK(hipMemcpyAsync(hb, m->d_bbox, sizeof hb, hipMemcpyDeviceToHost, m->stream));
+  CK(hipStreamSynchronize(m->stream));
+  for (int a = 0; a < 3; ++a) {
+    m->rt.bmin[a] = hb[a];
+    m->rt.bmax[a] = hb[3 + a] + 1;  // mBBox.max().offset(1)
+  }
+  m->has_isect = true;
+  return VDBGPU_OK;
+}
+
 int do_integrate(vdbgpu_map* m) {
   bool ch, cm;
   create_flags(m, &ch, &cm);
@@ -462,7 +523,7 @@ int do_integrate(vdbgpu_map* m) {
     {
       Timer t(m, kApply);
       CK(launch_map_insert(m->stream, g1, upd_view(s), s.list, s.pending, map_view(m), m->eidx, ch, cm,
-                           &m->d_st[1]));
+                           m->noop4096, vdbgpu_map::kNoopCap - 1, &m->d_st[1]));
       CK(launch_apply(m->stream, g2, upd_view(s), s.list, m->eidx, s.pending, map_view(m), m->lo,
                       &m->d_st[3]));
     }
@@ -471,6 +532,7 @@ int do_integrate(vdbgpu_map* m) {
     m->touched = true;
     s.pending = 0;
   }
+  if (m->fast_mode && m->touched) return build_snapshot(m);
   return VDBGPU_OK;
 }
 
@@ -556,6 +618,11 @@ int32_t vdbgpu_create(double resolution, vdbgpu_map** out) {
     return VDBGPU_DEVICE_ERROR;
   }
   memset(m->h_st, 0, 4 * sizeof(Status));
+  if (hipMalloc((void**)&m->noop4096, vdbgpu_map::kNoopCap * 8) != hipSuccess ||
+      launch_fill_u64(m->stream, m->noop4096, vdbgpu_map::kNoopCap, kEmpty) != hipSuccess) {
+    vdbgpu_destroy(m);
+    return VDBGPU_DEVICE_ERROR;
+  }
   *out = m;
   return VDBGPU_OK;
 }
@@ -581,6 +648,14 @@ void vdbgpu_destroy(vdbgpu_map* m) {
   dfree(m, &m->roff, m->ray_cap);
   dfree(m, &m->first_ray, m->chunk_cap);
   dfree(m, &m->tbl, (uint64_t)m->tbl_n * (m->tbl_n + 1) / 2);
+  dfree(m, &m->rt.leaf, m->rt_leaf_cap);
+  dfree(m, &m->rt.n128, m->rt_n128_cap);
+  dfree(m, &m->rt.n4096, m->rt_n4096_cap);
+  dfree(m, &m->d_bbox, 6);
+  dfree(m, &m->q_in, (size_t)m->q_cap * 7);
+  dfree(m, &m->q_succ, (size_t)m->q_cap);
+  dfree(m, &m->q_end, (size_t)m->q_cap * 3);
+  if (m->noop4096) (void)hipFree(m->noop4096);
   if (m->d_st) (void)hipFree(m->d_st);
   if (m->h_st) (void)hipHostFree(m->h_st);
   for (auto& t : m->timed) {
@@ -636,6 +711,16 @@ int32_t vdbgpu_add_source(vdbgpu_map* m, const char* id, double max_range, doubl
   return VDBGPU_OK;
 }
 
+int32_t vdbgpu_remove_source(vdbgpu_map* m, const char* id) {
+  if (!m || !id) return VDBGPU_BAD_ARGUMENT;
+  Guard g(m);
+  auto it = m->sources.find(id);
+  if (it == m->sources.end()) return fail(m, VDBGPU_UNKNOWN_SOURCE, std::string("no source ") + id);
+  free_update(m, it->second);
+  m->sources.erase(it);
+  return VDBGPU_OK;
+}
+
 int32_t vdbgpu_accumulate(vdbgpu_map* m, const char* id, const float* xyz, int64_t n, int64_t stride,
                           const double origin[3], int32_t on_device) {
   if (!m || !id) return VDBGPU_BAD_ARGUMENT;
@@ -675,6 +760,7 @@ int32_t vdbgpu_reset(vdbgpu_map* m) {
     CK(hipMemsetAsync(m->dirty, 0, m->lcap, m->stream));
   }
   if (m->mcap) CK(launch_fill_u64(m->stream, m->mkeys, m->mcap, kEmpty));
+  CK(launch_fill_u64(m->stream, m->noop4096, vdbgpu_map::kNoopCap, kEmpty));
   CK(hipMemsetAsync(&m->d_st[1], 0, sizeof(Status), m->stream));
   for (auto& kv : m->sources) {
     Source& s = kv.second;
@@ -809,12 +895,60 @@ int64_t vdbgpu_export_update(vdbgpu_map* m, const char* id, int32_t* origins, ui
   return (int64_t)order.size();
 }
 
+// Status of a finished raytrace launch: rejected queries are reported.
+static int32_t raytrace_status(vdbgpu_map* m) {
+  const int r = read_status(m);
+  if (r) return r;
+  if (m->h_st[2].error)
+    return fail(m, VDBGPU_BAD_ARGUMENT,
+                "query rays with non-finite values or more than 2^24 voxel steps were rejected "
+                "(success = 0, end point = NaN)");
+  return VDBGPU_OK;
+}
+
 int32_t vdbgpu_raytrace(vdbgpu_map* m, int64_t n, const double* o, const double* d,
                         const double* len, uint8_t* succ, double* ends) {
-  (void)n; (void)o; (void)d; (void)len; (void)succ; (void)ends;
-  if (!m) return VDBGPU_BAD_ARGUMENT;
+  if (!m || n < 0 || (n && (!o || !d || !len || !succ || !ends))) return VDBGPU_BAD_ARGUMENT;
   Guard g(m);
-  return fail(m, VDBGPU_NO_INTERSECTOR, "raytrace needs an integrate in fast mode first");
+  if (!m->has_isect)
+    return fail(m, VDBGPU_NO_INTERSECTOR, "raytrace needs an integrate in fast mode on a non-empty map");
+  if (n == 0) return VDBGPU_OK;
+  if (n > m->q_cap) {
+    dfree(m, &m->q_in, (size_t)m->q_cap * 7);
+    dfree(m, &m->q_succ, (size_t)m->q_cap);
+    dfree(m, &m->q_end, (size_t)m->q_cap * 3);
+    m->q_cap = n;
+    CK(dalloc(m, &m->q_in, (size_t)n * 7));
+    CK(dalloc(m, &m->q_succ, (size_t)n));
+    CK(dalloc(m, &m->q_end, (size_t)n * 3));
+  }
+  CK(hipMemcpyAsync(m->q_in, o, (size_t)n * 24, hipMemcpyHostToDevice, m->stream));
+  CK(hipMemcpyAsync(m->q_in + 3 * n, d, (size_t)n * 24, hipMemcpyHostToDevice, m->stream));
+  CK(hipMemcpyAsync(m->q_in + 6 * n, len, (size_t)n * 8, hipMemcpyHostToDevice, m->stream));
+  const int grid = (int)std::min<int64_t>((int64_t)m->num_cu * 16, (n + 255) / 256);
+  CK(hipMemsetAsync(&m->d_st[2].error, 0, sizeof(unsigned int), m->stream));
+  CK(launch_raytrace(m->stream, grid, m->rt, map_view(m), m->inv, m->res, n, m->q_in, m->q_in + 3 * n,
+                     m->q_in + 6 * n, m->q_succ, m->q_end, &m->d_st[2].error));
+  CK(hipMemcpyAsync(succ, m->q_succ, (size_t)n, hipMemcpyDeviceToHost, m->stream));
+  CK(hipMemcpyAsync(ends, m->q_end, (size_t)n * 24, hipMemcpyDeviceToHost, m->stream));
+  return raytrace_status(m);
+}
+
+int32_t vdbgpu_raytrace_device(vdbgpu_map* m, int64_t n, const double* o, const double* d,
+                               const double* len, uint8_t* succ, double* ends) {
+  if (!m || n < 0 || (n && (!o || !d || !len || !succ || !ends))) return VDBGPU_BAD_ARGUMENT;
+  Guard g(m);
+  if (!m->has_isect)
+    return fail(m, VDBGPU_NO_INTERSECTOR, "raytrace needs an integrate in fast mode on a non-empty map");
+  if (n == 0) return VDBGPU_OK;
+  const int grid = (int)std::min<int64_t>((int64_t)m->num_cu * 16, (n + 255) / 256);
+  CK(hipMemsetAsync(&m->d_st[2].error, 0, sizeof(unsigned int), m->stream));
+  {
+    Timer t(m, kRaytrace);
+    CK(launch_raytrace(m->stream, grid, m->rt, map_view(m), m->inv, m->res, n, o, d, len, succ, ends,
+                       &m->d_st[2].error));
+  }
+  return raytrace_status(m);
 }
 
 int32_t vdbgpu_device_alloc(int64_t bytes, void** out) {
