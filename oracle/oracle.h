@@ -11,6 +11,13 @@
  * The voxel arithmetic of the reference lives in OpenVDB, which is not part
  * of the reference tree; the restated OpenVDB semantics are listed in
  * oracle.cpp next to the code that uses them.
+ *
+ * Pinning (tests/test_oracle.py): the reference's known-answer tests, closed
+ * forms, an exact-rational DDA walk and brute-force set checks. Parity
+ * unpinned (no check here can compare them with OpenVDB itself): the rounding
+ * forms U1-U3, the double tie-break between axes of different extent (U5),
+ * raytrace clip/HDDA details (U6, U7, U9, U10) and leaf creation by no-op
+ * updates (U8).
  */
 #ifndef VDB_ORACLE_H
 #define VDB_ORACLE_H

@@ -522,20 +522,41 @@ __global__ __launch_bounds__(256) void k_traverse(const int4* __restrict__ rayin
   }
 }
 
+// Workgroup-aggregated append: returns the position of this lane's element
+// in a global list (valid lanes only), one atomic per workgroup call. Must be
+// called by every thread of the workgroup.
+__device__ __forceinline__ uint32_t block_append(bool valid, unsigned int* counter,
+                                                 uint32_t* s_wave, uint32_t* s_base) {
+  const uint64_t m = __ballot(valid);
+  const int w = threadIdx.x >> 6;
+  if (lane_id() == 0) s_wave[w] = (uint32_t)__popcll(m);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t tot = 0;
+    for (int i = 0; i < (int)(blockDim.x >> 6); ++i) {
+      const uint32_t c = s_wave[i];
+      s_wave[i] = tot;
+      tot += c;
+    }
+    *s_base = tot ? atomicAdd(counter, tot) : 0u;
+  }
+  __syncthreads();
+  const uint32_t pos = *s_base + s_wave[w] + (uint32_t)__popcll(m & ((1ull << lane_id()) - 1));
+  __syncthreads();
+  return pos;
+}
+
 // List the occupied update-set slots (keys only: coalesced 8-byte reads).
 __global__ __launch_bounds__(256) void k_compact(UpdView u, uint32_t cap,
                                                  uint32_t* __restrict__ list, Status* st) {
+  __shared__ uint32_t s_wave[4], s_base;
   const uint32_t G = gridDim.x * blockDim.x;
-  for (uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x; slot < cap; slot += G) {
-    const bool valid = u.keys[slot] != kEmpty;
-    const uint64_t m = __ballot(valid);
-    if (m) {
-      const int leader = __ffsll((long long)m) - 1;
-      uint32_t base = 0;
-      if (lane_id() == leader) base = atomicAdd(&st->entries, (unsigned int)__popcll(m));
-      base = __shfl(base, leader);
-      if (valid) list[base + __popcll(m & ((1ull << lane_id()) - 1))] = slot;
-    }
+  const uint32_t rounds = (cap + G - 1) / G;
+  for (uint32_t rd = 0; rd < rounds; ++rd) {
+    const uint32_t slot = rd * G + blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = slot < cap && u.keys[slot] != kEmpty;
+    const uint32_t pos = block_append(valid, &st->entries, s_wave, &s_base);
+    if (valid) list[pos] = slot;
   }
 }
 
@@ -548,6 +569,7 @@ __global__ __launch_bounds__(256) void k_map_insert(UpdView u, const uint32_t* _
                                                     uint32_t create_hit, uint32_t create_miss,
                                                     unsigned long long* noop_set,
                                                     uint32_t noop_mask, Status* st) {
+  __shared__ uint32_t s_wave[4], s_base;
   const uint32_t G = gridDim.x * blockDim.x;
   const uint32_t e0 = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t rounds = (count + G - 1) / G;
@@ -592,17 +614,11 @@ __global__ __launch_bounds__(256) void k_map_insert(UpdView u, const uint32_t* _
         set_insert(noop_set, noop_mask, node_key(o[0], o[1], o[2], 12));
       }
     }
-    const uint64_t cm = __ballot(created);
-    if (cm) {
-      const int leader = __ffsll((long long)cm) - 1;
-      uint32_t base = 0;
-      if (lane_id() == leader) base = atomicAdd(&st->leaf_count, (unsigned int)__popcll(cm));
-      base = __shfl(base, leader);
-      if (created) {
-        idx = base + __popcll(cm & ((1ull << lane_id()) - 1));
-        mp.idx[h] = idx;
-        mp.leaf_keys[idx] = key;
-      }
+    const uint32_t pos = block_append(created, &st->leaf_count, s_wave, &s_base);
+    if (created) {
+      idx = pos;
+      mp.idx[h] = idx;
+      mp.leaf_keys[idx] = key;
     }
     if (e < count) {
       eidx[e] = idx;
