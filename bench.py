@@ -9,6 +9,12 @@ scans of the stream, the timed steps integrate the following ones into the
 same map (scans are reused cyclically when warmup + steps > 100).
 
 Prints one JSON line (see --help); `value` is rays/s of the whole job.
+
+Other modes (--mode, not the default line) measure the north-star layers:
+  batch  the outdoor stream integrated in batches of --batch scans vs the plain path
+  mixed  64 scans of 10k-300k points with random poses and ranges: one batch vs per-scan
+  rgbd   200 RGB-D frames at 0.02 m, then 1M raytrace queries (device arrays)
+  serve  16 robot streams, Poisson arrivals at 10 Hz each, one map per stream
 """
 from __future__ import annotations
 
@@ -36,6 +42,12 @@ def parse():
     p.add_argument("--cpu-scans", type=int, default=2, help="scans timed on the CPU oracle (0 = skip)")
     p.add_argument("--no-profiling", action="store_true", help="do not time kernels with HIP events")
     p.add_argument("--tuning", type=str, default="", help="block,steps_per_lane,waves_per_cu,load_factor")
+    p.add_argument("--mode", choices=("plain", "batch", "mixed", "rgbd", "serve"), default="plain")
+    p.add_argument("--batch", type=int, default=8, help="scans per batch (batch mode)")
+    p.add_argument("--autotune", action="store_true", help="mixed mode: autotune first")
+    p.add_argument("--seconds", type=float, default=10.0, help="serve mode: arrival window")
+    p.add_argument("--frames", type=int, default=200, help="rgbd mode: depth frames")
+    p.add_argument("--queries", type=int, default=1_000_000, help="rgbd mode: raytrace queries")
     return p.parse_args()
 
 
@@ -57,6 +69,198 @@ def cpu_baseline(w, n_scans):
     return m, rays, t
 
 
+def new_map(V, w, tuning=None):
+    g = V.GpuMap(w.resolution)
+    g.set_config(V.make_config(max_range=w.max_range, **w.probs))
+    g.add_input_source("lidar", w.max_range)
+    if tuning:
+        b, spl, wpc, lf = tuning.split(",") if isinstance(tuning, str) else tuning
+        assert g.set_tuning(int(b), int(spl), int(wpc), float(lf)) == V.OK
+    return g
+
+
+def upload_scans(V, scans):
+    dev = V.DeviceBuffer(sum(s.points.nbytes for s in scans))
+    offs, off = [], 0
+    for s in scans:
+        dev.upload(s.points, off)
+        offs.append(off)
+        off += s.points.nbytes
+    return dev, offs
+
+
+def mode_batch(a, V, W):
+    """Outdoor stream: plain per-scan path vs batches of a.batch scans (same scans, fresh maps)."""
+    w = W.outdoor_stream_workload(a.scans)
+    scans = [w.scan(i) for i in range(a.scans)]
+    dev, offs = upload_scans(V, scans)
+    n_pts = scans[0].points.shape[0]
+    rays = sum(int((~np.isnan(s.points[:, :3]).any(1)).sum()) for s in scans)
+    out = {}
+    for path in ("plain", "batch"):
+        g = new_map(V, w, a.tuning)
+        # warm: size every table on the same stream, then start from an empty map
+        for j in range(0, a.scans, a.batch):
+            k = min(a.batch, a.scans - j)
+            g.insert_batch_device("lidar", dev.address + offs[j], [n_pts] * k, 16,
+                                  np.array([s.origin for s in scans[j:j + k]]), check=True)
+        g.reset()
+        g.synchronize()
+        t0 = time.perf_counter()
+        if path == "plain":
+            for j, s in enumerate(scans):
+                g.insert_device("lidar", dev.address + offs[j], n_pts, 16, s.origin, check=True)
+        else:
+            for j in range(0, a.scans, a.batch):
+                k = min(a.batch, a.scans - j)
+                g.insert_batch_device("lidar", dev.address + offs[j], [n_pts] * k, 16,
+                                      np.array([s.origin for s in scans[j:j + k]]), check=True)
+        g.synchronize()
+        out[path] = time.perf_counter() - t0
+        out[path + "_leaves"] = g.leaf_count()
+        g.close()
+    print(json.dumps({"metric": "batched integration of the outdoor stream", "value": round(rays / out["batch"], 1),
+                      "unit": "rays/s", "plain_rays_per_s": round(rays / out["plain"], 1),
+                      "speedup_vs_plain": round(out["plain"] / out["batch"], 3), "batch_scans": a.batch,
+                      "scans": a.scans, "same_leaf_count": out["plain_leaves"] == out["batch_leaves"],
+                      "config": {"workload": "outdoor_stream_262k_0.05m", "input": "device-resident"}}),
+          flush=True)
+
+
+def mode_mixed(a, V, W):
+    """64 scans, 10k-300k points, random poses, ranges U[20, 50]: one batch vs per-scan inserts."""
+    w = W.mixed_batch_workload(64)
+    scans = [w.scan(i) for i in range(64)]
+    counts = [s.points.shape[0] for s in scans]
+    dev, offs = upload_scans(V, scans)
+    origins = np.array([s.origin for s in scans])
+    ranges = np.array([s.max_range for s in scans])
+    tuning = a.tuning or None
+    tune_table = None
+    if a.autotune:
+        from vdb_mapping_amd.autotune import autotune
+        sample = list(range(0, 64, 4))
+        best, table = autotune(w.resolution, dict(max_range=w.max_range, **w.probs),
+                               [scans[i].points for i in sample], origins[sample], ranges[sample],
+                               batched=True)
+        tuning = (best["block"], best["steps_per_lane"], best["waves_per_cu"], best["load_factor"])
+        tune_table = [(t, round(s * 1e3, 3)) for t, s in sorted(table, key=lambda r: r[1])[:5]]
+    times = {}
+    for path in ("plain", "batch"):
+        g = new_map(V, w, tuning)
+        best = float("inf")
+        for rep in range(3):
+            g.reset()
+            g.synchronize()
+            t0 = time.perf_counter()
+            if path == "batch":
+                g.insert_batch_device("lidar", dev.address, counts, 16, origins, ranges, check=True)
+            else:
+                for j in range(64):
+                    g.add_input_source("lidar", float(ranges[j]))
+                    g.insert_device("lidar", dev.address + offs[j], counts[j], 16, origins[j], check=True)
+            g.synchronize()
+            best = min(best, time.perf_counter() - t0)
+        times[path] = best
+        g.close()
+    rays = int(sum(counts))
+    print(json.dumps({"metric": "batched integration of 64 mixed-size scans (one call)",
+                      "value": round(rays / times["batch"], 1), "unit": "rays/s",
+                      "plain_rays_per_s": round(rays / times["plain"], 1),
+                      "speedup_vs_plain": round(times["plain"] / times["batch"], 3),
+                      "batch_ms": round(times["batch"] * 1e3, 3), "plain_ms": round(times["plain"] * 1e3, 3),
+                      "rays": rays, "tuning": tuning, "autotune_top5": tune_table,
+                      "config": {"workload": "mixed_batch_64_0.05m", "input": "device-resident"}}), flush=True)
+
+
+def mode_rgbd(a, V, W):
+    """RGB-D frames at 0.02 m (batches of 16), then raytrace queries on device arrays."""
+    w = W.rgbd_workload(a.frames)
+    scans = [w.scan(i) for i in range(a.frames)]
+    dev, offs = upload_scans(V, scans)
+    n_pts = scans[0].points.shape[0]
+    g = new_map(V, w, a.tuning)
+    rays = sum(int((~np.isnan(s.points[:, :3]).any(1)).sum()) for s in scans)
+    g.synchronize()
+    t0 = time.perf_counter()
+    for j in range(0, a.frames, 16):
+        k = min(16, a.frames - j)
+        g.insert_batch_device("lidar", dev.address + offs[j], [n_pts] * k, 16,
+                              np.array([s.origin for s in scans[j:j + k]]), check=True)
+    g.synchronize()
+    t_int = time.perf_counter() - t0
+    g.set_config(V.make_config(max_range=w.max_range, fast_mode=True, **w.probs))
+    g.integrate(check=True)
+    org, dirs, lens = W.raytrace_queries(a.queries)
+    bo, bd, bl = (V.DeviceBuffer.from_array(np.ascontiguousarray(x, np.float64)) for x in (org, dirs, lens))
+    bs, be = V.DeviceBuffer(a.queries), V.DeviceBuffer(a.queries * 24)
+    g.raytrace_device(a.queries, bo.address, bd.address, bl.address, bs.address, be.address)  # warm
+    g.set_profiling(True)
+    st0 = g.stats()
+    t0 = time.perf_counter()
+    assert g.raytrace_device(a.queries, bo.address, bd.address, bl.address, bs.address, be.address) == V.OK
+    t_rt = time.perf_counter() - t0
+    st1 = g.stats()
+    ok = bs.download(np.zeros(a.queries, np.uint8))
+    print(json.dumps({"metric": "RGB-D integration + raytrace queries", "value": round(a.queries / t_rt, 1),
+                      "unit": "queries/s", "integrate_rays_per_s": round(rays / t_int, 1),
+                      "raytrace_kernel_ms": round(st1["ms_raytrace"] - st0["ms_raytrace"], 3),
+                      "success_fraction": round(float(ok.mean()), 4), "frames": a.frames,
+                      "map_leaves": g.leaf_count(),
+                      "config": {"workload": "rgbd_640x480_0.02m", "queries": a.queries}}), flush=True)
+
+
+def mode_serve(a, V, W):
+    """16 streams, Poisson arrivals (10 Hz each) of 20k-262k-point room scans, one map each."""
+    import math
+    import threading
+    rng = np.random.default_rng(1005)
+    scene = W.room_scene(1001)
+    pool = []
+    for i in range(48):
+        n = int(math.exp(rng.uniform(math.log(20_000), math.log(262_144))))
+        pose = np.array([rng.uniform(-8, 8), rng.uniform(-8, 8), rng.uniform(-0.5, 2.0),
+                         math.radians(rng.uniform(-3, 3)), math.radians(rng.uniform(-3, 3)),
+                         rng.uniform(-math.pi, math.pi)])
+        pts = W.lidar_scan(scene, pose, 64, (n + 63) // 64, n, -22.5, 22.5, 0.02, W.scan_seed(5, i))
+        pool.append((pts, pose[:3].copy()))
+    streams, rate = 16, 10.0
+    arrivals = []
+    for sidx in range(streams):
+        t = rng.exponential(1 / rate)
+        while t < a.seconds:
+            arrivals.append((t, sidx, int(rng.integers(0, len(pool)))))
+            t += rng.exponential(1 / rate)
+    arrivals.sort()
+    cfg = V.make_config(max_range=50.0, **W.PROBS)
+    srv = V.Server(streams, 0.05, cfg, 50.0, hbm_budget_bytes=200 << 30, max_batch=32)
+    # warm every map once (table sizing), then start the clock
+    for sidx in range(streams):
+        srv.submit(sidx, pool[sidx % len(pool)][0], pool[sidx % len(pool)][1])
+    srv.drain()
+    n_warm = len(srv.records())
+    t0 = time.perf_counter()
+    for t, sidx, k in arrivals:
+        delay = t0 + t - time.perf_counter()
+        if delay > 0:
+            time.sleep(delay)
+        srv.submit(sidx, pool[k][0], pool[k][1])
+    srv.drain()
+    elapsed = time.perf_counter() - t0
+    recs = srv.records()[n_warm:]
+    lat = np.array([r["latency_ms"] for r in recs])
+    rays = sum(r["points"] for r in recs)
+    print(json.dumps({"metric": "multi-robot serving: p50/p99 scan latency", "value": round(float(np.percentile(lat, 99)), 3),
+                      "unit": "ms (p99)", "higher_is_better": False,
+                      "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
+                      "max_ms": round(float(lat.max()), 3), "requests": len(recs), "streams": streams,
+                      "offered_hz": rate * streams, "rays_per_s": round(rays / elapsed, 1),
+                      "mean_batch": round(float(np.mean([r["batch_size"] for r in recs])), 3),
+                      "all_ok": all(r["status"] == 0 for r in recs), "device_gb": round(srv.device_bytes() / 1e9, 2),
+                      "config": {"workload": "multi_robot_16_streams_0.05m", "seconds": a.seconds}}), flush=True)
+    srv.close()
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -69,6 +273,9 @@ def main():
     if V.device_count() < 1:
         print("no gfx950 GPU visible", file=sys.stderr)
         sys.exit(1)
+    if a.mode != "plain":
+        {"batch": mode_batch, "mixed": mode_mixed, "rgbd": mode_rgbd, "serve": mode_serve}[a.mode](a, V, W)
+        return
     w = W.outdoor_stream_workload(a.scans)
     t_gen = time.perf_counter()
     scans = [w.scan(i) for i in range(a.scans)]

@@ -31,7 +31,8 @@ enum {
   VDBGPU_UNSUPPORTED = 7,    /* fast_mode raycasting (VM:520-527) is not provided */
   VDBGPU_DEVICE_ERROR = 8,   /* HIP runtime error or out of device memory */
   VDBGPU_BAD_ARGUMENT = 9,
-  VDBGPU_NO_INTERSECTOR = 10 /* raytrace before any fast_mode integrate (VM:698, :1438) */
+  VDBGPU_NO_INTERSECTOR = 10, /* raytrace before any fast_mode integrate (VM:698, :1438) */
+  VDBGPU_OVER_BUDGET = 11     /* serving: the maps' device footprint exceeds the HBM budget */
 };
 
 /* BaseConfig (VM:64-71) + Config (OCC:38-44), field for field.
@@ -116,6 +117,19 @@ int32_t vdbgpu_integrate(vdbgpu_map* map);
 int32_t vdbgpu_insert(vdbgpu_map* map, const char* source_id, const float* xyz, int64_t n,
                       int64_t stride_bytes, const double origin[3], int32_t points_on_device);
 
+/* Batched insertPointCloud: nscans clouds of one source, concatenated in
+ * `xyz` (counts[i] points each, same stride), integrated with a result
+ * identical to vdbgpu_insert on each scan in order (VM:399-406 per scan).
+ * origins: nscans x 3 doubles. max_ranges: per-scan max range, or NULL for
+ * the source's range (a range <= 0 casts nothing for that scan). Up to 128
+ * scans share one (leaf, scan)-keyed update set and one scan-ordered fold;
+ * longer lists are processed 128 at a time. No reference counterpart for the
+ * per-scan ranges. */
+int32_t vdbgpu_insert_batch(vdbgpu_map* map, const char* source_id, int32_t nscans,
+                            const float* xyz, const int64_t* counts, int64_t stride_bytes,
+                            const double* origins, const double* max_ranges,
+                            int32_t points_on_device);
+
 /* resetMap VM:174-186: empty map and empty update sets. */
 int32_t vdbgpu_reset(vdbgpu_map* map);
 /* GridT::empty() as used at VM:494 and VM:1438: 1 if no voxel was ever modified. */
@@ -174,6 +188,43 @@ int32_t vdbgpu_set_tuning(vdbgpu_map* map, const vdbgpu_tuning* tuning);
 int32_t vdbgpu_get_tuning(vdbgpu_map* map, vdbgpu_tuning* out);
 /* Text of the last error on this handle ("" when none). */
 const char* vdbgpu_last_error(vdbgpu_map* map);
+
+/* ---- Serving: many robot streams, one device map each ------------------
+ * No reference counterpart (the reference serves one map per process through
+ * its accumulation threads, VM:355-370, :1383-1430, which merge and drop
+ * scans). Here every submitted scan is integrated exactly once, in arrival
+ * order per map: a worker per map takes all requests queued while its map was
+ * busy and integrates them in one vdbgpu_insert_batch (same result as one by
+ * one); different maps run on their own streams and overlap on the device. A
+ * map that holds device memory gets no more work while the maps' summed
+ * footprint exceeds hbm_budget_bytes (its requests fail with
+ * VDBGPU_OVER_BUDGET); 0 = no budget. */
+typedef struct vdbgpu_server vdbgpu_server;
+
+typedef struct vdbgpu_request_record {
+  int64_t request_id;
+  int64_t points;
+  int32_t map_index;
+  int32_t batch_size; /* requests integrated together with this one */
+  int32_t status;
+  int32_t _pad;
+  double latency_ms;  /* submit -> device work finished */
+} vdbgpu_request_record;
+
+int32_t vdbgpu_server_create(int32_t n_maps, double resolution, const vdbgpu_config* config,
+                             double max_range, int64_t hbm_budget_bytes, int32_t max_batch,
+                             vdbgpu_server** out);
+/* Copies the cloud (host memory) and queues it for map `map_index`. */
+int32_t vdbgpu_server_submit(vdbgpu_server* server, int32_t map_index, const float* xyz, int64_t n,
+                             int64_t stride_bytes, const double origin[3], int64_t* request_id);
+/* Wait until every submitted request has finished; returns the first error. */
+int32_t vdbgpu_server_drain(vdbgpu_server* server);
+/* Finished requests in completion order (NULL out = count only). */
+int64_t vdbgpu_server_records(vdbgpu_server* server, vdbgpu_request_record* out, int64_t cap);
+/* The map of one stream (owned by the server; valid until destroy). */
+vdbgpu_map* vdbgpu_server_map(vdbgpu_server* server, int32_t map_index);
+int64_t vdbgpu_server_device_bytes(vdbgpu_server* server);
+void vdbgpu_server_destroy(vdbgpu_server* server);
 
 #ifdef __cplusplus
 }

@@ -19,19 +19,23 @@ INCLUDE_DIR = os.path.join(os.path.dirname(HERE), "include")
 
 OK, NOT_CONFIGURED, UNKNOWN_SOURCE, BAD_MAX_RANGE, BAD_PROB_MISS, BAD_PROB_HIT = 0, 1, 2, 3, 4, 5
 OUT_OF_RANGE, UNSUPPORTED, DEVICE_ERROR, BAD_ARGUMENT, NO_INTERSECTOR = 6, 7, 8, 9, 10
+OVER_BUDGET = 11
 
 # Every symbol include/vdbgpu.h declares.
 ABI_SYMBOLS = [
     "vdbgpu_version", "vdbgpu_status_string", "vdbgpu_device_count", "vdbgpu_create",
     "vdbgpu_remove_source",
     "vdbgpu_destroy", "vdbgpu_set_config", "vdbgpu_logodds", "vdbgpu_add_source",
-    "vdbgpu_accumulate", "vdbgpu_integrate", "vdbgpu_insert", "vdbgpu_reset", "vdbgpu_empty",
+    "vdbgpu_accumulate", "vdbgpu_integrate", "vdbgpu_insert", "vdbgpu_insert_batch", "vdbgpu_reset",
+    "vdbgpu_empty",
     "vdbgpu_leaf_count", "vdbgpu_export_leaves", "vdbgpu_export_dirty_leaves",
     "vdbgpu_update_leaf_count", "vdbgpu_export_update", "vdbgpu_raytrace", "vdbgpu_raytrace_device",
     "vdbgpu_synchronize",
     "vdbgpu_get_stats", "vdbgpu_set_profiling", "vdbgpu_set_tuning", "vdbgpu_get_tuning",
     "vdbgpu_last_error", "vdbgpu_device_alloc", "vdbgpu_device_free", "vdbgpu_device_upload",
-    "vdbgpu_device_download",
+    "vdbgpu_device_download", "vdbgpu_server_create", "vdbgpu_server_submit", "vdbgpu_server_drain",
+    "vdbgpu_server_records", "vdbgpu_server_map", "vdbgpu_server_device_bytes",
+    "vdbgpu_server_destroy",
 ]
 
 
@@ -74,6 +78,12 @@ class Tuning(ctypes.Structure):
                 ("waves_per_cu", ctypes.c_int32), ("load_factor", ctypes.c_double)]
 
 
+class RequestRecord(ctypes.Structure):
+    _fields_ = [("request_id", ctypes.c_int64), ("points", ctypes.c_int64), ("map_index", ctypes.c_int32),
+                ("batch_size", ctypes.c_int32), ("status", ctypes.c_int32), ("_pad", ctypes.c_int32),
+                ("latency_ms", ctypes.c_double)]
+
+
 class VdbGpuError(RuntimeError):
     pass
 
@@ -107,6 +117,8 @@ def lib():
         for f in (L.vdbgpu_accumulate, L.vdbgpu_insert):
             f.restype = I32
             f.argtypes = [P, ctypes.c_char_p, P, I64, I64, P, I32]
+        L.vdbgpu_insert_batch.restype = I32
+        L.vdbgpu_insert_batch.argtypes = [P, ctypes.c_char_p, I32, P, P, I64, P, P, I32]
         for f in (L.vdbgpu_integrate, L.vdbgpu_reset, L.vdbgpu_empty, L.vdbgpu_synchronize):
             f.restype = I32
             f.argtypes = [P]
@@ -139,6 +151,19 @@ def lib():
         for f in (L.vdbgpu_device_upload, L.vdbgpu_device_download):
             f.restype = I32
             f.argtypes = [P, P, I64]
+        L.vdbgpu_server_create.restype = I32
+        L.vdbgpu_server_create.argtypes = [I32, D, ctypes.POINTER(Config), D, I64, I32, ctypes.POINTER(P)]
+        L.vdbgpu_server_submit.restype = I32
+        L.vdbgpu_server_submit.argtypes = [P, I32, P, I64, I64, P, ctypes.POINTER(I64)]
+        L.vdbgpu_server_drain.restype = I32
+        L.vdbgpu_server_drain.argtypes = [P]
+        L.vdbgpu_server_records.restype = I64
+        L.vdbgpu_server_records.argtypes = [P, P, I64]
+        L.vdbgpu_server_map.restype = P
+        L.vdbgpu_server_map.argtypes = [P, I32]
+        L.vdbgpu_server_device_bytes.restype = I64
+        L.vdbgpu_server_device_bytes.argtypes = [P]
+        L.vdbgpu_server_destroy.argtypes = [P]
         _lib = L
     return _lib
 
@@ -163,15 +188,19 @@ class GpuMap:
     returns bool/void; `check=True` raises VdbGpuError on non-zero codes.
     """
 
-    def __init__(self, resolution: float):
+    def __init__(self, resolution: float, _borrowed_handle=None):
         self.h = ctypes.c_void_p()
-        rc = lib().vdbgpu_create(resolution, ctypes.byref(self.h))
-        if rc != OK:
-            raise VdbGpuError(f"vdbgpu_create failed: {status_string(rc)}")
+        self._owned = _borrowed_handle is None
+        if _borrowed_handle is not None:
+            self.h = ctypes.c_void_p(_borrowed_handle)
+        else:
+            rc = lib().vdbgpu_create(resolution, ctypes.byref(self.h))
+            if rc != OK:
+                raise VdbGpuError(f"vdbgpu_create failed: {status_string(rc)}")
         self.resolution = resolution
 
     def close(self):
-        if getattr(self, "h", None) and self.h.value:
+        if getattr(self, "h", None) and self.h.value and getattr(self, "_owned", True):
             lib().vdbgpu_destroy(self.h)
             self.h = ctypes.c_void_p()
 
@@ -232,6 +261,29 @@ class GpuMap:
         return self._check(rc, check)
 
     insert_point_cloud = insert
+
+    def insert_batch(self, source_id: str, clouds, origins, max_ranges=None, check=False) -> int:
+        """Batched insertPointCloud of several host clouds ((n_i, 3|4) float32 each)."""
+        clouds = [np.ascontiguousarray(c, dtype=np.float32) for c in clouds]
+        width = clouds[0].shape[1] if clouds else 4
+        if any(c.ndim != 2 or c.shape[1] != width for c in clouds):
+            raise ValueError("all clouds must be (n, 3) or all (n, 4) float32")
+        cat = np.concatenate(clouds) if clouds else np.zeros((0, width), np.float32)
+        counts = np.array([c.shape[0] for c in clouds], np.int64)
+        o = np.ascontiguousarray(origins, dtype=np.float64).reshape(-1, 3)
+        r = None if max_ranges is None else np.ascontiguousarray(max_ranges, dtype=np.float64)
+        rc = lib().vdbgpu_insert_batch(self.h, source_id.encode(), len(clouds), _ptr(cat), _ptr(counts),
+                                       width * 4, _ptr(o), _ptr(r), 0)
+        return self._check(rc, check)
+
+    def insert_batch_device(self, source_id: str, dev_ptr: int, counts, stride_bytes: int, origins,
+                            max_ranges=None, check=False) -> int:
+        counts = np.ascontiguousarray(counts, dtype=np.int64)
+        o = np.ascontiguousarray(origins, dtype=np.float64).reshape(-1, 3)
+        r = None if max_ranges is None else np.ascontiguousarray(max_ranges, dtype=np.float64)
+        rc = lib().vdbgpu_insert_batch(self.h, source_id.encode(), len(counts), ctypes.c_void_p(dev_ptr),
+                                       _ptr(counts), stride_bytes, _ptr(o), _ptr(r), 1)
+        return self._check(rc, check)
 
     def insert_device(self, source_id: str, dev_ptr: int, n: int, stride_bytes: int, origin,
                       check=False) -> int:
@@ -363,6 +415,52 @@ class DeviceBuffer:
 
     def __del__(self):
         self.free()
+
+
+class Server:
+    """vdbgpu_server: one device map per robot stream, continuous batching per map."""
+
+    def __init__(self, n_maps: int, resolution: float, cfg: Config, max_range: float,
+                 hbm_budget_bytes: int = 0, max_batch: int = 64):
+        self.h = ctypes.c_void_p()
+        rc = lib().vdbgpu_server_create(n_maps, resolution, ctypes.byref(cfg), max_range, hbm_budget_bytes,
+                                        max_batch, ctypes.byref(self.h))
+        if rc != OK:
+            raise VdbGpuError(f"vdbgpu_server_create failed: {status_string(rc)}")
+        self.n_maps, self.resolution = n_maps, resolution
+
+    def submit(self, map_index: int, points, origin) -> int:
+        pts = np.ascontiguousarray(points, dtype=np.float32)
+        o = np.ascontiguousarray(origin, dtype=np.float64)
+        rid = ctypes.c_int64()
+        rc = lib().vdbgpu_server_submit(self.h, map_index, _ptr(pts), pts.shape[0], pts.shape[1] * 4, _ptr(o),
+                                        ctypes.byref(rid))
+        if rc != OK:
+            raise VdbGpuError(f"submit failed: {status_string(rc)}")
+        return rid.value
+
+    def drain(self) -> int:
+        return lib().vdbgpu_server_drain(self.h)
+
+    def records(self):
+        n = lib().vdbgpu_server_records(self.h, None, 0)
+        arr = (RequestRecord * max(n, 1))()
+        got = lib().vdbgpu_server_records(self.h, arr, n)
+        return [{k: getattr(arr[i], k) for k, _ in RequestRecord._fields_ if k != "_pad"} for i in range(got)]
+
+    def map(self, i: int) -> "GpuMap":
+        return GpuMap(self.resolution, _borrowed_handle=lib().vdbgpu_server_map(self.h, i))
+
+    def device_bytes(self) -> int:
+        return int(lib().vdbgpu_server_device_bytes(self.h))
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            lib().vdbgpu_server_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        self.close()
 
 
 def device_count() -> int:

@@ -23,7 +23,12 @@ namespace vdbgpu {
 constexpr uint64_t kEmpty = ~0ull;
 constexpr uint32_t kNone = 0xFFFFFFFFu;
 constexpr int kCoordLimit = 1 << 23;  // voxel coords in [-2^23, 2^23): 21-bit leaf coords
-constexpr uint32_t kHitBit = 0x80000000u;
+constexpr uint32_t kHitBit = 0x80000000u;   // rayinfo.w: endpoint is a hit
+constexpr uint32_t kCntMask = 0xFFFFu;      // rayinfo.w bits 0-15: visit count (<= 1 + 3 * 8192)
+constexpr int kScanShift = 16;              // rayinfo.w bits 16-22: scan index in a batch
+constexpr uint32_t kScanMask = 0x7Fu;
+constexpr int kMaxBatchScans = 128;
+constexpr int kBatchCoordLimit = 1 << 21;   // batched keys: 19-bit leaf coords + 7-bit scan
 constexpr int kPrepBlock = 256;
 constexpr int kMaxQuerySteps = 1 << 24;  // per-axis voxel extent bound of a raytrace query
 
@@ -32,6 +37,7 @@ constexpr uint32_t kErrRange = 1u;  // a voxel index out of range / non-finite
 constexpr uint32_t kErrTable = 2u;  // crossing table or chunk table too short
 constexpr uint32_t kErrEntry = 4u;  // mid-ray DDA entry did not converge
 constexpr uint32_t kOvfHard = 1u;   // update-set probe found no free slot
+constexpr uint32_t kMapFull = 1u;   // Status::map_full: the fold needs a larger leaf pool / hash
 
 struct Status {
   unsigned long long total;     // visits of the scan (sum of counts)
@@ -41,13 +47,40 @@ struct Status {
   unsigned int overflow;
   unsigned int entries;         // update-set leaves (compact)
   unsigned int leaf_count;      // map leaves (map insert)
-  int s[3];                     // origin voxel of the scan
+  unsigned int groups;          // leaves of a batch (fold groups)
+  unsigned int map_full;
   unsigned int pad;
+};
+
+// One scan of a batch: sensor origin, max range, its rays [ray0, ray0 + nrays)
+// of the concatenated cloud; s = rounded origin voxel (written by k_prep).
+struct ScanInfo {
+  double o[3];
+  double R;
+  uint32_t ray0, nrays;
+  int s[3];
+  int pad;
+};
+
+// Sizes read by the kernels (device memory, so a captured graph can be replayed).
+struct BatchCtl {
+  uint32_t n_rays;
+  uint32_t nb;       // 256-ray blocks
+  uint32_t n_scans;
+  uint32_t batched;  // 1: update-set keys are (leaf, scan)
 };
 
 struct UpdView {
   unsigned long long* keys;
   unsigned long long* masks;  // 16 words per slot
+  uint32_t cap_mask;
+};
+
+// Batch fold groups: leaf (57-bit batched key) -> 128-bit scan mask + create flag.
+struct GroupView {
+  unsigned long long* keys;
+  unsigned long long* scans;  // 2 words per slot
+  unsigned int* need;
   uint32_t cap_mask;
 };
 
@@ -97,6 +130,24 @@ __host__ __device__ inline uint64_t fmix64(uint64_t k) {
 }
 
 __device__ inline bool coord_ok(int c) { return c >= -kCoordLimit && c < kCoordLimit; }
+
+// Update-set key of the leaf holding (x,y,z): the 63-bit leaf key, or in a
+// batch the 57-bit leaf key (19-bit coords) shifted left 7 with the scan index.
+__host__ __device__ inline uint64_t update_key(int x, int y, int z, uint32_t scan, bool batched) {
+  if (!batched) return leaf_key(x, y, z);
+  const uint64_t m = (1ull << 19) - 1;
+  return (((uint64_t)(uint32_t)(x >> 3) & m) << 45) | (((uint64_t)(uint32_t)(y >> 3) & m) << 26) |
+         (((uint64_t)(uint32_t)(z >> 3) & m) << 7) | scan;
+}
+
+// Canonical 63-bit leaf key of a batched update key (scan bits dropped).
+__host__ __device__ inline uint64_t batched_to_leaf_key(uint64_t k) {
+  const uint64_t m = (1ull << 19) - 1;
+  const int lx = (int)((int64_t)(((k >> 45) & m) << 45) >> 45);
+  const int ly = (int)((int64_t)(((k >> 26) & m) << 45) >> 45);
+  const int lz = (int)((int64_t)(((k >> 7) & m) << 45) >> 45);
+  return leaf_key(lx * 8, ly * 8, lz * 8);
+}
 
 // Key of the internal node (2^log2dim voxels per side) holding voxel (x,y,z).
 __host__ __device__ inline uint64_t node_key(int x, int y, int z, int log2dim) {

@@ -92,6 +92,7 @@ __device__ __forceinline__ int lds_slot(unsigned long long* lkey, uint64_t key) 
 // the global update set when the LDS table is full.
 struct Run {
   int lx, ly, lz;
+  uint32_t lb;   // scan of the current leaf entry (batched keys)
   int ls;        // LDS entry of the current leaf, -1 = global
   uint32_t gs;   // global slot when ls < 0
   int word;
@@ -99,6 +100,7 @@ struct Run {
   __device__ void init() {
     lx = INT_MIN;
     ly = lz = 0;
+    lb = 0;
     ls = -1;
     gs = kNone;
     word = 0;
@@ -113,11 +115,12 @@ struct Run {
     bits = 0;
   }
   __device__ __forceinline__ void add(const UpdView& u, unsigned int* ovf, unsigned long long* lkey,
-                                      unsigned long long* lvis, int x, int y, int z) {
+                                      unsigned long long* lvis, int x, int y, int z, uint32_t scan,
+                                      bool batched) {
     const int w = x & 7;
     const uint64_t b = 1ull << (((y & 7) << 3) | (z & 7));
     const int nx = x >> 3, ny = y >> 3, nz = z >> 3;
-    if (nx == lx && ny == ly && nz == lz) {
+    if (nx == lx && ny == ly && nz == lz && scan == lb) {
       if (w == word) {
         bits |= b;
         return;
@@ -128,7 +131,8 @@ struct Run {
       lx = nx;
       ly = ny;
       lz = nz;
-      const uint64_t key = leaf_key(x, y, z);
+      lb = scan;
+      const uint64_t key = update_key(x, y, z, scan, batched);
       ls = lds_slot(lkey, key);
       if (ls < 0) gs = upd_slot(u, key, ovf);
     }
@@ -283,17 +287,35 @@ __global__ void k_build_table(double* __restrict__ tbl, uint32_t n_lo, uint32_t 
 
 template <bool kVec4>
 __global__ __launch_bounds__(kPrepBlock) void k_prep(const float* __restrict__ pts,
-                                                     int64_t stride_f, uint32_t n, double ox,
-                                                     double oy, double oz, double R, double res,
+                                                     int64_t stride_f, ScanInfo* __restrict__ scans,
+                                                     const BatchCtl* __restrict__ ctl, double res,
                                                      double inv, uint32_t tbl_n,
                                                      int4* __restrict__ rayinfo,
                                                      uint32_t* __restrict__ pre,
                                                      uint32_t* __restrict__ bsum, Status* st) {
   __shared__ uint32_t wsum[kPrepBlock / 64];
+  const uint32_t n = ctl->n_rays;
+  if (blockIdx.x * kPrepBlock >= n) return;  // whole workgroup beyond the rays
+  const uint32_t nscans = ctl->n_scans;
+  const int lim = ctl->batched ? kBatchCoordLimit : kCoordLimit;
   const uint32_t i = blockIdx.x * kPrepBlock + threadIdx.x;
   uint32_t cnt = 0;
   bool valid = false;
   if (i < n) {
+    // scan of ray i: the last scan whose first ray is <= i
+    uint32_t b = 0;
+    if (nscans > 1) {
+      uint32_t lo = 0, hi = nscans - 1;
+      while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (scans[mid].ray0 <= i)
+          lo = mid;
+        else
+          hi = mid - 1;
+      }
+      b = lo;
+    }
+    const double ox = scans[b].o[0], oy = scans[b].o[1], oz = scans[b].o[2], R = scans[b].R;
     double px, py, pz;
     if (kVec4 && i + 1 < n) {  // the last point may end 4 bytes before a 16-byte boundary
       const float4 p = reinterpret_cast<const float4*>(pts)[i];
@@ -306,8 +328,11 @@ __global__ __launch_bounds__(kPrepBlock) void k_prep(const float* __restrict__ p
       py = p[1];
       pz = p[2];
     }
-    int4 ri = make_int4(0, 0, 0, 0);
-    valid = !(isnan(px) || isnan(py) || isnan(pz) || isnan(ox) || isnan(oy) || isnan(oz));
+    int4 ri = make_int4(0, 0, 0, (int)(b << kScanShift));
+    // a scan with max range <= 0 casts nothing (accumulateUpdate's gate, VDBMapping.hpp:331)
+    valid = R > 0.0 && !(isnan(px) || isnan(py) || isnan(pz) || isnan(ox) || isnan(oy) || isnan(oz));
+    const int sx = world_to_index1(ox, res, inv), sy = world_to_index1(oy, res, inv),
+              sz = world_to_index1(oz, res, inv);
     if (valid) {
       bool trunc = false;
       const double dx = px - ox, dy = py - oy, dz = pz - oz;
@@ -318,26 +343,27 @@ __global__ __launch_bounds__(kPrepBlock) void k_prep(const float* __restrict__ p
         pz = oz + (dz / len) * R;
         trunc = true;
       }
-      const int sx = world_to_index1(ox, res, inv), sy = world_to_index1(oy, res, inv),
-                sz = world_to_index1(oz, res, inv);
       const int ex = world_to_index1(px, res, inv), ey = world_to_index1(py, res, inv),
                 ez = world_to_index1(pz, res, inv);
-      if (!(coord_ok(sx) && coord_ok(sy) && coord_ok(sz) && coord_ok(ex) && coord_ok(ey) &&
-            coord_ok(ez))) {
+      if (!(sx >= -lim && sx < lim && sy >= -lim && sy < lim && sz >= -lim && sz < lim &&
+            ex >= -lim && ex < lim && ey >= -lim && ey < lim && ez >= -lim && ez < lim)) {
         atomicOr(&st->error, kErrRange);
       } else {
         const uint32_t nx = (uint32_t)abs(ex - sx), ny = (uint32_t)abs(ey - sy),
                        nz = (uint32_t)abs(ez - sz);
-        if (nx > tbl_n || ny > tbl_n || nz > tbl_n) atomicOr(&st->error, kErrTable);
-        cnt = (nx | ny | nz) ? 1u + nx + ny + nz : 0u;
-        ri = make_int4(ex, ey, ez, (int)(cnt | (trunc ? 0u : kHitBit)));
+        if (nx > tbl_n || ny > tbl_n || nz > tbl_n) {
+          atomicOr(&st->error, kErrTable);
+        } else {
+          cnt = (nx | ny | nz) ? 1u + nx + ny + nz : 0u;
+          ri = make_int4(ex, ey, ez, (int)(cnt | (b << kScanShift) | (trunc ? 0u : kHitBit)));
+        }
       }
     }
     rayinfo[i] = ri;
-    if (i == 0) {
-      st->s[0] = world_to_index1(ox, res, inv);
-      st->s[1] = world_to_index1(oy, res, inv);
-      st->s[2] = world_to_index1(oz, res, inv);
+    if (i == scans[b].ray0) {
+      scans[b].s[0] = sx;
+      scans[b].s[1] = sy;
+      scans[b].s[2] = sz;
     }
   }
   // valid-ray count, one atomic per wave
@@ -361,9 +387,11 @@ __global__ __launch_bounds__(kPrepBlock) void k_prep(const float* __restrict__ p
 }
 
 __global__ __launch_bounds__(1024) void k_scan_blocks(const uint32_t* __restrict__ bsum,
-                                                      uint64_t* __restrict__ boff, uint32_t nb,
+                                                      uint64_t* __restrict__ boff,
+                                                      const BatchCtl* __restrict__ ctl,
                                                       Status* st) {
   __shared__ unsigned long long wsum[16];
+  const uint32_t nb = ctl->nb;
   unsigned long long carry = 0;
   for (uint32_t base = 0; base < nb; base += 1024) {
     const uint32_t i = base + threadIdx.x;
@@ -393,13 +421,14 @@ __global__ __launch_bounds__(1024) void k_scan_blocks(const uint32_t* __restrict
 // the ray holding the chunk's first visit.
 __global__ __launch_bounds__(256) void k_ray_offsets(const int4* __restrict__ rayinfo,
                                                      const uint32_t* __restrict__ pre,
-                                                     const uint64_t* __restrict__ boff, uint32_t n,
+                                                     const uint64_t* __restrict__ boff,
+                                                     const BatchCtl* __restrict__ ctl,
                                                      uint32_t chunk, uint64_t* __restrict__ roff,
                                                      uint32_t* __restrict__ first_ray,
                                                      uint64_t max_chunks, Status* st) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const uint32_t cnt = (uint32_t)rayinfo[i].w & ~kHitBit;
+  if (i >= ctl->n_rays) return;
+  const uint32_t cnt = (uint32_t)rayinfo[i].w & kCntMask;
   const uint64_t off = boff[i / kPrepBlock] + pre[i] - cnt;
   roff[i] = off;
   if (cnt == 0) return;
@@ -414,8 +443,10 @@ __global__ __launch_bounds__(256) void k_ray_offsets(const int4* __restrict__ ra
 __global__ __launch_bounds__(256) void k_traverse(const int4* __restrict__ rayinfo,
                                                   const uint64_t* __restrict__ roff,
                                                   const uint32_t* __restrict__ first_ray,
-                                                  uint32_t n, const double* __restrict__ tbl,
-                                                  UpdView u, Status* st, uint32_t chunk) {
+                                                  const ScanInfo* __restrict__ scans,
+                                                  const BatchCtl* __restrict__ ctl,
+                                                  const double* __restrict__ tbl, UpdView u,
+                                                  Status* st, uint32_t chunk) {
   __shared__ unsigned long long lkey[kLdsEntries];
   __shared__ unsigned long long lvis[kLdsEntries * 8];
   __shared__ unsigned int s_stop;
@@ -425,7 +456,8 @@ __global__ __launch_bounds__(256) void k_traverse(const int4* __restrict__ rayin
   __syncthreads();
   if (s_stop) return;
   const uint64_t S = st->total;
-  const int sx = st->s[0], sy = st->s[1], sz = st->s[2];
+  const uint32_t n = ctl->n_rays;
+  const bool batched = ctl->batched != 0;
   const uint64_t G = (uint64_t)gridDim.x * blockDim.x;
   const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   unsigned int* ovf = &st->overflow;
@@ -439,7 +471,8 @@ __global__ __launch_bounds__(256) void k_traverse(const int4* __restrict__ rayin
   for (uint64_t i = gid; i < n; i += G) {
     const int4 r = rayinfo[i];
     if ((uint32_t)r.w & kHitBit) {
-      const uint32_t slot = upd_slot(u, leaf_key(r.x, r.y, r.z), ovf);
+      const uint32_t sc = ((uint32_t)r.w >> kScanShift) & kScanMask;
+      const uint32_t slot = upd_slot(u, update_key(r.x, r.y, r.z, sc, batched), ovf);
       const int w = r.x & 7;
       const uint64_t b = 1ull << (((r.y & 7) << 3) | (r.z & 7));
       upd_or(u, slot, w, b);
@@ -465,15 +498,24 @@ __global__ __launch_bounds__(256) void k_traverse(const int4* __restrict__ rayin
       uint32_t k = (uint32_t)(g - roff[i]);
       Run run;
       run.init();
+      uint32_t cur_scan = kNone;
+      int sx = 0, sy = 0, sz = 0;
       while (g < gend && i < n) {
         const int4 r = rayinfo[i];
-        const uint32_t cnt = (uint32_t)r.w & ~kHitBit;
+        const uint32_t cnt = (uint32_t)r.w & kCntMask;
         if (k >= cnt) {
           ++i;
           k = 0;
           continue;
         }
         const uint32_t m = (uint32_t)min((uint64_t)(cnt - k), gend - g);
+        const uint32_t sc = ((uint32_t)r.w >> kScanShift) & kScanMask;
+        if (sc != cur_scan) {
+          cur_scan = sc;
+          sx = scans[sc].s[0];
+          sy = scans[sc].s[1];
+          sz = scans[sc].s[2];
+        }
         const uint32_t nn[3] = {(uint32_t)abs(r.x - sx), (uint32_t)abs(r.y - sy),
                                 (uint32_t)abs(r.z - sz)};
         const int stx = r.x > sx ? 1 : (r.x < sx ? -1 : 0);
@@ -488,7 +530,7 @@ __global__ __launch_bounds__(256) void k_traverse(const int4* __restrict__ rayin
         d.init(nn, cc);
         int vx = sx + stx * (int)cc[0], vy = sy + sty * (int)cc[1], vz = sz + stz * (int)cc[2];
         for (uint32_t v = 0;;) {
-          run.add(u, ovf, lkey, lvis, vx, vy, vz);
+          run.add(u, ovf, lkey, lvis, vx, vy, vz, sc, batched);
           if (++v == m) break;
           const int a = d.next_axis(tbl);
           d.advance(a);
@@ -548,14 +590,16 @@ __device__ __forceinline__ uint32_t block_append(bool valid, unsigned int* count
 
 // List the occupied update-set slots (keys only: coalesced 8-byte reads).
 __global__ __launch_bounds__(256) void k_compact(UpdView u, uint32_t cap,
-                                                 uint32_t* __restrict__ list, Status* st) {
+                                                 uint32_t* __restrict__ list, unsigned int* counter,
+                                                 const Status* st) {
   __shared__ uint32_t s_wave[4], s_base;
+  if (st->error != 0 || st->overflow != 0) return;  // same for every thread
   const uint32_t G = gridDim.x * blockDim.x;
   const uint32_t rounds = (cap + G - 1) / G;
   for (uint32_t rd = 0; rd < rounds; ++rd) {
     const uint32_t slot = rd * G + blockIdx.x * blockDim.x + threadIdx.x;
     const bool valid = slot < cap && u.keys[slot] != kEmpty;
-    const uint32_t pos = block_append(valid, &st->entries, s_wave, &s_base);
+    const uint32_t pos = block_append(valid, counter, s_wave, &s_base);
     if (valid) list[pos] = slot;
   }
 }
@@ -775,45 +819,45 @@ hipError_t launch_build_table(hipStream_t s, double* tbl, uint32_t n_lo, uint32_
   return hipGetLastError();
 }
 
-hipError_t launch_prep(hipStream_t s, const float* pts, int64_t stride_bytes, uint32_t n,
-                       const double o[3], double R, double res, double inv, uint32_t tbl_n,
-                       int4* rayinfo, uint32_t* pre, uint32_t* bsum, Status* st) {
-  const uint32_t nb = (n + kPrepBlock - 1) / kPrepBlock;
-  const bool vec4 = stride_bytes == 16 && (reinterpret_cast<uintptr_t>(pts) & 15) == 0;
+hipError_t launch_prep(hipStream_t s, int grid, const float* pts, int64_t stride_bytes, bool vec4,
+                       ScanInfo* scans, const BatchCtl* ctl, double res, double inv,
+                       uint32_t tbl_n, int4* rayinfo, uint32_t* pre, uint32_t* bsum, Status* st) {
   if (vec4)
-    hipLaunchKernelGGL(k_prep<true>, dim3(nb), dim3(kPrepBlock), 0, s, pts, stride_bytes / 4, n,
-                       o[0], o[1], o[2], R, res, inv, tbl_n, rayinfo, pre, bsum, st);
+    hipLaunchKernelGGL(k_prep<true>, dim3(grid), dim3(kPrepBlock), 0, s, pts, stride_bytes / 4,
+                       scans, ctl, res, inv, tbl_n, rayinfo, pre, bsum, st);
   else
-    hipLaunchKernelGGL(k_prep<false>, dim3(nb), dim3(kPrepBlock), 0, s, pts, stride_bytes / 4, n,
-                       o[0], o[1], o[2], R, res, inv, tbl_n, rayinfo, pre, bsum, st);
+    hipLaunchKernelGGL(k_prep<false>, dim3(grid), dim3(kPrepBlock), 0, s, pts, stride_bytes / 4,
+                       scans, ctl, res, inv, tbl_n, rayinfo, pre, bsum, st);
   return hipGetLastError();
 }
 
-hipError_t launch_scan_blocks(hipStream_t s, const uint32_t* bsum, uint64_t* boff, uint32_t nb,
+hipError_t launch_scan_blocks(hipStream_t s, const uint32_t* bsum, uint64_t* boff,
+                              const BatchCtl* ctl, Status* st) {
+  hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, s, bsum, boff, ctl, st);
+  return hipGetLastError();
+}
+
+hipError_t launch_ray_offsets(hipStream_t s, int grid, const int4* rayinfo, const uint32_t* pre,
+                              const uint64_t* boff, const BatchCtl* ctl, uint32_t chunk,
+                              uint64_t* roff, uint32_t* first_ray, uint64_t max_chunks,
                               Status* st) {
-  hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, s, bsum, boff, nb, st);
-  return hipGetLastError();
-}
-
-hipError_t launch_ray_offsets(hipStream_t s, const int4* rayinfo, const uint32_t* pre,
-                              const uint64_t* boff, uint32_t n, uint32_t chunk, uint64_t* roff,
-                              uint32_t* first_ray, uint64_t max_chunks, Status* st) {
-  hipLaunchKernelGGL(k_ray_offsets, dim3((n + 255) / 256), dim3(256), 0, s, rayinfo, pre, boff, n,
-                     chunk, roff, first_ray, max_chunks, st);
+  hipLaunchKernelGGL(k_ray_offsets, dim3(grid), dim3(256), 0, s, rayinfo, pre, boff, ctl, chunk, roff,
+                     first_ray, max_chunks, st);
   return hipGetLastError();
 }
 
 hipError_t launch_traverse(hipStream_t s, int grid, int block, const int4* rayinfo,
-                           const uint64_t* roff, const uint32_t* first_ray, uint32_t n,
-                           const double* tbl, UpdView u, Status* st, uint32_t chunk) {
-  hipLaunchKernelGGL(k_traverse, dim3(grid), dim3(block), 0, s, rayinfo, roff, first_ray, n, tbl, u,
-                     st, chunk);
+                           const uint64_t* roff, const uint32_t* first_ray, const ScanInfo* scans,
+                           const BatchCtl* ctl, const double* tbl, UpdView u, Status* st,
+                           uint32_t chunk) {
+  hipLaunchKernelGGL(k_traverse, dim3(grid), dim3(block), 0, s, rayinfo, roff, first_ray, scans, ctl,
+                     tbl, u, st, chunk);
   return hipGetLastError();
 }
 
 hipError_t launch_compact(hipStream_t s, int grid, UpdView u, uint32_t cap, uint32_t* list,
-                          Status* st) {
-  hipLaunchKernelGGL(k_compact, dim3(grid), dim3(256), 0, s, u, cap, list, st);
+                          unsigned int* counter, const Status* st) {
+  hipLaunchKernelGGL(k_compact, dim3(grid), dim3(256), 0, s, u, cap, list, counter, st);
   return hipGetLastError();
 }
 

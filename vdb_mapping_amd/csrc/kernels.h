@@ -8,19 +8,34 @@
 namespace vdbgpu {
 
 hipError_t launch_build_table(hipStream_t s, double* tbl, uint32_t n_lo, uint32_t n_hi);
-hipError_t launch_prep(hipStream_t s, const float* pts, int64_t stride_bytes, uint32_t n,
-                       const double o[3], double R, double res, double inv, uint32_t tbl_n,
-                       int4* rayinfo, uint32_t* pre, uint32_t* bsum, Status* st);
-hipError_t launch_scan_blocks(hipStream_t s, const uint32_t* bsum, uint64_t* boff, uint32_t nb,
+hipError_t launch_prep(hipStream_t s, int grid, const float* pts, int64_t stride_bytes, bool vec4,
+                       ScanInfo* scans, const BatchCtl* ctl, double res, double inv,
+                       uint32_t tbl_n, int4* rayinfo, uint32_t* pre, uint32_t* bsum, Status* st);
+hipError_t launch_scan_blocks(hipStream_t s, const uint32_t* bsum, uint64_t* boff,
+                              const BatchCtl* ctl, Status* st);
+hipError_t launch_ray_offsets(hipStream_t s, int grid, const int4* rayinfo, const uint32_t* pre,
+                              const uint64_t* boff, const BatchCtl* ctl, uint32_t chunk,
+                              uint64_t* roff, uint32_t* first_ray, uint64_t max_chunks,
                               Status* st);
-hipError_t launch_ray_offsets(hipStream_t s, const int4* rayinfo, const uint32_t* pre,
-                              const uint64_t* boff, uint32_t n, uint32_t chunk, uint64_t* roff,
-                              uint32_t* first_ray, uint64_t max_chunks, Status* st);
 hipError_t launch_traverse(hipStream_t s, int grid, int block, const int4* rayinfo,
-                           const uint64_t* roff, const uint32_t* first_ray, uint32_t n,
-                           const double* tbl, UpdView u, Status* st, uint32_t chunk);
+                           const uint64_t* roff, const uint32_t* first_ray, const ScanInfo* scans,
+                           const BatchCtl* ctl, const double* tbl, UpdView u, Status* st,
+                           uint32_t chunk);
 hipError_t launch_compact(hipStream_t s, int grid, UpdView u, uint32_t cap, uint32_t* list,
-                          Status* st);
+                          unsigned int* counter, const Status* st);
+hipError_t launch_group_build(hipStream_t s, int grid, UpdView u, const uint32_t* list, GroupView gv,
+                              bool create_hit, bool create_miss, const Status* st);
+hipError_t launch_map_check(hipStream_t s, Status* st, const Status* stmap, uint32_t lcap,
+                            uint32_t mcap);
+hipError_t launch_group_insert(hipStream_t s, int grid, GroupView gv, const uint32_t* glist, MapView mp,
+                               uint32_t* gidx, unsigned long long* noop_set, uint32_t noop_mask,
+                               const Status* st, Status* stmap);
+hipError_t launch_fold(hipStream_t s, int grid, UpdView u, GroupView gv, const uint32_t* glist,
+                       const uint32_t* gidx, MapView mp, const float lo[6], const Status* st,
+                       Status* stcount);
+hipError_t launch_clear_keys(hipStream_t s, int grid, UpdView u, const uint32_t* list,
+                             const Status* st);
+hipError_t launch_zero_words(hipStream_t s, unsigned long long* p, uint32_t n);
 hipError_t launch_map_insert(hipStream_t s, int grid, UpdView u, const uint32_t* list,
                              uint32_t count, MapView mp, uint32_t* eidx, bool create_hit,
                              bool create_miss, unsigned long long* noop_set, uint32_t noop_mask,

@@ -91,6 +91,17 @@ struct vdbgpu_map {
   uint32_t rt_leaf_cap = 0, rt_n128_cap = 0, rt_n4096_cap = 0;
   int* d_bbox = nullptr;
   // raytrace query staging
+  // batch: scan table, control block, fold groups
+  ScanInfo* d_scans = nullptr;
+  BatchCtl* d_ctl = nullptr;
+  ScanInfo* h_scans = nullptr;
+  BatchCtl* h_ctl = nullptr;
+  unsigned long long* gkeys = nullptr;
+  unsigned long long* gscans = nullptr;
+  unsigned int* gneed = nullptr;
+  uint32_t* glist = nullptr;
+  uint32_t* gidx = nullptr;
+  uint32_t gcap = 0;
   double* q_in = nullptr;
   uint8_t* q_succ = nullptr;
   double* q_end = nullptr;
@@ -356,32 +367,82 @@ int fail(vdbgpu_map* m, int code, const std::string& msg) {
   return code;
 }
 
-int do_accumulate(vdbgpu_map* m, Source& src, const float* xyz, int64_t n, int64_t stride,
-                  const double origin[3], bool on_device) {
-  if (!m->config_set) return fail(m, VDBGPU_NOT_CONFIGURED, "Map not properly configured. Did you call setConfig method?");
-  if (m->fast_mode) return fail(m, VDBGPU_UNSUPPORTED, "fast_mode raycasting is not provided by this library");
-  if (n < 0 || stride < 12 || (stride & 3) || n > (int64_t)kMaxRays || !origin || (n && !xyz))
-    return fail(m, VDBGPU_BAD_ARGUMENT, "bad point buffer arguments");
-  if (n == 0) return VDBGPU_OK;
-  const double R = src.max_range;
-  const uint32_t nn = (uint32_t)n;
+void create_flags(const vdbgpu_map* m, bool* ch, bool* cm);
+
+int ensure_groups(vdbgpu_map* m, uint32_t cap) {
+  if (cap <= m->gcap) return VDBGPU_OK;
+  dfree(m, &m->gkeys, m->gcap);
+  dfree(m, &m->gscans, (size_t)m->gcap * 2);
+  dfree(m, &m->gneed, m->gcap);
+  dfree(m, &m->glist, m->gcap);
+  dfree(m, &m->gidx, m->gcap);
+  m->gcap = cap;
+  CK(dalloc(m, &m->gkeys, cap));
+  CK(dalloc(m, &m->gscans, (size_t)cap * 2));
+  CK(dalloc(m, &m->gneed, cap));
+  CK(dalloc(m, &m->glist, cap));
+  CK(dalloc(m, &m->gidx, cap));
+  CK(launch_fill_u64(m->stream, m->gkeys, cap, kEmpty));
+  CK(hipMemsetAsync(m->gscans, 0, (size_t)cap * 16, m->stream));
+  CK(hipMemsetAsync(m->gneed, 0, (size_t)cap * 4, m->stream));
+  return VDBGPU_OK;
+}
+
+GroupView group_view(vdbgpu_map* m) { return GroupView{m->gkeys, m->gscans, m->gneed, m->gcap - 1}; }
+
+int fold_grid(vdbgpu_map* m) { return m->num_cu * 8; }
+
+// Fold half of a batch: group the (leaf, scan) entries by leaf, make room in
+// the map, apply every scan in order, clear the update set.
+int launch_fold_stage(vdbgpu_map* m, Source& src, bool from_groups) {
+  bool ch, cm;
+  create_flags(m, &ch, &cm);
+  Status* st = &m->d_st[0];
+  if (from_groups) {
+    CK(launch_group_build(m->stream, fold_grid(m), upd_view(src), src.list, group_view(m), ch, cm, st));
+    CK(launch_compact(m->stream, std::min<int>(4096, (int)((m->gcap + 255) / 256)),
+                      UpdView{m->gkeys, nullptr, m->gcap - 1}, m->gcap, m->glist, &st->groups, st));
+  }
+  CK(launch_map_check(m->stream, st, &m->d_st[1], m->lcap, m->mcap));
+  CK(launch_group_insert(m->stream, fold_grid(m), group_view(m), m->glist, map_view(m), m->gidx,
+                         m->noop4096, vdbgpu_map::kNoopCap - 1, st, &m->d_st[1]));
+  CK(launch_fold(m->stream, fold_grid(m), upd_view(src), group_view(m), m->glist, m->gidx, map_view(m),
+                 m->lo, st, &m->d_st[3]));
+  CK(launch_clear_keys(m->stream, fold_grid(m), upd_view(src), src.list, st));
+  return VDBGPU_OK;
+}
+
+// Casts `scans` (rays concatenated in `xyz`) into `src`'s update set: prep,
+// visit-count scan, ray offsets, traversal, compaction. A batch (batched =
+// true, keys carry the scan index) continues with the scan-ordered fold into
+// the map. One status read at the end; an update-set overflow grows the set
+// and re-runs the traversal (marks are idempotent), a full map grows the leaf
+// pool and re-runs the fold (nothing has been applied before either check).
+int run_pipeline(vdbgpu_map* m, Source& src, const float* xyz, int64_t stride, bool on_device,
+                 const std::vector<ScanInfo>& scans, uint32_t n_total, bool batched) {
   int r;
-  if ((r = ensure_rays(m, nn))) return r;
-  const double ext = std::ceil(R * m->inv) + 2.0;
-  const uint32_t ext_n = ext < (double)kMaxTableN ? (uint32_t)ext : kMaxTableN;
-  if ((r = ensure_table(m, ext_n))) return r;
+  if ((r = ensure_rays(m, n_total))) return r;
+  uint32_t ext_max = 1;
+  uint64_t visits_bound = 0;
+  for (const ScanInfo& sc : scans) {
+    const double ext = std::ceil(sc.R * m->inv) + 2.0;
+    const uint32_t e = ext < (double)kMaxTableN ? (uint32_t)ext : kMaxTableN;
+    ext_max = std::max(ext_max, e);
+    visits_bound += (uint64_t)sc.nrays * (1 + 3ull * e);  // visits per ray <= 1 + 3 * extent
+  }
+  if ((r = ensure_table(m, ext_max))) return r;
   const uint32_t chunk = (uint32_t)m->tuning.steps_per_lane;
-  // visits per ray <= 1 + 3 * (per-axis extent)
-  const uint64_t max_chunks = ((uint64_t)nn * (1 + 3ull * ext_n) + chunk - 1) / chunk + 1;
+  const uint64_t max_chunks = (visits_bound + chunk - 1) / chunk + 1;
   if (max_chunks > (1ull << 29))
     return fail(m, VDBGPU_BAD_ARGUMENT, "steps_per_lane too small for this scan (chunk table > 2 GiB)");
   if ((r = ensure_chunks(m, max_chunks))) return r;
   if (src.cap == 0) {
-    if ((r = alloc_update(m, src, (uint32_t)std::max<uint64_t>(kMinUpdateCap, next_pow2(4 * (uint64_t)nn))))) return r;
+    if ((r = alloc_update(m, src, (uint32_t)std::max<uint64_t>(kMinUpdateCap, next_pow2(4 * (uint64_t)n_total))))) return r;
   }
+  if (batched && (r = ensure_groups(m, src.cap))) return r;
   const float* pts = xyz;
   if (!on_device) {
-    const size_t bytes = (size_t)(n - 1) * stride + 12;
+    const size_t bytes = (size_t)(n_total - 1) * stride + 12;
     if (bytes > m->points_bytes) {
       dfree(m, &m->d_points, m->points_bytes / 4);
       const size_t cap = std::max(bytes, m->points_bytes * 3 / 2) + 16;
@@ -391,60 +452,110 @@ int do_accumulate(vdbgpu_map* m, Source& src, const float* xyz, int64_t n, int64
     CK(hipMemcpyAsync(m->d_points, xyz, bytes, hipMemcpyHostToDevice, m->stream));
     pts = m->d_points;
   }
-  const uint32_t nb = (nn + kPrepBlock - 1) / kPrepBlock;
+  const uint32_t nb = (n_total + kPrepBlock - 1) / kPrepBlock;
+  std::copy(scans.begin(), scans.end(), m->h_scans);
+  *m->h_ctl = BatchCtl{n_total, nb, (uint32_t)scans.size(), batched ? 1u : 0u};
+  CK(hipMemcpyAsync(m->d_scans, m->h_scans, scans.size() * sizeof(ScanInfo), hipMemcpyHostToDevice, m->stream));
+  CK(hipMemcpyAsync(m->d_ctl, m->h_ctl, sizeof(BatchCtl), hipMemcpyHostToDevice, m->stream));
   CK(hipMemsetAsync(&m->d_st[0], 0, sizeof(Status), m->stream));
+  const bool vec4 = stride == 16 && (reinterpret_cast<uintptr_t>(pts) & 15) == 0;
   {
     Timer t(m, kPrep);
-    CK(launch_prep(m->stream, pts, stride, nn, origin, R, m->res, m->inv, m->tbl_n, m->rayinfo,
-                   m->pre, m->bsum, &m->d_st[0]));
-    CK(launch_scan_blocks(m->stream, m->bsum, m->boff, nb, &m->d_st[0]));
-    CK(launch_ray_offsets(m->stream, m->rayinfo, m->pre, m->boff, nn, chunk, m->roff, m->first_ray,
-                          m->chunk_cap, &m->d_st[0]));
+    CK(launch_prep(m->stream, (int)nb, pts, stride, vec4, m->d_scans, m->d_ctl, m->res, m->inv, m->tbl_n,
+                   m->rayinfo, m->pre, m->bsum, &m->d_st[0]));
+    CK(launch_scan_blocks(m->stream, m->bsum, m->boff, m->d_ctl, &m->d_st[0]));
+    CK(launch_ray_offsets(m->stream, (int)nb, m->rayinfo, m->pre, m->boff, m->d_ctl, chunk, m->roff,
+                          m->first_ray, m->chunk_cap, &m->d_st[0]));
   }
   const int block = m->tuning.block;
+  bool fold_only = false;
   for (int attempt = 0;; ++attempt) {
-    {
-      Timer t(m, kTraverse);
-      CK(launch_traverse(m->stream, traverse_grid(m, block), block, m->rayinfo, m->roff, m->first_ray, nn,
-                         m->tbl, upd_view(src), &m->d_st[0], chunk));
+    if (!fold_only) {
+      {
+        Timer t(m, kTraverse);
+        CK(launch_traverse(m->stream, traverse_grid(m, block), block, m->rayinfo, m->roff, m->first_ray,
+                           m->d_scans, m->d_ctl, m->tbl, upd_view(src), &m->d_st[0], chunk));
+      }
+      {
+        Timer t(m, kCompact);
+        CK(launch_compact(m->stream, std::min<int>(4096, (int)((src.cap + 255) / 256)), upd_view(src),
+                          src.cap, src.list, &m->d_st[0].entries, &m->d_st[0]));
+      }
     }
-    {
-      Timer t(m, kCompact);
-      CK(launch_compact(m->stream, std::min<int>(4096, (int)((src.cap + 255) / 256)), upd_view(src),
-                        src.cap, src.list, &m->d_st[0]));
+    if (batched) {
+      Timer t(m, kApply);
+      if ((r = launch_fold_stage(m, src, !fold_only))) return r;
     }
     if ((r = read_status(m))) return r;
     const Status& st = m->h_st[0];
     if (st.error & kErrRange)
-      return fail(m, VDBGPU_OUT_OF_RANGE, "a voxel index is outside [-2^23, 2^23) or not finite; scan rejected");
+      return fail(m, VDBGPU_OUT_OF_RANGE, batched ? "a voxel index is outside [-2^21, 2^21) (batch) or not finite; batch rejected"
+                                                  : "a voxel index is outside [-2^23, 2^23) or not finite; scan rejected");
     if (st.error & kErrTable)
       return fail(m, VDBGPU_UNSUPPORTED, "a ray spans more than 8192 voxels along one axis");
     if (st.error & kErrEntry)
       return fail(m, VDBGPU_DEVICE_ERROR, "mid-ray DDA entry did not converge");
+    if (attempt > 8) return fail(m, VDBGPU_DEVICE_ERROR, "update set or map cannot grow");
     if (st.overflow & kOvfHard) {
-      if (attempt > 6 || src.cap >= (1u << 30)) return fail(m, VDBGPU_DEVICE_ERROR, "update set cannot grow");
+      if (src.cap >= (1u << 30)) return fail(m, VDBGPU_DEVICE_ERROR, "update set cannot grow");
       if ((r = grow_update(m, src, src.cap * 4))) return r;
+      if (batched && (r = ensure_groups(m, src.cap))) return r;
       CK(hipMemsetAsync(&m->d_st[0].overflow, 0, sizeof(unsigned int), m->stream));
       CK(hipMemsetAsync(&m->d_st[0].entries, 0, sizeof(unsigned int), m->stream));
-      CK(hipMemsetAsync(&m->d_st[0].voxels, 0, sizeof(unsigned long long), m->stream));
+      CK(hipMemsetAsync(&m->d_st[0].groups, 0, sizeof(unsigned int), m->stream));
+      fold_only = false;
+      continue;
+    }
+    if (batched && st.map_full) {
+      if ((r = ensure_map(m, (uint64_t)m->h_st[1].leaf_count + st.groups))) return r;
+      CK(hipMemsetAsync(&m->d_st[0].map_full, 0, sizeof(unsigned int), m->stream));
+      fold_only = true;
       continue;
     }
     break;
   }
   const Status& st = m->h_st[0];
-  src.pending = st.entries;
-  m->stats.scans++;
+  for (const ScanInfo& sc : scans) m->stats.scans += (sc.nrays > 0 && sc.R > 0) ? 1 : 0;
   m->stats.rays += st.rays;
   m->stats.visits += st.total;
+  if (batched) {
+    m->stats.update_leaves += st.entries;
+    m->touched = m->touched || st.entries > 0;
+    src.pending = 0;
+  } else {
+    src.pending = st.entries;
+  }
   if ((double)st.entries > m->tuning.load_factor * src.cap && src.cap < (1u << 30)) {
+    const uint32_t entries = st.entries;
     if ((r = grow_update(m, src, src.cap * 2))) return r;
-    CK(hipMemsetAsync(&m->d_st[0], 0, sizeof(Status), m->stream));
-    CK(launch_compact(m->stream, std::min<int>(4096, (int)((src.cap + 255) / 256)), upd_view(src),
-                      src.cap, src.list, &m->d_st[0]));
-    if ((r = read_status(m))) return r;
-    src.pending = m->h_st[0].entries;
+    if (batched && (r = ensure_groups(m, src.cap))) return r;
+    if (!batched) {
+      CK(hipMemsetAsync(&m->d_st[0], 0, sizeof(Status), m->stream));
+      CK(launch_compact(m->stream, std::min<int>(4096, (int)((src.cap + 255) / 256)), upd_view(src),
+                        src.cap, src.list, &m->d_st[0].entries, &m->d_st[0]));
+      if ((r = read_status(m))) return r;
+      src.pending = m->h_st[0].entries;
+      (void)entries;
+    }
   }
   return VDBGPU_OK;
+}
+
+int do_accumulate(vdbgpu_map* m, Source& src, const float* xyz, int64_t n, int64_t stride,
+                  const double origin[3], bool on_device) {
+  if (!m->config_set) return fail(m, VDBGPU_NOT_CONFIGURED, "Map not properly configured. Did you call setConfig method?");
+  if (m->fast_mode) return fail(m, VDBGPU_UNSUPPORTED, "fast_mode raycasting is not provided by this library");
+  if (n < 0 || stride < 12 || (stride & 3) || n > (int64_t)kMaxRays || !origin || (n && !xyz))
+    return fail(m, VDBGPU_BAD_ARGUMENT, "bad point buffer arguments");
+  if (n == 0) return VDBGPU_OK;
+  ScanInfo sc{};
+  sc.o[0] = origin[0];
+  sc.o[1] = origin[1];
+  sc.o[2] = origin[2];
+  sc.R = src.max_range;
+  sc.ray0 = 0;
+  sc.nrays = (uint32_t)n;
+  return run_pipeline(m, src, xyz, stride, on_device, {sc}, (uint32_t)n, false);
 }
 
 // Creation rule for a missing leaf: the op must change a background voxel
@@ -584,6 +695,7 @@ const char* vdbgpu_status_string(int32_t s) {
     case VDBGPU_DEVICE_ERROR: return "device error";
     case VDBGPU_BAD_ARGUMENT: return "bad argument";
     case VDBGPU_NO_INTERSECTOR: return "no intersector";
+    case VDBGPU_OVER_BUDGET: return "over HBM budget";
     default: return "unknown status";
   }
 }
@@ -618,7 +730,11 @@ int32_t vdbgpu_create(double resolution, vdbgpu_map** out) {
     return VDBGPU_DEVICE_ERROR;
   }
   memset(m->h_st, 0, 4 * sizeof(Status));
-  if (hipMalloc((void**)&m->noop4096, vdbgpu_map::kNoopCap * 8) != hipSuccess ||
+  if (hipMalloc((void**)&m->d_scans, kMaxBatchScans * sizeof(ScanInfo)) != hipSuccess ||
+      hipMalloc((void**)&m->d_ctl, sizeof(BatchCtl)) != hipSuccess ||
+      hipHostMalloc((void**)&m->h_scans, kMaxBatchScans * sizeof(ScanInfo), hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void**)&m->h_ctl, sizeof(BatchCtl), hipHostMallocDefault) != hipSuccess ||
+      hipMalloc((void**)&m->noop4096, vdbgpu_map::kNoopCap * 8) != hipSuccess ||
       launch_fill_u64(m->stream, m->noop4096, vdbgpu_map::kNoopCap, kEmpty) != hipSuccess) {
     vdbgpu_destroy(m);
     return VDBGPU_DEVICE_ERROR;
@@ -656,6 +772,15 @@ void vdbgpu_destroy(vdbgpu_map* m) {
   dfree(m, &m->q_succ, (size_t)m->q_cap);
   dfree(m, &m->q_end, (size_t)m->q_cap * 3);
   if (m->noop4096) (void)hipFree(m->noop4096);
+  if (m->d_scans) (void)hipFree(m->d_scans);
+  if (m->d_ctl) (void)hipFree(m->d_ctl);
+  if (m->h_scans) (void)hipHostFree(m->h_scans);
+  if (m->h_ctl) (void)hipHostFree(m->h_ctl);
+  dfree(m, &m->gkeys, m->gcap);
+  dfree(m, &m->gscans, (size_t)m->gcap * 2);
+  dfree(m, &m->gneed, m->gcap);
+  dfree(m, &m->glist, m->gcap);
+  dfree(m, &m->gidx, m->gcap);
   if (m->d_st) (void)hipFree(m->d_st);
   if (m->h_st) (void)hipHostFree(m->h_st);
   for (auto& t : m->timed) {
@@ -749,6 +874,91 @@ int32_t vdbgpu_insert(vdbgpu_map* m, const char* id, const float* xyz, int64_t n
     status = do_accumulate(m, it->second, xyz, n, stride, origin, on_device != 0);
   const int r = do_integrate(m);
   return status ? status : r;
+}
+
+int32_t vdbgpu_insert_batch(vdbgpu_map* m, const char* id, int32_t nscans, const float* xyz,
+                            const int64_t* counts, int64_t stride, const double* origins,
+                            const double* ranges, int32_t on_device) {
+  if (!m || !id || nscans < 0 || (nscans && (!counts || !origins))) return VDBGPU_BAD_ARGUMENT;
+  Guard g(m);
+  auto it = m->sources.find(id);
+  if (it == m->sources.end() || !m->config_set || m->fast_mode || !(it->second.max_range > 0) ||
+      nscans == 0) {
+    // every insert would only integrate (nothing cast): one integrate does the same
+    int status = VDBGPU_OK;
+    if (nscans > 0) {
+      if (it == m->sources.end())
+        status = fail(m, VDBGPU_UNKNOWN_SOURCE, std::string("Tried to accumulate update for ") + id + ". Source not available");
+      else if (!m->config_set)
+        status = fail(m, VDBGPU_NOT_CONFIGURED, "Map not properly configured. Did you call setConfig method?");
+      else if (m->fast_mode)
+        status = fail(m, VDBGPU_UNSUPPORTED, "fast_mode raycasting is not provided by this library");
+    }
+    const int r = do_integrate(m);
+    return status ? status : r;
+  }
+  Source& src = it->second;
+  if (stride < 12 || (stride & 3) || (!xyz && nscans)) return fail(m, VDBGPU_BAD_ARGUMENT, "bad point buffer arguments");
+  std::vector<int64_t> off(nscans + 1, 0);
+  for (int i = 0; i < nscans; ++i) {
+    if (counts[i] < 0) return fail(m, VDBGPU_BAD_ARGUMENT, "negative point count");
+    off[i + 1] = off[i] + counts[i];
+  }
+  auto scan_range = [&](int i) { return ranges ? ranges[i] : src.max_range; };
+  auto scan_ptr = [&](int i) {
+    return reinterpret_cast<const float*>(reinterpret_cast<const char*>(xyz) + off[i] * stride);
+  };
+  // one scan through the plain path: accumulate (with the scan's range) + integrate
+  auto plain_insert = [&](int i) -> int {
+    const double keep = src.max_range;
+    src.max_range = scan_range(i);
+    int st = VDBGPU_OK;
+    if (src.max_range > 0 && counts[i] > 0)
+      st = do_accumulate(m, src, scan_ptr(i), counts[i], stride, origins + 3 * i, on_device != 0);
+    src.max_range = keep;
+    const int r = do_integrate(m);
+    return st ? st : r;
+  };
+  int i0 = 0;
+  bool pending = false;
+  for (auto& kv : m->sources) pending = pending || kv.second.pending > 0;
+  if (pending) {
+    // pending sets are applied with the first scan's integrate, in source-id order
+    const int r = plain_insert(0);
+    if (r) return r;
+    i0 = 1;
+  }
+  while (i0 < nscans) {
+    int i1 = i0;
+    uint64_t rays = 0;
+    while (i1 < nscans && i1 - i0 < kMaxBatchScans && rays + (uint64_t)counts[i1] <= kMaxRays) rays += counts[i1++];
+    if (i1 == i0) return fail(m, VDBGPU_BAD_ARGUMENT, "a scan has more than 2^30 points");
+    std::vector<ScanInfo> scans;
+    for (int i = i0; i < i1; ++i) {
+      ScanInfo sc{};
+      for (int a = 0; a < 3; ++a) sc.o[a] = origins[3 * i + a];
+      sc.R = scan_range(i);
+      sc.ray0 = (uint32_t)(off[i] - off[i0]);
+      sc.nrays = (uint32_t)counts[i];
+      scans.push_back(sc);
+    }
+    int r = rays ? run_pipeline(m, src, scan_ptr(i0), stride, on_device != 0, scans, (uint32_t)rays, true)
+                 : VDBGPU_OK;
+    if (r == VDBGPU_OUT_OF_RANGE) {
+      // outside the batched key range (or a bad scan): exact per-scan semantics
+      int first = VDBGPU_OK;
+      for (int i = i0; i < i1; ++i) {
+        const int ri = plain_insert(i);
+        if (ri && !first) first = ri;
+      }
+      if (first) return first;
+      r = VDBGPU_OK;
+    }
+    if (r) return r;
+    i0 = i1;
+  }
+  if (m->fast_mode && m->touched) return build_snapshot(m);
+  return VDBGPU_OK;
 }
 
 int32_t vdbgpu_reset(vdbgpu_map* m) {
