@@ -378,6 +378,8 @@ def main():
         "p99_scan_ms": round(float(np.percentile(lat_ms, 99)), 4),
         "kernel_ms_per_step": {k: round(d["ms_" + k] / max(a.steps, 1), 4)
                                for k in ("prep", "traverse", "compact", "apply")},
+        "update_leaves_per_step": round(d["update_leaves"] / max(a.steps, 1), 1),
+        "visits_per_step": round(visits / max(a.steps, 1), 1),
         "map_leaves": st1["leaf_count"],
         "device_gb": round(st1["device_bytes"] / 1e9, 2),
         "roofline": roofline,
