@@ -30,7 +30,6 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-PMC_SUMMARY = os.path.join(ROOT, "profiles", "traverse_outdoor_stream_pmc.json")
 
 
 def parse():
@@ -42,7 +41,8 @@ def parse():
     p.add_argument("--cpu-scans", type=int, default=2, help="scans timed on the CPU oracle (0 = skip)")
     p.add_argument("--no-profiling", action="store_true", help="do not time kernels with HIP events")
     p.add_argument("--tuning", type=str, default="", help="block,steps_per_lane,waves_per_cu,load_factor")
-    p.add_argument("--mode", choices=("plain", "batch", "mixed", "rgbd", "serve"), default="plain")
+    p.add_argument("--mode", choices=("plain", "batch", "mixed", "rgbd", "serve", "copy"), default="plain")
+    p.add_argument("--copy-mib", type=int, default=1024, help="copy mode: bytes per copy (MiB)")
     p.add_argument("--batch", type=int, default=8, help="scans per batch (batch mode)")
     p.add_argument("--autotune", action="store_true", help="mixed mode: autotune first")
     p.add_argument("--seconds", type=float, default=10.0, help="serve mode: arrival window")
@@ -261,6 +261,20 @@ def mode_serve(a, V, W):
     srv.close()
 
 
+def mode_copy(a, V, W):
+    """Plain device-to-device copies of --copy-mib MiB: the reference kernel for
+    calibrating the FETCH_SIZE / WRITE_SIZE counters."""
+    nbytes = a.copy_mib << 20
+    src, dst = V.DeviceBuffer(nbytes), V.DeviceBuffer(nbytes)
+    dst.copy_from(src, nbytes)  # warm
+    t0 = time.perf_counter()
+    for _ in range(3):
+        dst.copy_from(src, nbytes)
+    dt = (time.perf_counter() - t0) / 3
+    print(json.dumps({"metric": "device copy", "value": round(2 * nbytes / dt / 1e9, 1), "unit": "GB/s",
+                      "bytes_per_copy": nbytes, "copies": 4}), flush=True)
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -274,7 +288,8 @@ def main():
         print("no gfx950 GPU visible", file=sys.stderr)
         sys.exit(1)
     if a.mode != "plain":
-        {"batch": mode_batch, "mixed": mode_mixed, "rgbd": mode_rgbd, "serve": mode_serve}[a.mode](a, V, W)
+        {"batch": mode_batch, "mixed": mode_mixed, "rgbd": mode_rgbd, "serve": mode_serve,
+         "copy": mode_copy}[a.mode](a, V, W)
         return
     w = W.outdoor_stream_workload(a.scans)
     t_gen = time.perf_counter()
@@ -307,35 +322,50 @@ def main():
     g.synchronize()
     g.set_profiling(not a.no_profiling)
     st0 = g.stats()
-    lat = []
     g.synchronize()
     t0 = time.perf_counter()
     for k in range(a.steps):
-        ts = time.perf_counter()
-        step(a.warmup + k)
-        g.synchronize()
-        lat.append(time.perf_counter() - ts)
+        step(a.warmup + k)  # scans stream in: the next scan's cast overlaps this one's map update
     g.synchronize()
     elapsed = time.perf_counter() - t0
     st1 = g.stats()
     d = {k: st1[k] - st0[k] for k in st1 if isinstance(st1[k], (int, float))}
     rays, visits, upd = d["rays"], d["visits"], d["voxel_updates"]
+    # scan latency, outside the timed region: one scan at a time, insert to
+    # map updated (synchronous), over the next scans of the stream
+    g.set_profiling(False)
+    lat = []
+    for k in range(min(a.steps, 50)):
+        ts = time.perf_counter()
+        step(a.warmup + a.steps + k)
+        g.synchronize()
+        lat.append(time.perf_counter() - ts)
     lat_ms = np.array(lat) * 1e3
 
-    roofline = None
-    if d["n_traverse"] > 0 and d["ms_traverse"] > 0:
-        # traversal: 12 B of point data per ray + one 8-byte key touch per voxel visit
-        alg_bytes = 12.0 * rays + 8.0 * visits
-        achieved = alg_bytes / (d["ms_traverse"] * 1e-3) / 1e9
+    # Roofline of the dominant kernel (largest time per scan), with its
+    # algorithmic bytes: the voxel walk moves 12 B of point data per ray and
+    # touches one 8-byte key per voxel visit; the log-odds apply reads a
+    # key/flag and reads and writes a float per unique voxel (16 B).
+    candidates = []
+    if d["n_walk"] > 0 and d["ms_walk"] > 0:
+        candidates.append(("k_brick_walk", d["ms_walk"], d["n_walk"], 12.0 * rays + 8.0 * visits))
+    elif d["n_traverse"] > 0 and d["ms_traverse"] > 0:
+        candidates.append(("k_traverse", d["ms_traverse"], d["n_traverse"], 12.0 * rays + 8.0 * visits))
+    if d["n_apply"] > 0 and d["ms_apply"] > 0:
+        candidates.append(("k_apply", d["ms_apply"], d["n_apply"], 16.0 * upd))
+    rooflines = []
+    for name, ms, n, alg_bytes in candidates:
+        achieved = alg_bytes / (ms * 1e-3) / 1e9
         traffic = None
-        if os.path.exists(PMC_SUMMARY):
-            with open(PMC_SUMMARY) as f:
+        pmc = os.path.join(ROOT, "profiles", f"{name}_outdoor_stream_pmc.json")
+        if os.path.exists(pmc):
+            with open(pmc) as f:
                 traffic = json.load(f).get("hbm_bytes_per_launch")
-        roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                    "kernel": "k_traverse", "launches": d["n_traverse"],
-                    "avg_launch_ms": round(d["ms_traverse"] / d["n_traverse"], 4),
-                    "alg_bytes_per_launch": round(alg_bytes / d["n_traverse"])}
+        rooflines.append({"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                          "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": name,
+                          "launches": n, "avg_launch_ms": round(ms / n, 4),
+                          "alg_bytes_per_launch": round(alg_bytes / n)})
+    roofline = max(rooflines, key=lambda r: r["avg_launch_ms"]) if rooflines else None
 
     cpu = None
     checked = None
@@ -374,15 +404,16 @@ def main():
                    "input": "device-resident", "path": "plain per-scan insertPointCloud"},
         "voxel_updates_per_s": round(upd / elapsed, 1),
         "visits_per_s": round(visits / elapsed, 1),
-        "p50_scan_ms": round(float(np.percentile(lat_ms, 50)), 4),
+        "p50_scan_ms": round(float(np.percentile(lat_ms, 50)), 4),  # synchronous, one scan at a time
         "p99_scan_ms": round(float(np.percentile(lat_ms, 99)), 4),
         "kernel_ms_per_step": {k: round(d["ms_" + k] / max(a.steps, 1), 4)
-                               for k in ("prep", "traverse", "compact", "apply")},
+                               for k in ("prep", "traverse", "walk", "compact", "apply")},
         "update_leaves_per_step": round(d["update_leaves"] / max(a.steps, 1), 1),
         "visits_per_step": round(visits / max(a.steps, 1), 1),
         "map_leaves": st1["leaf_count"],
         "device_gb": round(st1["device_bytes"] / 1e9, 2),
         "roofline": roofline,
+        "roofline_other": [r for r in rooflines if r is not roofline],
         "cpu_baseline": cpu,
         "bit_exact_check": checked,
     }

@@ -58,8 +58,13 @@ typedef struct vdbgpu_stats {
   uint64_t update_leaves;  /* update-set leaves applied */
   uint64_t leaf_count;     /* leaves in the map now */
   uint64_t device_bytes;   /* device memory held by the handle */
-  double ms_prep, ms_traverse, ms_compact, ms_apply, ms_raytrace; /* summed kernel times */
+  /* summed kernel times: prep (points -> rays), traverse (hash traversal, or the brick cut),
+   * compact (update-leaf listing and map-leaf lookup/creation), apply (log-odds update),
+   * raytrace; ms_walk below: the brick walk */
+  double ms_prep, ms_traverse, ms_compact, ms_apply, ms_raytrace;
   uint64_t n_prep, n_traverse, n_compact, n_apply, n_raytrace;      /* launches timed */
+  double ms_walk;          /* brick path: the brick walk kernel (ms_traverse: the brick cut) */
+  uint64_t n_walk;
 } vdbgpu_stats;
 
 /* Launch parameters of the integration kernels (see vdbgpu_set_tuning). */
@@ -68,6 +73,8 @@ typedef struct vdbgpu_tuning {
   int32_t steps_per_lane; /* DDA visits per lane work item (chunk length) */
   int32_t waves_per_cu;   /* resident waves per CU requested for the traversal grid */
   double load_factor;     /* update-set hash target load (0.3..0.9) */
+  int32_t traversal;      /* plain inserts: 0 = brick walk (default), 1 = update-set hash traversal */
+  int32_t _pad;
 } vdbgpu_tuning;
 
 typedef struct vdbgpu_map vdbgpu_map;
@@ -173,6 +180,9 @@ int32_t vdbgpu_device_alloc(int64_t bytes, void** out);
 int32_t vdbgpu_device_free(void* ptr);
 int32_t vdbgpu_device_upload(void* dst, const void* src, int64_t bytes);
 int32_t vdbgpu_device_download(void* dst, const void* src, int64_t bytes);
+/* Device-to-device copy (the plain copy kernel the traffic counters are
+ * calibrated against). */
+int32_t vdbgpu_device_copy(void* dst, const void* src, int64_t bytes);
 
 /* raytrace with every array in device memory (same layout). */
 int32_t vdbgpu_raytrace_device(vdbgpu_map* map, int64_t n, const double* origins,

@@ -33,7 +33,7 @@ ABI_SYMBOLS = [
     "vdbgpu_synchronize",
     "vdbgpu_get_stats", "vdbgpu_set_profiling", "vdbgpu_set_tuning", "vdbgpu_get_tuning",
     "vdbgpu_last_error", "vdbgpu_device_alloc", "vdbgpu_device_free", "vdbgpu_device_upload",
-    "vdbgpu_device_download", "vdbgpu_server_create", "vdbgpu_server_submit", "vdbgpu_server_drain",
+    "vdbgpu_device_download", "vdbgpu_device_copy", "vdbgpu_server_create", "vdbgpu_server_submit", "vdbgpu_server_drain",
     "vdbgpu_server_records", "vdbgpu_server_map", "vdbgpu_server_device_bytes",
     "vdbgpu_server_destroy",
 ]
@@ -67,7 +67,8 @@ class Stats(ctypes.Structure):
                [(n, ctypes.c_double) for n in ("ms_prep", "ms_traverse", "ms_compact", "ms_apply",
                                                "ms_raytrace")] + \
                [(n, ctypes.c_uint64) for n in ("n_prep", "n_traverse", "n_compact", "n_apply",
-                                               "n_raytrace")]
+                                               "n_raytrace")] + \
+               [("ms_walk", ctypes.c_double), ("n_walk", ctypes.c_uint64)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
@@ -75,7 +76,8 @@ class Stats(ctypes.Structure):
 
 class Tuning(ctypes.Structure):
     _fields_ = [("block", ctypes.c_int32), ("steps_per_lane", ctypes.c_int32),
-                ("waves_per_cu", ctypes.c_int32), ("load_factor", ctypes.c_double)]
+                ("waves_per_cu", ctypes.c_int32), ("load_factor", ctypes.c_double),
+                ("traversal", ctypes.c_int32), ("_pad", ctypes.c_int32)]
 
 
 class RequestRecord(ctypes.Structure):
@@ -148,7 +150,7 @@ def lib():
         L.vdbgpu_device_alloc.argtypes = [I64, ctypes.POINTER(P)]
         L.vdbgpu_device_free.restype = I32
         L.vdbgpu_device_free.argtypes = [P]
-        for f in (L.vdbgpu_device_upload, L.vdbgpu_device_download):
+        for f in (L.vdbgpu_device_upload, L.vdbgpu_device_download, L.vdbgpu_device_copy):
             f.restype = I32
             f.argtypes = [P, P, I64]
         L.vdbgpu_server_create.restype = I32
@@ -363,14 +365,16 @@ class GpuMap:
     def set_profiling(self, on: bool) -> None:
         lib().vdbgpu_set_profiling(self.h, int(on))
 
-    def set_tuning(self, block=256, steps_per_lane=128, waves_per_cu=8, load_factor=0.5) -> int:
-        t = Tuning(block, steps_per_lane, waves_per_cu, load_factor)
+    def set_tuning(self, block=256, steps_per_lane=32, waves_per_cu=16, load_factor=0.5,
+                   traversal=0) -> int:
+        """traversal: 0 = brick walk for plain inserts, 1 = update-set hash traversal."""
+        t = Tuning(block, steps_per_lane, waves_per_cu, load_factor, traversal, 0)
         return lib().vdbgpu_set_tuning(self.h, ctypes.byref(t))
 
     def get_tuning(self) -> dict:
         t = Tuning()
         lib().vdbgpu_get_tuning(self.h, ctypes.byref(t))
-        return {n: getattr(t, n) for n, _ in t._fields_}
+        return {n: getattr(t, n) for n, _ in t._fields_ if n != "_pad"}
 
 
 class DeviceBuffer:
@@ -403,6 +407,11 @@ class DeviceBuffer:
         if rc != OK:
             raise VdbGpuError("download failed")
         return a
+
+    def copy_from(self, other: "DeviceBuffer", nbytes: int) -> None:
+        assert nbytes <= self.nbytes and nbytes <= other.nbytes
+        if lib().vdbgpu_device_copy(self.ptr, other.ptr, nbytes) != OK:
+            raise VdbGpuError("device copy failed")
 
     @property
     def address(self) -> int:

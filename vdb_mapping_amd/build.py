@@ -15,8 +15,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 
-HIP_SOURCES = ["integrate_kernels.hip", "batch_kernels.hip", "raytrace_kernels.hip", "vdbgpu.hip",
-               "server.hip"]
+HIP_SOURCES = ["integrate_kernels.hip", "brick_kernels.hip", "batch_kernels.hip", "raytrace_kernels.hip",
+               "vdbgpu.hip", "server.hip"]
 # -ffp-contract=off: no FMA contraction, every double op rounds on its own (bit-exact DDA).
 # -packed-fp32-ops off: no v_pk_*_f32 generation.
 HIPCC_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-ffp-contract=off", "-fPIC",

@@ -1,0 +1,582 @@
+// gfx950 kernels of the brick path of one scan (the raycast of
+// insertPointCloud, VDBMapping.hpp:466-566). Space is cut into bricks of
+// 32^3 voxels (4^3 leaves). Each ray is cut into per-brick segments in
+// closed form (dda_math.h), the segments are bucketed by brick, and one
+// workgroup owns a brick's visited/hit bitmap (8 KiB of LDS) while it walks
+// that brick's segments. The result is the scan's update set as a list of
+// leaf records (key + 8 visited + 8 hit mask words), every update leaf
+// exactly once, written with plain stores at fixed positions (no shared
+// counters); k_map_insert and k_apply take the dense list of filled
+// positions built from the per-item leaf masks. No global atomics touch voxel bits except for the few
+// bricks so crowded (more than kItemSegs segments, around the sensor) that
+// several workgroups share them.
+//
+//   k_seg_count     one lane per ray: enumerate its bricks, count segments
+//                   per brick (aggregated per workgroup in LDS)
+//   k_brick_sums    per 256 brick slots: sums of segments and work items
+//   k_scan_u64      exclusive scan of those sums (one 256-thread workgroup)
+//   k_brick_offsets segment offset and work items of each brick
+//   k_seg_fill      enumerate again, write each segment record at its
+//                   brick's next position
+//   k_brick_walk    one workgroup per work item: walk the item's segments
+//                   into the LDS bitmap, write the brick's leaf records at
+//                   the item's 64 fixed positions, or OR into the shared
+//                   bitmap of a split brick
+//   k_split_emit    leaf records of split bricks (positions after the items)
+//   k_mask_sums, k_mask_list  dense list of the filled record positions from
+//                   the per-item leaf masks (scan, no shared counters)
+//
+// Segment record (uint4): ray index; crossing count c0 (bits 0-13) | visits
+// m (bits 16-22) | end-voxel hit flag (bit 31); c1; c2. The walk of a
+// segment starts at voxel s + sign * c with the integer DDA state of those
+// counts and visits m voxels, all inside the brick (checked).
+#include "dda_math.h"
+#include "device_common.h"
+#include "kernels.h"
+
+namespace vdbgpu {
+
+namespace {
+
+constexpr int kLdsBricks = 1024;  // per-workgroup brick aggregation table
+constexpr int kLdsBrickProbe = 32;
+constexpr uint32_t kMaxBrickProbe = 512;
+constexpr int kBrickWords = 1024;  // 64 leaves x (8 visited + 8 hit) words
+
+__device__ __forceinline__ int bw_lane() { return (int)__lane_id(); }
+
+struct RayGeom {
+  int s[3], e[3];
+  uint32_t cnt;
+  bool hit;
+};
+
+// Geometry of ray i, false if it marks nothing (NaN, zero range, rejected).
+__device__ __forceinline__ bool ray_geom(const int4* __restrict__ rayinfo,
+                                         const ScanInfo* __restrict__ scans, uint32_t i,
+                                         RayGeom& g) {
+  const int4 r = rayinfo[i];
+  const uint32_t w = (uint32_t)r.w;
+  g.cnt = w & kCntMask;
+  g.hit = (w & kHitBit) != 0;
+  if (g.cnt == 0 && !g.hit) return false;
+  const uint32_t sc = (w >> kScanShift) & kScanMask;
+  g.s[0] = scans[sc].s[0];
+  g.s[1] = scans[sc].s[1];
+  g.s[2] = scans[sc].s[2];
+  g.e[0] = r.x;
+  g.e[1] = r.y;
+  g.e[2] = r.z;
+  return true;
+}
+
+__device__ __forceinline__ int lds_brick(unsigned long long* lkey, uint64_t key, bool insert) {
+  uint32_t h = (uint32_t)(fmix64(key) >> 40) & (kLdsBricks - 1);
+  for (int p = 0; p < kLdsBrickProbe; ++p) {
+    const unsigned long long k =
+        __hip_atomic_load(&lkey[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (k == key) return (int)h;
+    if (k == kEmpty) {
+      if (!insert) return -1;
+      const unsigned long long prev = atomicCAS(&lkey[h], kEmpty, (unsigned long long)key);
+      if (prev == kEmpty || prev == key) return (int)h;
+    }
+    h = (h + 1) & (kLdsBricks - 1);
+  }
+  return -1;
+}
+
+// Slot of a brick in the scan's brick table (inserted if `insert`), kNone if
+// absent or after kMaxBrickProbe probes (then the table is flagged full).
+__device__ __forceinline__ uint32_t brick_slot(const BrickView& bv, uint64_t key, bool insert,
+                                               unsigned int* ovf) {
+  uint32_t h = (uint32_t)fmix64(key) & bv.cap_mask;
+  const uint32_t limit = bv.cap_mask < kMaxBrickProbe ? bv.cap_mask + 1 : kMaxBrickProbe;
+  for (uint32_t p = 0; p < limit; ++p) {
+    const unsigned long long k =
+        __hip_atomic_load(&bv.keys[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (k == key) return h;
+    if (k == kEmpty) {
+      if (!insert) return kNone;
+      const unsigned long long prev = atomicCAS(&bv.keys[h], kEmpty, (unsigned long long)key);
+      if (prev == kEmpty || prev == key) return h;
+    }
+    h = (h + 1) & bv.cap_mask;
+  }
+  if (insert) atomicOr(ovf, kOvfBricks);
+  return kNone;
+}
+
+// Workgroup-uniform stop flag: error or any overflow already raised.
+__device__ __forceinline__ bool stopped(const Status* st, unsigned int* s_stop) {
+  if (threadIdx.x == 0)
+    *s_stop = __hip_atomic_load(&st->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) |
+              __hip_atomic_load(&st->overflow, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __syncthreads();
+  return *s_stop != 0;
+}
+
+__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+__device__ __forceinline__ uint64_t block_sum_u64(uint64_t v, uint64_t* s_w) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  if (bw_lane() == 0) s_w[threadIdx.x >> 6] = v;
+  __syncthreads();
+  uint64_t t = 0;
+  for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += s_w[w];
+  __syncthreads();
+  return t;
+}
+
+// Leaf records of the brick bitmap `bits` (brick origin o) at the fixed
+// positions base .. base + 63 (leaf l of the brick at base + l), for every
+// leaf with a visited voxel; *leaf_mask gets the set of filled positions
+// (bit l). Called by the whole workgroup.
+__device__ void emit_leaves(const unsigned long long* bits, int ox, int oy, int oz, UpdView rec,
+                            uint32_t base, uint32_t* s_pos, unsigned long long* leaf_mask) {
+  const int tid = threadIdx.x;
+  if (tid < 64) {  // lane l = leaf l of the brick
+    uint64_t any = 0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) any |= bits[tid * 8 + j];
+    const uint64_t m = __ballot(any != 0);
+    if (tid == 0) *leaf_mask = m;
+    s_pos[tid] = any ? base + tid : kNone;
+    if (any) rec.keys[base + tid] = leaf_key(ox + (tid >> 4) * 8, oy + ((tid >> 2) & 3) * 8, oz + (tid & 3) * 8);
+  }
+  __syncthreads();
+  // 16 records x 16 words per pass: contiguous 128-byte rows
+#pragma unroll
+  for (int p = 0; p < 4; ++p) {
+    const int l = p * 16 + (tid >> 4), w = tid & 15;
+    const uint32_t pos = s_pos[l];
+    if (pos != kNone) rec.masks[(size_t)pos * 16 + w] = bits[(w < 8 ? 0 : 512) + l * 8 + (w & 7)];
+  }
+}
+
+}  // namespace
+
+__global__ __launch_bounds__(256) void k_seg_count(const int4* __restrict__ rayinfo,
+                                                   const ScanInfo* __restrict__ scans,
+                                                   const BatchCtl* __restrict__ ctl,
+                                                   const double* __restrict__ tbl, BrickView bv,
+                                                   Status* st) {
+  __shared__ unsigned long long lkey[kLdsBricks];
+  __shared__ uint32_t lcnt[kLdsBricks];
+  __shared__ unsigned int s_stop;
+  if (stopped(st, &s_stop)) return;
+  for (int e = threadIdx.x; e < kLdsBricks; e += blockDim.x) {
+    lkey[e] = kEmpty;
+    lcnt[e] = 0;
+  }
+  __syncthreads();
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t nseg = 0;
+  RayGeom g;
+  if (i < ctl->n_rays && ray_geom(rayinfo, scans, i, g)) {
+    SegEnum en;
+    en.init(tbl, g.s, g.e, g.cnt);
+    Segment sg;
+    while (en.next(tbl, sg)) {
+      const uint64_t key = brick_key(sg.bx, sg.by, sg.bz);
+      const int ls = lds_brick(lkey, key, true);
+      if (ls >= 0) {
+        atomicAdd(&lcnt[ls], 1u);
+      } else {
+        const uint32_t gs = brick_slot(bv, key, true, &st->overflow);
+        if (gs != kNone) atomicAdd(&bv.cnt[gs], 1u);
+      }
+      ++nseg;
+    }
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < kLdsBricks; e += blockDim.x) {
+    const uint64_t key = lkey[e];
+    if (key == kEmpty) continue;
+    const uint32_t gs = brick_slot(bv, key, true, &st->overflow);
+    if (gs != kNone) atomicAdd(&bv.cnt[gs], lcnt[e]);
+  }
+  nseg = wave_sum_u32(nseg);
+  if (bw_lane() == 0 && nseg) atomicAdd(&st->segs, nseg);
+}
+
+// Per 256 slots: (work items << 32) | segments.
+__global__ __launch_bounds__(256) void k_brick_sums(BrickView bv, uint64_t* __restrict__ bsum,
+                                                    const Status* st) {
+  __shared__ uint64_t s_w[4];
+  const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t c = bv.cnt[slot];
+  const uint64_t v = ((uint64_t)((c + kItemSegs - 1) / kItemSegs) << 32) | c;
+  const uint64_t t = block_sum_u64(v, s_w);
+  if (threadIdx.x == 0) bsum[blockIdx.x] = t;
+}
+
+// Exclusive scan of nb packed sums (one workgroup of 1024); totals to st.
+__global__ __launch_bounds__(256) void k_scan_u64(const uint64_t* __restrict__ bsum,
+                                                  uint64_t* __restrict__ boff, uint32_t nb,
+                                                  unsigned int* total, int total_shift) {
+  __shared__ uint64_t wsum[4];
+  uint64_t carry = 0;
+  for (uint32_t base = 0; base < nb; base += 256) {
+    const uint32_t i = base + threadIdx.x;
+    const uint64_t x = i < nb ? bsum[i] : 0ull;
+    uint64_t v = x;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint64_t t = __shfl_up(v, o, 64);
+      if (bw_lane() >= o) v += t;
+    }
+    const int wv = threadIdx.x >> 6;
+    if (bw_lane() == 63) wsum[wv] = v;
+    __syncthreads();
+    uint64_t add = 0, tot = 0;
+    for (int w = 0; w < 4; ++w) {
+      if (w < wv) add += wsum[w];
+      tot += wsum[w];
+    }
+    if (i < nb) boff[i] = carry + add + v - x;
+    carry += tot;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *total = (unsigned int)(carry >> total_shift);
+}
+
+__global__ __launch_bounds__(256) void k_brick_offsets(BrickView bv, const uint64_t* __restrict__ boff,
+                                                       BrickItem* __restrict__ items,
+                                                       uint32_t seg_cap, uint32_t item_cap,
+                                                       uint32_t* __restrict__ split_slot,
+                                                       uint32_t split_cap, Status* st) {
+  __shared__ uint64_t s_w[4];
+  const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t c = bv.cnt[slot];
+  const uint32_t ni = (c + kItemSegs - 1) / kItemSegs;
+  const uint64_t x = ((uint64_t)ni << 32) | c;
+  uint64_t v = x;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint64_t t = __shfl_up(v, o, 64);
+    if (bw_lane() >= o) v += t;
+  }
+  const int wv = threadIdx.x >> 6;
+  if (bw_lane() == 63) s_w[wv] = v;
+  __syncthreads();
+  uint64_t add = 0;
+  for (int w = 0; w < wv; ++w) add += s_w[w];
+  const uint64_t off = boff[blockIdx.x] + add + v - x;
+  const uint32_t seg0 = (uint32_t)off, item0 = (uint32_t)(off >> 32);
+  bv.cur[slot] = seg0;
+  if (ni == 0) return;
+  if ((uint64_t)seg0 + c > seg_cap) {
+    atomicOr(&st->overflow, kOvfSegs);
+    return;
+  }
+  if ((uint64_t)item0 + ni > item_cap) {
+    atomicOr(&st->overflow, kOvfItems);
+    return;
+  }
+  uint32_t shared = kNone;
+  if (ni > 1) {
+    shared = atomicAdd(&st->splits, 1u);
+    if (shared >= split_cap) {
+      atomicOr(&st->overflow, kOvfSplit);
+      return;
+    }
+    split_slot[shared] = slot;
+  }
+  for (uint32_t j = 0; j < ni; ++j) {
+    const uint32_t a = seg0 + j * kItemSegs;
+    const uint32_t b = min(a + kItemSegs, seg0 + c);
+    items[item0 + j] = BrickItem{slot, a, b, shared};
+  }
+}
+
+__global__ __launch_bounds__(256) void k_seg_fill(const int4* __restrict__ rayinfo,
+                                                  const ScanInfo* __restrict__ scans,
+                                                  const BatchCtl* __restrict__ ctl,
+                                                  const double* __restrict__ tbl, BrickView bv,
+                                                  uint4* __restrict__ segs, uint32_t seg_cap,
+                                                  Status* st) {
+  __shared__ unsigned long long lkey[kLdsBricks];
+  __shared__ uint32_t lcnt[kLdsBricks];
+  __shared__ uint32_t lbase[kLdsBricks];
+  __shared__ unsigned int s_stop;
+  if (stopped(st, &s_stop)) return;
+  for (int e = threadIdx.x; e < kLdsBricks; e += blockDim.x) {
+    lkey[e] = kEmpty;
+    lcnt[e] = 0;
+  }
+  __syncthreads();
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  RayGeom g;
+  const bool valid = i < ctl->n_rays && ray_geom(rayinfo, scans, i, g);
+  // 1: local counts of the bricks that fit the LDS table
+  if (valid) {
+    SegEnum en;
+    en.init(tbl, g.s, g.e, g.cnt);
+    Segment sg;
+    while (en.next(tbl, sg)) {
+      const int ls = lds_brick(lkey, brick_key(sg.bx, sg.by, sg.bz), true);
+      if (ls >= 0) atomicAdd(&lcnt[ls], 1u);
+    }
+  }
+  __syncthreads();
+  // 2: one reservation per (workgroup, brick)
+  for (int e = threadIdx.x; e < kLdsBricks; e += blockDim.x) {
+    const uint64_t key = lkey[e];
+    if (key == kEmpty) continue;
+    const uint32_t gs = brick_slot(bv, key, false, &st->overflow);
+    if (gs == kNone) {
+      atomicOr(&st->error, kErrBrick);
+      lbase[e] = seg_cap;
+    } else {
+      lbase[e] = atomicAdd(&bv.cur[gs], lcnt[e]);
+    }
+    lcnt[e] = 0;
+  }
+  __syncthreads();
+  // 3: write the records
+  if (valid) {
+    SegEnum en;
+    en.init(tbl, g.s, g.e, g.cnt);
+    Segment sg;
+    while (en.next(tbl, sg)) {
+      const uint64_t key = brick_key(sg.bx, sg.by, sg.bz);
+      const int ls = lds_brick(lkey, key, false);
+      uint32_t pos;
+      if (ls >= 0) {
+        pos = lbase[ls] + atomicAdd(&lcnt[ls], 1u);
+      } else {
+        const uint32_t gs = brick_slot(bv, key, false, &st->overflow);
+        pos = gs == kNone ? seg_cap : atomicAdd(&bv.cur[gs], 1u);
+      }
+      if (pos >= seg_cap) {
+        atomicOr(&st->overflow, kOvfSegs);
+        continue;
+      }
+      segs[pos] = make_uint4(i, sg.c[0] | (sg.m << 16) | ((sg.last && g.hit) ? 0x80000000u : 0u),
+                             sg.c[1], sg.c[2]);
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_brick_walk(const BrickItem* __restrict__ items,
+                                                    const uint4* __restrict__ segs,
+                                                    const int4* __restrict__ rayinfo,
+                                                    const ScanInfo* __restrict__ scans,
+                                                    const double* __restrict__ tbl, BrickView bv,
+                                                    unsigned long long* __restrict__ shared_bits,
+                                                    UpdView rec, unsigned long long* __restrict__ leaf_mask,
+                                                    uint32_t rec_cap, Status* st) {
+  __shared__ unsigned long long bits[kBrickWords];
+  __shared__ uint32_t s_pos[64];
+  __shared__ unsigned int s_stop;
+  if (stopped(st, &s_stop)) return;
+  const uint32_t n_items = st->items;
+  if ((uint64_t)n_items * 64 > rec_cap) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(&st->overflow, kOvfRecords);
+    return;
+  }
+  // static assignment: item it -> workgroup it % grid (no shared queue counter)
+  for (uint32_t it = blockIdx.x; it < n_items; it += gridDim.x) {
+    const BrickItem item = items[it];
+    int b[3];
+    brick_coords(bv.keys[item.slot], b);
+    const int ox = b[0] * kBrickDim, oy = b[1] * kBrickDim, oz = b[2] * kBrickDim;
+    for (int w = threadIdx.x; w < kBrickWords; w += blockDim.x) bits[w] = 0;
+    __syncthreads();
+    for (uint32_t j = item.seg0 + threadIdx.x; j < item.seg1; j += blockDim.x) {
+      const uint4 sr = segs[j];
+      const int4 r = rayinfo[sr.x];
+      const uint32_t sc = ((uint32_t)r.w >> kScanShift) & kScanMask;
+      const int sx = scans[sc].s[0], sy = scans[sc].s[1], sz = scans[sc].s[2];
+      const uint32_t n[3] = {(uint32_t)abs(r.x - sx), (uint32_t)abs(r.y - sy), (uint32_t)abs(r.z - sz)};
+      const int stx = r.x > sx ? 1 : (r.x < sx ? -1 : 0);
+      const int sty = r.y > sy ? 1 : (r.y < sy ? -1 : 0);
+      const int stz = r.z > sz ? 1 : (r.z < sz ? -1 : 0);
+      const uint32_t c[3] = {sr.y & 0x3FFFu, sr.z, sr.w};
+      const uint32_t m = (sr.y >> 16) & 0x7Fu;
+      int lx = sx + stx * (int)c[0] - ox, ly = sy + sty * (int)c[1] - oy, lz = sz + stz * (int)c[2] - oz;
+      int cw = -1;
+      uint64_t cb = 0, lastb = 0;
+      bool ok = true;
+      IntDda d;
+      d.init(n, c);
+      for (uint32_t k = 0;;) {
+        if ((unsigned)lx >= (unsigned)kBrickDim || (unsigned)ly >= (unsigned)kBrickDim ||
+            (unsigned)lz >= (unsigned)kBrickDim) {
+          atomicOr(&st->error, kErrBrick);
+          ok = false;
+          break;
+        }
+        const int w = ((lx >> 3) << 7) | ((ly >> 3) << 5) | ((lz >> 3) << 3) | (lx & 7);
+        lastb = 1ull << (((ly & 7) << 3) | (lz & 7));
+        if (w != cw) {
+          if (cb) atomicOr(&bits[cw], (unsigned long long)cb);
+          cw = w;
+          cb = 0;
+        }
+        cb |= lastb;
+        if (++k >= m) break;  // m == 0: the endpoint of a zero-length ray
+        const int a = d.next_axis(tbl);
+        d.advance(a);
+        lx += a == 0 ? stx : 0;
+        ly += a == 1 ? sty : 0;
+        lz += a == 2 ? stz : 0;
+      }
+      if (ok) {
+        if (cb) atomicOr(&bits[cw], (unsigned long long)cb);
+        // setValueOn(end, true) (VDBMapping.hpp:533-536): the last visit is the end voxel
+        if (sr.y & 0x80000000u) atomicOr(&bits[512 + cw], (unsigned long long)lastb);
+      }
+    }
+    __syncthreads();
+    if (item.shared == kNone) {
+      emit_leaves(bits, ox, oy, oz, rec, it * 64, s_pos, &leaf_mask[it]);
+    } else {
+      unsigned long long* dst = shared_bits + (size_t)item.shared * kBrickWords;
+      for (int w = threadIdx.x; w < kBrickWords; w += blockDim.x) {
+        const uint64_t v = bits[w];
+        if (v) atomicOr(&dst[w], (unsigned long long)v);
+      }
+      if (threadIdx.x == 0) leaf_mask[it] = 0;  // its records come from k_split_emit
+    }
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(256) void k_split_emit(const uint32_t* __restrict__ split_slot,
+                                                    unsigned long long* __restrict__ shared_bits,
+                                                    BrickView bv, UpdView rec,
+                                                    unsigned long long* __restrict__ leaf_mask,
+                                                    uint32_t rec_cap, Status* st) {
+  __shared__ unsigned long long bits[kBrickWords];
+  __shared__ uint32_t s_pos[64];
+  __shared__ unsigned int s_stop;
+  if (stopped(st, &s_stop)) return;
+  const uint32_t ns = st->splits, n_items = st->items;
+  if ((uint64_t)(n_items + ns) * 64 > rec_cap) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(&st->overflow, kOvfRecords);
+    return;
+  }
+  for (uint32_t k = blockIdx.x; k < ns; k += gridDim.x) {
+    unsigned long long* src = shared_bits + (size_t)k * kBrickWords;
+    for (int w = threadIdx.x; w < kBrickWords; w += blockDim.x) {
+      bits[w] = src[w];
+      src[w] = 0;
+    }
+    __syncthreads();
+    int b[3];
+    brick_coords(bv.keys[split_slot[k]], b);
+    emit_leaves(bits, b[0] * kBrickDim, b[1] * kBrickDim, b[2] * kBrickDim, rec, (n_items + k) * 64, s_pos,
+                &leaf_mask[n_items + k]);
+    __syncthreads();
+  }
+}
+
+// Record list from the per-group (64 positions) leaf masks: per 256 groups
+// the number of filled positions, then after the scan of those sums every
+// group writes its positions in order. No shared counters.
+__global__ __launch_bounds__(256) void k_mask_sums(const unsigned long long* __restrict__ leaf_mask,
+                                                   uint64_t* __restrict__ bsum, const Status* st) {
+  __shared__ uint64_t s_w[4];
+  __shared__ unsigned int s_stop;
+  if (stopped(st, &s_stop)) return;
+  const uint32_t ng = st->items + st->splits;
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t c = g < ng ? (uint64_t)__popcll(leaf_mask[g]) : 0;
+  const uint64_t t = block_sum_u64(c, s_w);
+  if (threadIdx.x == 0) bsum[blockIdx.x] = t;
+}
+
+__global__ __launch_bounds__(256) void k_mask_list(const unsigned long long* __restrict__ leaf_mask,
+                                                   const uint64_t* __restrict__ boff,
+                                                   uint32_t* __restrict__ list, const Status* st) {
+  __shared__ uint64_t s_w[4];
+  __shared__ unsigned int s_stop;
+  if (stopped(st, &s_stop)) return;
+  const uint32_t ng = st->items + st->splits;
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t m = g < ng ? leaf_mask[g] : 0;
+  const uint64_t x = (uint64_t)__popcll(m);
+  uint64_t v = x;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint64_t t = __shfl_up(v, o, 64);
+    if (bw_lane() >= o) v += t;
+  }
+  const int wv = threadIdx.x >> 6;
+  if (bw_lane() == 63) s_w[wv] = v;
+  __syncthreads();
+  uint64_t add = 0;
+  for (int w = 0; w < wv; ++w) add += s_w[w];
+  uint32_t pos = (uint32_t)(boff[blockIdx.x] + add + v - x);
+  for (uint64_t r = m; r; r &= r - 1) list[pos++] = g * 64 + (uint32_t)__builtin_ctzll(r);
+}
+
+// ---------------------------------------------------------------- launchers --
+
+hipError_t launch_seg_count(hipStream_t s, int grid, const int4* rayinfo, const ScanInfo* scans,
+                            const BatchCtl* ctl, const double* tbl, BrickView bv, Status* st) {
+  hipLaunchKernelGGL(k_seg_count, dim3(grid), dim3(256), 0, s, rayinfo, scans, ctl, tbl, bv, st);
+  return hipGetLastError();
+}
+
+hipError_t launch_brick_offsets(hipStream_t s, BrickView bv, uint64_t* bsum, uint64_t* boff,
+                                BrickItem* items, uint32_t seg_cap, uint32_t item_cap,
+                                uint32_t* split_slot, uint32_t split_cap, Status* st) {
+  const uint32_t nb = (bv.cap_mask + 1) / 256;
+  hipLaunchKernelGGL(k_brick_sums, dim3(nb), dim3(256), 0, s, bv, bsum, st);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_scan_u64, dim3(1), dim3(256), 0, s, bsum, boff, nb, &st->items, 32);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_brick_offsets, dim3(nb), dim3(256), 0, s, bv, boff, items, seg_cap, item_cap,
+                     split_slot, split_cap, st);
+  return hipGetLastError();
+}
+
+hipError_t launch_seg_fill(hipStream_t s, int grid, const int4* rayinfo, const ScanInfo* scans,
+                           const BatchCtl* ctl, const double* tbl, BrickView bv, uint4* segs,
+                           uint32_t seg_cap, Status* st) {
+  hipLaunchKernelGGL(k_seg_fill, dim3(grid), dim3(256), 0, s, rayinfo, scans, ctl, tbl, bv, segs,
+                     seg_cap, st);
+  return hipGetLastError();
+}
+
+hipError_t launch_brick_walk(hipStream_t s, int grid, const BrickItem* items, const uint4* segs,
+                             const int4* rayinfo, const ScanInfo* scans, const double* tbl,
+                             BrickView bv, unsigned long long* shared_bits, UpdView rec,
+                             unsigned long long* leaf_mask, uint32_t rec_cap, Status* st) {
+  hipLaunchKernelGGL(k_brick_walk, dim3(grid), dim3(256), 0, s, items, segs, rayinfo, scans, tbl, bv,
+                     shared_bits, rec, leaf_mask, rec_cap, st);
+  return hipGetLastError();
+}
+
+// nb_cap: 256-group blocks the bsum/boff arrays hold (>= (items + splits) / 256).
+hipError_t launch_rec_list(hipStream_t s, const unsigned long long* leaf_mask, uint64_t* bsum, uint64_t* boff,
+                           uint32_t nb_cap, uint32_t* list, Status* st) {
+  hipLaunchKernelGGL(k_mask_sums, dim3(nb_cap), dim3(256), 0, s, leaf_mask, bsum, st);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_scan_u64, dim3(1), dim3(256), 0, s, bsum, boff, nb_cap, &st->records, 0);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_mask_list, dim3(nb_cap), dim3(256), 0, s, leaf_mask, boff, list, st);
+  return hipGetLastError();
+}
+
+hipError_t launch_split_emit(hipStream_t s, int grid, const uint32_t* split_slot,
+                             unsigned long long* shared_bits, BrickView bv, UpdView rec,
+                             unsigned long long* leaf_mask, uint32_t rec_cap, Status* st) {
+  hipLaunchKernelGGL(k_split_emit, dim3(grid), dim3(256), 0, s, split_slot, shared_bits, bv, rec,
+                     leaf_mask, rec_cap, st);
+  return hipGetLastError();
+}
+
+}  // namespace vdbgpu

@@ -36,8 +36,17 @@ constexpr int kMaxQuerySteps = 1 << 24;  // per-axis voxel extent bound of a ray
 constexpr uint32_t kErrRange = 1u;  // a voxel index out of range / non-finite
 constexpr uint32_t kErrTable = 2u;  // crossing table or chunk table too short
 constexpr uint32_t kErrEntry = 4u;  // mid-ray DDA entry did not converge
+constexpr uint32_t kErrBrick = 8u;  // brick path inconsistency (a segment left its brick)
 constexpr uint32_t kOvfHard = 1u;   // update-set probe found no free slot
 constexpr uint32_t kMapFull = 1u;   // Status::map_full: the fold needs a larger leaf pool / hash
+// Brick path overflow bits (Status::overflow): a table too small for this
+// scan; the host grows it and re-runs the brick stages.
+constexpr uint32_t kOvfBricks = 2u;   // brick hash
+constexpr uint32_t kOvfSegs = 4u;     // segment array
+constexpr uint32_t kOvfItems = 8u;    // work items
+constexpr uint32_t kOvfSplit = 16u;   // shared-brick bitmaps
+constexpr uint32_t kOvfRecords = 32u; // update-leaf records
+constexpr uint32_t kItemSegs = 1024;  // segments per brick work item
 
 struct Status {
   unsigned long long total;     // visits of the scan (sum of counts)
@@ -49,6 +58,11 @@ struct Status {
   unsigned int leaf_count;      // map leaves (map insert)
   unsigned int groups;          // leaves of a batch (fold groups)
   unsigned int map_full;
+  unsigned int segs;            // brick segments of the scan (brick path)
+  unsigned int items;           // brick work items
+  unsigned int splits;          // bricks shared by several work items
+  unsigned int records;         // update-leaf records listed by the brick path
+  unsigned int reserved;
   unsigned int pad;
 };
 
@@ -84,6 +98,19 @@ struct GroupView {
   uint32_t cap_mask;
 };
 
+// Bricks (32^3 voxels) of one scan: key -> segment count and fill cursor.
+struct BrickView {
+  unsigned long long* keys;  // kEmpty = free
+  uint32_t* cnt;             // segments in the brick
+  uint32_t* cur;             // next free segment position (fill stage)
+  uint32_t cap_mask;
+};
+// One work item of the brick walk: segments [seg0, seg1) of brick `slot`;
+// `shared` indexes the global bitmap the items of a split brick OR into
+// (kNone: the item owns the brick).
+struct BrickItem {
+  uint32_t slot, seg0, seg1, shared;
+};
 struct MapView {
   unsigned long long* keys;
   uint32_t* idx;
@@ -92,6 +119,7 @@ struct MapView {
   unsigned long long* active;
   unsigned long long* leaf_keys;
   uint8_t* dirty;
+  uint32_t lcap;  // leaf pool capacity
 };
 
 // Topology snapshot used by raytrace: hash sets of leaf, 128^3-node and
@@ -149,6 +177,19 @@ __host__ __device__ inline uint64_t batched_to_leaf_key(uint64_t k) {
   return leaf_key(lx * 8, ly * 8, lz * 8);
 }
 
+// Key of the brick (32^3 voxels) with brick coordinates (bx,by,bz) = voxel >> 5
+// (19-bit fields: voxel coords in [-2^23, 2^23)).
+__host__ __device__ inline uint64_t brick_key(int bx, int by, int bz) {
+  const uint64_t m = (1ull << 19) - 1;
+  return ((uint64_t)(uint32_t)bx & m) << 38 | ((uint64_t)(uint32_t)by & m) << 19 |
+         ((uint64_t)(uint32_t)bz & m);
+}
+__host__ __device__ inline void brick_coords(uint64_t k, int b[3]) {
+  const uint64_t m = (1ull << 19) - 1;
+  b[0] = (int)((int64_t)(((k >> 38) & m) << 45) >> 45);
+  b[1] = (int)((int64_t)(((k >> 19) & m) << 45) >> 45);
+  b[2] = (int)((int64_t)((k & m) << 45) >> 45);
+}
 // Key of the internal node (2^log2dim voxels per side) holding voxel (x,y,z).
 __host__ __device__ inline uint64_t node_key(int x, int y, int z, int log2dim) {
   return leaf_key((x >> log2dim) * 8, (y >> log2dim) * 8, (z >> log2dim) * 8);

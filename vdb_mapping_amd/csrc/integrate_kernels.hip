@@ -11,15 +11,15 @@
 //                 source's update set at leaf-word granularity
 //   k_compact     list of occupied update-set slots + visited-voxel count
 //   k_map_insert  one lane per update leaf: find or create the map leaf
-//   k_apply       one wave per update leaf: clamped log-odds update of the
-//                 visited voxels (OccupancyVDBMapping.hpp:92-117); lane l owns
-//                 bit l of each 64-bit mask word, so the new value mask word is
-//                 one ballot
+//   k_apply       sixteen lanes per 64-voxel mask word of an update leaf:
+//                 clamped log-odds update of the visited voxels
+//                 (OccupancyVDBMapping.hpp:92-117), one float4 per lane
 //
 // Build with -ffp-contract=off: every double operation must round on its own,
 // exactly like the reference's x86-64 build.
 #include <float.h>
 
+#include "dda_math.h"
 #include "device_common.h"
 #include "kernels.h"
 
@@ -138,133 +138,6 @@ struct Run {
     }
     word = w;
     bits = b;
-  }
-};
-
-// Exact order of DDA crossing events without floating point.
-//
-// An axis with |delta| = n (ray from a voxel centre, VDBMapping.hpp:555-564)
-// crosses voxel faces at the times math::DDA accumulates: T(n, 0) = 0.5*fl(1/n),
-// T(n, j) = fl(T(n, j-1) + fl(1/n)); the rational value is (2j+1)/(2n). For
-// n <= 8192, |T(n, j) - (2j+1)/(2n)| <= 2n * 2^-53 < 1.9e-12, while two
-// different rational times differ by at least 1/(2 n_a n_b) >= 7.4e-9. So the
-// double-precision order equals the rational order, which is decided exactly
-// by the integer sign of (2c_b+1) n_a - (2c_a+1) n_b, except at exact rational
-// ties. There the doubles decide (read from the crossing table), and equal
-// doubles go to the larger axis index (math::MinIndex). Axes with equal n have
-// identical double sequences, so their ties need no table read.
-
-// Event (b, cb) precedes event (a, ca) in DDA order.
-__device__ __forceinline__ bool ev_before(const double* __restrict__ tbl, int b, uint32_t nb,
-                                          uint32_t cb, int a, uint32_t na, uint32_t ca) {
-  const int64_t d = (int64_t)(2 * cb + 1) * na - (int64_t)(2 * ca + 1) * nb;
-  if (d != 0) return d < 0;
-  if (nb == na) return b > a;
-  const double fb = tbl[(((uint64_t)nb * (nb - 1)) >> 1) + cb];
-  const double fa = tbl[(((uint64_t)na * (na - 1)) >> 1) + ca];
-  return fb < fa || (fb == fa && b > a);
-}
-
-// Crossing counts c[3] after k steps of a ray with |delta| n[3] (k <= sum n):
-// the first k events of the merged crossing sequences. Starts from the
-// rational estimate at t = k / N (off by at most one event) and repairs it
-// until every included event precedes every excluded one; each repair moves
-// one misplaced event, so a handful of rounds suffice. Returns false only if
-// the repair did not converge (reported as a kernel error, never silently).
-__device__ bool dda_enter(const double* __restrict__ tbl, const uint32_t n[3], uint32_t k,
-                          uint32_t c[3]) {
-  const uint64_t N = (uint64_t)n[0] + n[1] + n[2];
-  int sum = 0;
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    // #{j : (2j+1)/(2n) <= k/N} = floor((2 n k + N) / (2N))
-    const uint64_t g = (2ull * n[a] * k + N) / (2ull * N);
-    c[a] = (uint32_t)(g < n[a] ? g : n[a]);
-    sum += (int)c[a];
-  }
-  for (int round = 0; round < 64; ++round) {
-    // included event that comes last (amax) and excluded event that comes first (amin)
-    int amax = -1, amin = -1;
-    uint32_t nmax = 0, cmax = 0, nmin = 0, cmin = 0;
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      if (c[a] > 0 && (amax < 0 || ev_before(tbl, amax, nmax, cmax - 1, a, n[a], c[a] - 1))) {
-        amax = a;
-        nmax = n[a];
-        cmax = c[a];
-      }
-      if (c[a] < n[a] && (amin < 0 || ev_before(tbl, a, n[a], c[a], amin, nmin, cmin))) {
-        amin = a;
-        nmin = n[a];
-        cmin = c[a];
-      }
-    }
-    int dec = -1, inc = -1;
-    if (sum > (int)k) {
-      dec = amax;
-    } else if (sum < (int)k) {
-      inc = amin;
-    } else if (amax >= 0 && amin >= 0 && ev_before(tbl, amin, nmin, cmin, amax, nmax, cmax - 1)) {
-      dec = amax;
-      inc = amin;
-    } else {
-      return true;
-    }
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      if (a == dec) c[a]--;
-      if (a == inc) c[a]++;
-    }
-    sum += (inc >= 0) - (dec >= 0);
-  }
-  return false;
-}
-
-// Integer DDA state of one ray: crossing counts and the pairwise order
-// differences D_yx = (2c_y+1) n_x - (2c_x+1) n_y, D_zx, D_zy.
-struct IntDda {
-  uint32_t n0, n1, n2, c0, c1, c2;
-  int dyx, dzx, dzy;
-  __device__ __forceinline__ void init(const uint32_t n[3], const uint32_t c[3]) {
-    n0 = n[0];
-    n1 = n[1];
-    n2 = n[2];
-    c0 = c[0];
-    c1 = c[1];
-    c2 = c[2];
-    dyx = (int)((2 * c1 + 1) * n0) - (int)((2 * c0 + 1) * n1);
-    dzx = (int)((2 * c2 + 1) * n0) - (int)((2 * c0 + 1) * n2);
-    dzy = (int)((2 * c2 + 1) * n1) - (int)((2 * c1 + 1) * n2);
-  }
-  // Float tie-break: does (b, cb) come no later than (a, ca) under MinIndex?
-  __device__ __forceinline__ bool tie_le(const double* __restrict__ tbl, uint32_t nb, uint32_t cb,
-                                         uint32_t na, uint32_t ca) const {
-    if (nb == na) return true;
-    if (cb >= nb) return false;
-    if (ca >= na) return true;
-    return tbl[(((uint64_t)nb * (nb - 1)) >> 1) + cb] <= tbl[(((uint64_t)na * (na - 1)) >> 1) + ca];
-  }
-  // Axis of the next step (0, 1, 2).
-  __device__ __forceinline__ int next_axis(const double* __restrict__ tbl) const {
-    const bool yx = dyx < 0 || (dyx == 0 && tie_le(tbl, n1, c1, n0, c0));
-    const int dzm = yx ? dzy : dzx;
-    const bool zm = dzm < 0 || (dzm == 0 && (yx ? tie_le(tbl, n2, c2, n1, c1) : tie_le(tbl, n2, c2, n0, c0)));
-    return zm ? 2 : (yx ? 1 : 0);
-  }
-  __device__ __forceinline__ void advance(int a) {
-    if (a == 0) {
-      ++c0;
-      dyx -= 2 * (int)n1;
-      dzx -= 2 * (int)n2;
-    } else if (a == 1) {
-      ++c1;
-      dyx += 2 * (int)n0;
-      dzy -= 2 * (int)n2;
-    } else {
-      ++c2;
-      dzx += 2 * (int)n0;
-      dzy += 2 * (int)n1;
-    }
   }
 };
 
@@ -612,8 +485,9 @@ __global__ __launch_bounds__(256) void k_map_insert(UpdView u, const uint32_t* _
                                                     uint32_t* __restrict__ eidx,
                                                     uint32_t create_hit, uint32_t create_miss,
                                                     unsigned long long* noop_set,
-                                                    uint32_t noop_mask, Status* st) {
+                                                    uint32_t noop_mask, Status* st, Status* stc) {
   __shared__ uint32_t s_wave[4], s_base;
+  uint32_t listed = 0;
   const uint32_t G = gridDim.x * blockDim.x;
   const uint32_t e0 = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t rounds = (count + G - 1) / G;
@@ -622,8 +496,9 @@ __global__ __launch_bounds__(256) void k_map_insert(UpdView u, const uint32_t* _
     bool created = false;
     uint32_t h = kNone, idx = kNone;
     uint64_t key = 0;
-    if (e < count) {
-      const uint32_t slot = list[e];
+    if (e < count && (list || u.keys[e] != kEmpty)) {
+      ++listed;
+      const uint32_t slot = list ? list[e] : e;
       key = u.keys[slot];
       const unsigned long long* w = &u.masks[(size_t)slot * 16];
       uint64_t anyhit = 0, anymiss = 0;
@@ -660,81 +535,120 @@ __global__ __launch_bounds__(256) void k_map_insert(UpdView u, const uint32_t* _
     }
     const uint32_t pos = block_append(created, &st->leaf_count, s_wave, &s_base);
     if (created) {
-      idx = pos;
-      mp.idx[h] = idx;
-      mp.leaf_keys[idx] = key;
+      if (pos < mp.lcap) {
+        idx = pos;
+        mp.idx[h] = idx;
+        mp.leaf_keys[idx] = key;
+      } else {  // the host sizes the pool first; never write past it
+        mp.idx[h] = kNone;
+        atomicOr(&st->map_full, 1u);
+      }
     }
     if (e < count) {
       eidx[e] = idx;
       if (idx != kNone) mp.dirty[idx] = 1;
     }
   }
+  // update leaves of this call: one atomic per workgroup
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) listed += __shfl_xor(listed, o, 64);
+  if (lane_id() == 0) s_wave[threadIdx.x >> 6] = listed;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += s_wave[w];
+    if (t) atomicAdd(&stc->entries, t);
+  }
 }
 
-// One wave per update leaf (updateMap VDBMapping.hpp:764-783 with
-// updateOccupiedNode / updateFreeNode OccupancyVDBMapping.hpp:92-117):
-// hit -> x += l_hit, above t_max: active, clamp at c_max;
-// miss -> x += l_miss, below t_min: inactive, clamp at c_min.
-// Lane l owns bit l of the 8 mask words, i.e. voxels l, 64+l, ..., 448+l:
-// all loads of a leaf are issued before the first update. Clears the
-// update-set slot afterwards and counts the updated voxels.
+// Clamped log-odds update of one voxel (updateOccupiedNode / updateFreeNode,
+// OccupancyVDBMapping.hpp:92-117): hit -> x += l_hit, above t_max: active,
+// clamp at c_max; miss -> x += l_miss, below t_min: inactive, clamp at c_min.
+__device__ __forceinline__ void update_voxel(float& x, uint32_t& ab, int k, bool visited, bool hit,
+                                             float l_hit, float l_miss, float t_min, float t_max,
+                                             float c_max, float c_min) {
+  if (!visited) return;
+  float v = x;
+  if (hit) {
+    v += l_hit;
+    if (v > t_max) {
+      ab |= 1u << k;
+      if (v > c_max) v = c_max;
+    }
+  } else {
+    v += l_miss;
+    if (v < t_min) {
+      ab &= ~(1u << k);
+      if (v < c_min) v = c_min;
+    }
+  }
+  x = v;
+}
+
+// updateMap (VDBMapping.hpp:764-783): every visited voxel of every listed
+// update leaf gets one clamped log-odds update. Sixteen lanes own one 64-voxel
+// mask word (one x-slab of the leaf, 256 contiguous bytes of values): lane l
+// updates voxels 4l .. 4l+3 with one float4 load and store and takes its
+// four visited / hit / active bits straight from the word (no cross-lane
+// mask transposes); the new active word is OR-reduced over the sixteen
+// lanes. Words with no visited voxel are skipped. Optionally clears the
+// update-set slots afterwards; counts the updated voxels.
 __global__ __launch_bounds__(256) void k_apply(UpdView u, const uint32_t* __restrict__ list,
                                                const uint32_t* __restrict__ eidx, uint32_t count,
                                                MapView mp, float l_hit, float l_miss, float t_min,
-                                               float t_max, float c_max, float c_min, Status* st) {
-  const uint32_t waves = gridDim.x * (blockDim.x >> 6);
-  const int lane = lane_id();
+                                               float t_max, float c_max, float c_min, Status* st,
+                                               uint32_t clear) {
+  const uint64_t G = (uint64_t)gridDim.x * blockDim.x;
+  const uint64_t total = (uint64_t)count * 8 * 16;  // (record, word, lane) triples
+  const int l = lane_id() & 15;
   unsigned long long vox = 0;
-  for (uint32_t e = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); e < count; e += waves) {
-    const uint32_t slot = list[e];
-    const uint32_t idx = eidx[e];
+  for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += G) {
+    // the sixteen lanes of a word group take the same branches below
+    const uint32_t ws = (uint32_t)(t >> 4);
+    const uint32_t e = ws >> 3;
+    const int j = (int)(ws & 7);
+    const uint32_t slot = list ? list[e] : (u.keys[e] != kEmpty ? e : kNone);
+    if (slot == kNone) continue;
     unsigned long long* w = &u.masks[(size_t)slot * 16];
-    uint64_t vis[8], hit[8];
-#pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      vis[j] = w[j];
-      hit[j] = w[8 + j];
-      vox += __popcll(vis[j]);
-    }
-    if (idx != kNone) {
-      float* vals = mp.values + (size_t)idx * 512;
-      unsigned long long* act = mp.active + (size_t)idx * 8;
-      float x[8];
-      uint64_t aw[8];
-#pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        aw[j] = act[j];
-        x[j] = ((vis[j] >> lane) & 1) ? vals[j * 64 + lane] : 0.0f;
+    const uint64_t vis = w[j];
+    const uint64_t hit = w[8 + j];
+    if (clear && l == 0) {
+      if (vis) {
+        w[j] = 0;
+        w[8 + j] = 0;
       }
-#pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        if (vis[j] == 0) continue;
-        bool a = (aw[j] >> lane) & 1;
-        if ((vis[j] >> lane) & 1) {
-          float v = x[j];
-          if ((hit[j] >> lane) & 1) {
-            v += l_hit;
-            if (v > t_max) {
-              a = true;
-              if (v > c_max) v = c_max;
-            }
-          } else {
-            v += l_miss;
-            if (v < t_min) {
-              a = false;
-              if (v < c_min) v = c_min;
-            }
-          }
-          vals[j * 64 + lane] = v;
-        }
-        const uint64_t na = __ballot(a);
-        if (lane == 0) act[j] = na;
-      }
+      if (j == 0) u.keys[slot] = kEmpty;
     }
-    if (lane < 16) w[lane] = 0;
-    if (lane == 0) u.keys[slot] = kEmpty;
+    if (vis == 0) continue;
+    if (l == 0) vox += __popcll(vis);
+    const uint32_t idx = eidx[e];
+    if (idx == kNone) continue;
+    float4* px = reinterpret_cast<float4*>(mp.values + (size_t)idx * 512 + j * 64) + l;
+    unsigned long long* pa = mp.active + (size_t)idx * 8 + j;
+    const uint64_t aw = *pa;
+    float4 x = *px;
+    const uint32_t vb = (uint32_t)(vis >> (4 * l)) & 15u, hb = (uint32_t)(hit >> (4 * l)) & 15u;
+    uint32_t ab = (uint32_t)(aw >> (4 * l)) & 15u;
+    update_voxel(x.x, ab, 0, vb & 1, hb & 1, l_hit, l_miss, t_min, t_max, c_max, c_min);
+    update_voxel(x.y, ab, 1, vb & 2, hb & 2, l_hit, l_miss, t_min, t_max, c_max, c_min);
+    update_voxel(x.z, ab, 2, vb & 4, hb & 4, l_hit, l_miss, t_min, t_max, c_max, c_min);
+    update_voxel(x.w, ab, 3, vb & 8, hb & 8, l_hit, l_miss, t_min, t_max, c_max, c_min);
+    *px = x;  // whole 16 bytes: unvisited voxels are written back unchanged
+    uint64_t na = (uint64_t)ab << (4 * l);
+#pragma unroll
+    for (int o = 1; o < 16; o <<= 1) na |= __shfl_xor(na, o, 16);
+    if (l == 0) *pa = na;
   }
-  if (lane == 0 && vox) atomicAdd(&st->voxels, vox);
+  __shared__ unsigned long long s_vox[4];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) vox += __shfl_xor(vox, o, 64);
+  if (lane_id() == 0) s_vox[threadIdx.x >> 6] = vox;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long tv = 0;
+    for (int k = 0; k < (int)(blockDim.x >> 6); ++k) tv += s_vox[k];
+    if (tv) atomicAdd(&st->voxels, tv);
+  }
 }
 
 // Move every entry of an update set into a larger one.
@@ -864,17 +778,17 @@ hipError_t launch_compact(hipStream_t s, int grid, UpdView u, uint32_t cap, uint
 hipError_t launch_map_insert(hipStream_t s, int grid, UpdView u, const uint32_t* list,
                              uint32_t count, MapView mp, uint32_t* eidx, bool create_hit,
                              bool create_miss, unsigned long long* noop_set, uint32_t noop_mask,
-                             Status* st) {
+                             Status* st, Status* stc) {
   hipLaunchKernelGGL(k_map_insert, dim3(grid), dim3(256), 0, s, u, list, count, mp, eidx,
-                     (uint32_t)create_hit, (uint32_t)create_miss, noop_set, noop_mask, st);
+                     (uint32_t)create_hit, (uint32_t)create_miss, noop_set, noop_mask, st, stc);
   return hipGetLastError();
 }
 
 hipError_t launch_apply(hipStream_t s, int grid, UpdView u, const uint32_t* list,
                         const uint32_t* eidx, uint32_t count, MapView mp, const float lo[6],
-                        Status* st) {
+                        Status* st, bool clear) {
   hipLaunchKernelGGL(k_apply, dim3(grid), dim3(256), 0, s, u, list, eidx, count, mp, lo[0], lo[1],
-                     lo[2], lo[3], lo[4], lo[5], st);
+                     lo[2], lo[3], lo[4], lo[5], st, (uint32_t)clear);
   return hipGetLastError();
 }
 

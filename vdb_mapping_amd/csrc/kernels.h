@@ -39,10 +39,10 @@ hipError_t launch_zero_words(hipStream_t s, unsigned long long* p, uint32_t n);
 hipError_t launch_map_insert(hipStream_t s, int grid, UpdView u, const uint32_t* list,
                              uint32_t count, MapView mp, uint32_t* eidx, bool create_hit,
                              bool create_miss, unsigned long long* noop_set, uint32_t noop_mask,
-                             Status* st);
+                             Status* st, Status* stc);
 hipError_t launch_apply(hipStream_t s, int grid, UpdView u, const uint32_t* list,
                         const uint32_t* eidx, uint32_t count, MapView mp, const float lo[6],
-                        Status* st);
+                        Status* st, bool clear);
 hipError_t launch_rehash_update(hipStream_t s, int grid, UpdView from, uint32_t from_cap,
                                 UpdView to, Status* st);
 hipError_t launch_rehash_map(hipStream_t s, int grid, MapView mp, uint32_t leaf_count);
@@ -52,6 +52,24 @@ hipError_t launch_dirty_list(hipStream_t s, int grid, uint8_t* dirty, uint32_t l
 hipError_t launch_gather_leaves(hipStream_t s, int grid, const uint32_t* list, uint32_t count,
                                 MapView mp, unsigned long long* keys_out, float* vals_out,
                                 unsigned long long* act_out);
+
+hipError_t launch_seg_count(hipStream_t s, int grid, const int4* rayinfo, const ScanInfo* scans,
+                            const BatchCtl* ctl, const double* tbl, BrickView bv, Status* st);
+hipError_t launch_brick_offsets(hipStream_t s, BrickView bv, uint64_t* bsum, uint64_t* boff,
+                                BrickItem* items, uint32_t seg_cap, uint32_t item_cap,
+                                uint32_t* split_slot, uint32_t split_cap, Status* st);
+hipError_t launch_seg_fill(hipStream_t s, int grid, const int4* rayinfo, const ScanInfo* scans,
+                           const BatchCtl* ctl, const double* tbl, BrickView bv, uint4* segs,
+                           uint32_t seg_cap, Status* st);
+hipError_t launch_brick_walk(hipStream_t s, int grid, const BrickItem* items, const uint4* segs,
+                             const int4* rayinfo, const ScanInfo* scans, const double* tbl,
+                             BrickView bv, unsigned long long* shared_bits, UpdView rec,
+                             unsigned long long* leaf_mask, uint32_t rec_cap, Status* st);
+hipError_t launch_rec_list(hipStream_t s, const unsigned long long* leaf_mask, uint64_t* bsum, uint64_t* boff,
+                           uint32_t nb_cap, uint32_t* list, Status* st);
+hipError_t launch_split_emit(hipStream_t s, int grid, const uint32_t* split_slot,
+                             unsigned long long* shared_bits, BrickView bv, UpdView rec,
+                             unsigned long long* leaf_mask, uint32_t rec_cap, Status* st);
 
 hipError_t launch_snapshot(hipStream_t s, MapView mp, uint32_t leaf_count, RtView rt, int* bbox);
 hipError_t launch_copy_set(hipStream_t s, const unsigned long long* src, uint32_t src_cap,

@@ -32,7 +32,7 @@ constexpr uint32_t kMinUpdateCap = 1u << 12;
 constexpr uint32_t kMaxRays = 1u << 30;
 constexpr int kTraverseMaxBlock = 256;  // __launch_bounds__ of k_traverse
 
-enum TimedKind { kPrep = 0, kTraverse = 1, kCompact = 2, kApply = 3, kRaytrace = 4 };
+enum TimedKind { kPrep = 0, kTraverse = 1, kCompact = 2, kApply = 3, kRaytrace = 4, kWalk = 5 };
 
 struct TimedPair {
   int kind;
@@ -61,7 +61,8 @@ struct vdbgpu_map {
   std::mutex mu;
   int device = 0;
   int num_cu = 256;
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;  // map stream: every map read / update, the hash path
+  hipStream_t cast = nullptr;    // brick path casting (prep .. record list) of plain inserts
   double res = 0, inv = 0;
 
   // config (VDBMapping.hpp:1456-1469, OccupancyVDBMapping.hpp:59-89)
@@ -81,6 +82,13 @@ struct vdbgpu_map {
   uint8_t* dirty = nullptr;
   uint32_t lcap = 0;
   uint64_t leaf_count_ub = 0;  // upper bound of the device leaf count
+  // Records whose map insert may still be running on the map stream, for the
+  // leaf-count bound taken between full syncs: win_a = those enqueued after
+  // the previous brick-path apply up to and including the last one, win_b =
+  // those enqueued after the last one. The cast of a brick-path scan waits for
+  // the previous brick apply, so a leaf-count snapshot taken there misses at
+  // most win_a + win_b leaves.
+  uint64_t win_a = 0, win_b = 0;
   bool touched = false;        // any voxel modify since creation/reset
   unsigned long long* noop4096 = nullptr;  // root children of updates that created no leaf
   static constexpr uint32_t kNoopCap = 4096;
@@ -120,6 +128,32 @@ struct vdbgpu_map {
   uint64_t chunk_cap = 0;
   uint32_t* eidx = nullptr;
   uint32_t eidx_cap = 0;
+  // brick path (plain inserts): brick table, segments, work items, shared
+  // bitmaps of split bricks, update-leaf records
+  unsigned long long* bkeys = nullptr;
+  uint32_t* bcnt = nullptr;  // [0, bcap): segment counts, [bcap, 2 bcap): fill cursors
+  uint64_t* bbsum = nullptr;
+  uint64_t* bboff = nullptr;
+  uint32_t bcap = 0;
+  uint4* segs = nullptr;
+  uint32_t seg_cap = 0;
+  BrickItem* items = nullptr;
+  uint32_t item_cap = 0;
+  uint32_t* split_slot = nullptr;
+  unsigned long long* shared_bits = nullptr;
+  uint32_t split_cap = 0;
+  // update-leaf records, double-buffered: the cast of scan i+1 fills one set
+  // while the map stream still applies scan i from the other
+  unsigned long long* rkeys[2] = {nullptr, nullptr};
+  unsigned long long* rmasks[2] = {nullptr, nullptr};
+  uint32_t* rlist[2] = {nullptr, nullptr};  // filled record positions (dense)
+  hipEvent_t rec_free[2] = {nullptr, nullptr};  // recorded on the map stream after the last read
+  int rec_cur = 0;
+  uint32_t rec_cap = 0;        // record positions: 64 per work item and per split brick
+  unsigned long long* leaf_mask = nullptr;  // filled positions per 64-position group
+  uint64_t* lbsum = nullptr;
+  uint64_t* lboff = nullptr;
+  uint32_t group_cap = 0;      // groups: work items + split bricks
   double* tbl = nullptr;
   uint32_t tbl_n = 0;
 
@@ -127,7 +161,7 @@ struct vdbgpu_map {
   Status* d_st = nullptr;
   Status* h_st = nullptr;
 
-  vdbgpu_tuning tuning{256, 32, 16, 0.5};
+  vdbgpu_tuning tuning{256, 32, 16, 0.5, 0, 0};
   bool profiling = false;
   std::vector<TimedPair> timed;
   std::vector<hipEvent_t> event_pool;
@@ -179,25 +213,32 @@ hipEvent_t get_event(vdbgpu_map* m) {
 struct Timer {
   vdbgpu_map* m;
   int kind;
+  hipStream_t s;
   hipEvent_t a = nullptr;
-  Timer(vdbgpu_map* mm, int k) : m(mm), kind(k) {
+  Timer(vdbgpu_map* mm, int k, hipStream_t st = nullptr) : m(mm), kind(k), s(st ? st : mm->stream) {
     if (m->profiling) {
       a = get_event(m);
-      (void)hipEventRecord(a, m->stream);
+      (void)hipEventRecord(a, s);
     }
   }
   ~Timer() {
     if (a) {
       hipEvent_t b = get_event(m);
-      (void)hipEventRecord(b, m->stream);
+      (void)hipEventRecord(b, s);
       m->timed.push_back({kind, a, b});
     }
   }
 };
 
-// Fold finished event pairs into the stats (call after a stream sync).
+// Fold finished event pairs into the stats (pairs still in flight on the
+// other stream stay queued).
 void collect_timing(vdbgpu_map* m) {
+  std::vector<TimedPair> pending;
   for (auto& t : m->timed) {
+    if (hipEventQuery(t.b) != hipSuccess) {
+      pending.push_back(t);
+      continue;
+    }
     float ms = 0;
     if (hipEventElapsedTime(&ms, t.a, t.b) == hipSuccess) {
       switch (t.kind) {
@@ -205,13 +246,19 @@ void collect_timing(vdbgpu_map* m) {
         case kTraverse: m->stats.ms_traverse += ms; m->stats.n_traverse++; break;
         case kCompact: m->stats.ms_compact += ms; m->stats.n_compact++; break;
         case kRaytrace: m->stats.ms_raytrace += ms; m->stats.n_raytrace++; break;
+        case kWalk: m->stats.ms_walk += ms; m->stats.n_walk++; break;
         default: m->stats.ms_apply += ms; m->stats.n_apply++; break;
       }
     }
     m->event_pool.push_back(t.a);
     m->event_pool.push_back(t.b);
   }
-  m->timed.clear();
+  m->timed.swap(pending);
+}
+
+int fail(vdbgpu_map* m, int code, const std::string& msg) {
+  m->last_error = msg;
+  return code;
 }
 
 int read_status(vdbgpu_map* m) {
@@ -219,6 +266,9 @@ int read_status(vdbgpu_map* m) {
   CK(hipStreamSynchronize(m->stream));
   collect_timing(m);
   m->leaf_count_ub = m->h_st[1].leaf_count;
+  m->win_a = m->win_b = 0;
+  if (m->h_st[1].map_full)
+    return fail(m, VDBGPU_DEVICE_ERROR, "leaf pool overflow in a map insert (internal bound violated)");
   return VDBGPU_OK;
 }
 
@@ -236,7 +286,7 @@ int traverse_grid(vdbgpu_map* m, int block) {
 UpdView upd_view(const Source& s) { return UpdView{s.keys, s.masks, s.cap - 1}; }
 
 MapView map_view(vdbgpu_map* m) {
-  return MapView{m->mkeys, m->midx, m->mcap - 1, m->values, m->active, m->leaf_keys, m->dirty};
+  return MapView{m->mkeys, m->midx, m->mcap - 1, m->values, m->active, m->leaf_keys, m->dirty, m->lcap};
 }
 
 int alloc_update(vdbgpu_map* m, Source& s, uint32_t cap) {
@@ -300,14 +350,14 @@ int ensure_chunks(vdbgpu_map* m, uint64_t need) {
   return VDBGPU_OK;
 }
 
-int ensure_table(vdbgpu_map* m, uint32_t need) {
+int ensure_table(vdbgpu_map* m, uint32_t need, hipStream_t stream) {
   need = std::min(need, kMaxTableN);
   if (need <= m->tbl_n) return VDBGPU_OK;
   const uint32_t n = std::min(kMaxTableN, (need + 255u) & ~255u);
   const uint64_t old_entries = (uint64_t)m->tbl_n * (m->tbl_n + 1) / 2;
   dfree(m, &m->tbl, old_entries);
   CK(dalloc(m, &m->tbl, (uint64_t)n * (n + 1) / 2));
-  CK(launch_build_table(m->stream, m->tbl, 1, n));
+  CK(launch_build_table(stream, m->tbl, 1, n));
   m->tbl_n = n;
   return VDBGPU_OK;
 }
@@ -362,11 +412,6 @@ int ensure_map(vdbgpu_map* m, uint64_t need) {
   return VDBGPU_OK;
 }
 
-int fail(vdbgpu_map* m, int code, const std::string& msg) {
-  m->last_error = msg;
-  return code;
-}
-
 void create_flags(const vdbgpu_map* m, bool* ch, bool* cm);
 
 int ensure_groups(vdbgpu_map* m, uint32_t cap) {
@@ -418,28 +463,24 @@ int launch_fold_stage(vdbgpu_map* m, Source& src, bool from_groups) {
 // the map. One status read at the end; an update-set overflow grows the set
 // and re-runs the traversal (marks are idempotent), a full map grows the leaf
 // pool and re-runs the fold (nothing has been applied before either check).
-int run_pipeline(vdbgpu_map* m, Source& src, const float* xyz, int64_t stride, bool on_device,
-                 const std::vector<ScanInfo>& scans, uint32_t n_total, bool batched) {
+// Stage the points and the scan table, run the per-point prep (rays, end
+// voxels, visit counts and their block prefix, st[0].total). Returns the
+// visit bound of the scans in *visits_bound.
+int stage_scans(vdbgpu_map* m, hipStream_t stream, const float* xyz, int64_t stride, bool on_device,
+                const std::vector<ScanInfo>& scans, uint32_t n_total, bool batched,
+                uint64_t* visits_bound) {
   int r;
   if ((r = ensure_rays(m, n_total))) return r;
   uint32_t ext_max = 1;
-  uint64_t visits_bound = 0;
+  uint64_t vb = 0;
   for (const ScanInfo& sc : scans) {
     const double ext = std::ceil(sc.R * m->inv) + 2.0;
     const uint32_t e = ext < (double)kMaxTableN ? (uint32_t)ext : kMaxTableN;
     ext_max = std::max(ext_max, e);
-    visits_bound += (uint64_t)sc.nrays * (1 + 3ull * e);  // visits per ray <= 1 + 3 * extent
+    vb += (uint64_t)sc.nrays * (1 + 3ull * e);  // visits per ray <= 1 + 3 * extent
   }
-  if ((r = ensure_table(m, ext_max))) return r;
-  const uint32_t chunk = (uint32_t)m->tuning.steps_per_lane;
-  const uint64_t max_chunks = (visits_bound + chunk - 1) / chunk + 1;
-  if (max_chunks > (1ull << 29))
-    return fail(m, VDBGPU_BAD_ARGUMENT, "steps_per_lane too small for this scan (chunk table > 2 GiB)");
-  if ((r = ensure_chunks(m, max_chunks))) return r;
-  if (src.cap == 0) {
-    if ((r = alloc_update(m, src, (uint32_t)std::max<uint64_t>(kMinUpdateCap, next_pow2(4 * (uint64_t)n_total))))) return r;
-  }
-  if (batched && (r = ensure_groups(m, src.cap))) return r;
+  *visits_bound = vb;
+  if ((r = ensure_table(m, ext_max, stream))) return r;
   const float* pts = xyz;
   if (!on_device) {
     const size_t bytes = (size_t)(n_total - 1) * stride + 12;
@@ -449,21 +490,57 @@ int run_pipeline(vdbgpu_map* m, Source& src, const float* xyz, int64_t stride, b
       CK(dalloc(m, &m->d_points, (cap + 3) / 4));
       m->points_bytes = ((cap + 3) / 4) * 4;
     }
-    CK(hipMemcpyAsync(m->d_points, xyz, bytes, hipMemcpyHostToDevice, m->stream));
+    CK(hipMemcpyAsync(m->d_points, xyz, bytes, hipMemcpyHostToDevice, stream));
     pts = m->d_points;
   }
   const uint32_t nb = (n_total + kPrepBlock - 1) / kPrepBlock;
   std::copy(scans.begin(), scans.end(), m->h_scans);
   *m->h_ctl = BatchCtl{n_total, nb, (uint32_t)scans.size(), batched ? 1u : 0u};
-  CK(hipMemcpyAsync(m->d_scans, m->h_scans, scans.size() * sizeof(ScanInfo), hipMemcpyHostToDevice, m->stream));
-  CK(hipMemcpyAsync(m->d_ctl, m->h_ctl, sizeof(BatchCtl), hipMemcpyHostToDevice, m->stream));
-  CK(hipMemsetAsync(&m->d_st[0], 0, sizeof(Status), m->stream));
+  CK(hipMemcpyAsync(m->d_scans, m->h_scans, scans.size() * sizeof(ScanInfo), hipMemcpyHostToDevice, stream));
+  CK(hipMemcpyAsync(m->d_ctl, m->h_ctl, sizeof(BatchCtl), hipMemcpyHostToDevice, stream));
+  CK(hipMemsetAsync(&m->d_st[0], 0, sizeof(Status), stream));
   const bool vec4 = stride == 16 && (reinterpret_cast<uintptr_t>(pts) & 15) == 0;
+  Timer t(m, kPrep, stream);
+  CK(launch_prep(stream, (int)nb, pts, stride, vec4, m->d_scans, m->d_ctl, m->res, m->inv, m->tbl_n,
+                 m->rayinfo, m->pre, m->bsum, &m->d_st[0]));
+  CK(launch_scan_blocks(stream, m->bsum, m->boff, m->d_ctl, &m->d_st[0]));
+  return VDBGPU_OK;
+}
+
+// Status code of the kernel error bits of a scan (0 = none).
+int scan_error(vdbgpu_map* m, const Status& st, bool batched) {
+  if (st.error & kErrRange)
+    return fail(m, VDBGPU_OUT_OF_RANGE, batched ? "a voxel index is outside [-2^21, 2^21) (batch) or not finite; batch rejected"
+                                                : "a voxel index is outside [-2^23, 2^23) or not finite; scan rejected");
+  if (st.error & kErrTable) return fail(m, VDBGPU_UNSUPPORTED, "a ray spans more than 8192 voxels along one axis");
+  if (st.error & kErrEntry) return fail(m, VDBGPU_DEVICE_ERROR, "mid-ray DDA entry did not converge");
+  if (st.error & kErrBrick) return fail(m, VDBGPU_DEVICE_ERROR, "brick walk inconsistency (a segment left its brick)");
+  return VDBGPU_OK;
+}
+
+// Casts `scans` (rays concatenated in `xyz`) into `src`'s update set: prep,
+// visit-count scan, ray offsets, traversal, compaction. A batch (batched =
+// true, keys carry the scan index) continues with the scan-ordered fold into
+// the map. One status read at the end; an update-set overflow grows the set
+// and re-runs the traversal (marks are idempotent), a full map grows the leaf
+// pool and re-runs the fold (nothing has been applied before either check).
+int run_pipeline(vdbgpu_map* m, Source& src, const float* xyz, int64_t stride, bool on_device,
+                 const std::vector<ScanInfo>& scans, uint32_t n_total, bool batched) {
+  int r;
+  uint64_t visits_bound = 0;
+  if ((r = stage_scans(m, m->stream, xyz, stride, on_device, scans, n_total, batched, &visits_bound))) return r;
+  const uint32_t chunk = (uint32_t)m->tuning.steps_per_lane;
+  const uint64_t max_chunks = (visits_bound + chunk - 1) / chunk + 1;
+  if (max_chunks > (1ull << 29))
+    return fail(m, VDBGPU_BAD_ARGUMENT, "steps_per_lane too small for this scan (chunk table > 2 GiB)");
+  if ((r = ensure_chunks(m, max_chunks))) return r;
+  if (src.cap == 0) {
+    if ((r = alloc_update(m, src, (uint32_t)std::max<uint64_t>(kMinUpdateCap, next_pow2(4 * (uint64_t)n_total))))) return r;
+  }
+  if (batched && (r = ensure_groups(m, src.cap))) return r;
+  const uint32_t nb = (n_total + kPrepBlock - 1) / kPrepBlock;
   {
     Timer t(m, kPrep);
-    CK(launch_prep(m->stream, (int)nb, pts, stride, vec4, m->d_scans, m->d_ctl, m->res, m->inv, m->tbl_n,
-                   m->rayinfo, m->pre, m->bsum, &m->d_st[0]));
-    CK(launch_scan_blocks(m->stream, m->bsum, m->boff, m->d_ctl, &m->d_st[0]));
     CK(launch_ray_offsets(m->stream, (int)nb, m->rayinfo, m->pre, m->boff, m->d_ctl, chunk, m->roff,
                           m->first_ray, m->chunk_cap, &m->d_st[0]));
   }
@@ -488,13 +565,7 @@ int run_pipeline(vdbgpu_map* m, Source& src, const float* xyz, int64_t stride, b
     }
     if ((r = read_status(m))) return r;
     const Status& st = m->h_st[0];
-    if (st.error & kErrRange)
-      return fail(m, VDBGPU_OUT_OF_RANGE, batched ? "a voxel index is outside [-2^21, 2^21) (batch) or not finite; batch rejected"
-                                                  : "a voxel index is outside [-2^23, 2^23) or not finite; scan rejected");
-    if (st.error & kErrTable)
-      return fail(m, VDBGPU_UNSUPPORTED, "a ray spans more than 8192 voxels along one axis");
-    if (st.error & kErrEntry)
-      return fail(m, VDBGPU_DEVICE_ERROR, "mid-ray DDA entry did not converge");
+    if ((r = scan_error(m, st, batched))) return r;
     if (attempt > 8) return fail(m, VDBGPU_DEVICE_ERROR, "update set or map cannot grow");
     if (st.overflow & kOvfHard) {
       if (src.cap >= (1u << 30)) return fail(m, VDBGPU_DEVICE_ERROR, "update set cannot grow");
@@ -556,6 +627,199 @@ int do_accumulate(vdbgpu_map* m, Source& src, const float* xyz, int64_t n, int64
   sc.ray0 = 0;
   sc.nrays = (uint32_t)n;
   return run_pipeline(m, src, xyz, stride, on_device, {sc}, (uint32_t)n, false);
+}
+
+// Brick path tables large enough for the given counts (grown, never shrunk).
+int ensure_bricks(vdbgpu_map* m, uint32_t bcap, uint32_t seg_cap, uint32_t split_cap, uint32_t rec_cap) {
+  if (bcap > m->bcap) {
+    dfree(m, &m->bkeys, m->bcap);
+    dfree(m, &m->bcnt, (size_t)m->bcap * 2);
+    dfree(m, &m->bbsum, m->bcap / 256);
+    dfree(m, &m->bboff, m->bcap / 256);
+    m->bcap = bcap;
+    CK(dalloc(m, &m->bkeys, bcap));
+    CK(dalloc(m, &m->bcnt, (size_t)bcap * 2));
+    CK(dalloc(m, &m->bbsum, bcap / 256));
+    CK(dalloc(m, &m->bboff, bcap / 256));
+  }
+  if (seg_cap > m->seg_cap) {
+    dfree(m, &m->segs, m->seg_cap);
+    m->seg_cap = seg_cap;
+    CK(dalloc(m, &m->segs, seg_cap));
+  }
+  // work items: at most one partial item per brick plus seg_cap / kItemSegs full ones
+  const uint32_t item_need = m->bcap + m->seg_cap / kItemSegs + 1;
+  if (item_need > m->item_cap) {
+    dfree(m, &m->items, m->item_cap);
+    m->item_cap = item_need;
+    CK(dalloc(m, &m->items, item_need));
+  }
+  if (split_cap > m->split_cap) {
+    dfree(m, &m->split_slot, m->split_cap);
+    dfree(m, &m->shared_bits, (size_t)m->split_cap * 1024);
+    m->split_cap = split_cap;
+    CK(dalloc(m, &m->split_slot, split_cap));
+    CK(dalloc(m, &m->shared_bits, (size_t)split_cap * 1024));
+    CK(launch_fill_u64(m->cast, m->shared_bits, (uint64_t)split_cap * 1024, 0));
+  }
+  if (rec_cap > m->rec_cap) {
+    for (int i = 0; i < 2; ++i) {
+      dfree(m, &m->rkeys[i], m->rec_cap);
+      dfree(m, &m->rmasks[i], (size_t)m->rec_cap * 16);
+      dfree(m, &m->rlist[i], m->rec_cap);
+    }
+    m->rec_cap = rec_cap;
+    for (int i = 0; i < 2; ++i) {
+      CK(dalloc(m, &m->rkeys[i], rec_cap));
+      CK(dalloc(m, &m->rmasks[i], (size_t)rec_cap * 16));
+      CK(dalloc(m, &m->rlist[i], rec_cap));
+    }
+  }
+  const uint32_t groups = (m->item_cap + m->split_cap + 255) & ~255u;
+  if (groups > m->group_cap) {
+    dfree(m, &m->leaf_mask, m->group_cap);
+    dfree(m, &m->lbsum, m->group_cap / 256);
+    dfree(m, &m->lboff, m->group_cap / 256);
+    m->group_cap = groups;
+    CK(dalloc(m, &m->leaf_mask, groups));
+    CK(dalloc(m, &m->lbsum, groups / 256));
+    CK(dalloc(m, &m->lboff, groups / 256));
+  }
+  return VDBGPU_OK;
+}
+
+uint32_t grow_cap(uint64_t need, uint32_t cur) {
+  const uint64_t want = std::max<uint64_t>(need + need / 4, (uint64_t)cur * 2);
+  return (uint32_t)std::min<uint64_t>(want, 0xC0000000ull);
+}
+
+// insertPointCloud of one scan when no source holds pending updates: the
+// brick path casts the scan into leaf records and applies them at once
+// (equal to accumulateUpdate + integrateUpdate). Table overflows grow the
+// table and re-run the brick stages (nothing is applied before the check).
+int run_brick(vdbgpu_map* m, Source& src, const float* xyz, int64_t n, int64_t stride,
+              const double origin[3], bool on_device) {
+  ScanInfo sc{};
+  for (int a = 0; a < 3; ++a) sc.o[a] = origin[a];
+  sc.R = src.max_range;
+  sc.ray0 = 0;
+  sc.nrays = (uint32_t)n;
+  int r;
+  uint64_t vb = 0;
+  hipStream_t cs = m->cast;
+  if ((r = stage_scans(m, cs, xyz, stride, on_device, {sc}, (uint32_t)n, false, &vb))) return r;
+  if (m->bcap == 0) {
+    const uint64_t segs0 = std::min<uint64_t>(vb, 16ull * (uint64_t)n);
+    if ((r = ensure_bricks(m, (uint32_t)std::max<uint64_t>(4096, next_pow2((uint64_t)n / 2)),
+                           (uint32_t)std::max<uint64_t>(1u << 16, std::min<uint64_t>(segs0, 0xC0000000ull)), 256,
+                           (uint32_t)std::max<uint64_t>(1u << 16, std::min<uint64_t>(8ull * n, 0xC0000000ull)))))
+      return r;
+  }
+  const int nb = (int)((n + kPrepBlock - 1) / kPrepBlock);
+  // record set X: free once the map stream has applied the scan that last used it
+  const int X = m->rec_cur;
+  CK(hipStreamWaitEvent(cs, m->rec_free[X], 0));
+  for (int attempt = 0;; ++attempt) {
+    const BrickView bv{m->bkeys, m->bcnt, m->bcnt + m->bcap, m->bcap - 1};
+    const UpdView rec{m->rkeys[X], m->rmasks[X], 0};
+    CK(launch_fill_u64(cs, m->bkeys, m->bcap, kEmpty));
+    CK(launch_fill_u64(cs, reinterpret_cast<unsigned long long*>(m->bcnt), m->bcap, 0));
+    {
+      Timer t(m, kTraverse, cs);
+      CK(launch_seg_count(cs, nb, m->rayinfo, m->d_scans, m->d_ctl, m->tbl, bv, &m->d_st[0]));
+      CK(launch_brick_offsets(cs, bv, m->bbsum, m->bboff, m->items, m->seg_cap, m->item_cap,
+                              m->split_slot, m->split_cap, &m->d_st[0]));
+      CK(launch_seg_fill(cs, nb, m->rayinfo, m->d_scans, m->d_ctl, m->tbl, bv, m->segs, m->seg_cap,
+                         &m->d_st[0]));
+    }
+    {
+      Timer t(m, kWalk, cs);
+      CK(launch_brick_walk(cs, m->num_cu * 16, m->items, m->segs, m->rayinfo, m->d_scans, m->tbl, bv,
+                           m->shared_bits, rec, m->leaf_mask, m->rec_cap, &m->d_st[0]));
+    }
+    {
+      Timer t(m, kCompact, cs);
+      CK(launch_split_emit(cs, m->num_cu, m->split_slot, m->shared_bits, bv, rec, m->leaf_mask,
+                           m->rec_cap, &m->d_st[0]));
+      CK(launch_rec_list(cs, m->leaf_mask, m->lbsum, m->lboff, m->group_cap / 256, m->rlist[X],
+                         &m->d_st[0]));
+    }
+    // the scan's status and a snapshot of the leaf count: the map stream keeps
+    // applying the previous scan meanwhile
+    CK(hipMemcpyAsync(m->h_st, m->d_st, 2 * sizeof(Status), hipMemcpyDeviceToHost, cs));
+    CK(hipStreamSynchronize(cs));
+    collect_timing(m);
+    m->leaf_count_ub = (uint64_t)m->h_st[1].leaf_count + m->win_a + m->win_b;
+    if (m->h_st[1].map_full)
+      return fail(m, VDBGPU_DEVICE_ERROR, "leaf pool overflow in a map insert (internal bound violated)");
+    const Status st = m->h_st[0];
+    if ((r = scan_error(m, st, false))) return r;
+    if (st.overflow == 0) break;
+    if (attempt > 8) return fail(m, VDBGPU_DEVICE_ERROR, "brick tables cannot grow");
+    const uint32_t bcap = (st.overflow & kOvfBricks) ? (uint32_t)std::min<uint64_t>(1u << 30, (uint64_t)m->bcap * 4) : m->bcap;
+    const uint32_t seg_cap = st.segs > m->seg_cap ? grow_cap(st.segs, m->seg_cap) : m->seg_cap;
+    const uint32_t split_cap = st.splits > m->split_cap ? grow_cap(st.splits, m->split_cap) : m->split_cap;
+    const uint64_t positions = (uint64_t)(st.items + st.splits) * 64;
+    const uint32_t rec_cap = positions > m->rec_cap ? grow_cap(positions, m->rec_cap) : m->rec_cap;
+    if ((st.overflow & kOvfBricks) && bcap == m->bcap) return fail(m, VDBGPU_DEVICE_ERROR, "brick table cannot grow");
+    if ((r = ensure_bricks(m, bcap, seg_cap, split_cap, rec_cap))) return r;
+    // split-brick bitmaps may hold partial ORs of the stopped run
+    CK(launch_fill_u64(cs, m->shared_bits, (uint64_t)m->split_cap * 1024, 0));
+    CK(hipMemsetAsync(&m->d_st[0].overflow, 0, sizeof(unsigned int), cs));
+    CK(hipMemsetAsync(&m->d_st[0].segs, 0, 6 * sizeof(unsigned int), cs));  // segs .. pad
+  }
+  const Status& st = m->h_st[0];
+  const uint32_t recs = st.records;  // listed in m->rlist[X]
+  m->stats.scans += 1;
+  m->stats.rays += st.rays;
+  m->stats.visits += st.total;
+  m->rec_cur ^= 1;
+  if (recs == 0) {
+    CK(hipEventRecord(m->rec_free[X], m->stream));  // orders the next scan's snapshot as usual
+    m->win_a = m->win_b;
+    m->win_b = 0;
+    return VDBGPU_OK;
+  }
+  if ((r = ensure_map(m, m->leaf_count_ub + recs))) return r;
+  if (recs > m->eidx_cap) {
+    dfree(m, &m->eidx, m->eidx_cap);
+    m->eidx_cap = (uint32_t)next_pow2(recs);
+    CK(dalloc(m, &m->eidx, m->eidx_cap));
+  }
+  bool ch, cm;
+  create_flags(m, &ch, &cm);
+  const UpdView rec{m->rkeys[X], m->rmasks[X], 0};
+  const int g1 = std::min<int>(4096, (int)((recs + 255) / 256));
+  const int g2 = std::min<int>(m->num_cu * 16, (int)((recs + 1) / 2));  // 128 lanes per leaf
+  {
+    Timer t(m, kCompact);
+    CK(launch_map_insert(m->stream, g1, rec, m->rlist[X], recs, map_view(m), m->eidx, ch, cm, m->noop4096,
+                         vdbgpu_map::kNoopCap - 1, &m->d_st[1], &m->d_st[3]));
+  }
+  {
+    Timer t(m, kApply);
+    CK(launch_apply(m->stream, g2, rec, m->rlist[X], m->eidx, recs, map_view(m), m->lo, &m->d_st[3], false));
+  }
+  CK(hipEventRecord(m->rec_free[X], m->stream));
+  m->leaf_count_ub += recs;
+  m->win_a = m->win_b + recs;
+  m->win_b = 0;
+  m->touched = true;
+  return VDBGPU_OK;
+}
+
+// The raycast half of insertPointCloud: the brick path when it applies
+// (no pending updates anywhere, tuning.traversal == 0), else the update-set
+// traversal (accumulateUpdate); the caller integrates afterwards.
+int insert_cast(vdbgpu_map* m, Source& src, const float* xyz, int64_t n, int64_t stride,
+                const double origin[3], bool on_device) {
+  bool pending = false;
+  for (auto& kv : m->sources) pending = pending || kv.second.pending > 0;
+  const bool args_ok = !(n < 0 || stride < 12 || (stride & 3) || n > (int64_t)kMaxRays || !origin || (n && !xyz));
+  if (m->tuning.traversal == 0 && !pending && m->config_set && !m->fast_mode && args_ok && n > 0 &&
+      src.max_range > 0)
+    return run_brick(m, src, xyz, n, stride, origin, on_device);
+  return do_accumulate(m, src, xyz, n, stride, origin, on_device);
 }
 
 // Creation rule for a missing leaf: the op must change a background voxel
@@ -630,16 +894,19 @@ int do_integrate(vdbgpu_map* m) {
       CK(dalloc(m, &m->eidx, m->eidx_cap));
     }
     const int g1 = std::min<int>(4096, (int)((s.pending + 255) / 256));
-    const int g2 = std::min<int>(m->num_cu * 16, (int)((s.pending + 3) / 4));
+    const int g2 = std::min<int>(m->num_cu * 16, (int)((s.pending + 1) / 2));  // 128 lanes per leaf
+    {
+      Timer t(m, kCompact);
+      CK(launch_map_insert(m->stream, g1, upd_view(s), s.list, s.pending, map_view(m), m->eidx, ch, cm,
+                           m->noop4096, vdbgpu_map::kNoopCap - 1, &m->d_st[1], &m->d_st[3]));
+    }
     {
       Timer t(m, kApply);
-      CK(launch_map_insert(m->stream, g1, upd_view(s), s.list, s.pending, map_view(m), m->eidx, ch, cm,
-                           m->noop4096, vdbgpu_map::kNoopCap - 1, &m->d_st[1]));
       CK(launch_apply(m->stream, g2, upd_view(s), s.list, m->eidx, s.pending, map_view(m), m->lo,
-                      &m->d_st[3]));
+                      &m->d_st[3], true));
     }
     m->leaf_count_ub += s.pending;
-    m->stats.update_leaves += s.pending;
+    m->win_b += s.pending;
     m->touched = true;
     s.pending = 0;
   }
@@ -723,6 +990,9 @@ int32_t vdbgpu_create(double resolution, vdbgpu_map** out) {
   if (hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, m->device) == hipSuccess && cu > 0)
     m->num_cu = cu;
   if (hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&m->cast, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&m->rec_free[0], hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&m->rec_free[1], hipEventDisableTiming) != hipSuccess ||
       hipMalloc((void**)&m->d_st, 4 * sizeof(Status)) != hipSuccess ||
       hipHostMalloc((void**)&m->h_st, 4 * sizeof(Status), hipHostMallocDefault) != hipSuccess ||
       hipMemsetAsync(m->d_st, 0, 4 * sizeof(Status), m->stream) != hipSuccess) {
@@ -747,6 +1017,7 @@ void vdbgpu_destroy(vdbgpu_map* m) {
   if (!m) return;
   (void)hipSetDevice(m->device);
   if (m->stream) (void)hipStreamSynchronize(m->stream);
+  if (m->cast) (void)hipStreamSynchronize(m->cast);
   for (auto& kv : m->sources) free_update(m, kv.second);
   dfree(m, &m->mkeys, m->mcap);
   dfree(m, &m->midx, m->mcap);
@@ -761,6 +1032,23 @@ void vdbgpu_destroy(vdbgpu_map* m) {
   dfree(m, &m->bsum, nb);
   dfree(m, &m->boff, nb);
   dfree(m, &m->eidx, m->eidx_cap);
+  dfree(m, &m->bkeys, m->bcap);
+  dfree(m, &m->bcnt, (size_t)m->bcap * 2);
+  dfree(m, &m->bbsum, m->bcap / 256);
+  dfree(m, &m->bboff, m->bcap / 256);
+  dfree(m, &m->segs, m->seg_cap);
+  dfree(m, &m->items, m->item_cap);
+  dfree(m, &m->split_slot, m->split_cap);
+  dfree(m, &m->shared_bits, (size_t)m->split_cap * 1024);
+  for (int i = 0; i < 2; ++i) {
+    dfree(m, &m->rkeys[i], m->rec_cap);
+    dfree(m, &m->rmasks[i], (size_t)m->rec_cap * 16);
+    dfree(m, &m->rlist[i], m->rec_cap);
+    if (m->rec_free[i]) (void)hipEventDestroy(m->rec_free[i]);
+  }
+  dfree(m, &m->leaf_mask, m->group_cap);
+  dfree(m, &m->lbsum, m->group_cap / 256);
+  dfree(m, &m->lboff, m->group_cap / 256);
   dfree(m, &m->roff, m->ray_cap);
   dfree(m, &m->first_ray, m->chunk_cap);
   dfree(m, &m->tbl, (uint64_t)m->tbl_n * (m->tbl_n + 1) / 2);
@@ -789,6 +1077,7 @@ void vdbgpu_destroy(vdbgpu_map* m) {
   }
   for (auto e : m->event_pool) (void)hipEventDestroy(e);
   if (m->stream) (void)hipStreamDestroy(m->stream);
+  if (m->cast) (void)hipStreamDestroy(m->cast);
   delete m;
 }
 
@@ -871,7 +1160,7 @@ int32_t vdbgpu_insert(vdbgpu_map* m, const char* id, const float* xyz, int64_t n
   if (it == m->sources.end())
     status = fail(m, VDBGPU_UNKNOWN_SOURCE, std::string("Tried to accumulate update for ") + id + ". Source not available");
   else if (it->second.max_range > 0)
-    status = do_accumulate(m, it->second, xyz, n, stride, origin, on_device != 0);
+    status = insert_cast(m, it->second, xyz, n, stride, origin, on_device != 0);
   const int r = do_integrate(m);
   return status ? status : r;
 }
@@ -914,7 +1203,7 @@ int32_t vdbgpu_insert_batch(vdbgpu_map* m, const char* id, int32_t nscans, const
     src.max_range = scan_range(i);
     int st = VDBGPU_OK;
     if (src.max_range > 0 && counts[i] > 0)
-      st = do_accumulate(m, src, scan_ptr(i), counts[i], stride, origins + 3 * i, on_device != 0);
+      st = insert_cast(m, src, scan_ptr(i), counts[i], stride, origins + 3 * i, on_device != 0);
     src.max_range = keep;
     const int r = do_integrate(m);
     return st ? st : r;
@@ -981,6 +1270,7 @@ int32_t vdbgpu_reset(vdbgpu_map* m) {
     s.pending = 0;
   }
   m->leaf_count_ub = 0;
+  m->win_a = m->win_b = 0;
   m->touched = false;
   CK(hipStreamSynchronize(m->stream));
   return VDBGPU_OK;
@@ -1184,9 +1474,16 @@ int32_t vdbgpu_device_download(void* dst, const void* src, int64_t bytes) {
                                                                                  : VDBGPU_DEVICE_ERROR;
 }
 
+int32_t vdbgpu_device_copy(void* dst, const void* src, int64_t bytes) {
+  if (!dst || !src || bytes < 0) return VDBGPU_BAD_ARGUMENT;
+  return hipMemcpy(dst, src, (size_t)bytes, hipMemcpyDeviceToDevice) == hipSuccess ? VDBGPU_OK
+                                                                                   : VDBGPU_DEVICE_ERROR;
+}
+
 int32_t vdbgpu_synchronize(vdbgpu_map* m) {
   if (!m) return VDBGPU_BAD_ARGUMENT;
   Guard g(m);
+  CK(hipStreamSynchronize(m->cast));
   return read_status(m);
 }
 
@@ -1199,6 +1496,8 @@ int32_t vdbgpu_get_stats(vdbgpu_map* m, vdbgpu_stats* out) {
   m->stats.voxel_updates = m->h_st[3].voxels;
   m->stats.device_bytes = m->device_bytes;
   *out = m->stats;
+  // batch folds count on the host, every k_map_insert on the device
+  out->update_leaves = m->stats.update_leaves + m->h_st[3].entries;
   return VDBGPU_OK;
 }
 
@@ -1213,7 +1512,7 @@ int32_t vdbgpu_set_tuning(vdbgpu_map* m, const vdbgpu_tuning* t) {
   if (!m || !t) return VDBGPU_BAD_ARGUMENT;
   if (t->block < 64 || t->block > kTraverseMaxBlock || (t->block & 63) || t->steps_per_lane < 1 ||
       t->steps_per_lane > 1 << 20 || t->waves_per_cu < 1 || t->waves_per_cu > 64 ||
-      !(t->load_factor >= 0.2 && t->load_factor <= 0.95))
+      !(t->load_factor >= 0.2 && t->load_factor <= 0.95) || t->traversal < 0 || t->traversal > 1)
     return VDBGPU_BAD_ARGUMENT;
   Guard g(m);
   m->tuning = *t;
