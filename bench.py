@@ -40,7 +40,8 @@ def parse():
     p.add_argument("--scans", type=int, default=100, help="distinct scans in the stream")
     p.add_argument("--cpu-scans", type=int, default=2, help="scans timed on the CPU oracle (0 = skip)")
     p.add_argument("--no-profiling", action="store_true", help="do not time kernels with HIP events")
-    p.add_argument("--tuning", type=str, default="", help="block,steps_per_lane,waves_per_cu,load_factor")
+    p.add_argument("--tuning", type=str, default="",
+                   help="block,steps_per_lane,waves_per_cu,load_factor[,traversal]")
     p.add_argument("--mode", choices=("plain", "batch", "mixed", "rgbd", "serve", "copy"), default="plain")
     p.add_argument("--copy-mib", type=int, default=1024, help="copy mode: bytes per copy (MiB)")
     p.add_argument("--batch", type=int, default=8, help="scans per batch (batch mode)")
@@ -73,9 +74,13 @@ def new_map(V, w, tuning=None):
     g = V.GpuMap(w.resolution)
     g.set_config(V.make_config(max_range=w.max_range, **w.probs))
     g.add_input_source("lidar", w.max_range)
-    if tuning:
-        b, spl, wpc, lf = tuning.split(",") if isinstance(tuning, str) else tuning
-        assert g.set_tuning(int(b), int(spl), int(wpc), float(lf)) == V.OK
+    if isinstance(tuning, dict):
+        assert g.set_tuning(**tuning) == V.OK
+    elif tuning:
+        parts = tuning.split(",")
+        b, spl, wpc, lf = parts[:4]
+        trav = int(parts[4]) if len(parts) > 4 else 0
+        assert g.set_tuning(int(b), int(spl), int(wpc), float(lf), trav) == V.OK
     return g
 
 
@@ -97,8 +102,10 @@ def mode_batch(a, V, W):
     n_pts = scans[0].points.shape[0]
     rays = sum(int((~np.isnan(s.points[:, :3]).any(1)).sum()) for s in scans)
     out = {}
-    for path in ("plain", "batch"):
+    for path in ("plain", "batch", "fold"):
         g = new_map(V, w, a.tuning)
+        if path == "fold":  # the (leaf, scan)-keyed traversal and scan-ordered fold
+            assert g.set_tuning(traversal=1) == V.OK
         # warm: size every table on the same stream, then start from an empty map
         for j in range(0, a.scans, a.batch):
             k = min(a.batch, a.scans - j)
@@ -121,8 +128,10 @@ def mode_batch(a, V, W):
         g.close()
     print(json.dumps({"metric": "batched integration of the outdoor stream", "value": round(rays / out["batch"], 1),
                       "unit": "rays/s", "plain_rays_per_s": round(rays / out["plain"], 1),
+                      "fold_rays_per_s": round(rays / out["fold"], 1),
                       "speedup_vs_plain": round(out["plain"] / out["batch"], 3), "batch_scans": a.batch,
-                      "scans": a.scans, "same_leaf_count": out["plain_leaves"] == out["batch_leaves"],
+                      "scans": a.scans,
+                      "same_leaf_count": out["plain_leaves"] == out["batch_leaves"] == out["fold_leaves"],
                       "config": {"workload": "outdoor_stream_262k_0.05m", "input": "device-resident"}}),
           flush=True)
 
@@ -143,17 +152,19 @@ def mode_mixed(a, V, W):
         best, table = autotune(w.resolution, dict(max_range=w.max_range, **w.probs),
                                [scans[i].points for i in sample], origins[sample], ranges[sample],
                                batched=True)
-        tuning = (best["block"], best["steps_per_lane"], best["waves_per_cu"], best["load_factor"])
+        tuning = best
         tune_table = [(t, round(s * 1e3, 3)) for t, s in sorted(table, key=lambda r: r[1])[:5]]
     times = {}
-    for path in ("plain", "batch"):
+    for path in ("plain", "batch", "fold"):
         g = new_map(V, w, tuning)
+        if path == "fold":  # the (leaf, scan)-keyed traversal and scan-ordered fold
+            assert g.set_tuning(traversal=1) == V.OK
         best = float("inf")
         for rep in range(3):
             g.reset()
             g.synchronize()
             t0 = time.perf_counter()
-            if path == "batch":
+            if path != "plain":
                 g.insert_batch_device("lidar", dev.address, counts, 16, origins, ranges, check=True)
             else:
                 for j in range(64):
@@ -167,6 +178,7 @@ def mode_mixed(a, V, W):
     print(json.dumps({"metric": "batched integration of 64 mixed-size scans (one call)",
                       "value": round(rays / times["batch"], 1), "unit": "rays/s",
                       "plain_rays_per_s": round(rays / times["plain"], 1),
+                      "fold_rays_per_s": round(rays / times["fold"], 1),
                       "speedup_vs_plain": round(times["plain"] / times["batch"], 3),
                       "batch_ms": round(times["batch"] * 1e3, 3), "plain_ms": round(times["plain"] * 1e3, 3),
                       "rays": rays, "tuning": tuning, "autotune_top5": tune_table,
@@ -310,8 +322,10 @@ def main():
     g.set_config(V.make_config(max_range=w.max_range, **w.probs))
     g.add_input_source("lidar", w.max_range)
     if a.tuning:
-        b, spl, wpc, lf = a.tuning.split(",")
-        assert g.set_tuning(int(b), int(spl), int(wpc), float(lf)) == V.OK
+        parts = a.tuning.split(",")
+        b, spl, wpc, lf = parts[:4]
+        trav = int(parts[4]) if len(parts) > 4 else 0
+        assert g.set_tuning(int(b), int(spl), int(wpc), float(lf), trav) == V.OK
 
     def step(i):
         j = i % a.scans

@@ -1,15 +1,32 @@
 """Batched integration (vdbgpu_insert_batch) must equal inserting the scans one
 after another: bit-exact against the oracle's sequential inserts and against
-the GPU plain path."""
+the GPU plain path. Every test runs on both batch paths: traversal 0 streams
+the scans through the brick path, traversal 1 runs the (leaf, scan)-keyed
+traversal and the scan-ordered fold."""
 import numpy as np
 import pytest
 
 import oracle as O
 import vdb_mapping_amd as V
-from parity import pair, same_map
+import parity
+from parity import same_map
 from vdb_mapping_amd import workloads as W
 
 pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(params=[0, 1], ids=["streamed", "fold"], autouse=True)
+def traversal(request):
+    return request.param
+
+
+@pytest.fixture
+def pair(traversal):
+    def make(*args, **kw):
+        g, o = parity.pair(*args, **kw)
+        assert g.set_tuning(traversal=traversal) == V.OK
+        return g, o
+    return make
 
 
 def oracle_sequential(o, source, clouds, origins, ranges):
@@ -33,7 +50,7 @@ def random_scans(rng, n_scans, R, sizes=(0, 3000)):
 
 
 @pytest.mark.parametrize("seed", range(4))
-def test_batch_equals_sequential(seed):
+def test_batch_equals_sequential(seed, pair):
     rng = np.random.default_rng(900 + seed)
     res = [0.1, 0.05, 0.2, 0.5][seed]
     R = [5.0, 3.0, 12.0, 25.0][seed]
@@ -46,7 +63,7 @@ def test_batch_equals_sequential(seed):
     assert st["scans"] == sum(1 for c in clouds if c.shape[0] > 0)
 
 
-def test_batch_equals_gpu_plain_path():
+def test_batch_equals_gpu_plain_path(traversal):
     rng = np.random.default_rng(77)
     clouds, origins, ranges = random_scans(rng, 12, 6.0, sizes=(100, 5000))
     gb = V.GpuMap(0.05)
@@ -54,6 +71,7 @@ def test_batch_equals_gpu_plain_path():
     for g in (gb, gs):
         g.set_config(V.make_config(max_range=6.0, **W.PROBS))
         g.add_input_source("s", 6.0)
+    assert gb.set_tuning(traversal=traversal) == V.OK
     gb.insert_batch("s", clouds, origins, ranges, check=True)
     for c, org, r in zip(clouds, origins, ranges):
         gs.add_input_source("s", r)
@@ -63,7 +81,7 @@ def test_batch_equals_gpu_plain_path():
     assert np.array_equal(a[2], b[2])
 
 
-def test_batch_clamps_across_scans():
+def test_batch_clamps_across_scans(pair):
     """Seven hits on one voxel inside one batch: the clamp and activation
     happen at the same scans as with sequential inserts."""
     g, o = pair(1.0, max_range=10.0, **W.PROBS)
@@ -75,7 +93,7 @@ def test_batch_clamps_across_scans():
     same_map(g, o)
 
 
-def test_batch_more_than_128_scans():
+def test_batch_more_than_128_scans(pair):
     rng = np.random.default_rng(5)
     clouds, origins, ranges = random_scans(rng, 300, 2.0, sizes=(0, 60))
     g, o = pair(0.1, max_range=2.0)
@@ -84,7 +102,7 @@ def test_batch_more_than_128_scans():
     same_map(g, o)
 
 
-def test_batch_after_pending_updates_of_other_sources():
+def test_batch_after_pending_updates_of_other_sources(pair):
     """Pending update sets are integrated with the first scan, in source-id order."""
     g, o = pair(0.1, max_range=4.0, sources=("a", "m", "z"))
     rng = np.random.default_rng(8)
@@ -98,7 +116,7 @@ def test_batch_after_pending_updates_of_other_sources():
     same_map(g, o)
 
 
-def test_batch_beyond_batched_key_range_falls_back():
+def test_batch_beyond_batched_key_range_falls_back(pair):
     """Voxel indices past 2^21 (but inside 2^23) cannot use (leaf, scan) keys;
     the batch is re-done scan by scan with the same result."""
     g, o = pair(1.0, max_range=0.0)
@@ -113,7 +131,7 @@ def test_batch_beyond_batched_key_range_falls_back():
     same_map(g, o)
 
 
-def test_batch_growth_paths():
+def test_batch_growth_paths(pair):
     """Long rays into an empty map: the update set overflows its first
     capacity and the map is empty at the fold, so both re-run paths run."""
     rng = np.random.default_rng(6)
@@ -129,7 +147,7 @@ def test_batch_growth_paths():
     same_map(g, o)
 
 
-def test_mixed_batch_workload_subset():
+def test_mixed_batch_workload_subset(pair):
     """Eight scans of the mixed-size batch workload (every 8th point)."""
     w = W.mixed_batch_workload(8)
     scans = [w.scan(i) for i in range(8)]
@@ -142,7 +160,7 @@ def test_mixed_batch_workload_subset():
     assert same_map(g, o) > 1000
 
 
-def test_batch_device_resident():
+def test_batch_device_resident(pair):
     rng = np.random.default_rng(3)
     clouds, origins, ranges = random_scans(rng, 9, 5.0, sizes=(1, 2000))
     cat = np.concatenate(clouds)

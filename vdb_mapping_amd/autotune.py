@@ -1,11 +1,11 @@
 """Launch-parameter autotuner of the integration kernels.
 
-Sweeps the traversal workgroup size, the visits per lane work item (chunk)
-and the update-set hash load factor (plus resident waves per CU) on a sample
-of the caller's own scans, timing the real insert path on fresh maps, and
-returns the fastest setting (vdbgpu_set_tuning). Results never depend on the
-tuning (tests/test_gpu_parity.py::test_tuning_does_not_change_results), only
-the time does.
+Times the brick path (traversal 0) and a sweep of the update-set hash
+traversal (traversal 1: workgroup size, visits per lane work item, resident
+waves per CU, hash load factor) on a sample of the caller's own scans, on
+fresh maps, and returns the fastest setting (vdbgpu_set_tuning). Results never
+depend on the tuning (tests/test_gpu_parity.py::test_tuning_does_not_change_results,
+tests/test_gpu_brick.py), only the time does.
 """
 from __future__ import annotations
 
@@ -61,8 +61,9 @@ def autotune(resolution, cfg_kwargs, clouds, origins, ranges=None, batched=True,
     ranges = np.full(len(clouds), cfg_kwargs["max_range"]) if ranges is None else np.asarray(ranges)
     keys = list(grid)
     table = []
-    for values in itertools.product(*(grid[k] for k in keys)):
-        tuning = dict(zip(keys, values))
+    candidates = [dict(traversal=0)] + [dict(zip(keys, values), traversal=1)
+                                        for values in itertools.product(*(grid[k] for k in keys))]
+    for tuning in candidates:
         t = _time_once(resolution, cfg_kwargs, tuning, clouds, origins, ranges, batched, repeats)
         if t is None:
             continue

@@ -374,6 +374,8 @@ __global__ __launch_bounds__(256) void k_brick_walk(const BrickItem* __restrict_
                                                     uint32_t rec_cap, Status* st) {
   __shared__ unsigned long long bits[kBrickWords];
   __shared__ uint32_t s_pos[64];
+  __shared__ uint32_t s_hist[128];
+  __shared__ uint16_t s_order[kItemSegs];
   __shared__ unsigned int s_stop;
   if (stopped(st, &s_stop)) return;
   const uint32_t n_items = st->items;
@@ -388,9 +390,34 @@ __global__ __launch_bounds__(256) void k_brick_walk(const BrickItem* __restrict_
     brick_coords(bv.keys[item.slot], b);
     const int ox = b[0] * kBrickDim, oy = b[1] * kBrickDim, oz = b[2] * kBrickDim;
     for (int w = threadIdx.x; w < kBrickWords; w += blockDim.x) bits[w] = 0;
+    if (threadIdx.x < 128) s_hist[threadIdx.x] = 0;
     __syncthreads();
-    for (uint32_t j = item.seg0 + threadIdx.x; j < item.seg1; j += blockDim.x) {
-      const uint4 sr = segs[j];
+    // Counting sort of the item's segments by visit count, longest first, so
+    // the lanes of a wave walk segments of about the same length.
+    const uint32_t nseg = item.seg1 - item.seg0;  // <= kItemSegs
+    for (uint32_t k = threadIdx.x; k < nseg; k += blockDim.x)
+      atomicAdd(&s_hist[127 - ((segs[item.seg0 + k].y >> 16) & 0x7Fu)], 1u);
+    __syncthreads();
+    if (threadIdx.x < 64) {  // exclusive scan of the 128 bins, two per lane
+      const uint32_t h0 = s_hist[2 * threadIdx.x], h1 = s_hist[2 * threadIdx.x + 1];
+      uint32_t v = h0 + h1;
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t t = __shfl_up(v, o, 64);
+        if ((int)threadIdx.x >= o) v += t;
+      }
+      const uint32_t ex = v - h0 - h1;
+      s_hist[2 * threadIdx.x] = ex;
+      s_hist[2 * threadIdx.x + 1] = ex + h0;
+    }
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < nseg; k += blockDim.x) {
+      const uint32_t pos = atomicAdd(&s_hist[127 - ((segs[item.seg0 + k].y >> 16) & 0x7Fu)], 1u);
+      s_order[pos] = (uint16_t)k;
+    }
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < nseg; k += blockDim.x) {
+      const uint4 sr = segs[item.seg0 + s_order[k]];
       const int4 r = rayinfo[sr.x];
       const uint32_t sc = ((uint32_t)r.w >> kScanShift) & kScanMask;
       const int sx = scans[sc].s[0], sy = scans[sc].s[1], sz = scans[sc].s[2];
@@ -401,19 +428,17 @@ __global__ __launch_bounds__(256) void k_brick_walk(const BrickItem* __restrict_
       const uint32_t c[3] = {sr.y & 0x3FFFu, sr.z, sr.w};
       const uint32_t m = (sr.y >> 16) & 0x7Fu;
       int lx = sx + stx * (int)c[0] - ox, ly = sy + sty * (int)c[1] - oy, lz = sz + stz * (int)c[2] - oz;
+      // The path is monotone along every axis, so it stays inside the brick
+      // when its first and last voxels do: both are checked, and the LDS
+      // word index is masked into the brick so no access can leave it.
+      const bool ok0 = (unsigned)lx < (unsigned)kBrickDim && (unsigned)ly < (unsigned)kBrickDim &&
+                       (unsigned)lz < (unsigned)kBrickDim;
       int cw = -1;
       uint64_t cb = 0, lastb = 0;
-      bool ok = true;
       IntDda d;
       d.init(n, c);
       for (uint32_t k = 0;;) {
-        if ((unsigned)lx >= (unsigned)kBrickDim || (unsigned)ly >= (unsigned)kBrickDim ||
-            (unsigned)lz >= (unsigned)kBrickDim) {
-          atomicOr(&st->error, kErrBrick);
-          ok = false;
-          break;
-        }
-        const int w = ((lx >> 3) << 7) | ((ly >> 3) << 5) | ((lz >> 3) << 3) | (lx & 7);
+        const int w = (((lx >> 3) & 3) << 7) | (((ly >> 3) & 3) << 5) | (((lz >> 3) & 3) << 3) | (lx & 7);
         lastb = 1ull << (((ly & 7) << 3) | (lz & 7));
         if (w != cw) {
           if (cb) atomicOr(&bits[cw], (unsigned long long)cb);
@@ -428,10 +453,14 @@ __global__ __launch_bounds__(256) void k_brick_walk(const BrickItem* __restrict_
         ly += a == 1 ? sty : 0;
         lz += a == 2 ? stz : 0;
       }
-      if (ok) {
-        if (cb) atomicOr(&bits[cw], (unsigned long long)cb);
+      if (cb) atomicOr(&bits[cw], (unsigned long long)cb);
+      const bool ok1 = (unsigned)lx < (unsigned)kBrickDim && (unsigned)ly < (unsigned)kBrickDim &&
+                       (unsigned)lz < (unsigned)kBrickDim;
+      if (!(ok0 && ok1)) {
+        atomicOr(&st->error, kErrBrick);  // the scan is rejected
+      } else if (sr.y & 0x80000000u) {
         // setValueOn(end, true) (VDBMapping.hpp:533-536): the last visit is the end voxel
-        if (sr.y & 0x80000000u) atomicOr(&bits[512 + cw], (unsigned long long)lastb);
+        atomicOr(&bits[512 + cw], (unsigned long long)lastb);
       }
     }
     __syncthreads();

@@ -33,10 +33,11 @@ VDB_HD double crossing_time(const double* __restrict__ tbl, uint32_t n, uint32_t
   return tbl[(((uint64_t)n * (n - 1)) >> 1) + c];
 }
 
-// Event (b, cb) precedes event (a, ca) in DDA order.
+// Event (b, cb) precedes event (a, ca) in DDA order. Every product below is
+// under (2*8192+1)*8192 < 2^31: 32-bit arithmetic (no emulated 64-bit ops).
 VDB_HD bool ev_before(const double* __restrict__ tbl, int b, uint32_t nb, uint32_t cb, int a,
                       uint32_t na, uint32_t ca) {
-  const int64_t d = (int64_t)(2 * cb + 1) * na - (int64_t)(2 * ca + 1) * nb;
+  const int32_t d = (int32_t)((2 * cb + 1) * na) - (int32_t)((2 * ca + 1) * nb);
   if (d != 0) return d < 0;
   if (nb == na) return b > a;
   const double fb = crossing_time(tbl, nb, cb);
@@ -50,15 +51,15 @@ VDB_HD bool ev_before(const double* __restrict__ tbl, int b, uint32_t nb, uint32
 VDB_HD uint32_t count_before(const double* __restrict__ tbl, int b, uint32_t nb, int a, uint32_t na,
                              uint32_t ca) {
   if (nb == 0) return 0;
-  const uint64_t X = (uint64_t)(2 * ca + 1) * nb;
-  const uint64_t q = X / na, r = X - q * na;
-  uint64_t cnt = r == 0 ? q / 2 : (q + 1) / 2;
+  const uint32_t X = (2 * ca + 1) * nb;  // < 2^28
+  const uint32_t q = X / na, r = X - q * na;
+  uint32_t cnt = r == 0 ? q / 2 : (q + 1) / 2;
   if (cnt > nb) cnt = nb;
   if (r == 0 && (q & 1)) {
-    const uint32_t ct = (uint32_t)((q - 1) / 2);
+    const uint32_t ct = (q - 1) / 2;
     if (ct < nb && ev_before(tbl, b, nb, ct, a, na, ca)) ++cnt;
   }
-  return (uint32_t)cnt;
+  return cnt;
 }
 
 // Crossing counts c[3] after k steps of a ray with |delta| n[3] (k <= sum n):
@@ -68,11 +69,11 @@ VDB_HD uint32_t count_before(const double* __restrict__ tbl, int b, uint32_t nb,
 // one misplaced event, so a handful of rounds suffice. Returns false only if
 // the repair did not converge (reported as a kernel error, never silently).
 VDB_HD bool dda_enter(const double* __restrict__ tbl, const uint32_t n[3], uint32_t k, uint32_t c[3]) {
-  const uint64_t N = (uint64_t)n[0] + n[1] + n[2];
+  const uint32_t N = n[0] + n[1] + n[2];
   int sum = 0;
   for (int a = 0; a < 3; ++a) {
-    // #{j : (2j+1)/(2n) <= k/N} = floor((2 n k + N) / (2N))
-    const uint64_t g = (2ull * n[a] * k + N) / (2ull * N);
+    // #{j : (2j+1)/(2n) <= k/N} = floor((2 n k + N) / (2N)); 2 n k + N < 2^31
+    const uint32_t g = (2 * n[a] * k + N) / (2 * N);
     c[a] = (uint32_t)(g < n[a] ? g : n[a]);
     sum += (int)c[a];
   }

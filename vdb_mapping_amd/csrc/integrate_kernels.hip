@@ -259,14 +259,14 @@ __global__ __launch_bounds__(kPrepBlock) void k_prep(const float* __restrict__ p
   if (threadIdx.x == kPrepBlock - 1) bsum[blockIdx.x] = v;
 }
 
-__global__ __launch_bounds__(1024) void k_scan_blocks(const uint32_t* __restrict__ bsum,
-                                                      uint64_t* __restrict__ boff,
-                                                      const BatchCtl* __restrict__ ctl,
-                                                      Status* st) {
-  __shared__ unsigned long long wsum[16];
+__global__ __launch_bounds__(256) void k_scan_blocks(const uint32_t* __restrict__ bsum,
+                                                     uint64_t* __restrict__ boff,
+                                                     const BatchCtl* __restrict__ ctl,
+                                                     Status* st) {
+  __shared__ unsigned long long wsum[4];
   const uint32_t nb = ctl->nb;
   unsigned long long carry = 0;
-  for (uint32_t base = 0; base < nb; base += 1024) {
+  for (uint32_t base = 0; base < nb; base += 256) {
     const uint32_t i = base + threadIdx.x;
     const unsigned long long x = i < nb ? bsum[i] : 0ull;
     unsigned long long v = x;
@@ -279,7 +279,7 @@ __global__ __launch_bounds__(1024) void k_scan_blocks(const uint32_t* __restrict
     if (lane_id() == 63) wsum[wv] = v;
     __syncthreads();
     unsigned long long add = 0, tot = 0;
-    for (int w = 0; w < 16; ++w) {
+    for (int w = 0; w < 4; ++w) {
       if (w < wv) add += wsum[w];
       tot += wsum[w];
     }
@@ -626,14 +626,15 @@ __global__ __launch_bounds__(256) void k_apply(UpdView u, const uint32_t* __rest
     float4* px = reinterpret_cast<float4*>(mp.values + (size_t)idx * 512 + j * 64) + l;
     unsigned long long* pa = mp.active + (size_t)idx * 8 + j;
     const uint64_t aw = *pa;
-    float4 x = *px;
     const uint32_t vb = (uint32_t)(vis >> (4 * l)) & 15u, hb = (uint32_t)(hit >> (4 * l)) & 15u;
     uint32_t ab = (uint32_t)(aw >> (4 * l)) & 15u;
+    // only float4s holding a visited voxel are read and written (partial rows)
+    float4 x = vb ? *px : make_float4(0.f, 0.f, 0.f, 0.f);
     update_voxel(x.x, ab, 0, vb & 1, hb & 1, l_hit, l_miss, t_min, t_max, c_max, c_min);
     update_voxel(x.y, ab, 1, vb & 2, hb & 2, l_hit, l_miss, t_min, t_max, c_max, c_min);
     update_voxel(x.z, ab, 2, vb & 4, hb & 4, l_hit, l_miss, t_min, t_max, c_max, c_min);
     update_voxel(x.w, ab, 3, vb & 8, hb & 8, l_hit, l_miss, t_min, t_max, c_max, c_min);
-    *px = x;  // whole 16 bytes: unvisited voxels are written back unchanged
+    if (vb) *px = x;  // the 16 bytes: unvisited voxels of it are written back unchanged
     uint64_t na = (uint64_t)ab << (4 * l);
 #pragma unroll
     for (int o = 1; o < 16; o <<= 1) na |= __shfl_xor(na, o, 16);
@@ -747,7 +748,7 @@ hipError_t launch_prep(hipStream_t s, int grid, const float* pts, int64_t stride
 
 hipError_t launch_scan_blocks(hipStream_t s, const uint32_t* bsum, uint64_t* boff,
                               const BatchCtl* ctl, Status* st) {
-  hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, s, bsum, boff, ctl, st);
+  hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(256), 0, s, bsum, boff, ctl, st);
   return hipGetLastError();
 }
 

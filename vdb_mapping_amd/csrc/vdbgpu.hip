@@ -790,7 +790,8 @@ int run_brick(vdbgpu_map* m, Source& src, const float* xyz, int64_t n, int64_t s
   create_flags(m, &ch, &cm);
   const UpdView rec{m->rkeys[X], m->rmasks[X], 0};
   const int g1 = std::min<int>(4096, (int)((recs + 255) / 256));
-  const int g2 = std::min<int>(m->num_cu * 16, (int)((recs + 1) / 2));  // 128 lanes per leaf
+  // 128 lanes per leaf; six workgroups per CU leave wave slots for the next scan's cast
+  const int g2 = std::min<int>(m->num_cu * 6, (int)((recs + 1) / 2));
   {
     Timer t(m, kCompact);
     CK(launch_map_insert(m->stream, g1, rec, m->rlist[X], recs, map_view(m), m->eidx, ch, cm, m->noop4096,
@@ -1211,6 +1212,16 @@ int32_t vdbgpu_insert_batch(vdbgpu_map* m, const char* id, int32_t nscans, const
   int i0 = 0;
   bool pending = false;
   for (auto& kv : m->sources) pending = pending || kv.second.pending > 0;
+  if (m->tuning.traversal == 0 && !pending) {
+    // Streaming every scan through the brick path (the next scan's cast
+    // overlaps this one's map update) is faster than the (leaf, scan) fold.
+    for (int i = 0; i < nscans; ++i) {
+      const int r = plain_insert(i);
+      if (r) return r;
+    }
+    if (m->fast_mode && m->touched) return build_snapshot(m);
+    return VDBGPU_OK;
+  }
   if (pending) {
     // pending sets are applied with the first scan's integrate, in source-id order
     const int r = plain_insert(0);
