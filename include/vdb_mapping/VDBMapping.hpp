@@ -11,9 +11,11 @@
 // include/vdbgpu.h (no voxel arithmetic on the host); worldToIndex, a
 // standalone helper in the reference, is restated on the host.
 //
+// Fast mode (:520-527, :577-602) is provided by the library: accumulates cast
+// through the topology snapshot of the last fast-mode integrate.
+//
 // Not provided (each has no call on this path): the async accumulation and
-// integration threads (:133, :355-370, :1383-1430), fast-mode raycasting
-// (:520-527, :577-602), persistence and PCD I/O (:193-307), map sections,
+// integration threads (:133, :355-370, :1383-1430), persistence and PCD I/O (:193-307), map sections,
 // byte arrays and morphology (:810-1338), direct point edits and artificial
 // areas (:413-447, :1152-1236), castRayIntoGrid (:550-566, per-ray host call)
 // and updateMap (:731-792, reached through integrateUpdate).

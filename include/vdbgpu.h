@@ -28,10 +28,12 @@ enum {
   VDBGPU_BAD_PROB_MISS = 4,  /* OCC:64-69: prob_miss > 0.5 */
   VDBGPU_BAD_PROB_HIT = 5,   /* OCC:70-75: prob_hit < 0.5 */
   VDBGPU_OUT_OF_RANGE = 6,   /* a voxel index outside [-2^23, 2^23) or non-finite: scan rejected */
-  VDBGPU_UNSUPPORTED = 7,    /* fast_mode raycasting (VM:520-527) is not provided */
+  VDBGPU_UNSUPPORTED = 7,    /* a ray longer than 8192 voxels along one axis */
   VDBGPU_DEVICE_ERROR = 8,   /* HIP runtime error or out of device memory */
   VDBGPU_BAD_ARGUMENT = 9,
-  VDBGPU_NO_INTERSECTOR = 10, /* raytrace before any fast_mode integrate (VM:698, :1438) */
+  VDBGPU_NO_INTERSECTOR = 10, /* raytrace, or a fast-mode cast into a non-empty map, before the
+                               * source has an intersector (VM:590, :698, :1438); the reference
+                               * dereferences a null pointer there */
   VDBGPU_OVER_BUDGET = 11     /* serving: the maps' device footprint exceeds the HBM budget */
 };
 

@@ -21,6 +21,9 @@
 //      that holds an active voxel.
 //   U10 Ray::valid() uses eps = Delta<float> (1e-5f); TimeSpan::valid() uses
 //      Delta<double> (1e-9).
+//   U11 VolumeHDDA::hits (fast-mode raycast) walks like the march without its
+//      Ray::valid() pre-check and early exit, pushes every valid span, and
+//      closes an open span at the root level's max time.
 //
 // Compile with -ffp-contract=off (no FMA contraction): the reference is built
 // for baseline x86-64 (Release, no -march flags, VM CMakeLists.txt:8-11),
@@ -195,6 +198,7 @@ struct Source {
   std::string id;
   double max_range = 0;
   UpdateSet upd;
+  bool has_isect = false;  // volume_ray_intersector set (VM:1440-1446), null after addInputSource
 };
 
 // Snapshot taken by updateVolumeRayIntersectors (VM:1436-1449): topology of
@@ -324,11 +328,18 @@ void build_intersector(Map& m) {
   m.isect = std::move(is);
 }
 
-// raycastPointCloud (VM:466-539) in normal (non-fast) mode.
+template <class Visit>
+void cast_ray_fast(const Map& m, const Intersector& is, const int32_t s[3], const int32_t e[3],
+                   Visit&& visit);
+
+// raycastPointCloud (VM:466-539); fast mode casts with castRayIntoGridFast
+// (VM:577-602) and only when the map is not empty (VM:495, :520-527).
 int raycast(Map& m, Source& src, const float* xyz, int64_t n, int64_t stride,
             const double origin[3]) {
   if (!m.config_set) return ORC_NOT_CONFIGURED;  // VM:478-482
-  if (m.fast_mode) return ORC_UNSUPPORTED;
+  const bool grid_empty = m.n4096.empty();
+  // the reference dereferences a null intersector here (crash); reported instead
+  if (m.fast_mode && !grid_empty && !(src.has_isect && m.isect)) return ORC_NO_INTERSECTOR;
   const double R = src.max_range;
   for (int a = 0; a < 3; ++a)
     if (std::isnan(origin[a])) return ORC_OK;  // every point is skipped (VM:505-510)
@@ -369,11 +380,15 @@ int raycast(Map& m, Source& src, const float* xyz, int64_t n, int64_t stride,
   for (int64_t i = 0; i < n; ++i) {
     if (!flags[i]) continue;
     const int32_t* e = &ends[(size_t)i * 3];
-    cast_ray(s, e, [&](int32_t x, int32_t y, int32_t z) {
+    auto mark = [&](int32_t x, int32_t y, int32_t z) {
       UpdLeaf& l = u.get(leaf_key(x, y, z));
       const int off = leaf_offset(x, y, z);
       l.vis[off >> 6] |= 1ull << (off & 63);
-    });
+    };
+    if (!m.fast_mode)
+      cast_ray(s, e, mark);
+    else if (!grid_empty)
+      cast_ray_fast(m, *m.isect, s, e, mark);
     if (flags[i] == 2) {
       UpdLeaf& l = u.get(leaf_key(e[0], e[1], e[2]));
       const int off = leaf_offset(e[0], e[1], e[2]);
@@ -461,6 +476,78 @@ bool hdda_4096(const Intersector& is, RayState& ray, Span& t) {
   return false;
 }
 
+// Ray::clip to the padded active bbox (VolumeRayIntersector::setIndexRay) [U6];
+// the times change only when the ray hits the box.
+void clip_ray(const Intersector& is, RayState& ray) {
+  double t0 = ray.t0, t1 = ray.t1;
+  bool hit = true;
+  for (int a = 0; a < 3 && hit; ++a) {
+    double lo = (double)is.bmin[a], hi = (double)is.bmax[a] + ORC_U6_CLIP_PAD;
+    double ta = (lo - ray.eye[a]) * ray.inv[a];
+    double tb = (hi - ray.eye[a]) * ray.inv[a];
+    if (ta > tb) std::swap(ta, tb);
+    if (ta > t0) t0 = ta;
+    if (tb < t1) t1 = tb;
+    if (t0 > t1) hit = false;
+  }
+  if (hit) {
+    ray.t0 = t0;
+    ray.t1 = t1;
+  }
+}
+
+// VolumeHDDA::hits (openvdb/math/DDA.h): every run of existing leaves along
+// the ray, as time spans [U7]; the same walk as the march above without its
+// early exit.
+void hits_leaf(const Intersector& is, RayState& ray, std::vector<Span>& out, Span& t) {
+  Dda d;
+  d.init(ray.eye, ray.dir, ray.inv, ray.t0, ray.t1, 3);
+  do {
+    if (is.leaves.count(leaf_key(d.vox[0], d.vox[1], d.vox[2]))) {
+      if (t.t0 < 0) t.t0 = d.time();
+    } else if (t.t0 >= 0) {
+      t.t1 = d.time();
+      if (t.valid()) out.push_back(t);
+      t.t0 = t.t1 = -1;
+    }
+  } while (d.step());
+  if (t.t0 >= 0) t.t1 = d.max_time();
+}
+
+void hits_128(const Intersector& is, RayState& ray, std::vector<Span>& out, Span& t) {
+  Dda d;
+  d.init(ray.eye, ray.dir, ray.inv, ray.t0, ray.t1, 7);
+  do {
+    if (is.n128.count(key128(d.vox[0], d.vox[1], d.vox[2]))) {
+      ray.t0 = d.time();
+      ray.t1 = d.next();
+      hits_leaf(is, ray, out, t);
+    } else if (t.t0 >= 0) {
+      t.t1 = d.time();
+      if (t.valid()) out.push_back(t);
+      t.t0 = t.t1 = -1;
+    }
+  } while (d.step());
+  if (t.t0 >= 0) t.t1 = d.max_time();
+}
+
+void hits_4096(const Intersector& is, RayState& ray, std::vector<Span>& out, Span& t) {
+  Dda d;
+  d.init(ray.eye, ray.dir, ray.inv, ray.t0, ray.t1, 12);
+  do {
+    if (is.n4096.count(key4096(d.vox[0], d.vox[1], d.vox[2]))) {
+      ray.t0 = d.time();
+      ray.t1 = d.next();
+      hits_128(is, ray, out, t);
+    } else if (t.t0 >= 0) {
+      t.t1 = d.time();
+      if (t.valid()) out.push_back(t);
+      t.t0 = t.t1 = -1;
+    }
+  } while (d.step());
+  if (t.t0 >= 0) t.t1 = d.max_time();
+}
+
 bool map_is_on(const Map& m, int32_t x, int32_t y, int32_t z) {
   if (x < -kCoordLimit || x >= kCoordLimit || y < -kCoordLimit || y >= kCoordLimit ||
       z < -kCoordLimit || z >= kCoordLimit)
@@ -469,6 +556,36 @@ bool map_is_on(const Map& m, int32_t x, int32_t y, int32_t z) {
   if (it == m.leaves.end()) return false;
   const int off = leaf_offset(x, y, z);
   return (it->second.act[off >> 6] >> (off & 63)) & 1;
+}
+
+// castRayIntoGridFast (VM:577-602): Ray(s + 0.5, e - s, 0, 1) through the
+// intersector (clip, hits), then for every span a voxel DDA over
+// [t0, t1] that marks the voxels active in the live map. No e == s guard:
+// a zero-length ray still probes the origin voxel.
+template <class Visit>
+void cast_ray_fast(const Map& m, const Intersector& is, const int32_t s[3], const int32_t e[3],
+                   Visit&& visit) {
+  RayState ray;
+  for (int a = 0; a < 3; ++a) {
+    ray.eye[a] = (double)s[a] + 0.5;
+    ray.dir[a] = (double)e[a] - (double)s[a];
+    ray.inv[a] = 1.0 / ray.dir[a];
+  }
+  ray.t0 = 0.0;
+  ray.t1 = 1.0;
+  clip_ray(is, ray);  // setIndexRay; its result is ignored (VM:590)
+  std::vector<Span> spans;
+  Span t;
+  RayState r = ray;
+  hits_4096(is, r, spans, t);
+  if (t.valid()) spans.push_back(t);
+  for (const Span& sp : spans) {
+    Dda d;
+    d.init(ray.eye, ray.dir, ray.inv, sp.t0, sp.t1, 0);
+    do {
+      if (map_is_on(m, d.vox[0], d.vox[1], d.vox[2])) visit(d.vox[0], d.vox[1], d.vox[2]);
+    } while (d.step());
+  }
 }
 
 }  // namespace
@@ -518,6 +635,7 @@ int orc_add_source(void* h, const char* id, double max_range, double max_rate) {
   s.id = id;
   s.max_range = (max_range == 0) ? m.max_range : max_range;
   s.upd.clear();
+  s.has_isect = false;  // a new InputSource has no intersector yet
   return ORC_OK;
 }
 
@@ -538,7 +656,10 @@ int orc_integrate(void* h) {
     apply_update(m, kv.second.upd);
     kv.second.upd.clear();
   }
-  if (m.fast_mode && !m.n4096.empty()) build_intersector(m);
+  if (m.fast_mode && !m.n4096.empty()) {
+    build_intersector(m);  // updateVolumeRayIntersectors: every source gets a copy
+    for (auto& kv : m.sources) kv.second.has_isect = true;
+  }
   return ORC_OK;
 }
 
@@ -663,24 +784,7 @@ int orc_raytrace(void* h, int64_t n, const double* origins, const double* dirs,
     }
     ray.t0 = 0.0;
     ray.t1 = 1.0;
-    // setIndexRay: clip to the padded bbox; the result is ignored (VM:698) [U6].
-    {
-      double t0 = ray.t0, t1 = ray.t1;
-      bool hit = true;
-      for (int a = 0; a < 3 && hit; ++a) {
-        double lo = (double)is.bmin[a], hi = (double)is.bmax[a] + ORC_U6_CLIP_PAD;
-        double ta = (lo - ray.eye[a]) * ray.inv[a];
-        double tb = (hi - ray.eye[a]) * ray.inv[a];
-        if (ta > tb) std::swap(ta, tb);
-        if (ta > t0) t0 = ta;
-        if (tb < t1) t1 = tb;
-        if (t0 > t1) hit = false;
-      }
-      if (hit) {
-        ray.t0 = t0;
-        ray.t1 = t1;
-      }
-    }
+    clip_ray(is, ray);  // setIndexRay: the result is ignored (VM:698) [U6]
     Span t;
     if ((ray.t1 - ray.t0) > (double)1e-5f) hdda_4096(is, ray, t);  // Ray::valid [U10]
     double* out = ends + 3 * i;

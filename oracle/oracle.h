@@ -16,8 +16,8 @@
  * forms, an exact-rational DDA walk and brute-force set checks. Parity
  * unpinned (no check here can compare them with OpenVDB itself): the rounding
  * forms U1-U3, the double tie-break between axes of different extent (U5),
- * raytrace clip/HDDA details (U6, U7, U9, U10) and leaf creation by no-op
- * updates (U8).
+ * raytrace clip/HDDA details (U6, U7, U9, U10), the fast-mode hits walk
+ * (U11) and leaf creation by no-op updates (U8).
  */
 #ifndef VDB_ORACLE_H
 #define VDB_ORACLE_H

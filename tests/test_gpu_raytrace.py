@@ -5,7 +5,7 @@ import pytest
 
 import oracle as O
 import vdb_mapping_amd as V
-from parity import both, pair, same_map
+from parity import same_update, both, pair, same_map
 from vdb_mapping_amd import workloads as W
 
 pytestmark = pytest.mark.gpu
@@ -102,9 +102,11 @@ def test_raytrace_device_arrays():
     assert np.array_equal(ok, oko) and np.array_equal(end.view(np.uint64), eo.view(np.uint64))
 
 
-def test_fast_mode_accumulate_is_unsupported():
+def test_fast_mode_accumulate_after_switch():
+    """After switching to fast mode and integrating, accumulates cast through
+    the intersector (tests/test_gpu_fast.py covers the walk itself)."""
     g, o = pair(0.1, max_range=5.0)
     both(g, o, "insert", "test", np.array([[1, 0, 0]], np.float32), [0, 0, 0])
     switch_to_fast_mode(g, o, 5.0)
-    assert g.accumulate("test", np.array([[0, 1, 0]], np.float32), [0, 0, 0]) == V.UNSUPPORTED
-    assert o.accumulate("test", np.array([[0, 1, 0]], np.float32), [0, 0, 0]) == O.UNSUPPORTED
+    both(g, o, "accumulate", "test", np.array([[0, 1, 0], [1.05, 0, 0]], np.float32), [0, 0, 0])
+    same_update(g, o, "test")

@@ -409,3 +409,100 @@ def test_raytrace_miss_reports_segment_end():
     st, ok, end = m.raytrace([[0.5, 30.5, 0.5]], [[0, 1, 0]], [4.0])
     assert st == O.OK and not ok[0]
     assert end[0].tolist() == [0.5, 34.5, 0.5]
+
+
+# ------------------------------------------------------------ fast mode --
+# castRayIntoGridFast (VDBMapping.hpp:577-602): pinned by closed forms and
+# invariants; its agreement with OpenVDB's VolumeHDDA::hits itself (U11) is
+# unpinned.
+
+def _voxels(origins, words):
+    """Set of voxel coords of an exported update (origins + 8 mask words)."""
+    out = set()
+    for o, w in zip(origins, words):
+        for j in range(8):
+            bits = int(w[j])
+            while bits:
+                b = (bits & -bits).bit_length() - 1
+                bits &= bits - 1
+                n = j * 64 + b
+                out.add((int(o[0]) + (n >> 6), int(o[1]) + ((n >> 3) & 7), int(o[2]) + (n & 7)))
+    return out
+
+
+def _active_block_map(fast):
+    """1 m voxels; the leaf-aligned block [0,16)^3 made active by one scan whose
+    endpoints are the block's voxel centres (one hit activates at thres 0.8)."""
+    m = O.OracleMap(1.0)
+    m.set_config(O.make_config(max_range=1000.0, prob_hit=0.9, prob_miss=0.45, prob_thres_min=0.2,
+                               prob_thres_max=0.8))
+    m.add_input_source("s", 1000.0)
+    g = np.stack(np.meshgrid(*(np.arange(16),) * 3, indexing="ij"), -1).reshape(-1, 3) + 0.25
+    assert m.insert("s", g.astype(np.float32), [-30.25, -30.25, -30.25]) == O.OK
+    _, _, act = m.export_leaves()
+    block = {(x, y, z) for x in range(16) for y in range(16) for z in range(16)}
+    on = set()
+    o, _, a = m.export_leaves()
+    on = _voxels(o, a)
+    assert block <= on
+    if fast:
+        assert m.set_config(O.make_config(max_range=1000.0, fast_mode=True, prob_hit=0.9, prob_miss=0.45,
+                                          prob_thres_min=0.2, prob_thres_max=0.8)) == O.OK
+        assert m.integrate() == O.OK  # builds the intersector (updateVolumeRayIntersectors)
+    return m, on
+
+
+def test_fast_mode_axis_ray_through_active_block_closed_form():
+    m, on = _active_block_map(True)
+    # along x at y = 3, z = 5 from x = -40 to x = 40 (truncated at range 1000: no)
+    pts = np.array([[40.25, 3.25, 5.25]], np.float32)
+    assert m.accumulate("s", pts, [-39.75, 3.25, 5.25]) == O.OK
+    o, vis, hit = m.export_update("s")
+    got = _voxels(o, vis)
+    want = {(x, 3, 5) for x in range(16)} | {(40, 3, 5)}  # the block voxels on the line + the endpoint
+    assert got == want
+    assert _voxels(o, hit) == {(40, 3, 5)}
+
+
+def test_fast_mode_marks_only_previously_active_voxels():
+    m, on = _active_block_map(True)
+    rng = np.random.default_rng(5)
+    org = np.array([-7.7, 21.3, 8.1])
+    pts = (rng.uniform(-10, 26, (400, 3))).astype(np.float32)
+    assert m.accumulate("s", pts, org) == O.OK
+    o, vis, hit = m.export_update("s")
+    ends = {tuple(int(v) for v in np.floor(p.astype(np.float64) + 0.5)) for p in pts}
+    visited = _voxels(o, vis)
+    assert visited - _voxels(o, hit) <= on
+    assert _voxels(o, hit) <= ends
+    assert len(visited & on) > 100
+
+
+def test_fast_mode_empty_map_marks_endpoints_only():
+    m = O.OracleMap(0.1)
+    m.set_config(O.make_config(max_range=5.0, fast_mode=True))
+    m.add_input_source("s", 5.0)
+    pts = np.array([[1.0, 0.2, 0.3], [0.5, -0.5, 0.1], [9.0, 0.0, 0.0]], np.float32)
+    assert m.accumulate("s", pts, [0, 0, 0]) == O.OK
+    o, vis, hit = m.export_update("s")
+    assert _voxels(o, vis) == _voxels(o, hit) == {(10, 2, 3), (5, -5, 1)}  # the truncated ray marks nothing
+
+
+def test_fast_mode_zero_length_ray_probes_origin_voxel():
+    m, on = _active_block_map(True)
+    assert m.accumulate("s", np.array([[4.3, 4.4, 4.1]], np.float32), [4.2, 4.3, 4.4]) == O.OK
+    o, vis, hit = m.export_update("s")
+    assert _voxels(o, vis) == _voxels(o, hit) == {(4, 4, 4)}
+
+
+def test_fast_mode_without_intersector_is_rejected():
+    """Normal-mode map, then fast mode without an integrate: the reference
+    dereferences a null intersector; the oracle reports NO_INTERSECTOR. A source
+    added after the intersector was built has none either."""
+    m, on = _active_block_map(False)
+    assert m.set_config(O.make_config(max_range=1000.0, fast_mode=True)) == O.OK
+    assert m.accumulate("s", np.array([[1, 1, 1]], np.float32), [0, 0, 0]) == O.NO_INTERSECTOR
+    assert m.integrate() == O.OK
+    assert m.accumulate("s", np.array([[1, 1, 1]], np.float32), [0, 0, 0]) == O.OK
+    m.add_input_source("late", 100.0)
+    assert m.accumulate("late", np.array([[1, 1, 1]], np.float32), [0, 0, 0]) == O.NO_INTERSECTOR

@@ -24,6 +24,7 @@ constexpr uint64_t kEmpty = ~0ull;
 constexpr uint32_t kNone = 0xFFFFFFFFu;
 constexpr int kCoordLimit = 1 << 23;  // voxel coords in [-2^23, 2^23): 21-bit leaf coords
 constexpr uint32_t kHitBit = 0x80000000u;   // rayinfo.w: endpoint is a hit
+constexpr uint32_t kValidBit = 0x40000000u; // rayinfo.w: the ray is cast (not NaN, range > 0)
 constexpr uint32_t kCntMask = 0xFFFFu;      // rayinfo.w bits 0-15: visit count (<= 1 + 3 * 8192)
 constexpr int kScanShift = 16;              // rayinfo.w bits 16-22: scan index in a batch
 constexpr uint32_t kScanMask = 0x7Fu;
@@ -37,6 +38,7 @@ constexpr uint32_t kErrRange = 1u;  // a voxel index out of range / non-finite
 constexpr uint32_t kErrTable = 2u;  // crossing table or chunk table too short
 constexpr uint32_t kErrEntry = 4u;  // mid-ray DDA entry did not converge
 constexpr uint32_t kErrBrick = 8u;  // brick path inconsistency (a segment left its brick)
+constexpr uint32_t kErrWalk = 16u;  // fast-mode walk exceeded its step bound
 constexpr uint32_t kOvfHard = 1u;   // update-set probe found no free slot
 constexpr uint32_t kMapFull = 1u;   // Status::map_full: the fold needs a larger leaf pool / hash
 // Brick path overflow bits (Status::overflow): a table too small for this

@@ -22,6 +22,7 @@
 #include "dda_math.h"
 #include "device_common.h"
 #include "kernels.h"
+#include "rt_common.h"
 
 namespace vdbgpu {
 
@@ -228,7 +229,8 @@ __global__ __launch_bounds__(kPrepBlock) void k_prep(const float* __restrict__ p
           atomicOr(&st->error, kErrTable);
         } else {
           cnt = (nx | ny | nz) ? 1u + nx + ny + nz : 0u;
-          ri = make_int4(ex, ey, ez, (int)(cnt | (b << kScanShift) | (trunc ? 0u : kHitBit)));
+          ri = make_int4(ex, ey, ez,
+                         (int)(cnt | (b << kScanShift) | kValidBit | (trunc ? 0u : kHitBit)));
         }
       }
     }
@@ -434,6 +436,101 @@ __global__ __launch_bounds__(256) void k_traverse(const int4* __restrict__ rayin
       lkey[e] = kEmpty;
     }
     __syncthreads();
+  }
+}
+
+// Visited (and, for an endpoint, hit) mark of one voxel in an update set.
+__device__ __forceinline__ void mark_voxel(const UpdView& u, unsigned int* ovf, int x, int y, int z,
+                                           bool hit) {
+  const uint32_t slot = upd_slot(u, leaf_key(x, y, z), ovf);
+  const int w = x & 7;
+  const uint64_t b = 1ull << (((y & 7) << 3) | (z & 7));
+  upd_or(u, slot, w, b);
+  if (hit) upd_or(u, slot, 8 + w, b);
+}
+
+// Fast-mode raycast (castRayIntoGridFast, VDBMapping.hpp:577-602), one lane
+// per ray: Ray(s + 0.5, e - s, 0, 1) clipped to the snapshot's padded active
+// bbox (result ignored, VDBMapping.hpp:590), VolumeHDDA::hits over the
+// topology snapshot (4096 -> 128 -> 8: every run of existing leaves, as a time
+// span), and for every span a voxel DDA over [t0, t1] that marks the voxels
+// active in the live map as visited. No e == s guard: a zero-length ray
+// probes its origin voxel. Endpoints inside max range are hits as in the
+// normal path (VDBMapping.hpp:533-536). cast == 0 (the map is empty,
+// VDBMapping.hpp:495, :522): endpoints only.
+__global__ __launch_bounds__(256) void k_fast_cast(const int4* __restrict__ rayinfo,
+                                                   const ScanInfo* __restrict__ scans,
+                                                   const BatchCtl* __restrict__ ctl, RtView rt,
+                                                   MapView mp, UpdView u, uint32_t cast,
+                                                   Status* st) {
+  const uint32_t n = ctl->n_rays;
+  unsigned int* ovf = &st->overflow;
+  const uint32_t G = gridDim.x * blockDim.x;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += G) {
+    const int4 r = rayinfo[i];
+    const uint32_t wbits = (uint32_t)r.w;
+    if (!(wbits & kValidBit)) continue;
+    if (wbits & kHitBit) mark_voxel(u, ovf, r.x, r.y, r.z, true);
+    if (!cast) continue;
+    const uint32_t sc = (wbits >> kScanShift) & kScanMask;
+    const int s[3] = {scans[sc].s[0], scans[sc].s[1], scans[sc].s[2]};
+    const int e[3] = {r.x, r.y, r.z};
+    double eye[3], dir[3], idir[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      eye[a] = (double)s[a] + 0.5;
+      dir[a] = (double)e[a] - (double)s[a];
+      idir[a] = 1.0 / dir[a];
+    }
+    // every DDA level takes at most |dir|_1 / dim + 3 steps over t in [0, 1]
+    int budget = 4 * ((int)(fabs(dir[0]) + fabs(dir[1]) + fabs(dir[2])) + 8);
+    double rt0 = 0.0, rt1 = 1.0;
+    rt_clip(rt, eye, idir, rt0, rt1);
+    ActiveProbe probe;
+    auto walk_span = [&](double t0, double t1) {
+      RtDda fd;
+      fd.init(eye, dir, idir, t0, t1, 0);
+      do {
+        if (probe.on(mp, fd.vox[0], fd.vox[1], fd.vox[2])) mark_voxel(u, ovf, fd.vox[0], fd.vox[1], fd.vox[2], false);
+      } while (fd.step() && --budget > 0);
+    };
+    double s0 = -1, s1 = -1;  // the open TimeSpan
+    RtDda d12;
+    d12.init(eye, dir, idir, rt0, rt1, 12);
+    do {
+      if (set_has(rt.n4096, rt.n4096_mask, node_key(d12.vox[0], d12.vox[1], d12.vox[2], 12))) {
+        RtDda d7;
+        d7.init(eye, dir, idir, d12.t0, d12.next(), 7);
+        do {
+          if (set_has(rt.n128, rt.n128_mask, node_key(d7.vox[0], d7.vox[1], d7.vox[2], 7))) {
+            RtDda d3;
+            d3.init(eye, dir, idir, d7.t0, d7.next(), 3);
+            do {
+              if (set_has(rt.leaf, rt.leaf_mask, leaf_key(d3.vox[0], d3.vox[1], d3.vox[2]))) {
+                if (s0 < 0) s0 = d3.t0;
+              } else if (s0 >= 0) {
+                s1 = d3.t0;
+                if (span_valid(s0, s1)) walk_span(s0, s1);
+                s0 = s1 = -1;
+              }
+            } while (d3.step() && --budget > 0);
+            if (s0 >= 0) s1 = d3.t1;
+          } else if (s0 >= 0) {
+            s1 = d7.t0;
+            if (span_valid(s0, s1)) walk_span(s0, s1);
+            s0 = s1 = -1;
+          }
+        } while (d7.step() && --budget > 0);
+        if (s0 >= 0) s1 = d7.t1;
+      } else if (s0 >= 0) {
+        s1 = d12.t0;
+        if (span_valid(s0, s1)) walk_span(s0, s1);
+        s0 = s1 = -1;
+      }
+    } while (d12.step() && --budget > 0);
+    if (s0 >= 0) s1 = d12.t1;
+    if (span_valid(s0, s1)) walk_span(s0, s1);
+    if (budget <= 0) atomicOr(&st->error, kErrWalk);
   }
 }
 
@@ -767,6 +864,14 @@ hipError_t launch_traverse(hipStream_t s, int grid, int block, const int4* rayin
                            uint32_t chunk) {
   hipLaunchKernelGGL(k_traverse, dim3(grid), dim3(block), 0, s, rayinfo, roff, first_ray, scans, ctl,
                      tbl, u, st, chunk);
+  return hipGetLastError();
+}
+
+hipError_t launch_fast_cast(hipStream_t s, int grid, const int4* rayinfo, const ScanInfo* scans,
+                            const BatchCtl* ctl, RtView rt, MapView mp, UpdView u, bool cast,
+                            Status* st) {
+  hipLaunchKernelGGL(k_fast_cast, dim3(grid), dim3(256), 0, s, rayinfo, scans, ctl, rt, mp, u,
+                     (uint32_t)cast, st);
   return hipGetLastError();
 }
 

@@ -21,6 +21,9 @@ hipError_t launch_traverse(hipStream_t s, int grid, int block, const int4* rayin
                            const uint64_t* roff, const uint32_t* first_ray, const ScanInfo* scans,
                            const BatchCtl* ctl, const double* tbl, UpdView u, Status* st,
                            uint32_t chunk);
+hipError_t launch_fast_cast(hipStream_t s, int grid, const int4* rayinfo, const ScanInfo* scans,
+                            const BatchCtl* ctl, RtView rt, MapView mp, UpdView u, bool cast,
+                            Status* st);
 hipError_t launch_compact(hipStream_t s, int grid, UpdView u, uint32_t cap, uint32_t* list,
                           unsigned int* counter, const Status* st);
 hipError_t launch_group_build(hipStream_t s, int grid, UpdView u, const uint32_t* list, GroupView gv,

@@ -47,6 +47,7 @@ struct Source {
   uint32_t* list = nullptr;
   uint32_t cap = 0;
   uint32_t pending = 0;         // listed update leaves not yet integrated
+  bool has_isect = false;       // volume_ray_intersector set (fast mode); none after addInputSource
 };
 
 uint64_t next_pow2(uint64_t v) {
@@ -515,6 +516,7 @@ int scan_error(vdbgpu_map* m, const Status& st, bool batched) {
   if (st.error & kErrTable) return fail(m, VDBGPU_UNSUPPORTED, "a ray spans more than 8192 voxels along one axis");
   if (st.error & kErrEntry) return fail(m, VDBGPU_DEVICE_ERROR, "mid-ray DDA entry did not converge");
   if (st.error & kErrBrick) return fail(m, VDBGPU_DEVICE_ERROR, "brick walk inconsistency (a segment left its brick)");
+  if (st.error & kErrWalk) return fail(m, VDBGPU_DEVICE_ERROR, "fast-mode walk exceeded its step bound");
   return VDBGPU_OK;
 }
 
@@ -612,13 +614,64 @@ int run_pipeline(vdbgpu_map* m, Source& src, const float* xyz, int64_t stride, b
   return VDBGPU_OK;
 }
 
+// raycastPointCloud in fast mode (VDBMapping.hpp:520-527, castRayIntoGridFast
+// :577-602) into the source's update set: k_fast_cast marks the map-active
+// voxels along each ray's runs of existing leaves (topology snapshot of the
+// last fast-mode integrate) plus the endpoint hits; nothing is cast while the
+// map is empty. An update-set overflow grows the set and re-runs (marks are
+// idempotent).
+int run_fast(vdbgpu_map* m, Source& src, const float* xyz, int64_t n, int64_t stride,
+             const double origin[3], bool on_device) {
+  const bool cast = m->touched;  // m_vdb_grid->empty() at the start of the raycast
+  if (cast && !(src.has_isect && m->has_isect))
+    return fail(m, VDBGPU_NO_INTERSECTOR,
+                "fast mode: this source has no volume ray intersector yet (integrate in fast mode first)");
+  ScanInfo sc{};
+  for (int a = 0; a < 3; ++a) sc.o[a] = origin[a];
+  sc.R = src.max_range;
+  sc.ray0 = 0;
+  sc.nrays = (uint32_t)n;
+  int r;
+  uint64_t vb = 0;
+  if ((r = stage_scans(m, m->stream, xyz, stride, on_device, {sc}, (uint32_t)n, false, &vb))) return r;
+  if (src.cap == 0) {
+    if ((r = alloc_update(m, src, (uint32_t)std::max<uint64_t>(kMinUpdateCap, next_pow2(4 * (uint64_t)n))))) return r;
+  }
+  const int grid = std::min<int>(m->num_cu * 8, (int)((n + 255) / 256));
+  for (int attempt = 0;; ++attempt) {
+    {
+      Timer t(m, kTraverse);
+      CK(launch_fast_cast(m->stream, grid, m->rayinfo, m->d_scans, m->d_ctl, m->rt, map_view(m),
+                          upd_view(src), cast, &m->d_st[0]));
+    }
+    {
+      Timer t(m, kCompact);
+      CK(launch_compact(m->stream, std::min<int>(4096, (int)((src.cap + 255) / 256)), upd_view(src),
+                        src.cap, src.list, &m->d_st[0].entries, &m->d_st[0]));
+    }
+    if ((r = read_status(m))) return r;
+    const Status& st = m->h_st[0];
+    if ((r = scan_error(m, st, false))) return r;
+    if (!(st.overflow & kOvfHard)) break;
+    if (attempt > 8 || src.cap >= (1u << 30)) return fail(m, VDBGPU_DEVICE_ERROR, "update set cannot grow");
+    if ((r = grow_update(m, src, src.cap * 4))) return r;
+    CK(hipMemsetAsync(&m->d_st[0].overflow, 0, sizeof(unsigned int), m->stream));
+    CK(hipMemsetAsync(&m->d_st[0].entries, 0, sizeof(unsigned int), m->stream));
+  }
+  const Status& st = m->h_st[0];
+  m->stats.scans += 1;
+  m->stats.rays += st.rays;
+  src.pending = st.entries;
+  return VDBGPU_OK;
+}
+
 int do_accumulate(vdbgpu_map* m, Source& src, const float* xyz, int64_t n, int64_t stride,
                   const double origin[3], bool on_device) {
   if (!m->config_set) return fail(m, VDBGPU_NOT_CONFIGURED, "Map not properly configured. Did you call setConfig method?");
-  if (m->fast_mode) return fail(m, VDBGPU_UNSUPPORTED, "fast_mode raycasting is not provided by this library");
   if (n < 0 || stride < 12 || (stride & 3) || n > (int64_t)kMaxRays || !origin || (n && !xyz))
     return fail(m, VDBGPU_BAD_ARGUMENT, "bad point buffer arguments");
   if (n == 0) return VDBGPU_OK;
+  if (m->fast_mode) return run_fast(m, src, xyz, n, stride, origin, on_device);
   ScanInfo sc{};
   sc.o[0] = origin[0];
   sc.o[1] = origin[1];
@@ -878,6 +931,7 @@ int build_snapshot(vdbgpu_map* m) {
     m->rt.bmax[a] = hb[3 + a] + 1;  // mBBox.max().offset(1)
   }
   m->has_isect = true;
+  for (auto& kv : m->sources) kv.second.has_isect = true;  // every source gets a copy (VDBMapping.hpp:1441-1446)
   return VDBGPU_OK;
 }
 
@@ -1123,6 +1177,7 @@ int32_t vdbgpu_add_source(vdbgpu_map* m, const char* id, double max_range, doubl
     free_update(m, s);
     s.pending = 0;
   }
+  s.has_isect = false;  // a new InputSource has no intersector (VDBMapping.hpp:1352-1375)
   return VDBGPU_OK;
 }
 
@@ -1172,8 +1227,10 @@ int32_t vdbgpu_insert_batch(vdbgpu_map* m, const char* id, int32_t nscans, const
   if (!m || !id || nscans < 0 || (nscans && (!counts || !origins))) return VDBGPU_BAD_ARGUMENT;
   Guard g(m);
   auto it = m->sources.find(id);
-  if (it == m->sources.end() || !m->config_set || m->fast_mode || !(it->second.max_range > 0) ||
-      nscans == 0) {
+  bool any_range = false;
+  if (it != m->sources.end())
+    for (int i = 0; i < nscans && !any_range; ++i) any_range = (ranges ? ranges[i] : it->second.max_range) > 0;
+  if (it == m->sources.end() || !m->config_set || !any_range || nscans == 0) {
     // every insert would only integrate (nothing cast): one integrate does the same
     int status = VDBGPU_OK;
     if (nscans > 0) {
@@ -1181,8 +1238,6 @@ int32_t vdbgpu_insert_batch(vdbgpu_map* m, const char* id, int32_t nscans, const
         status = fail(m, VDBGPU_UNKNOWN_SOURCE, std::string("Tried to accumulate update for ") + id + ". Source not available");
       else if (!m->config_set)
         status = fail(m, VDBGPU_NOT_CONFIGURED, "Map not properly configured. Did you call setConfig method?");
-      else if (m->fast_mode)
-        status = fail(m, VDBGPU_UNSUPPORTED, "fast_mode raycasting is not provided by this library");
     }
     const int r = do_integrate(m);
     return status ? status : r;
@@ -1212,9 +1267,10 @@ int32_t vdbgpu_insert_batch(vdbgpu_map* m, const char* id, int32_t nscans, const
   int i0 = 0;
   bool pending = false;
   for (auto& kv : m->sources) pending = pending || kv.second.pending > 0;
-  if (m->tuning.traversal == 0 && !pending) {
+  if ((m->tuning.traversal == 0 && !pending) || m->fast_mode) {
     // Streaming every scan through the brick path (the next scan's cast
-    // overlaps this one's map update) is faster than the (leaf, scan) fold.
+    // overlaps this one's map update) is faster than the (leaf, scan) fold;
+    // a fast-mode cast depends on the map state, so it runs scan by scan.
     for (int i = 0; i < nscans; ++i) {
       const int r = plain_insert(i);
       if (r) return r;
