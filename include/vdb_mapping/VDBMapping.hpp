@@ -14,8 +14,10 @@
 // Fast mode (:520-527, :577-602) is provided by the library: accumulates cast
 // through the topology snapshot of the last fast-mode integrate.
 //
-// Not provided (each has no call on this path): the async accumulation and
-// integration threads (:133, :355-370, :1383-1430), persistence and PCD I/O (:193-307), map sections,
+// addDataToAccumulate with its accumulation and integration threads (:133,
+// :355-370, :1383-1430) runs in the library (vdbgpu_async_*).
+//
+// Not provided (each has no call on this path): persistence and PCD I/O (:193-307), map sections,
 // byte arrays and morphology (:810-1338), direct point edits and artificial
 // areas (:413-447, :1152-1236), castRayIntoGrid (:550-566, per-ray host call)
 // and updateMap (:731-792, reached through integrateUpdate).
@@ -276,7 +278,44 @@ public:
   {
     vdbgpu_add_source(m_map, source_id.c_str(), max_range, max_rate);
     m_sources[source_id] = (max_range == 0) ? m_max_range : max_range;
+    m_rates[source_id]   = max_rate;
   }
+
+  /*!
+   * \brief Hands a cloud to the source's accumulation thread (reference
+   * :355-370): latest only, accumulated by the source's worker and integrated
+   * by the integration thread every accumulation_period. The threads run in
+   * the library (vdbgpu_async_*) and start with the first call.
+   */
+  void addDataToAccumulate(const PointCloudT::ConstPtr& cloud,
+                           const Eigen::Matrix<double, 3, 1>& origin,
+                           const std::string source_id)
+  {
+    auto it = m_rates.find(source_id);
+    if (it == m_rates.end())
+    {
+      std::cout << "Tried to add data for accumulation of " << source_id
+                << ". Source not available" << std::endl;
+      return;
+    }
+    vdbgpu_async_start(m_map, m_accumulation_period);
+    const double o[3] = {origin.x(), origin.y(), origin.z()};
+    const float* xyz  = cloud->points.empty() ? nullptr : &cloud->points[0].x;
+    report(vdbgpu_async_submit(m_map,
+                               source_id.c_str(),
+                               xyz,
+                               static_cast<int64_t>(cloud->points.size()),
+                               static_cast<int64_t>(sizeof(PointT)),
+                               o,
+                               it->second));
+  }
+
+  /*!
+   * \brief Waits until the accumulation threads have taken and accumulated
+   * every handed cloud, then integrates (no reference counterpart: the
+   * reference offers no way to wait for its threads).
+   */
+  void flushAccumulation() { report(vdbgpu_async_flush(m_map)); }
 
   virtual void setConfig(const TConfig& config)
   {
@@ -286,10 +325,11 @@ public:
                 << std::endl;
       return;
     }
-    m_max_range          = config.max_range;
-    m_map_directory_path = config.map_directory_path;
-    m_fast_mode          = config.fast_mode;
-    m_config_set         = true;
+    m_max_range           = config.max_range;
+    m_map_directory_path  = config.map_directory_path;
+    m_fast_mode           = config.fast_mode;
+    m_accumulation_period = config.accumulation_period;
+    m_config_set          = true;
   }
 
   /*!
@@ -331,6 +371,8 @@ protected:
   std::string m_map_directory_path;
   bool m_config_set;
   std::map<std::string, double> m_sources;
+  std::map<std::string, double> m_rates;
+  double m_accumulation_period = 0;
   mutable std::shared_ptr<std::shared_mutex> m_map_mutex;
 };
 

@@ -238,6 +238,28 @@ vdbgpu_map* vdbgpu_server_map(vdbgpu_server* server, int32_t map_index);
 int64_t vdbgpu_server_device_bytes(vdbgpu_server* server);
 void vdbgpu_server_destroy(vdbgpu_server* server);
 
+/* ---- asynchronous ingestion (opt-in) ----------------------------------------
+ * The reference's addDataToAccumulate (VM:355-370), per-source accumulation
+ * threads and integration thread (VM:133, :1383-1430) as an engine per map:
+ * each source has a latest-only slot (a new cloud replaces one not yet taken)
+ * and a worker that accumulates it and then waits out 1/max_rate; one thread
+ * integrates every accumulation_period seconds. Scans accumulated within a
+ * period OR-merge and replaced scans are dropped, exactly the reference's
+ * semantics; for per-scan exactness use vdbgpu_insert or the server. Not
+ * started until vdbgpu_async_start (the reference starts its threads in the
+ * constructor and addInputSource). Calls must not race vdbgpu_async_stop. */
+int32_t vdbgpu_async_start(vdbgpu_map* map, double accumulation_period_s);
+/* Copies the cloud into the source's slot (xyz, stride as vdbgpu_insert). */
+int32_t vdbgpu_async_submit(vdbgpu_map* map, const char* source_id, const float* xyz, int64_t n,
+                            int64_t stride_bytes, const double origin[3], double max_rate);
+/* Waits until every slot is empty and its accumulate done, then integrates;
+ * returns the first error any engine call met. */
+int32_t vdbgpu_async_flush(vdbgpu_map* map);
+/* out: submitted, replaced (dropped), accumulated, integrations. */
+int32_t vdbgpu_async_counters(vdbgpu_map* map, int64_t out[4]);
+/* Stops and joins the threads (vdbgpu_destroy does it too). */
+int32_t vdbgpu_async_stop(vdbgpu_map* map);
+
 #ifdef __cplusplus
 }
 #endif

@@ -161,6 +161,38 @@ static void raytrace_test()
   CHECK(end[0] == 0.5 && end[1] == 34.5 && end[2] == 0.5);
 }
 
+static void add_data_to_accumulate_test()
+{
+  // the asynchronous front end: one cloud through addDataToAccumulate, then a
+  // flush, gives the same map as insertPointCloud of that cloud
+  OccupancyVDBMapping a(1), b(1);
+  Config conf              = make_config(20);
+  conf.accumulation_period = 60.0;  // the integration thread idles; the flush integrates
+  a.setConfig(conf);
+  b.setConfig(conf);
+  a.addInputSource("test", conf.max_range, 0);
+  b.addInputSource("test", conf.max_range, 0);
+  OccupancyVDBMapping::PointCloudT::Ptr cloud(new OccupancyVDBMapping::PointCloudT);
+  for (int k = -3; k <= 3; ++k) cloud->points.emplace_back(6, k, 2 * k);
+  Eigen::Matrix<double, 3, 1> origin(0.2, 0.1, 0.3);
+  a.addDataToAccumulate(cloud, origin, "test");
+  a.flushAccumulation();
+  b.insertPointCloud(cloud, origin, "test");
+  auto ga = a.getGrid()->getAccessor();
+  auto gb = b.getGrid()->getAccessor();
+  int same = 0;
+  for (int x = -1; x <= 7; ++x)
+    for (int y = -4; y <= 4; ++y)
+      for (int z = -7; z <= 7; ++z)
+      {
+        const openvdb::Coord c(x, y, z);
+        same += ga.getValue(c) == gb.getValue(c) && ga.isValueOn(c) == gb.isValueOn(c);
+      }
+  CHECK(same == 9 * 9 * 15);
+  CHECK(ga.isValueOn(openvdb::Coord(6, 3, 6)));
+  a.addDataToAccumulate(cloud, origin, "missing");  // prints, does nothing
+}
+
 int main()
 {
   struct T
@@ -175,6 +207,7 @@ int main()
     {"ResetMap", reset_map_test},
     {"RaycastPointCloud", raycast_point_cloud_test},
     {"Raytrace", raytrace_test},
+    {"AddDataToAccumulate", add_data_to_accumulate_test},
   };
   for (const T& t : tests)
   {

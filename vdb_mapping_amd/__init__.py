@@ -33,7 +33,8 @@ ABI_SYMBOLS = [
     "vdbgpu_synchronize",
     "vdbgpu_get_stats", "vdbgpu_set_profiling", "vdbgpu_set_tuning", "vdbgpu_get_tuning",
     "vdbgpu_last_error", "vdbgpu_device_alloc", "vdbgpu_device_free", "vdbgpu_device_upload",
-    "vdbgpu_device_download", "vdbgpu_device_copy", "vdbgpu_server_create", "vdbgpu_server_submit", "vdbgpu_server_drain",
+    "vdbgpu_device_download", "vdbgpu_device_copy", "vdbgpu_async_start", "vdbgpu_async_submit",
+    "vdbgpu_async_flush", "vdbgpu_async_counters", "vdbgpu_async_stop", "vdbgpu_server_create", "vdbgpu_server_submit", "vdbgpu_server_drain",
     "vdbgpu_server_records", "vdbgpu_server_map", "vdbgpu_server_device_bytes",
     "vdbgpu_server_destroy",
 ]
@@ -150,6 +151,15 @@ def lib():
         L.vdbgpu_device_alloc.argtypes = [I64, ctypes.POINTER(P)]
         L.vdbgpu_device_free.restype = I32
         L.vdbgpu_device_free.argtypes = [P]
+        L.vdbgpu_async_start.restype = I32
+        L.vdbgpu_async_start.argtypes = [P, D]
+        L.vdbgpu_async_submit.restype = I32
+        L.vdbgpu_async_submit.argtypes = [P, ctypes.c_char_p, P, I64, I64, P, D]
+        for f in (L.vdbgpu_async_flush, L.vdbgpu_async_stop):
+            f.restype = I32
+            f.argtypes = [P]
+        L.vdbgpu_async_counters.restype = I32
+        L.vdbgpu_async_counters.argtypes = [P, P]
         for f in (L.vdbgpu_device_upload, L.vdbgpu_device_download, L.vdbgpu_device_copy):
             f.restype = I32
             f.argtypes = [P, P, I64]
@@ -356,6 +366,27 @@ class GpuMap:
         return lib().vdbgpu_raytrace_device(self.h, n, ctypes.c_void_p(d_origins),
                                             ctypes.c_void_p(d_dirs), ctypes.c_void_p(d_lens),
                                             ctypes.c_void_p(d_success), ctypes.c_void_p(d_ends))
+
+    # asynchronous ingestion (addDataToAccumulate + worker/integration threads)
+    def async_start(self, accumulation_period_s: float) -> int:
+        return lib().vdbgpu_async_start(self.h, accumulation_period_s)
+
+    def add_data_to_accumulate(self, source_id: str, points, origin, max_rate: float = 0.0) -> int:
+        pts, stride = self._pts(points)
+        o = np.ascontiguousarray(origin, dtype=np.float64)
+        return lib().vdbgpu_async_submit(self.h, source_id.encode(), _ptr(pts), pts.shape[0],
+                                         stride, _ptr(o), max_rate)
+
+    def async_flush(self) -> int:
+        return lib().vdbgpu_async_flush(self.h)
+
+    def async_counters(self) -> dict:
+        out = np.zeros(4, np.int64)
+        lib().vdbgpu_async_counters(self.h, _ptr(out))
+        return dict(zip(("submitted", "replaced", "accumulated", "integrations"), out.tolist()))
+
+    def async_stop(self) -> int:
+        return lib().vdbgpu_async_stop(self.h)
 
     def stats(self) -> dict:
         s = Stats()

@@ -91,6 +91,7 @@ struct vdbgpu_map {
   // most win_a + win_b leaves.
   uint64_t win_a = 0, win_b = 0;
   bool touched = false;        // any voxel modify since creation/reset
+  bool snap_dirty = true;      // the map changed since the last topology snapshot
   unsigned long long* noop4096 = nullptr;  // root children of updates that created no leaf
   static constexpr uint32_t kNoopCap = 4096;
 
@@ -594,6 +595,7 @@ int run_pipeline(vdbgpu_map* m, Source& src, const float* xyz, int64_t stride, b
   if (batched) {
     m->stats.update_leaves += st.entries;
     m->touched = m->touched || st.entries > 0;
+    m->snap_dirty = m->snap_dirty || st.entries > 0;
     src.pending = 0;
   } else {
     src.pending = st.entries;
@@ -859,6 +861,7 @@ int run_brick(vdbgpu_map* m, Source& src, const float* xyz, int64_t n, int64_t s
   m->win_a = m->win_b + recs;
   m->win_b = 0;
   m->touched = true;
+  m->snap_dirty = true;
   return VDBGPU_OK;
 }
 
@@ -931,7 +934,19 @@ int build_snapshot(vdbgpu_map* m) {
     m->rt.bmax[a] = hb[3 + a] + 1;  // mBBox.max().offset(1)
   }
   m->has_isect = true;
+  m->snap_dirty = false;
   for (auto& kv : m->sources) kv.second.has_isect = true;  // every source gets a copy (VDBMapping.hpp:1441-1446)
+  return VDBGPU_OK;
+}
+
+// updateVolumeRayIntersectors after an integrate (VDBMapping.hpp:1436-1449):
+// in fast mode with a non-empty map every source gets the intersector; the
+// snapshot is rebuilt only when the map changed since the last one (an
+// identical topology copy otherwise).
+int refresh_snapshot(vdbgpu_map* m) {
+  if (!m->fast_mode || !m->touched) return VDBGPU_OK;
+  if (m->snap_dirty || !m->has_isect) return build_snapshot(m);
+  for (auto& kv : m->sources) kv.second.has_isect = true;
   return VDBGPU_OK;
 }
 
@@ -963,9 +978,10 @@ int do_integrate(vdbgpu_map* m) {
     m->leaf_count_ub += s.pending;
     m->win_b += s.pending;
     m->touched = true;
+    m->snap_dirty = true;
     s.pending = 0;
   }
-  if (m->fast_mode && m->touched) return build_snapshot(m);
+  return refresh_snapshot(m);
   return VDBGPU_OK;
 }
 
@@ -1070,6 +1086,7 @@ int32_t vdbgpu_create(double resolution, vdbgpu_map** out) {
 
 void vdbgpu_destroy(vdbgpu_map* m) {
   if (!m) return;
+  (void)vdbgpu_async_stop(m);  // the ingestion threads call into this handle
   (void)hipSetDevice(m->device);
   if (m->stream) (void)hipStreamSynchronize(m->stream);
   if (m->cast) (void)hipStreamSynchronize(m->cast);
@@ -1275,7 +1292,7 @@ int32_t vdbgpu_insert_batch(vdbgpu_map* m, const char* id, int32_t nscans, const
       const int r = plain_insert(i);
       if (r) return r;
     }
-    if (m->fast_mode && m->touched) return build_snapshot(m);
+    return refresh_snapshot(m);
     return VDBGPU_OK;
   }
   if (pending) {
@@ -1313,7 +1330,7 @@ int32_t vdbgpu_insert_batch(vdbgpu_map* m, const char* id, int32_t nscans, const
     if (r) return r;
     i0 = i1;
   }
-  if (m->fast_mode && m->touched) return build_snapshot(m);
+  return refresh_snapshot(m);
   return VDBGPU_OK;
 }
 
@@ -1339,6 +1356,7 @@ int32_t vdbgpu_reset(vdbgpu_map* m) {
   m->leaf_count_ub = 0;
   m->win_a = m->win_b = 0;
   m->touched = false;
+  m->snap_dirty = true;
   CK(hipStreamSynchronize(m->stream));
   return VDBGPU_OK;
 }
