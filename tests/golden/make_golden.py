@@ -1,0 +1,203 @@
+"""Writes the golden fixtures of this directory.
+
+  known_answers.json  the reference's five known-answer tests
+                      (tests/mapping.cpp:6-158) as data: inputs and the
+                      expected voxel values as float32 bit patterns, derived
+                      from the closed-form log-odds (math.log), not from the
+                      oracle.
+  oracle_cases.npz    small cases with their full expected outputs, written
+                      by the oracle once it is pinned by tests/test_oracle.py:
+                      tie-heavy rays (update set and map), a fast-mode map
+                      with raytrace queries, and the map edits.
+  digests.json        SHA-256 digests of the oracle's map export after five
+                      32,768-ray room scans (res 0.1, R 15) and of the update
+                      set of the first scan's first 4,096 rays.
+
+The fixtures are committed; the tests compare both the oracle (drift check)
+and the GPU library (parity without running the oracle) with them. Re-run
+this script only when the restated semantics change on purpose:
+    python tests/golden/make_golden.py
+"""
+from __future__ import annotations
+
+import hashlib
+import json
+import math
+import os
+import struct
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+import oracle as O  # noqa: E402
+from vdb_mapping_amd import workloads as W  # noqa: E402
+
+
+def f2u(x) -> int:
+    return struct.unpack("<I", struct.pack("<f", float(np.float32(x))))[0]
+
+
+def logit32(p) -> int:
+    return f2u(np.float32(math.log(p) - math.log(1 - p)))
+
+
+# ---------------------------------------------------------------- cases --
+
+# Default probabilities of the reference's tests (tests/mapping.cpp:18-25).
+DEFAULT_PROBS = dict(prob_hit=0.9, prob_miss=0.1, prob_thres_min=0.49, prob_thres_max=0.51)
+
+TIE_ENDS = [(a, a, 0) for a in (1, 2, 7, 64, 333)] + [(a, a, a) for a in (1, 3, 8, 100, 999)] + \
+    [(a, 2 * a, 0) for a in (1, 5, 50)] + [(3, 1, 0), (5, 15, 0), (15, 5, 3), (30, 10, 6),
+                                           (1000, 999, 1), (7, -7, 7), (-999, 333, -111)]
+
+
+def tie_points():
+    ends = []
+    for e in TIE_ENDS:
+        for sx in (1, -1):
+            for sy in (1, -1):
+                ends.append((sx * e[0] + 0.25, sy * e[1] + 0.25, e[2] + 0.25))
+    return np.array(ends, np.float32), np.array([0.25, 0.25, 0.25])
+
+
+def fast_case():
+    """A map built in fast mode from two scans of a 1 m room corner, then
+    raytrace queries along and across it (res 0.1, R 6)."""
+    rng = np.random.default_rng(77)
+    pts = []
+    for _ in range(1500):
+        u, v = rng.uniform(-2.0, 2.0, 2)
+        k = rng.integers(3)
+        p = [u, v, 1.5] if k == 0 else ([1.5, u, v] if k == 1 else [u, 1.5, v])
+        pts.append(p)
+    pts = np.array(pts, np.float32)
+    origins = [np.array([0.02, -0.03, 0.01]), np.array([0.3, 0.2, -0.1])]
+    q = rng.normal(size=(300, 3))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    q_orig = np.repeat(origins[1][None, :], 300, axis=0) + rng.uniform(-0.2, 0.2, (300, 3))
+    q_len = rng.uniform(0.5, 4.0, 300)
+    return pts, origins, q_orig, q, q_len
+
+
+def edit_case():
+    """Point edits and artificial areas on a scanned map (res 0.1, R 5)."""
+    rng = np.random.default_rng(91)
+    scan = (rng.normal(0, 1.5, (4000, 3))).astype(np.float32)
+    add = rng.uniform(-3, 3, (600, 3)).astype(np.float32)
+    rem = np.concatenate([scan[::7], rng.uniform(-3, 3, (200, 3)).astype(np.float32)])
+    polygons = [np.array([[-1.0, -1.0, 0.0], [1.2, -1.0, 0.0], [1.2, 0.8, 0.0]]),
+                np.array([[2.05, 2.0, -0.3], [-2.0, 2.55, -0.3]])]
+    return scan, add, rem, polygons
+
+
+ROOM_SCANS = 5
+
+
+def map_arrays(m, prefix, out):
+    o, v, a = m.export_leaves()
+    out[prefix + "_origins"] = o
+    out[prefix + "_values"] = v.view(np.uint32)
+    out[prefix + "_masks"] = a
+
+
+def digest(*arrays) -> str:
+    h = hashlib.sha256()
+    for a in arrays:
+        h.update(np.ascontiguousarray(a).tobytes())
+    return h.hexdigest()
+
+
+def known_answers():
+    """tests/mapping.cpp:6-158 as data. Voxels absent from a case's map read
+    as (0.0, inactive)."""
+    hit, miss = logit32(0.9), logit32(0.1)
+    cases = [
+        {"name": "set_config", "ref": "tests/mapping.cpp:6-30", "res": 1.0, "max_range": 10.0,
+         "points": [[0, 0, 1]], "origin": [0, 0, 0],
+         "expect": [[[0, 0, 0], miss, False], [[0, 0, 1], hit, True]]},
+    ]
+    for sign in (1, -1):
+        exp = [[[0, 0, sign * i], miss, False] for i in range(5)] + [[[0, 0, sign * 5], hit, True]]
+        cases.append({"name": "insert_point_%s" % ("pos" if sign > 0 else "neg"),
+                      "ref": "tests/mapping.cpp:32-98", "res": 0.1, "max_range": 10.0,
+                      "points": [[0, 0, float(np.float32(sign * 5 * 0.1))]], "origin": [0, 0, 0],
+                      "expect": exp})
+    exp = [[[0, 0, i], miss, False] for i in range(0, 6)] + [[[0, 0, 6], 0, False]]
+    cases.append({"name": "max_range", "ref": "tests/mapping.cpp:100-131", "res": 0.1,
+                  "max_range": 0.5, "points": [[0, 0, float(np.float32(7 * 0.1))]],
+                  "origin": [0, 0, 0], "expect": exp})
+    cases.append({"name": "reset", "ref": "tests/mapping.cpp:133-158", "res": 1.0, "max_range": 10.0,
+                  "points": [[0, 0, 1]], "origin": [0, 0, 0], "reset": True,
+                  "expect_before_reset": [[[0, 0, 1], hit, True]],
+                  "expect": [[[0, 0, 1], 0, False]]})
+    return {"probs": DEFAULT_PROBS, "cases": cases}
+
+
+def main():
+    O.build()
+    W.build()
+    with open(os.path.join(HERE, "known_answers.json"), "w") as f:
+        json.dump(known_answers(), f, indent=1)
+
+    out = {}
+    # tie-heavy rays at res 1: update set after accumulate, map after integrate
+    m = O.OracleMap(1.0)
+    m.set_config(O.make_config(max_range=5000.0))
+    m.add_input_source("test", 5000.0)
+    pts, origin = tie_points()
+    assert m.accumulate("test", pts, origin) == O.OK
+    uo, uv, uh = m.export_update("test")
+    out["tie_points"], out["tie_origin"] = pts, origin
+    out["tie_update_origins"], out["tie_update_visited"], out["tie_update_hit"] = uo, uv, uh
+    assert m.integrate() == O.OK
+    map_arrays(m, "tie_map", out)
+
+    # fast mode: two scans, then raytrace
+    pts, origins, q_o, q_d, q_l = fast_case()
+    m = O.OracleMap(0.1)
+    m.set_config(O.make_config(max_range=6.0, fast_mode=True, **W.PROBS))
+    m.add_input_source("test", 6.0)
+    for o in origins:
+        assert m.insert("test", pts, o) == O.OK
+    st, succ, ends = m.raytrace(q_o, q_d, q_l)
+    assert st == O.OK
+    out["fast_points"], out["fast_origins"] = pts, np.array(origins)
+    map_arrays(m, "fast_map", out)
+    out["rt_origins"], out["rt_dirs"], out["rt_lengths"] = q_o, q_d, q_l
+    out["rt_success"], out["rt_ends"] = succ.astype(np.uint8), ends
+
+    np.savez_compressed(os.path.join(HERE, "oracle_cases.npz"), **out)
+
+    # room scan digests
+    w = W.room_scan_workload()
+    s = w.scan(0)
+    m = O.OracleMap(w.resolution)
+    m.set_config(O.make_config(max_range=w.max_range, **W.PROBS))
+    m.add_input_source("test", w.max_range)
+    assert m.accumulate("test", s.points[:4096], s.origin) == O.OK
+    uo, uv, uh = m.export_update("test")
+    upd = {"leaves": int(uo.shape[0]), "visited_voxels": int(sum(bin(int(x)).count("1") for x in uv.ravel())),
+           "sha256": digest(uo, uv, uh)}
+    m.reset()
+    for i in range(ROOM_SCANS):
+        s = w.scan(i)
+        assert m.insert("test", s.points, s.origin) == O.OK
+    o, v, a = m.export_leaves()
+    res = {"room_scan": {"workload": "room scan, scans 0-4 inserted in order, 32 x 1024 rays each, "
+                                     "res 0.1, R 15, p 0.7/0.4/0.12/0.97",
+                         "scans": ROOM_SCANS,
+                         "leaves": int(o.shape[0]),
+                         "active_voxels": int(sum(bin(int(x)).count("1") for x in a.ravel())),
+                         "sha256": digest(o, v.view(np.uint32), a)},
+           "room_scan_first_4096_update": upd}
+    with open(os.path.join(HERE, "digests.json"), "w") as f:
+        json.dump(res, f, indent=1)
+    print("wrote", sorted(os.listdir(HERE)))
+
+
+if __name__ == "__main__":
+    main()
