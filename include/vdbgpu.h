@@ -260,6 +260,66 @@ int32_t vdbgpu_async_counters(vdbgpu_map* map, int64_t out[4]);
 /* Stops and joins the threads (vdbgpu_destroy does it too). */
 int32_t vdbgpu_async_stop(vdbgpu_map* map);
 
+/* ---- direct map edits --------------------------------------------------------
+ * addPointsToGrid (occupied = 1) / removePointsFromGrid (occupied = 0),
+ * VM:413-447: the voxel floor(p / resolution) of every point (no half-voxel
+ * shift) becomes (max log-odds, active) or (min log-odds, inactive)
+ * (setNodeToOccupied / setNodeToFree, OCC:118-129). Pending update sets are
+ * not touched. A NaN or out-of-range point rejects the call
+ * (VDBGPU_OUT_OF_RANGE) before any change. */
+int32_t vdbgpu_set_points(vdbgpu_map* map, const float* xyz, int64_t n, int64_t stride_bytes,
+                          int32_t occupied);
+
+/* addArtificialAreas (VM:1175-1236): restores the previous areas
+ * (restoreMapIntegrity), then walls between consecutive vertices (and last
+ * to first) of each polygon, rounded like sensor origins (VM:612-631), cast
+ * for every height layer k in [(int)(negative_height / res),
+ * (int)(positive_height / res)) into the artificial grid. Polygon p has
+ * counts[p] vertices, taken in order from xyz (x, y, z per vertex). The
+ * artificial voxels are set active at the end of every later non-empty
+ * map update (VM:785-789). Walls longer than 8192 voxels along one axis give
+ * VDBGPU_UNSUPPORTED; a rejected call changes nothing. */
+int32_t vdbgpu_add_artificial_areas(vdbgpu_map* map, int32_t n_polygons, const int32_t* counts,
+                                    const double* xyz, double negative_height,
+                                    double positive_height);
+/* restoreMapIntegrity (VM:1152-1166): every artificial voxel gets
+ * active = value > thres_max (setNodeState, OCC:130-134); the grid is cleared. */
+int32_t vdbgpu_restore_map_integrity(vdbgpu_map* map);
+/* The artificial grid: leaf origins and voxel masks, sorted by origin (the
+ * m_artificial_area_grid the reference keeps); null outputs return the count,
+ * a short cap returns -count. */
+int64_t vdbgpu_export_artificial(vdbgpu_map* map, int32_t* origins, uint64_t* masks, int64_t cap);
+
+/* ---- map sections -------------------------------------------------------------
+ * getMapSectionGrid / getMapSectionUpdateGrid (VM:884-1014) over the inclusive
+ * index box bbox = {min x, y, z, max x, y, z} (createIndexBoundingBox, see
+ * vdbgpu_index_bbox). full_grid = 1: the inside voxels of every overlapping
+ * leaf with value and state (extractFullLeaf); 0: the inside active voxels
+ * with value 1.0 (extractSparseLeaf; true in an update grid). Voxels outside
+ * the box are (0, inactive). Leaves sorted by origin as in
+ * vdbgpu_export_leaves; null outputs return the count, a short cap returns
+ * -count, an error -1. */
+int64_t vdbgpu_export_section(vdbgpu_map* map, const int32_t bbox[6], int32_t full_grid,
+                              int32_t* origins, float* values, uint64_t* masks, int64_t cap);
+/* applyMapSectionGrid without smoothing (VM:1022-1050): every voxel of every
+ * section leaf is written to the map with its value and state, and the
+ * inactive tiles of the section grid's internal nodes write (0, inactive) at
+ * their origins, as the reference's cbeginValueAll loop does. Origins must be
+ * distinct multiples of 8. smooth_map = true (OpenVDB morphology) is not
+ * offered. */
+int32_t vdbgpu_apply_section(vdbgpu_map* map, int64_t n, const int32_t* origins,
+                             const float* values, const uint64_t* masks);
+/* applyMapSectionUpdateGrid without smoothing (VM:1058-1085): active map
+ * voxels inside bbox (the section's bb_min / bb_max) are deactivated, then
+ * every active section voxel is activated (a missing leaf is created with
+ * value 0). */
+int32_t vdbgpu_apply_section_update(vdbgpu_map* map, const int32_t bbox[6], int64_t n,
+                                    const int32_t* origins, const uint64_t* masks);
+/* createIndexBoundingBox (VM:811-874): tf is the row-major 4x4 map-to-reference
+ * transform; out = {min x, y, z, max x, y, z}. */
+int32_t vdbgpu_index_bbox(double resolution, const double min_boundary[3],
+                          const double max_boundary[3], const double tf[16], int32_t out[6]);
+
 #ifdef __cplusplus
 }
 #endif

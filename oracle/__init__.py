@@ -17,7 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liborc.so")
 
 OK, NOT_CONFIGURED, UNKNOWN_SOURCE, BAD_MAX_RANGE, BAD_PROB_MISS, BAD_PROB_HIT = 0, 1, 2, 3, 4, 5
-OUT_OF_RANGE, UNSUPPORTED, NO_INTERSECTOR = 6, 7, 10
+OUT_OF_RANGE, UNSUPPORTED, BAD_ARGUMENT, NO_INTERSECTOR = 6, 7, 9, 10
 
 BUILD_CMD = [
     "g++", "-O2", "-std=c++17", "-ffp-contract=off", "-fno-fast-math", "-fPIC", "-shared",
@@ -86,6 +86,16 @@ def lib():
         L.orc_cast_ray.restype = I64
         L.orc_cast_ray.argtypes = [P, P, P, I64]
         L.orc_raytrace.argtypes = [P, I64, P, P, P, P, P]
+        L.orc_set_points.argtypes = [P, P, I64, I64, I32]
+        L.orc_add_artificial_areas.argtypes = [P, I32, P, P, D, D]
+        L.orc_restore_map_integrity.argtypes = [P]
+        L.orc_export_artificial.restype = I64
+        L.orc_export_artificial.argtypes = [P, P, P, I64]
+        L.orc_export_section.restype = I64
+        L.orc_export_section.argtypes = [P, P, I32, P, P, P, I64]
+        L.orc_apply_section.argtypes = [P, I64, P, P, P]
+        L.orc_apply_section_update.argtypes = [P, P, I64, P, P]
+        L.orc_index_bbox.argtypes = [D, P, P, P, P]
         _lib = L
     return _lib
 
@@ -108,6 +118,24 @@ def cast_ray(s, e) -> np.ndarray:
     out = np.zeros((max(n, 1), 3), np.int32)
     lib().orc_cast_ray(_ptr(s), _ptr(e), _ptr(out), n)
     return out[:n]
+
+
+def index_bbox(res: float, min_boundary, max_boundary, tf=None) -> np.ndarray:
+    """createIndexBoundingBox: inclusive index box (min xyz, max xyz)."""
+    mn = np.ascontiguousarray(min_boundary, dtype=np.float64)
+    mx = np.ascontiguousarray(max_boundary, dtype=np.float64)
+    t = np.ascontiguousarray(np.eye(4) if tf is None else tf, dtype=np.float64)
+    out = np.zeros(6, np.int32)
+    lib().orc_index_bbox(res, _ptr(mn), _ptr(mx), _ptr(t), _ptr(out))
+    return out
+
+
+def polygons_arg(polygons):
+    """Flatten a list of (k, 3+) vertex arrays into (counts int32, xyz float64)."""
+    counts = np.array([len(p) for p in polygons], np.int32)
+    rows = [np.asarray(p, np.float64).reshape(len(p), -1)[:, :3] for p in polygons if len(p)]
+    xyz = np.ascontiguousarray(np.concatenate(rows) if rows else np.zeros((0, 3)), dtype=np.float64)
+    return counts, xyz
 
 
 class OracleMap:
@@ -186,6 +214,55 @@ class OracleMap:
         act = np.zeros(1, np.int32)
         v = lib().orc_get_value(self.h, _ptr(c), _ptr(act))
         return np.float32(v), bool(act[0])
+
+    # map edits and sections
+    def set_points(self, points, occupied: bool) -> int:
+        pts, stride = self._pts(points)
+        return lib().orc_set_points(self.h, _ptr(pts), pts.shape[0], stride, int(bool(occupied)))
+
+    def add_points_to_grid(self, points) -> int:
+        return self.set_points(points, True)
+
+    def remove_points_from_grid(self, points) -> int:
+        return self.set_points(points, False)
+
+    def add_artificial_areas(self, polygons, negative_height: float, positive_height: float) -> int:
+        counts, xyz = polygons_arg(polygons)
+        return lib().orc_add_artificial_areas(self.h, len(counts), _ptr(counts), _ptr(xyz),
+                                              negative_height, positive_height)
+
+    def restore_map_integrity(self) -> int:
+        return lib().orc_restore_map_integrity(self.h)
+
+    def export_artificial(self):
+        n = lib().orc_export_artificial(self.h, None, None, 0)
+        o = np.zeros((n, 3), np.int32)
+        m = np.zeros((n, 8), np.uint64)
+        assert lib().orc_export_artificial(self.h, _ptr(o), _ptr(m), n) == n
+        return o, m
+
+    def export_section(self, bbox, full_grid: bool):
+        b = np.ascontiguousarray(bbox, dtype=np.int32)
+        n = lib().orc_export_section(self.h, _ptr(b), int(bool(full_grid)), None, None, None, 0)
+        o = np.zeros((n, 3), np.int32)
+        v = np.zeros((n, 512), np.float32)
+        m = np.zeros((n, 8), np.uint64)
+        got = lib().orc_export_section(self.h, _ptr(b), int(bool(full_grid)), _ptr(o), _ptr(v),
+                                       _ptr(m), n)
+        assert got == n
+        return o, v, m
+
+    def apply_section(self, origins, values, masks) -> int:
+        o = np.ascontiguousarray(origins, dtype=np.int32).reshape(-1, 3)
+        v = np.ascontiguousarray(values, dtype=np.float32).reshape(-1, 512)
+        m = np.ascontiguousarray(masks, dtype=np.uint64).reshape(-1, 8)
+        return lib().orc_apply_section(self.h, o.shape[0], _ptr(o), _ptr(v), _ptr(m))
+
+    def apply_section_update(self, bbox, origins, masks) -> int:
+        b = np.ascontiguousarray(bbox, dtype=np.int32)
+        o = np.ascontiguousarray(origins, dtype=np.int32).reshape(-1, 3)
+        m = np.ascontiguousarray(masks, dtype=np.uint64).reshape(-1, 8)
+        return lib().orc_apply_section_update(self.h, _ptr(b), o.shape[0], _ptr(o), _ptr(m))
 
     def raytrace(self, origins, dirs, lengths):
         o = np.ascontiguousarray(origins, dtype=np.float64).reshape(-1, 3)

@@ -24,6 +24,10 @@
 //   U11 VolumeHDDA::hits (fast-mode raycast) walks like the march without its
 //      Ray::valid() pre-check and early exit, pushes every valid span, and
 //      closes an open span at the root level's max time.
+//   U12 Grid::cbeginValueAll visits the inactive tiles of internal nodes (one
+//      value per tile, at the tile's origin) besides the leaf voxels.
+//   U13 pcl::transformPointCloud with a double matrix computes each output
+//      coordinate as ((m0 p0 + m1 p1) + m2 p2) + m3 in double, rounded to float.
 //
 // Compile with -ffp-contract=off (no FMA contraction): the reference is built
 // for baseline x86-64 (Release, no -march flags, VM CMakeLists.txt:8-11),
@@ -220,6 +224,8 @@ struct Map {
   std::unordered_map<uint64_t, Leaf> leaves;
   std::unordered_set<uint64_t> n4096;  // root children (created even by no-op modifies)
   std::unique_ptr<Intersector> isect;
+  UpdateSet art;             // m_artificial_area_grid (VM:132): visited = artificial voxel
+  bool art_present = false;  // m_artificial_areas_present (VM:1529)
 };
 
 // Keys of the 4096^3 (root child) and 128^3 internal nodes holding voxel (x,y,z).
@@ -246,9 +252,32 @@ inline void op_miss(const Map& m, float& x, bool& act) {
   }
 }
 
+Leaf& touch_leaf(Map& m, uint64_t k) {
+  auto it = m.leaves.find(k);
+  if (it != m.leaves.end()) return it->second;
+  Leaf z;
+  std::memset(&z, 0, sizeof z);
+  return m.leaves.emplace(k, z).first->second;
+}
+
+// setActiveState(xyz, true) over every artificial voxel (VM:785-789): a
+// missing leaf is created (value 0) since the state changes.
+void activate_artificial(Map& m) {
+  for (auto& kv : m.art.leaves) {
+    int32_t o[3];
+    key_origin(kv.first, o);
+    m.n4096.insert(key4096(o[0], o[1], o[2]));
+    Leaf& leaf = touch_leaf(m, kv.first);
+    for (int w = 0; w < 8; ++w) leaf.act[w] |= kv.second.vis[w];
+  }
+}
+
 // updateMap (VM:731-792) over one source's update set. Leaves are created
-// only when the op would change a background tile [U8].
+// only when the op would change a background tile [U8]. An empty update set
+// returns at once (VM:735-738); otherwise the artificial voxels are set
+// active after the update (VM:785-789).
 void apply_update(Map& m, UpdateSet& u) {
+  if (u.leaves.empty()) return;
   for (auto& kv : u.leaves) {
     const uint64_t k = kv.first;
     const UpdLeaf& ul = kv.second;
@@ -296,6 +325,7 @@ void apply_update(Map& m, UpdateSet& u) {
       }
     }
   }
+  activate_artificial(m);
 }
 
 // updateVolumeRayIntersectors (VM:1436-1449) for the state that raytrace needs.
@@ -810,6 +840,348 @@ int orc_raytrace(void* h, int64_t n, const double* origins, const double* dirs,
     }
   }
   return ORC_OK;
+}
+
+// ------------------------------------------------------------ map edits --
+
+// addPointsToGrid / removePointsFromGrid (VM:413-447): each point's voxel
+// Coord::floor(worldToIndex(p)) [U1; no half-voxel shift, unlike VM:612-631]
+// gets setNodeToOccupied (OCC:124-129: max log-odds, active) or setNodeToFree
+// (OCC:118-123: min log-odds, inactive) through modifyValueAndActiveState
+// [U8]. NaN or out-of-range points reject the call before any change (the
+// reference converts them to undefined coordinates).
+int orc_set_points(void* h, const float* xyz, int64_t n, int64_t stride, int32_t occupied) {
+  Map& m = *(Map*)h;
+  if (n < 0 || stride < 12 || (stride & 3) || (n && !xyz)) return ORC_BAD_ARGUMENT;
+  std::vector<int32_t> vox((size_t)n * 3);
+  const char* base = (const char*)xyz;
+  for (int64_t i = 0; i < n; ++i) {
+    const float* pt = (const float*)(base + i * stride);
+    int32_t* c = &vox[(size_t)i * 3];
+    for (int a = 0; a < 3; ++a) {
+      const double f = std::floor((double)pt[a] * m.inv);
+      if (!(f >= -(double)kCoordLimit && f < (double)kCoordLimit)) return ORC_OUT_OF_RANGE;
+      c[a] = (int32_t)f;
+    }
+  }
+  const float val = occupied ? m.c_max : m.c_min;
+  const bool need = occupied || val != 0.0f;  // does the op change a background voxel?
+  for (int64_t i = 0; i < n; ++i) {
+    const int32_t* c = &vox[(size_t)i * 3];
+    m.n4096.insert(key4096(c[0], c[1], c[2]));
+    const uint64_t k = leaf_key(c[0], c[1], c[2]);
+    if (!need && !m.leaves.count(k)) continue;
+    Leaf& leaf = touch_leaf(m, k);
+    const int off = leaf_offset(c[0], c[1], c[2]);
+    leaf.v[off] = val;
+    if (occupied)
+      leaf.act[off >> 6] |= 1ull << (off & 63);
+    else
+      leaf.act[off >> 6] &= ~(1ull << (off & 63));
+  }
+  return ORC_OK;
+}
+
+namespace {
+
+// restoreMapIntegrity (VM:1152-1166): setNodeState (OCC:130-134: active =
+// value > t_max) through modifyValueAndActiveState on every artificial voxel
+// [U8], then the artificial grid is cleared.
+void restore_integrity(Map& m) {
+  const bool need = 0.0f > m.t_max;  // would a background voxel change?
+  for (auto& kv : m.art.leaves) {
+    int32_t o[3];
+    key_origin(kv.first, o);
+    m.n4096.insert(key4096(o[0], o[1], o[2]));
+    if (!need && !m.leaves.count(kv.first)) continue;
+    Leaf& leaf = touch_leaf(m, kv.first);
+    for (int w = 0; w < 8; ++w) {
+      uint64_t vis = kv.second.vis[w];
+      while (vis) {
+        const int b = __builtin_ctzll(vis);
+        vis &= vis - 1;
+        if (leaf.v[w * 64 + b] > m.t_max)
+          leaf.act[w] |= 1ull << b;
+        else
+          leaf.act[w] &= ~(1ull << b);
+      }
+    }
+  }
+  m.art.clear();
+  m.art_present = false;
+}
+
+}  // namespace
+
+int orc_restore_map_integrity(void* h) {
+  restore_integrity(*(Map*)h);
+  return ORC_OK;
+}
+
+// addArtificialAreas (VM:1175-1190) -> addArtificialPolygon (VM:1198-1207)
+// -> addArtificialWall (VM:1217-1236): restore first, then for every polygon
+// edge (i, i+1 mod size) a castRayIntoGrid between the rounded edge ends
+// (VM:612-631) shifted by (0, 0, k) for k in [(int)(neg / res), (int)(pos /
+// res)), marking the artificial grid. Polygons are `counts[p]` consecutive
+// (x, y, z) vertices of `xyz`. Arguments are validated before any change.
+int orc_add_artificial_areas(void* h, int32_t n_polygons, const int32_t* counts, const double* xyz,
+                             double negative_height, double positive_height) {
+  Map& m = *(Map*)h;
+  if (n_polygons < 0 || (n_polygons && !counts)) return ORC_BAD_ARGUMENT;
+  const double qn = negative_height / m.res, qp = positive_height / m.res;
+  if (!(qn > -2147483649.0 && qn < 2147483648.0) || !(qp > -2147483649.0 && qp < 2147483648.0))
+    return ORC_BAD_ARGUMENT;
+  const int32_t k0 = (int32_t)qn, k1 = (int32_t)qp;
+  std::vector<std::array<int32_t, 6>> walls;
+  int64_t v0 = 0;
+  for (int32_t p = 0; p < n_polygons; ++p) {
+    const int32_t cnt = counts[p];
+    if (cnt < 0 || (cnt && !xyz)) return ORC_BAD_ARGUMENT;
+    for (int32_t i = 0; i < cnt; ++i) {
+      std::array<int32_t, 6> w;
+      world_to_index(m.res, m.inv, xyz + 3 * (v0 + i), w.data());
+      world_to_index(m.res, m.inv, xyz + 3 * (v0 + (i + 1) % cnt), w.data() + 3);
+      if (k0 < k1) {
+        for (int j = 0; j < 6; ++j) {
+          const int64_t lo = (int64_t)w[j] + ((j % 3 == 2) ? k0 : 0);
+          const int64_t hi = (int64_t)w[j] + ((j % 3 == 2) ? k1 - 1 : 0);
+          if (w[j] == INT32_MIN || lo < -kCoordLimit || hi >= kCoordLimit) return ORC_OUT_OF_RANGE;
+        }
+        walls.push_back(w);
+      }
+    }
+    v0 += cnt;
+  }
+  restore_integrity(m);
+  m.art_present = true;
+  for (const auto& w : walls) {
+    for (int32_t k = k0; k < k1; ++k) {
+      const int32_t s[3] = {w[0], w[1], w[2] + k}, e[3] = {w[3], w[4], w[5] + k};
+      cast_ray(s, e, [&](int32_t x, int32_t y, int32_t z) {
+        UpdLeaf& l = m.art.get(leaf_key(x, y, z));
+        const int off = leaf_offset(x, y, z);
+        l.vis[off >> 6] |= 1ull << (off & 63);
+      });
+    }
+  }
+  return ORC_OK;
+}
+
+// The artificial grid (visited mask per leaf), sorted by origin.
+int64_t orc_export_artificial(void* h, int32_t* origins, uint64_t* masks, int64_t cap) {
+  Map& m = *(Map*)h;
+  std::vector<uint64_t> keys, sorted;
+  for (auto& kv : m.art.leaves) keys.push_back(kv.first);
+  const int64_t n = export_sorted_keys(keys, sorted);
+  if (!origins && !masks) return n;
+  if (n > cap) return -n;
+  for (int64_t i = 0; i < n; ++i) {
+    key_origin(sorted[i], origins + 3 * i);
+    std::memcpy(masks + 8 * i, m.art.leaves.at(sorted[i]).vis, 64);
+  }
+  return n;
+}
+
+// ----------------------------------------------------------- map sections --
+
+namespace {
+
+// Voxels of the leaf at origin o inside the inclusive index box b.
+void inside_mask(const int32_t o[3], const int32_t b[6], uint64_t out[8]) {
+  for (int w = 0; w < 8; ++w) out[w] = 0;
+  for (int n = 0; n < 512; ++n) {
+    const int32_t x = o[0] + (n >> 6), y = o[1] + ((n >> 3) & 7), z = o[2] + (n & 7);
+    if (x >= b[0] && x <= b[3] && y >= b[1] && y <= b[4] && z >= b[2] && z <= b[5])
+      out[n >> 6] |= 1ull << (n & 63);
+  }
+}
+
+}  // namespace
+
+// getMapSection (VM:922-969) with extractFullLeaf (VM:971-992) or
+// extractSparseLeaf (VM:1000-1014) for every map leaf overlapping the box.
+// The section grid gets a leaf when a voxel inside the box is set on, or set
+// off with a value other than its background 0 (setValueOff on a tile keeps
+// the tile when the value equals it); voxels of that leaf visited before its
+// creation keep the background +0. Full: inside voxels keep value and state.
+// Sparse: inside active voxels are set on with value true (1.0 in a float
+// grid). Everything else in a section leaf is background (0, off). Sorted by
+// origin; a null output returns the count; a short cap returns -count.
+int64_t orc_export_section(void* h, const int32_t bbox[6], int32_t full, int32_t* origins,
+                           float* values, uint64_t* masks, int64_t cap) {
+  Map& m = *(Map*)h;
+  std::vector<uint64_t> keys, sorted;
+  std::vector<Leaf> outs;
+  std::unordered_map<uint64_t, size_t> at;
+  for (auto& kv : m.leaves) {
+    int32_t o[3];
+    key_origin(kv.first, o);
+    uint64_t in[8];
+    inside_mask(o, bbox, in);
+    Leaf out;
+    std::memset(&out, 0, sizeof out);
+    bool created = false;
+    for (int n = 0; n < 512; ++n) {
+      if (!((in[n >> 6] >> (n & 63)) & 1)) continue;
+      const bool on = (kv.second.act[n >> 6] >> (n & 63)) & 1;
+      const float v = kv.second.v[n];
+      if (full) {
+        if (!created && !on && v == 0.0f) continue;  // setValueOff(xyz, 0) on the background tile
+        created = true;
+        out.v[n] = v;
+        if (on) out.act[n >> 6] |= 1ull << (n & 63);
+      } else if (on) {
+        created = true;
+        out.v[n] = 1.0f;
+        out.act[n >> 6] |= 1ull << (n & 63);
+      }
+    }
+    if (!created) continue;
+    at[kv.first] = outs.size();
+    outs.push_back(out);
+    keys.push_back(kv.first);
+  }
+  const int64_t n = export_sorted_keys(keys, sorted);
+  if (!origins && !values && !masks) return n;
+  if (n > cap) return -n;
+  for (int64_t i = 0; i < n; ++i) {
+    const Leaf& l = outs[at[sorted[i]]];
+    key_origin(sorted[i], origins + 3 * i);
+    std::memcpy(values + 512 * i, l.v, sizeof l.v);
+    std::memcpy(masks + 8 * i, l.act, sizeof l.act);
+  }
+  return n;
+}
+
+// applyMapSectionGrid without smoothing (VM:1022-1050): every value of the
+// section grid, cbeginValueAll, is written to the map with its state
+// (setValueOn / setValueOff). For a section leaf that is all 512 voxels; a
+// missing map leaf is created at the first voxel that is on or non-zero (the
+// voxels before it stay background). The iteration also visits the inactive
+// tiles of the section's internal nodes [U12]: every 8^3 slot of a 128^3
+// node without a section leaf, and every 128^3 slot of a 4096^3 node without
+// a 128^3 child, yields setValueOff(slot origin, 0), which sets that voxel of
+// an existing map leaf to (0, off) and changes nothing where the map has no
+// leaf. `origins` are the section's leaf origins (multiples of 8, distinct).
+int orc_apply_section(void* h, int64_t n, const int32_t* origins, const float* values,
+                      const uint64_t* masks) {
+  Map& m = *(Map*)h;
+  if (n < 0 || (n && (!origins || !values || !masks))) return ORC_BAD_ARGUMENT;
+  std::unordered_set<uint64_t> sleaf, s128, s4096;
+  for (int64_t i = 0; i < n; ++i) {
+    const int32_t* o = origins + 3 * i;
+    if ((o[0] | o[1] | o[2]) & 7) return ORC_BAD_ARGUMENT;
+    if (!in_range(o)) return ORC_OUT_OF_RANGE;
+    if (!sleaf.insert(leaf_key(o[0], o[1], o[2])).second) return ORC_BAD_ARGUMENT;
+  }
+  for (int64_t i = 0; i < n; ++i) {
+    const int32_t* o = origins + 3 * i;
+    const uint64_t k = leaf_key(o[0], o[1], o[2]);
+    s128.insert(key128(o[0], o[1], o[2]));
+    s4096.insert(key4096(o[0], o[1], o[2]));
+    const float* v = values + 512 * i;
+    const uint64_t* a = masks + 8 * i;
+    int first = 0;
+    if (!m.leaves.count(k)) {
+      first = 512;
+      for (int j = 0; j < 512; ++j)
+        if (((a[j >> 6] >> (j & 63)) & 1) || v[j] != 0.0f) {
+          first = j;
+          break;
+        }
+      if (first == 512) continue;
+      m.n4096.insert(key4096(o[0], o[1], o[2]));
+    }
+    Leaf& leaf = touch_leaf(m, k);
+    for (int j = first; j < 512; ++j) {
+      leaf.v[j] = v[j];
+      const uint64_t bit = 1ull << (j & 63);
+      leaf.act[j >> 6] = (leaf.act[j >> 6] & ~bit) | (a[j >> 6] & bit);
+    }
+  }
+  auto clobber = [&](int32_t x, int32_t y, int32_t z) {
+    auto it = m.leaves.find(leaf_key(x, y, z));
+    if (it == m.leaves.end()) return;
+    it->second.v[leaf_offset(x, y, z)] = 0.0f;
+    const int off = leaf_offset(x, y, z);
+    it->second.act[off >> 6] &= ~(1ull << (off & 63));
+  };
+  for (uint64_t k : s128) {
+    int32_t o[3];
+    key_origin(k, o);  // key128 packs origin / 128 * 8
+    for (int a = 0; a < 3; ++a) o[a] = o[a] / 8 * 128;
+    for (int j = 0; j < 4096; ++j) {
+      const int32_t x = o[0] + 8 * (j >> 8), y = o[1] + 8 * ((j >> 4) & 15), z = o[2] + 8 * (j & 15);
+      if (!sleaf.count(leaf_key(x, y, z))) clobber(x, y, z);
+    }
+  }
+  for (uint64_t k : s4096) {
+    int32_t o[3];
+    key_origin(k, o);
+    for (int a = 0; a < 3; ++a) o[a] = o[a] / 8 * 4096;
+    for (int j = 0; j < 32768; ++j) {
+      const int32_t x = o[0] + 128 * (j >> 10), y = o[1] + 128 * ((j >> 5) & 31),
+                    z = o[2] + 128 * (j & 31);
+      if (!s128.count(key128(x, y, z))) clobber(x, y, z);
+    }
+  }
+  return ORC_OK;
+}
+
+// applyMapSectionUpdateGrid without smoothing (VM:1058-1085): every active
+// map voxel inside the inclusive box (the section's bb_min / bb_max) is set
+// inactive, then every active section voxel is set active (a missing map leaf
+// is created with value 0).
+int orc_apply_section_update(void* h, const int32_t bbox[6], int64_t n, const int32_t* origins,
+                             const uint64_t* masks) {
+  Map& m = *(Map*)h;
+  if (!bbox || n < 0 || (n && (!origins || !masks))) return ORC_BAD_ARGUMENT;
+  for (int64_t i = 0; i < n; ++i) {
+    const int32_t* o = origins + 3 * i;
+    if ((o[0] | o[1] | o[2]) & 7) return ORC_BAD_ARGUMENT;
+    if (!in_range(o)) return ORC_OUT_OF_RANGE;
+  }
+  for (auto& kv : m.leaves) {
+    int32_t o[3];
+    key_origin(kv.first, o);
+    uint64_t in[8];
+    inside_mask(o, bbox, in);
+    for (int w = 0; w < 8; ++w) kv.second.act[w] &= ~in[w];
+  }
+  for (int64_t i = 0; i < n; ++i) {
+    const int32_t* o = origins + 3 * i;
+    const uint64_t* a = masks + 8 * i;
+    uint64_t any = 0;
+    for (int w = 0; w < 8; ++w) any |= a[w];
+    if (!any) continue;
+    m.n4096.insert(key4096(o[0], o[1], o[2]));
+    Leaf& leaf = touch_leaf(m, leaf_key(o[0], o[1], o[2]));
+    for (int w = 0; w < 8; ++w) leaf.act[w] |= a[w];
+  }
+  return ORC_OK;
+}
+
+// createIndexBoundingBox (VM:858-874) over createWorldBoundingBox (VM:811-850):
+// the 8 corners of [min, max] as float, transformed by the row-major 4x4
+// matrix in double and stored as float [U13: pcl::transformPointCloud], their
+// float min / max, then Coord::floor(worldToIndex) [U1] of each.
+void orc_index_bbox(double res, const double mn[3], const double mx[3], const double tf[16],
+                    int32_t out[6]) {
+  float lo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, hi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+  for (int c = 0; c < 8; ++c) {
+    const double p[3] = {(double)(float)((c & 4) ? mx[0] : mn[0]), (double)(float)((c & 2) ? mx[1] : mn[1]),
+                         (double)(float)((c & 1) ? mx[2] : mn[2])};
+    for (int r = 0; r < 3; ++r) {
+      const float q = (float)(tf[4 * r] * p[0] + tf[4 * r + 1] * p[1] + tf[4 * r + 2] * p[2] + tf[4 * r + 3]);
+      lo[r] = std::min(lo[r], q);
+      hi[r] = std::max(hi[r], q);
+    }
+  }
+  const double inv = 1.0 / res;
+  for (int r = 0; r < 3; ++r) {
+    out[r] = (int32_t)std::floor((double)lo[r] * inv);
+    out[3 + r] = (int32_t)std::floor((double)hi[r] * inv);
+  }
 }
 
 }  // extern "C"

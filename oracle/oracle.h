@@ -17,7 +17,8 @@
  * unpinned (no check here can compare them with OpenVDB itself): the rounding
  * forms U1-U3, the double tie-break between axes of different extent (U5),
  * raytrace clip/HDDA details (U6, U7, U9, U10), the fast-mode hits walk
- * (U11) and leaf creation by no-op updates (U8).
+ * (U11), leaf creation by no-op updates (U8), the section-apply visit of
+ * internal-node tiles (U12) and the bounding-box corner transform (U13).
  */
 #ifndef VDB_ORACLE_H
 #define VDB_ORACLE_H
@@ -50,6 +51,7 @@ enum {
   ORC_BAD_PROB_HIT = 5,
   ORC_OUT_OF_RANGE = 6,
   ORC_UNSUPPORTED = 7,
+  ORC_BAD_ARGUMENT = 9,
   ORC_NO_INTERSECTOR = 10
 };
 
@@ -87,6 +89,30 @@ int64_t orc_cast_ray(const int32_t s[3], const int32_t e[3], int32_t* out_xyz, i
  * builds in integrateUpdate when fast_mode is set (VM:1436-1449). */
 int orc_raytrace(void* h, int64_t n, const double* origins, const double* dirs,
                  const double* max_lengths, uint8_t* success, double* end_points);
+
+/* addPointsToGrid (occupied = 1) / removePointsFromGrid (occupied = 0), VM:413-447. */
+int orc_set_points(void* h, const float* xyz, int64_t n, int64_t stride_bytes, int32_t occupied);
+
+/* addArtificialAreas (VM:1175-1236): polygons of counts[p] (x, y, z) vertices. */
+int orc_add_artificial_areas(void* h, int32_t n_polygons, const int32_t* counts,
+                             const double* xyz, double negative_height, double positive_height);
+/* restoreMapIntegrity (VM:1152-1166). */
+int orc_restore_map_integrity(void* h);
+/* The artificial grid: origins and voxel masks per leaf, sorted by origin. */
+int64_t orc_export_artificial(void* h, int32_t* origins, uint64_t* masks, int64_t cap);
+
+/* getMapSection (VM:922-1014) over the inclusive index box bbox = (min xyz, max xyz). */
+int64_t orc_export_section(void* h, const int32_t bbox[6], int32_t full_grid, int32_t* origins,
+                           float* values, uint64_t* masks, int64_t cap);
+/* applyMapSectionGrid without smoothing (VM:1022-1050). */
+int orc_apply_section(void* h, int64_t n, const int32_t* origins, const float* values,
+                      const uint64_t* masks);
+/* applyMapSectionUpdateGrid without smoothing (VM:1058-1085). */
+int orc_apply_section_update(void* h, const int32_t bbox[6], int64_t n, const int32_t* origins,
+                             const uint64_t* masks);
+/* createIndexBoundingBox (VM:811-874); tf is a row-major 4x4 matrix. */
+void orc_index_bbox(double resolution, const double min_boundary[3],
+                    const double max_boundary[3], const double tf[16], int32_t out[6]);
 
 #ifdef __cplusplus
 }

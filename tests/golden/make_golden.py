@@ -32,8 +32,10 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.dirname(HERE))
 
 import oracle as O  # noqa: E402
+from golden_cases import EDIT_BOX, run_edit_case  # noqa: E402
 from vdb_mapping_amd import workloads as W  # noqa: E402
 
 
@@ -83,20 +85,7 @@ def fast_case():
     return pts, origins, q_orig, q, q_len
 
 
-def edit_case():
-    """Point edits and artificial areas on a scanned map (res 0.1, R 5)."""
-    rng = np.random.default_rng(91)
-    scan = (rng.normal(0, 1.5, (4000, 3))).astype(np.float32)
-    add = rng.uniform(-3, 3, (600, 3)).astype(np.float32)
-    rem = np.concatenate([scan[::7], rng.uniform(-3, 3, (200, 3)).astype(np.float32)])
-    polygons = [np.array([[-1.0, -1.0, 0.0], [1.2, -1.0, 0.0], [1.2, 0.8, 0.0]]),
-                np.array([[2.05, 2.0, -0.3], [-2.0, 2.55, -0.3]])]
-    return scan, add, rem, polygons
-
-
 ROOM_SCANS = 5
-
-
 def map_arrays(m, prefix, out):
     o, v, a = m.export_leaves()
     out[prefix + "_origins"] = o
@@ -169,6 +158,15 @@ def main():
     map_arrays(m, "fast_map", out)
     out["rt_origins"], out["rt_dirs"], out["rt_lengths"] = q_o, q_d, q_l
     out["rt_success"], out["rt_ends"] = succ.astype(np.uint8), ends
+
+    # edits: scan, point edits, artificial areas, another scan, sections
+    m = run_edit_case(O.OracleMap, O.make_config)
+    map_arrays(m, "edit_map", out)
+    out["edit_art_origins"], out["edit_art_masks"] = m.export_artificial()
+    for full in (0, 1):
+        o, v, a = m.export_section(EDIT_BOX, bool(full))
+        out["edit_section%d_origins" % full], out["edit_section%d_values" % full] = o, v.view(np.uint32)
+        out["edit_section%d_masks" % full] = a
 
     np.savez_compressed(os.path.join(HERE, "oracle_cases.npz"), **out)
 

@@ -57,6 +57,49 @@ def check_map(m, data, prefix):
     return o.shape[0]
 
 
+EDIT_BOX = (-12, -9, -6, 15, 10, 7)
+EDIT_PROBS = dict(prob_hit=0.7, prob_miss=0.4, prob_thres_min=0.12, prob_thres_max=0.97)
+
+
+def edit_case():
+    """Inputs of the edit fixture (res 0.1, R 5): a scan, points to add and
+    remove, and two artificial polygons."""
+    rng = np.random.default_rng(91)
+    scan = (rng.normal(0, 1.5, (4000, 3))).astype(np.float32)
+    add = rng.uniform(-3, 3, (600, 3)).astype(np.float32)
+    rem = np.concatenate([scan[::7], rng.uniform(-3, 3, (200, 3)).astype(np.float32)])
+    polygons = [np.array([[-1.0, -1.0, 0.0], [1.2, -1.0, 0.0], [1.2, 0.8, 0.0]]),
+                np.array([[2.05, 2.0, -0.3], [-2.0, 2.55, -0.3]])]
+    return scan, add, rem, polygons
+
+
+def run_edit_case(make_map, make_config):
+    """The edit sequence of the fixture on a map class with the reference's
+    method names: scan, addPointsToGrid, removePointsFromGrid,
+    addArtificialAreas, another scan."""
+    scan, add, rem, polygons = edit_case()
+    m = make_map(0.1)
+    assert m.set_config(make_config(max_range=5.0, **EDIT_PROBS)) == 0
+    assert m.add_input_source("test", 5.0) == 0
+    assert m.insert("test", scan, [0.02, 0.01, -0.03]) == 0
+    assert m.add_points_to_grid(add) == 0
+    assert m.remove_points_from_grid(rem) == 0
+    assert m.add_artificial_areas(polygons, -0.3, 0.45) == 0
+    assert m.insert("test", scan[::2] * 0.8, [0.3, -0.2, 0.1]) == 0
+    return m
+
+
+def check_edit_case(m, data):
+    check_map(m, data, "edit_map")
+    o, a = m.export_artificial()
+    assert np.array_equal(o, data["edit_art_origins"]) and np.array_equal(a, data["edit_art_masks"])
+    for full in (0, 1):
+        o, v, a = m.export_section(EDIT_BOX, bool(full))
+        assert np.array_equal(o, data["edit_section%d_origins" % full])
+        assert np.array_equal(v.view(np.uint32), data["edit_section%d_values" % full])
+        assert np.array_equal(a, data["edit_section%d_masks" % full])
+
+
 def run_known_answer(make_map, case, probs):
     """One reference known-answer case against map factory make_map(res,
     max_range, probs) -> map with source "test"."""

@@ -69,3 +69,9 @@ def test_room_scan_digests():
     o, v, a = m.export_leaves()
     assert (o.shape[0], popcount(a)) == (r["leaves"], r["active_voxels"])
     assert digest(o, v.view(np.uint32), a) == r["sha256"]
+
+
+def test_edit_fixture():
+    from golden_cases import check_edit_case, run_edit_case
+    m = run_edit_case(O.OracleMap, O.make_config)
+    check_edit_case(m, cases())

@@ -80,3 +80,14 @@ def test_room_scan_digests(traversal):
     o, v, a = g.export_leaves()
     assert (o.shape[0], popcount(a)) == (r["leaves"], r["active_voxels"])
     assert digest(o, v.view(np.uint32), a) == r["sha256"]
+
+
+@pytest.mark.parametrize("traversal", [0, 1])
+def test_edit_fixture(traversal):
+    from golden_cases import check_edit_case, run_edit_case
+
+    def make(res):
+        g = V.GpuMap(res)
+        assert g.set_tuning(traversal=traversal) == V.OK
+        return g
+    check_edit_case(run_edit_case(make, V.make_config), cases())

@@ -36,7 +36,9 @@ ABI_SYMBOLS = [
     "vdbgpu_device_download", "vdbgpu_device_copy", "vdbgpu_async_start", "vdbgpu_async_submit",
     "vdbgpu_async_flush", "vdbgpu_async_counters", "vdbgpu_async_stop", "vdbgpu_server_create", "vdbgpu_server_submit", "vdbgpu_server_drain",
     "vdbgpu_server_records", "vdbgpu_server_map", "vdbgpu_server_device_bytes",
-    "vdbgpu_server_destroy",
+    "vdbgpu_server_destroy", "vdbgpu_set_points", "vdbgpu_add_artificial_areas",
+    "vdbgpu_restore_map_integrity", "vdbgpu_export_artificial", "vdbgpu_export_section",
+    "vdbgpu_apply_section", "vdbgpu_apply_section_update", "vdbgpu_index_bbox",
 ]
 
 
@@ -176,6 +178,22 @@ def lib():
         L.vdbgpu_server_device_bytes.restype = I64
         L.vdbgpu_server_device_bytes.argtypes = [P]
         L.vdbgpu_server_destroy.argtypes = [P]
+        L.vdbgpu_set_points.restype = I32
+        L.vdbgpu_set_points.argtypes = [P, P, I64, I64, I32]
+        L.vdbgpu_add_artificial_areas.restype = I32
+        L.vdbgpu_add_artificial_areas.argtypes = [P, I32, P, P, D, D]
+        L.vdbgpu_restore_map_integrity.restype = I32
+        L.vdbgpu_restore_map_integrity.argtypes = [P]
+        L.vdbgpu_export_artificial.restype = I64
+        L.vdbgpu_export_artificial.argtypes = [P, P, P, I64]
+        L.vdbgpu_export_section.restype = I64
+        L.vdbgpu_export_section.argtypes = [P, P, I32, P, P, P, I64]
+        L.vdbgpu_apply_section.restype = I32
+        L.vdbgpu_apply_section.argtypes = [P, I64, P, P, P]
+        L.vdbgpu_apply_section_update.restype = I32
+        L.vdbgpu_apply_section_update.argtypes = [P, P, I64, P, P]
+        L.vdbgpu_index_bbox.restype = I32
+        L.vdbgpu_index_bbox.argtypes = [D, P, P, P, P]
         _lib = L
     return _lib
 
@@ -186,6 +204,27 @@ def _ptr(a):
     if isinstance(a, int):
         return ctypes.c_void_p(a)
     return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def index_bbox(resolution: float, min_boundary, max_boundary, tf=None) -> np.ndarray:
+    """createIndexBoundingBox: inclusive index box (min xyz, max xyz) of the
+    world box [min, max] seen through the 4x4 map-to-reference transform."""
+    mn = np.ascontiguousarray(min_boundary, dtype=np.float64)
+    mx = np.ascontiguousarray(max_boundary, dtype=np.float64)
+    t = np.ascontiguousarray(np.eye(4) if tf is None else tf, dtype=np.float64)
+    out = np.zeros(6, np.int32)
+    rc = lib().vdbgpu_index_bbox(resolution, _ptr(mn), _ptr(mx), _ptr(t), _ptr(out))
+    if rc:
+        raise VdbGpuError(f"index_bbox: {status_string(rc)}")
+    return out
+
+
+def polygons_arg(polygons):
+    """Flatten a list of (k, 3+) vertex arrays into (counts int32, xyz float64)."""
+    counts = np.array([len(p) for p in polygons], np.int32)
+    rows = [np.asarray(p, np.float64).reshape(len(p), -1)[:, :3] for p in polygons if len(p)]
+    xyz = np.ascontiguousarray(np.concatenate(rows) if rows else np.zeros((0, 3)), dtype=np.float64)
+    return counts, xyz
 
 
 def status_string(code: int) -> str:
@@ -350,6 +389,68 @@ class GpuMap:
         if got != n:
             raise VdbGpuError(f"export_update returned {got}, expected {n}: {self.last_error()}")
         return o, vis, hit
+
+    # direct edits, artificial areas and map sections
+    def set_points(self, points, occupied: bool, check=False) -> int:
+        pts, stride = self._pts(points)
+        rc = lib().vdbgpu_set_points(self.h, _ptr(pts), pts.shape[0], stride, int(bool(occupied)))
+        return self._check(rc, check)
+
+    def add_points_to_grid(self, points, check=False) -> int:
+        return self.set_points(points, True, check)
+
+    def remove_points_from_grid(self, points, check=False) -> int:
+        return self.set_points(points, False, check)
+
+    def add_artificial_areas(self, polygons, negative_height: float, positive_height: float,
+                             check=False) -> int:
+        counts, xyz = polygons_arg(polygons)
+        rc = lib().vdbgpu_add_artificial_areas(self.h, len(counts), _ptr(counts), _ptr(xyz),
+                                               negative_height, positive_height)
+        return self._check(rc, check)
+
+    def restore_map_integrity(self, check=False) -> int:
+        return self._check(lib().vdbgpu_restore_map_integrity(self.h), check)
+
+    def export_artificial(self):
+        n = lib().vdbgpu_export_artificial(self.h, None, None, 0)
+        if n < 0:
+            raise VdbGpuError(f"export_artificial failed: {self.last_error()}")
+        o = np.zeros((n, 3), np.int32)
+        m = np.zeros((n, 8), np.uint64)
+        if lib().vdbgpu_export_artificial(self.h, _ptr(o), _ptr(m), n) != n:
+            raise VdbGpuError(f"export_artificial failed: {self.last_error()}")
+        return o, m
+
+    def export_section(self, bbox, full_grid: bool):
+        """getMapSectionGrid (full_grid) / getMapSectionUpdateGrid over an
+        inclusive index box: origins (n,3), values (n,512), masks (n,8)."""
+        b = np.ascontiguousarray(bbox, dtype=np.int32)
+        n = lib().vdbgpu_export_section(self.h, _ptr(b), int(bool(full_grid)), None, None, None, 0)
+        if n < 0:
+            raise VdbGpuError(f"export_section failed: {self.last_error()}")
+        o = np.zeros((n, 3), np.int32)
+        v = np.zeros((n, 512), np.float32)
+        m = np.zeros((n, 8), np.uint64)
+        got = lib().vdbgpu_export_section(self.h, _ptr(b), int(bool(full_grid)), _ptr(o), _ptr(v),
+                                          _ptr(m), n)
+        if got != n:
+            raise VdbGpuError(f"export_section returned {got}, expected {n}: {self.last_error()}")
+        return o, v, m
+
+    def apply_section(self, origins, values, masks, check=False) -> int:
+        o = np.ascontiguousarray(origins, dtype=np.int32).reshape(-1, 3)
+        v = np.ascontiguousarray(values, dtype=np.float32).reshape(-1, 512)
+        m = np.ascontiguousarray(masks, dtype=np.uint64).reshape(-1, 8)
+        return self._check(lib().vdbgpu_apply_section(self.h, o.shape[0], _ptr(o), _ptr(v), _ptr(m)),
+                           check)
+
+    def apply_section_update(self, bbox, origins, masks, check=False) -> int:
+        b = np.ascontiguousarray(bbox, dtype=np.int32)
+        o = np.ascontiguousarray(origins, dtype=np.int32).reshape(-1, 3)
+        m = np.ascontiguousarray(masks, dtype=np.uint64).reshape(-1, 8)
+        return self._check(lib().vdbgpu_apply_section_update(self.h, _ptr(b), o.shape[0], _ptr(o),
+                                                             _ptr(m)), check)
 
     def raytrace(self, origins, dirs, lengths):
         o = np.ascontiguousarray(origins, dtype=np.float64).reshape(-1, 3)

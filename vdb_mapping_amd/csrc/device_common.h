@@ -50,6 +50,12 @@ constexpr uint32_t kOvfSplit = 16u;   // shared-brick bitmaps
 constexpr uint32_t kOvfRecords = 32u; // update-leaf records
 constexpr uint32_t kItemSegs = 1024;  // segments per brick work item
 
+// Direct edit ops on the marked voxels of an edit set (k_set_voxels).
+constexpr int kEditOccupied = 0;  // setNodeToOccupied: (max log-odds, on)
+constexpr int kEditFree = 1;      // setNodeToFree: (min log-odds, off)
+constexpr int kEditState = 2;     // setNodeState: on = value > t_max
+constexpr int kEditActivate = 3;  // setActiveState(on)
+
 struct Status {
   unsigned long long total;     // visits of the scan (sum of counts)
   unsigned long long voxels;    // visited voxels in the update set (compact)

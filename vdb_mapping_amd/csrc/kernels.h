@@ -81,4 +81,33 @@ hipError_t launch_raytrace(hipStream_t s, int grid, RtView rt, MapView mp, doubl
                            int64_t n, const double* origins, const double* dirs,
                            const double* lens, uint8_t* success, double* ends, unsigned int* err);
 
+hipError_t launch_mark_points(hipStream_t s, const float* pts, int64_t stride_bytes, uint32_t n, double inv,
+                              UpdView u, Status* st);
+hipError_t launch_mark_walls(hipStream_t s, const double* verts, const int* walls, uint32_t n_walls, int k0,
+                             int k1, double res, double inv, const double* tbl, uint32_t tbl_n, UpdView u,
+                             Status* st);
+hipError_t launch_set_voxels(hipStream_t s, UpdView u, const uint32_t* list, const uint32_t* eidx,
+                             uint32_t count, MapView mp, int op, float val, float t_max, bool clear);
+hipError_t launch_section_pick(hipStream_t s, MapView mp, uint32_t leaf_count, const int* bbox, bool full,
+                               uint32_t* out, unsigned int* counter);
+hipError_t launch_section_gather(hipStream_t s, MapView mp, const uint32_t* list, uint32_t count,
+                                 const int* bbox, bool full, unsigned long long* keys_out, float* vals_out,
+                                 unsigned long long* act_out);
+hipError_t launch_upsert_leaves(hipStream_t s, const unsigned long long* keys, const uint8_t* need, uint32_t n,
+                                MapView mp, uint32_t* eidx, uint16_t* first, Status* st);
+hipError_t launch_section_write(hipStream_t s, const float* vals, const unsigned long long* act,
+                                const uint32_t* eidx, const uint16_t* first, uint32_t n, MapView mp);
+hipError_t launch_section_tiles(hipStream_t s, const unsigned long long* n128, uint32_t c128,
+                                const unsigned long long* n4096, uint32_t c4096,
+                                const unsigned long long* sleaf, uint32_t sleaf_mask,
+                                const unsigned long long* s128, uint32_t s128_mask, MapView mp);
+hipError_t launch_section_clear(hipStream_t s, MapView mp, uint32_t leaf_count, const int* bbox);
+hipError_t launch_section_prep(hipStream_t s, const int* origins, const float* vals, const unsigned long long* act,
+                               uint32_t n, unsigned long long* keys, uint8_t* need, uint16_t* first,
+                               unsigned long long* sleaf, uint32_t sleaf_mask, unsigned long long* s128,
+                               uint32_t s128_mask, unsigned long long* n128, unsigned long long* s4096,
+                               uint32_t s4096_mask, unsigned long long* n4096, unsigned int* counters);
+hipError_t launch_section_or(hipStream_t s, const unsigned long long* act, const uint32_t* eidx, uint32_t n,
+                             MapView mp);
+
 }  // namespace vdbgpu

@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cfloat>
 #include <array>
 #include <cmath>
 #include <cstdio>
@@ -158,6 +159,15 @@ struct vdbgpu_map {
   uint32_t group_cap = 0;      // groups: work items + split bricks
   double* tbl = nullptr;
   uint32_t tbl_n = 0;
+
+  // direct edits (addPointsToGrid / removePointsFromGrid) and artificial areas
+  Source edit;                 // marked voxels of one edit call (cleared after it)
+  Source art, art_tmp;         // artificial grid (VM:132); art_tmp receives a new one
+  uint32_t art_count = 0;      // listed leaves of the artificial grid (art.list)
+  uint32_t* art_eidx = nullptr;
+  uint32_t art_eidx_cap = 0;
+  bool art_present = false;    // m_artificial_areas_present (VM:1529)
+  bool art_leaves = false;     // every artificial leaf exists in the map
 
   // status: [0] scan, [1] map (leaf_count), [2] auxiliary, [3] cumulative counters
   Status* d_st = nullptr;
@@ -415,6 +425,7 @@ int ensure_map(vdbgpu_map* m, uint64_t need) {
 }
 
 void create_flags(const vdbgpu_map* m, bool* ch, bool* cm);
+int activate_artificial(vdbgpu_map* m);
 
 int ensure_groups(vdbgpu_map* m, uint32_t cap) {
   if (cap <= m->gcap) return VDBGPU_OK;
@@ -597,6 +608,10 @@ int run_pipeline(vdbgpu_map* m, Source& src, const float* xyz, int64_t stride, b
     m->touched = m->touched || st.entries > 0;
     m->snap_dirty = m->snap_dirty || st.entries > 0;
     src.pending = 0;
+    // the scans' updateMap calls each end by activating the artificial voxels
+    // (VM:785-789); values do not depend on active states, so once after the
+    // fold gives the same map
+    if (st.entries > 0 && (r = activate_artificial(m))) return r;
   } else {
     src.pending = st.entries;
   }
@@ -835,7 +850,8 @@ int run_brick(vdbgpu_map* m, Source& src, const float* xyz, int64_t n, int64_t s
     m->win_b = 0;
     return VDBGPU_OK;
   }
-  if ((r = ensure_map(m, m->leaf_count_ub + recs))) return r;
+  const uint32_t art_new = m->art_leaves ? 0 : m->art_count;  // leaves the activation may create
+  if ((r = ensure_map(m, m->leaf_count_ub + recs + art_new))) return r;
   if (recs > m->eidx_cap) {
     dfree(m, &m->eidx, m->eidx_cap);
     m->eidx_cap = (uint32_t)next_pow2(recs);
@@ -856,9 +872,11 @@ int run_brick(vdbgpu_map* m, Source& src, const float* xyz, int64_t n, int64_t s
     Timer t(m, kApply);
     CK(launch_apply(m->stream, g2, rec, m->rlist[X], m->eidx, recs, map_view(m), m->lo, &m->d_st[3], false));
   }
-  CK(hipEventRecord(m->rec_free[X], m->stream));
   m->leaf_count_ub += recs;
-  m->win_a = m->win_b + recs;
+  m->win_b += recs;
+  if ((r = activate_artificial(m))) return r;  // counts its new leaves into win_b
+  CK(hipEventRecord(m->rec_free[X], m->stream));
+  m->win_a = m->win_b;
   m->win_b = 0;
   m->touched = true;
   m->snap_dirty = true;
@@ -953,6 +971,7 @@ int refresh_snapshot(vdbgpu_map* m) {
 int do_integrate(vdbgpu_map* m) {
   bool ch, cm;
   create_flags(m, &ch, &cm);
+  bool applied = false;
   for (auto& kv : m->sources) {
     Source& s = kv.second;
     if (s.pending == 0) continue;
@@ -980,8 +999,147 @@ int do_integrate(vdbgpu_map* m) {
     m->touched = true;
     m->snap_dirty = true;
     s.pending = 0;
+    applied = true;
+  }
+  // every non-empty updateMap ends by activating the artificial voxels
+  // (VM:785-789); values do not depend on active states, so once after the
+  // last source gives the same map as after each one
+  if (applied) {
+    const int r = activate_artificial(m);
+    if (r) return r;
   }
   return refresh_snapshot(m);
+}
+
+// ---------------------------------------------------------- direct edits --
+
+// Empty an edit set (keys EMPTY, masks 0).
+int clear_set(vdbgpu_map* m, Source& s) {
+  if (!s.cap) return VDBGPU_OK;
+  CK(launch_fill_u64(m->stream, s.keys, s.cap, kEmpty));
+  CK(hipMemsetAsync(s.masks, 0, (size_t)s.cap * 16 * sizeof(unsigned long long), m->stream));
+  return VDBGPU_OK;
+}
+
+// Status of the kernel error bits of an edit call.
+int edit_error(vdbgpu_map* m, const Status& st) {
+  if (st.error & kErrRange)
+    return fail(m, VDBGPU_OUT_OF_RANGE, "a voxel index is outside [-2^23, 2^23) or not finite; edit rejected");
+  if (st.error & kErrTable) return fail(m, VDBGPU_UNSUPPORTED, "a wall spans more than 8192 voxels along one axis");
+  return st.error ? fail(m, VDBGPU_DEVICE_ERROR, "edit kernel error") : VDBGPU_OK;
+}
+
+// Mark an edit set with `mark` (idempotent marks), list it, read the status
+// (the call's only sync). A full set is cleared, grown 4x and marked again; a
+// kernel error clears the set and returns its status. *count = listed leaves.
+template <class Mark>
+int mark_set(vdbgpu_map* m, Source& s, uint32_t cap0, Mark&& mark, uint32_t* count) {
+  int r;
+  if (s.cap < cap0) {
+    free_update(m, s);
+    if ((r = alloc_update(m, s, cap0))) return r;
+  }
+  for (int attempt = 0;; ++attempt) {
+    CK(hipMemsetAsync(&m->d_st[2], 0, sizeof(Status), m->stream));
+    if ((r = mark())) return r;
+    CK(launch_compact(m->stream, std::min<int>(4096, (int)((s.cap + 255) / 256)), upd_view(s), s.cap, s.list,
+                      &m->d_st[2].entries, &m->d_st[2]));
+    if ((r = read_status(m))) return r;
+    const Status st = m->h_st[2];
+    if (st.error) {
+      if ((r = clear_set(m, s))) return r;
+      return edit_error(m, st);
+    }
+    if (!(st.overflow & kOvfHard)) {
+      *count = st.entries;
+      return VDBGPU_OK;
+    }
+    if (attempt > 6 || s.cap >= (1u << 30)) return fail(m, VDBGPU_DEVICE_ERROR, "edit set cannot grow");
+    const uint32_t cap = s.cap * 4;
+    free_update(m, s);
+    if ((r = alloc_update(m, s, cap))) return r;
+  }
+}
+
+// Find or create the map leaf of each listed edit-set leaf (created only when
+// `need`: the op changes a background voxel; a no-op modify still gives the
+// root its 4096^3 child), then run `op` on the marked voxels.
+int apply_edit(vdbgpu_map* m, Source& s, uint32_t count, uint32_t* eidx, int op, float val, bool need,
+               bool clear) {
+  if (count == 0) return VDBGPU_OK;
+  const int g1 = std::min<int>(4096, (int)((count + 255) / 256));
+  CK(launch_map_insert(m->stream, g1, upd_view(s), s.list, count, map_view(m), eidx, need, need, m->noop4096,
+                       vdbgpu_map::kNoopCap - 1, &m->d_st[1], &m->d_st[2]));
+  CK(launch_set_voxels(m->stream, upd_view(s), s.list, eidx, count, map_view(m), op, val, m->lo[3], clear));
+  m->touched = true;
+  m->snap_dirty = true;
+  return VDBGPU_OK;
+}
+
+int ensure_eidx(vdbgpu_map* m, uint32_t n) {
+  if (n <= m->eidx_cap) return VDBGPU_OK;
+  dfree(m, &m->eidx, m->eidx_cap);
+  m->eidx_cap = (uint32_t)next_pow2(n);
+  CK(dalloc(m, &m->eidx, m->eidx_cap));
+  return VDBGPU_OK;
+}
+
+// Last step of a non-empty updateMap (VM:785-789): setActiveState(on) for
+// every artificial voxel (a missing leaf is created with value 0). The
+// artificial leaves are created once; later activations create none.
+int activate_artificial(vdbgpu_map* m) {
+  if (m->art_count == 0) return VDBGPU_OK;
+  int r;
+  const uint32_t fresh = m->art_leaves ? 0 : m->art_count;
+  if (fresh && (r = ensure_map(m, m->leaf_count_ub + fresh))) return r;
+  if ((r = apply_edit(m, m->art, m->art_count, m->art_eidx, kEditActivate, 0.0f, true, false))) return r;
+  m->leaf_count_ub += fresh;
+  m->win_b += fresh;
+  m->art_leaves = true;
+  return VDBGPU_OK;
+}
+
+// restoreMapIntegrity (VM:1152-1166): setNodeState through
+// modifyValueAndActiveState on every artificial voxel, then the grid is cleared.
+int restore_integrity(vdbgpu_map* m) {
+  int r;
+  if (m->art_count) {
+    const bool need = 0.0f > m->lo[3];  // would a background voxel change?
+    if (need && (r = ensure_map(m, m->leaf_count_ub + m->art_count))) return r;
+    if ((r = apply_edit(m, m->art, m->art_count, m->art_eidx, kEditState, 0.0f, need, true))) return r;
+    if (need) {
+      m->leaf_count_ub += m->art_count;
+      m->win_b += m->art_count;
+    }
+  }
+  m->art_count = 0;
+  m->art_present = false;
+  m->art_leaves = false;
+  return VDBGPU_OK;
+}
+
+// Stage `bytes` of host data in a fresh device buffer (freed by the caller).
+template <class T>
+int upload(vdbgpu_map* m, T** dst, const T* src, size_t count) {
+  CK(dalloc(m, dst, count));
+  if (count) CK(hipMemcpyAsync(*dst, src, count * sizeof(T), hipMemcpyHostToDevice, m->stream));
+  return VDBGPU_OK;
+}
+
+// Validate section leaf origins: multiples of 8, inside the key range, distinct.
+int check_origins(vdbgpu_map* m, int64_t n, const int32_t* origins) {
+  std::vector<unsigned long long> keys((size_t)n);
+  for (int64_t i = 0; i < n; ++i) {
+    const int32_t* o = origins + 3 * i;
+    if ((o[0] | o[1] | o[2]) & 7) return fail(m, VDBGPU_BAD_ARGUMENT, "section leaf origin is not a multiple of 8");
+    for (int a = 0; a < 3; ++a)
+      if (o[a] < -kCoordLimit || o[a] >= kCoordLimit)
+        return fail(m, VDBGPU_OUT_OF_RANGE, "section leaf origin outside [-2^23, 2^23)");
+    keys[(size_t)i] = leaf_key(o[0], o[1], o[2]);
+  }
+  std::sort(keys.begin(), keys.end());
+  if (std::adjacent_find(keys.begin(), keys.end()) != keys.end())
+    return fail(m, VDBGPU_BAD_ARGUMENT, "section leaf origins repeat");
   return VDBGPU_OK;
 }
 
@@ -1091,6 +1249,10 @@ void vdbgpu_destroy(vdbgpu_map* m) {
   if (m->stream) (void)hipStreamSynchronize(m->stream);
   if (m->cast) (void)hipStreamSynchronize(m->cast);
   for (auto& kv : m->sources) free_update(m, kv.second);
+  free_update(m, m->edit);
+  free_update(m, m->art);
+  free_update(m, m->art_tmp);
+  dfree(m, &m->art_eidx, m->art_eidx_cap);
   dfree(m, &m->mkeys, m->mcap);
   dfree(m, &m->midx, m->mcap);
   dfree(m, &m->values, (size_t)m->lcap * 512);
@@ -1293,7 +1455,6 @@ int32_t vdbgpu_insert_batch(vdbgpu_map* m, const char* id, int32_t nscans, const
       if (r) return r;
     }
     return refresh_snapshot(m);
-    return VDBGPU_OK;
   }
   if (pending) {
     // pending sets are applied with the first scan's integrate, in source-id order
@@ -1331,7 +1492,6 @@ int32_t vdbgpu_insert_batch(vdbgpu_map* m, const char* id, int32_t nscans, const
     i0 = i1;
   }
   return refresh_snapshot(m);
-  return VDBGPU_OK;
 }
 
 int32_t vdbgpu_reset(vdbgpu_map* m) {
@@ -1357,6 +1517,7 @@ int32_t vdbgpu_reset(vdbgpu_map* m) {
   m->win_a = m->win_b = 0;
   m->touched = false;
   m->snap_dirty = true;
+  m->art_leaves = false;  // resetMap keeps the artificial grid (VM:174-186), not its leaves
   CK(hipStreamSynchronize(m->stream));
   return VDBGPU_OK;
 }
@@ -1612,5 +1773,337 @@ int32_t vdbgpu_get_tuning(vdbgpu_map* m, vdbgpu_tuning* out) {
 }
 
 const char* vdbgpu_last_error(vdbgpu_map* m) { return m ? m->last_error.c_str() : ""; }
+
+// ------------------------------------------------------------ map edits --
+
+int32_t vdbgpu_set_points(vdbgpu_map* m, const float* xyz, int64_t n, int64_t stride, int32_t occupied) {
+  if (!m) return VDBGPU_BAD_ARGUMENT;
+  Guard g(m);
+  if (n < 0 || stride < 12 || (stride & 3) || n > (int64_t)kMaxRays || (n && !xyz))
+    return fail(m, VDBGPU_BAD_ARGUMENT, "bad point buffer arguments");
+  if (n == 0) return VDBGPU_OK;
+  int r;
+  const size_t bytes = (size_t)(n - 1) * stride + 12;
+  float* pts = nullptr;
+  CK(dalloc(m, &pts, (bytes + 3) / 4));
+  CK(hipMemcpyAsync(pts, xyz, bytes, hipMemcpyHostToDevice, m->stream));
+  uint32_t count = 0;
+  // at most n leaves; large edits start smaller and grow on overflow
+  const uint32_t cap0 = (uint32_t)std::max<uint64_t>(kMinUpdateCap, next_pow2(std::min<uint64_t>(2 * (uint64_t)n, 1u << 22)));
+  r = mark_set(m, m->edit, cap0, [&]() -> int {
+    CK(launch_mark_points(m->stream, pts, stride, (uint32_t)n, m->inv, upd_view(m->edit), &m->d_st[2]));
+    return VDBGPU_OK;
+  }, &count);
+  if (r == VDBGPU_OK) {
+    const float val = occupied ? m->lo[4] : m->lo[5];
+    const bool need = occupied || val != 0.0f;  // does the op change a background voxel?
+    if (!(r = ensure_map(m, m->leaf_count_ub + count)) && !(r = ensure_eidx(m, count)) &&
+        !(r = apply_edit(m, m->edit, count, m->eidx, occupied ? kEditOccupied : kEditFree, val, need, true))) {
+      m->leaf_count_ub += count;
+      m->win_b += count;
+    } else {
+      (void)clear_set(m, m->edit);  // no stale marks for the next call
+    }
+  }
+  dfree(m, &pts, (bytes + 3) / 4);  // hipFree waits for the kernels that read it
+  return r;
+}
+
+int32_t vdbgpu_add_artificial_areas(vdbgpu_map* m, int32_t n_polygons, const int32_t* counts,
+                                    const double* xyz, double negative_height, double positive_height) {
+  if (!m) return VDBGPU_BAD_ARGUMENT;
+  Guard g(m);
+  if (n_polygons < 0 || (n_polygons && !counts)) return fail(m, VDBGPU_BAD_ARGUMENT, "bad polygon arguments");
+  const double qn = negative_height / m->res, qp = positive_height / m->res;
+  if (!(qn > -2147483649.0 && qn < 2147483648.0) || !(qp > -2147483649.0 && qp < 2147483648.0))
+    return fail(m, VDBGPU_BAD_ARGUMENT, "height / resolution is outside the int range");
+  const int k0 = (int)qn, k1 = (int)qp;  // (int)(height / m_resolution), VM:1227-1228
+  std::vector<int> walls;
+  int64_t nv = 0;
+  double ext = 0;
+  for (int32_t p = 0; p < n_polygons; ++p) {
+    const int32_t cnt = counts[p];
+    if (cnt < 0 || (cnt && !xyz)) return fail(m, VDBGPU_BAD_ARGUMENT, "bad polygon vertex count");
+    for (int32_t i = 0; i < cnt; ++i) {
+      const int64_t a = nv + i, b = nv + (i + 1) % cnt;
+      walls.push_back((int)a);
+      walls.push_back((int)b);
+      for (int c = 0; c < 3; ++c) {
+        const double d = std::fabs(xyz[3 * b + c] - xyz[3 * a + c]) * m->inv;
+        if (d > ext || std::isnan(d)) ext = std::isnan(d) ? 0 : d;
+      }
+    }
+    nv += cnt;
+    if (nv > (1 << 28)) return fail(m, VDBGPU_BAD_ARGUMENT, "too many polygon vertices");
+  }
+  const uint32_t n_walls = (uint32_t)(walls.size() / 2);
+  const bool marks = n_walls > 0 && k0 < k1;
+  if (marks && (int64_t)k1 - k0 > (1 << 24)) return fail(m, VDBGPU_BAD_ARGUMENT, "more than 2^24 height layers");
+  int r;
+  CK(hipStreamSynchronize(m->cast));
+  uint32_t count = 0;
+  if (marks) {
+    if ((r = ensure_table(m, (uint32_t)std::min<double>(ext + 2.0, (double)kMaxTableN), m->stream))) return r;
+    double* d_verts = nullptr;
+    int* d_walls = nullptr;
+    if ((r = upload(m, &d_verts, xyz, (size_t)nv * 3)) || (r = upload(m, &d_walls, walls.data(), walls.size())))
+      return r;
+    r = mark_set(m, m->art_tmp, kMinUpdateCap, [&]() -> int {
+      CK(launch_mark_walls(m->stream, d_verts, d_walls, n_walls, k0, k1, m->res, m->inv, m->tbl, m->tbl_n,
+                           upd_view(m->art_tmp), &m->d_st[2]));
+      return VDBGPU_OK;
+    }, &count);
+    dfree(m, &d_verts, (size_t)nv * 3);
+    dfree(m, &d_walls, walls.size());
+    if (r) return r;  // the live artificial grid and the map are unchanged
+  }
+  // restoreMapIntegrity first (VM:1181), then the new grid replaces the old
+  if ((r = restore_integrity(m))) {
+    (void)clear_set(m, m->art_tmp);
+    return r;
+  }
+  std::swap(m->art, m->art_tmp);
+  m->art_count = count;
+  m->art_present = true;
+  m->art_leaves = false;
+  if (count > m->art_eidx_cap) {
+    dfree(m, &m->art_eidx, m->art_eidx_cap);
+    m->art_eidx_cap = (uint32_t)next_pow2(count);
+    CK(dalloc(m, &m->art_eidx, m->art_eidx_cap));
+  }
+  return VDBGPU_OK;
+}
+
+int32_t vdbgpu_restore_map_integrity(vdbgpu_map* m) {
+  if (!m) return VDBGPU_BAD_ARGUMENT;
+  Guard g(m);
+  return restore_integrity(m);
+}
+
+// CK for the exports: a HIP error returns -1.
+#define CKN(expr)                                                                \
+  do {                                                                           \
+    hipError_t e_ = (expr);                                                      \
+    if (e_ != hipSuccess) {                                                      \
+      m->last_error = std::string(#expr) + ": " + hipGetErrorString(e_);         \
+      return -1;                                                                 \
+    }                                                                            \
+  } while (0)
+
+int64_t vdbgpu_export_artificial(vdbgpu_map* m, int32_t* origins, uint64_t* masks, int64_t cap) {
+  if (!m) return -1;
+  Guard g(m);
+  const uint32_t n = m->art_count;
+  std::vector<unsigned long long> keys(n), act((size_t)n * 8);
+  std::vector<float> none;
+  if (n && (origins || masks)) {
+    std::vector<uint32_t> list(n);
+    std::vector<unsigned long long> all_keys(m->art.cap);
+    CKN(hipMemcpyAsync(list.data(), m->art.list, n * sizeof(uint32_t), hipMemcpyDeviceToHost, m->stream));
+    CKN(hipMemcpyAsync(all_keys.data(), m->art.keys, m->art.cap * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                      m->stream));
+    std::vector<unsigned long long> all_masks((size_t)m->art.cap * 16);
+    CKN(hipMemcpyAsync(all_masks.data(), m->art.masks, all_masks.size() * sizeof(unsigned long long),
+                      hipMemcpyDeviceToHost, m->stream));
+    CKN(hipStreamSynchronize(m->stream));
+    for (uint32_t i = 0; i < n; ++i) {
+      keys[i] = all_keys[list[i]];
+      std::memcpy(&act[(size_t)i * 8], &all_masks[(size_t)list[i] * 16], 64);
+    }
+  }
+  return export_sorted(m, keys, none, act, origins, nullptr, masks, cap, false);
+}
+
+// ---------------------------------------------------------- map sections --
+
+int64_t vdbgpu_export_section(vdbgpu_map* m, const int32_t bbox[6], int32_t full_grid, int32_t* origins,
+                              float* values, uint64_t* masks, int64_t cap) {
+  if (!m || !bbox) return -1;
+  Guard g(m);
+  CKN(hipStreamSynchronize(m->cast));
+  if (read_status(m)) return -1;
+  const uint32_t L = m->h_st[1].leaf_count;
+  std::vector<unsigned long long> keys, act;
+  std::vector<float> vals;
+  if (L) {
+    int* d_box = nullptr;
+    uint32_t* list = nullptr;
+    if (upload(m, &d_box, bbox, 6)) return -1;
+    CKN(dalloc(m, &list, L));
+    CKN(hipMemsetAsync(&m->d_st[2], 0, sizeof(Status), m->stream));
+    CKN(launch_section_pick(m->stream, map_view(m), L, d_box, full_grid != 0, list, &m->d_st[2].entries));
+    if (read_status(m)) return -1;
+    const uint32_t n = m->h_st[2].entries;
+    const bool want = origins || values || masks;
+    if (n && want && n <= cap) {
+      unsigned long long *dk = nullptr, *da = nullptr;
+      float* dv = nullptr;
+      CKN(dalloc(m, &dk, n));
+      CKN(dalloc(m, &da, (size_t)n * 8));
+      CKN(dalloc(m, &dv, (size_t)n * 512));
+      CKN(launch_section_gather(m->stream, map_view(m), list, n, d_box, full_grid != 0, dk, dv, da));
+      keys.resize(n);
+      act.resize((size_t)n * 8);
+      vals.resize((size_t)n * 512);
+      CKN(hipMemcpyAsync(keys.data(), dk, n * 8, hipMemcpyDeviceToHost, m->stream));
+      CKN(hipMemcpyAsync(act.data(), da, (size_t)n * 64, hipMemcpyDeviceToHost, m->stream));
+      CKN(hipMemcpyAsync(vals.data(), dv, (size_t)n * 2048, hipMemcpyDeviceToHost, m->stream));
+      CKN(hipStreamSynchronize(m->stream));
+      dfree(m, &dk, n);
+      dfree(m, &da, (size_t)n * 8);
+      dfree(m, &dv, (size_t)n * 512);
+    } else {
+      keys.resize(n);  // the count is all that is returned
+    }
+    dfree(m, &d_box, 6);
+    dfree(m, &list, L);
+  }
+  return export_sorted(m, keys, vals, act, origins, values, masks, cap, true);
+}
+
+namespace {
+
+// Upsert the map leaves of n section leaves (device arrays) and hand back
+// their pool indices in *eidx_out (device, n entries; freed by the caller).
+int section_upsert(vdbgpu_map* m, const unsigned long long* keys, const uint8_t* need, uint32_t n,
+                   uint16_t* first, uint32_t** eidx_out) {
+  int r;
+  if ((r = ensure_map(m, m->leaf_count_ub + n))) return r;
+  CK(dalloc(m, eidx_out, n));
+  CK(launch_upsert_leaves(m->stream, keys, need, n, map_view(m), *eidx_out, first, &m->d_st[1]));
+  m->leaf_count_ub += n;
+  m->win_b += n;
+  m->snap_dirty = true;
+  return VDBGPU_OK;
+}
+
+}  // namespace
+
+int32_t vdbgpu_apply_section(vdbgpu_map* m, int64_t n, const int32_t* origins, const float* values,
+                             const uint64_t* masks) {
+  if (!m) return VDBGPU_BAD_ARGUMENT;
+  Guard g(m);
+  if (n < 0 || n >= (1 << 28) || (n && (!origins || !values || !masks)))
+    return fail(m, VDBGPU_BAD_ARGUMENT, "bad section arguments");
+  int r;
+  if ((r = check_origins(m, n, origins))) return r;
+  if (n == 0) return VDBGPU_OK;
+  CK(hipStreamSynchronize(m->cast));
+  const uint32_t N = (uint32_t)n;
+  const uint32_t scap = (uint32_t)std::max<uint64_t>(1024, next_pow2(2 * (uint64_t)N));
+  int* d_o = nullptr;
+  float* d_v = nullptr;
+  unsigned long long *d_a = nullptr, *keys = nullptr, *sets = nullptr, *lists = nullptr;
+  uint8_t* need = nullptr;
+  uint16_t* first = nullptr;
+  uint32_t* eidx = nullptr;
+  if ((r = upload(m, &d_o, origins, (size_t)N * 3)) || (r = upload(m, &d_v, values, (size_t)N * 512)) ||
+      (r = upload(m, &d_a, reinterpret_cast<const unsigned long long*>(masks), (size_t)N * 8)))
+    return r;
+  CK(dalloc(m, &keys, N));
+  CK(dalloc(m, &need, N));
+  CK(dalloc(m, &first, N));
+  CK(dalloc(m, &sets, (size_t)scap * 3));  // leaf, 128^3 and 4096^3 key sets
+  CK(dalloc(m, &lists, (size_t)N * 2));    // new 128^3 and 4096^3 node keys
+  CK(launch_fill_u64(m->stream, sets, (uint64_t)scap * 3, kEmpty));
+  CK(hipMemsetAsync(&m->d_st[2], 0, sizeof(Status), m->stream));
+  unsigned int* counters = &m->d_st[2].segs;  // [0] 128^3 nodes, [1] 4096^3 nodes
+  CK(launch_section_prep(m->stream, d_o, d_v, d_a, N, keys, need, first, sets, scap - 1, sets + scap, scap - 1,
+                         lists, sets + 2 * (size_t)scap, scap - 1, lists + N, counters));
+  if ((r = read_status(m))) return r;
+  const uint32_t c128 = m->h_st[2].segs, c4096 = m->h_st[2].items;
+  if ((r = section_upsert(m, keys, need, N, first, &eidx))) return r;
+  CK(launch_section_write(m->stream, d_v, d_a, eidx, first, N, map_view(m)));
+  // the inactive tiles of the section's internal nodes (VM:1034 cbeginValueAll)
+  CK(launch_section_tiles(m->stream, lists, c128, lists + N, c4096, sets, scap - 1, sets + scap, scap - 1,
+                          map_view(m)));
+  if ((r = read_status(m))) return r;
+  m->touched = m->touched || m->h_st[1].leaf_count > 0;
+  dfree(m, &d_o, (size_t)N * 3);
+  dfree(m, &d_v, (size_t)N * 512);
+  dfree(m, &d_a, (size_t)N * 8);
+  dfree(m, &keys, N);
+  dfree(m, &need, N);
+  dfree(m, &first, N);
+  dfree(m, &sets, (size_t)scap * 3);
+  dfree(m, &lists, (size_t)N * 2);
+  dfree(m, &eidx, N);
+  return VDBGPU_OK;
+}
+
+int32_t vdbgpu_apply_section_update(vdbgpu_map* m, const int32_t bbox[6], int64_t n, const int32_t* origins,
+                                    const uint64_t* masks) {
+  if (!m) return VDBGPU_BAD_ARGUMENT;
+  Guard g(m);
+  if (!bbox || n < 0 || n >= (1 << 28) || (n && (!origins || !masks)))
+    return fail(m, VDBGPU_BAD_ARGUMENT, "bad section arguments");
+  int r;
+  if ((r = check_origins(m, n, origins))) return r;
+  CK(hipStreamSynchronize(m->cast));
+  if ((r = read_status(m))) return r;
+  const uint32_t L = m->h_st[1].leaf_count;
+  int* d_box = nullptr;
+  if ((r = upload(m, &d_box, bbox, 6))) return r;
+  CK(launch_section_clear(m->stream, map_view(m), L, d_box));
+  if (n) {
+    const uint32_t N = (uint32_t)n;
+    int* d_o = nullptr;
+    unsigned long long *d_a = nullptr, *keys = nullptr;
+    uint8_t* need = nullptr;
+    uint16_t* first = nullptr;
+    uint32_t* eidx = nullptr;
+    if ((r = upload(m, &d_o, origins, (size_t)N * 3)) ||
+        (r = upload(m, &d_a, reinterpret_cast<const unsigned long long*>(masks), (size_t)N * 8)))
+      return r;
+    CK(dalloc(m, &keys, N));
+    CK(dalloc(m, &need, N));
+    CK(dalloc(m, &first, N));
+    CK(launch_section_prep(m->stream, d_o, nullptr, d_a, N, keys, need, first, nullptr, 0, nullptr, 0, nullptr,
+                           nullptr, 0, nullptr, nullptr));
+    if ((r = section_upsert(m, keys, need, N, nullptr, &eidx))) return r;
+    CK(launch_section_or(m->stream, d_a, eidx, N, map_view(m)));
+    if ((r = read_status(m))) return r;
+    m->touched = m->touched || m->h_st[1].leaf_count > 0;
+    dfree(m, &d_o, (size_t)N * 3);
+    dfree(m, &d_a, (size_t)N * 8);
+    dfree(m, &keys, N);
+    dfree(m, &need, N);
+    dfree(m, &first, N);
+    dfree(m, &eidx, N);
+  }
+  CK(hipStreamSynchronize(m->stream));
+  dfree(m, &d_box, 6);
+  return VDBGPU_OK;
+}
+
+// createIndexBoundingBox (VM:811-874): corners of [min, max] as float, each
+// transformed by the row-major 4x4 matrix in double (((m0 p0 + m1 p1) + m2 p2)
+// + m3, rounded to float: pcl::transformPointCloud), float min / max,
+// Coord::floor(value * 1/res).
+int32_t vdbgpu_index_bbox(double resolution, const double min_boundary[3], const double max_boundary[3],
+                          const double tf[16], int32_t out[6]) {
+  if (!(resolution > 0) || !min_boundary || !max_boundary || !tf || !out) return VDBGPU_BAD_ARGUMENT;
+  float lo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, hi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+  for (int c = 0; c < 8; ++c) {
+    const double p[3] = {(double)(float)((c & 4) ? max_boundary[0] : min_boundary[0]),
+                         (double)(float)((c & 2) ? max_boundary[1] : min_boundary[1]),
+                         (double)(float)((c & 1) ? max_boundary[2] : min_boundary[2])};
+    for (int r = 0; r < 3; ++r) {
+      const double acc = tf[4 * r] * p[0] + tf[4 * r + 1] * p[1] + tf[4 * r + 2] * p[2] + tf[4 * r + 3];
+      const float q = (float)acc;
+      lo[r] = std::min(lo[r], q);
+      hi[r] = std::max(hi[r], q);
+    }
+  }
+  const double inv = 1.0 / resolution;
+  for (int r = 0; r < 3; ++r) {
+    const double a = std::floor((double)lo[r] * inv), b = std::floor((double)hi[r] * inv);
+    if (!(a >= -2147483648.0 && a < 2147483648.0 && b >= -2147483648.0 && b < 2147483648.0))
+      return VDBGPU_OUT_OF_RANGE;
+    out[r] = (int32_t)a;
+    out[3 + r] = (int32_t)b;
+  }
+  return VDBGPU_OK;
+}
 
 }  // extern "C"
