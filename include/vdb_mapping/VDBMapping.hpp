@@ -17,13 +17,24 @@
 // addDataToAccumulate with its accumulation and integration threads (:133,
 // :355-370, :1383-1430) runs in the library (vdbgpu_async_*).
 //
-// Not provided (each has no call on this path): persistence and PCD I/O (:193-307), map sections,
-// byte arrays and morphology (:810-1338), direct point edits and artificial
-// areas (:413-447, :1152-1236), castRayIntoGrid (:550-566, per-ray host call)
-// and updateMap (:731-792, reached through integrateUpdate).
+// Direct edits (addPointsToGrid / removePointsFromGrid :413-447), artificial
+// areas (restoreMapIntegrity, addArtificialAreas / Polygon / Wall
+// :1152-1236) and map sections (createWorldBoundingBox /
+// createIndexBoundingBox, getMapSectionGrid / UpdateGrid, applyMapSectionGrid
+// / UpdateGrid :811-1085, without morphological smoothing) run in the library
+// too (vdbgpu_set_points, vdbgpu_*artificial*, vdbgpu_*section*).
+//
+// Not provided: persistence and PCD I/O (:193-307) and the byte-array wire
+// format (:1246-1338), which need OpenVDB's io::Stream; the morphology
+// operators (:1095-1140, OpenVDB tools); castRayIntoGrid (:550-566, a
+// per-ray host call) and updateMap (:731-792, reached through
+// integrateUpdate).
 #ifndef VDB_MAPPING_VDB_MAPPING_H_INCLUDED
 #define VDB_MAPPING_VDB_MAPPING_H_INCLUDED
 
+#include <algorithm>
+#include <array>
+#include <cfloat>
 #include <cmath>
 #include <cstring>
 #include <iostream>
@@ -317,6 +328,234 @@ public:
    */
   void flushAccumulation() { report(vdbgpu_async_flush(m_map)); }
 
+  /*!
+   * \brief Sets the voxel of every point free (min log-odds, inactive),
+   * reference :413-427.
+   */
+  bool removePointsFromGrid(const PointCloudT::ConstPtr& cloud) { return editPoints(*cloud, 0); }
+
+  /*!
+   * \brief Sets the voxel of every point occupied (max log-odds, active),
+   * reference :434-447.
+   */
+  bool addPointsToGrid(const PointCloudT::ConstPtr& cloud) { return editPoints(*cloud, 1); }
+
+  /*!
+   * \brief World box around the 8 transformed corners of [min, max]
+   * (reference :811-850).
+   */
+  openvdb::BBoxd createWorldBoundingBox(const Eigen::Matrix<double, 3, 1>& min_boundary,
+                                        const Eigen::Matrix<double, 3, 1>& max_boundary,
+                                        const Eigen::Matrix<double, 4, 4>& map_to_reference_tf) const
+  {
+    float lo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, hi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+    for (int c = 0; c < 8; ++c)
+    {
+      const double p[3] = {(double)(float)((c & 4) ? max_boundary.x() : min_boundary.x()),
+                           (double)(float)((c & 2) ? max_boundary.y() : min_boundary.y()),
+                           (double)(float)((c & 1) ? max_boundary.z() : min_boundary.z())};
+      for (int r = 0; r < 3; ++r)
+      {
+        const float q = (float)(map_to_reference_tf(r, 0) * p[0] + map_to_reference_tf(r, 1) * p[1] +
+                                map_to_reference_tf(r, 2) * p[2] + map_to_reference_tf(r, 3));
+        lo[r] = std::min(lo[r], q);
+        hi[r] = std::max(hi[r], q);
+      }
+    }
+    return openvdb::BBoxd(openvdb::Vec3d(lo[0], lo[1], lo[2]), openvdb::Vec3d(hi[0], hi[1], hi[2]));
+  }
+
+  /*!
+   * \brief Index box of createWorldBoundingBox (reference :858-874).
+   */
+  openvdb::CoordBBox createIndexBoundingBox(const Eigen::Matrix<double, 3, 1>& min_boundary,
+                                            const Eigen::Matrix<double, 3, 1>& max_boundary,
+                                            const Eigen::Matrix<double, 4, 4>& map_to_reference_tf) const
+  {
+    const int32_t* b = indexBox(min_boundary, max_boundary, map_to_reference_tf).data();
+    return openvdb::CoordBBox(openvdb::Coord(b[0], b[1], b[2]), openvdb::Coord(b[3], b[4], b[5]));
+  }
+
+  /*!
+   * \brief Map section as an update grid (reference :884-894, :922-1014).
+   */
+  typename UpdateGridT::Ptr getMapSectionUpdateGrid(const Eigen::Matrix<double, 3, 1>& min_boundary,
+                                                    const Eigen::Matrix<double, 3, 1>& max_boundary,
+                                                    const Eigen::Matrix<double, 4, 4>& map_to_reference_tf,
+                                                    const bool full_grid = false) const
+  {
+    const std::array<int32_t, 6> box = indexBox(min_boundary, max_boundary, map_to_reference_tf);
+    std::vector<int32_t> o;
+    std::vector<float> v;
+    std::vector<uint64_t> a;
+    exportSection(box, full_grid, o, v, a);
+    typename UpdateGridT::Ptr grid = UpdateGridT::create(false);
+    auto acc                       = grid->getAccessor();
+    for (size_t i = 0; i < o.size() / 3; ++i)
+    {
+      uint64_t val[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // true where the float value is non-zero
+      for (int n = 0; n < 512; ++n)
+      {
+        if (v[512 * i + n] != 0.0f)
+        {
+          val[n >> 6] |= 1ull << (n & 63);
+        }
+      }
+      acc.mergeLeaf(&o[3 * i], &a[8 * i], val);
+    }
+    grid->insertMeta("bb_min", openvdb::Vec3d(box[0], box[1], box[2]));
+    grid->insertMeta("bb_max", openvdb::Vec3d(box[3], box[4], box[5]));
+    return grid;
+  }
+
+  /*!
+   * \brief Map section as a grid (reference :902-912, :922-1014).
+   */
+  typename GridT::Ptr getMapSectionGrid(const Eigen::Matrix<double, 3, 1>& min_boundary,
+                                        const Eigen::Matrix<double, 3, 1>& max_boundary,
+                                        const Eigen::Matrix<double, 4, 4>& map_to_reference_tf,
+                                        const bool full_grid = false) const
+  {
+    const std::array<int32_t, 6> box = indexBox(min_boundary, max_boundary, map_to_reference_tf);
+    std::vector<int32_t> o;
+    std::vector<float> v;
+    std::vector<uint64_t> a;
+    exportSection(box, full_grid, o, v, a);
+    typename GridT::Ptr grid = GridT::create(m_resolution);
+    for (size_t i = 0; i < o.size() / 3; ++i)
+    {
+      HostLeaf l;
+      std::memcpy(l.origin, &o[3 * i], sizeof l.origin);
+      std::memcpy(l.values, &v[512 * i], sizeof l.values);
+      std::memcpy(l.mask, &a[8 * i], sizeof l.mask);
+      grid->insertLeaf(l);
+    }
+    grid->insertMeta("bb_min", openvdb::Vec3d(box[0], box[1], box[2]));
+    grid->insertMeta("bb_max", openvdb::Vec3d(box[3], box[4], box[5]));
+    return grid;
+  }
+
+  /*!
+   * \brief Writes a section grid into the map (reference :1022-1050).
+   * Morphological smoothing (smooth_map) is not offered: such a call is
+   * refused with a message and changes nothing.
+   */
+  void applyMapSectionGrid(const typename GridT::Ptr section,
+                           bool smooth_map          = false,
+                           int smoothing_iterations = 2)
+  {
+    (void)smoothing_iterations;
+    if (smooth_map)
+    {
+      std::cerr << "applyMapSectionGrid: morphological smoothing is not supported" << std::endl;
+      return;
+    }
+    std::vector<int32_t> o;
+    std::vector<float> v;
+    std::vector<uint64_t> a;
+    section->forEachLeaf([&](const HostLeaf& l) {
+      o.insert(o.end(), l.origin, l.origin + 3);
+      v.insert(v.end(), l.values, l.values + 512);
+      a.insert(a.end(), l.mask, l.mask + 8);
+    });
+    std::unique_lock map_lock(*m_map_mutex);
+    report(vdbgpu_apply_section(m_map, (int64_t)(o.size() / 3), o.data(), v.data(), a.data()));
+    m_vdb_grid->markStale();
+  }
+
+  /*!
+   * \brief Replaces the active states inside the section's box by the
+   * section's (reference :1058-1085). Smoothing is not offered (see above).
+   */
+  void applyMapSectionUpdateGrid(const typename UpdateGridT::Ptr section,
+                                 bool smooth_map          = false,
+                                 int smoothing_iterations = 2)
+  {
+    (void)smoothing_iterations;
+    if (smooth_map)
+    {
+      std::cerr << "applyMapSectionUpdateGrid: morphological smoothing is not supported" << std::endl;
+      return;
+    }
+    const openvdb::Vec3d mn = section->template metaValue<openvdb::Vec3d>("bb_min");
+    const openvdb::Vec3d mx = section->template metaValue<openvdb::Vec3d>("bb_max");
+    const int32_t box[6]    = {(int32_t)std::floor(mn.x()), (int32_t)std::floor(mn.y()),
+                               (int32_t)std::floor(mn.z()), (int32_t)std::floor(mx.x()),
+                               (int32_t)std::floor(mx.y()), (int32_t)std::floor(mx.z())};
+    std::vector<int32_t> o;
+    std::vector<uint64_t> a;
+    section->forEachLeaf([&](const typename UpdateGridT::Leaf& l) {
+      o.insert(o.end(), l.origin, l.origin + 3);
+      a.insert(a.end(), l.active, l.active + 8);
+    });
+    std::unique_lock map_lock(*m_map_mutex);
+    report(vdbgpu_apply_section_update(m_map, box, (int64_t)(o.size() / 3), o.data(), a.data()));
+    m_vdb_grid->markStale();
+  }
+
+  /*!
+   * \brief Restores the state of every artificial voxel (reference :1152-1166).
+   */
+  void restoreMapIntegrity()
+  {
+    std::unique_lock map_lock(*m_map_mutex);
+    report(vdbgpu_restore_map_integrity(m_map));
+    m_vdb_grid->markStale();
+  }
+
+  /*!
+   * \brief Replaces the artificial areas (reference :1175-1190).
+   */
+  void addArtificialAreas(const std::vector<std::vector<Eigen::Matrix<double, 4, 1> > >& artificial_areas,
+                          const double negative_height,
+                          const double positive_height)
+  {
+    std::vector<int32_t> counts;
+    std::vector<double> xyz;
+    for (const auto& area : artificial_areas)
+    {
+      counts.push_back((int32_t)area.size());
+      for (const auto& p : area)
+      {
+        xyz.insert(xyz.end(), {p.x(), p.y(), p.z()});
+      }
+    }
+    std::unique_lock map_lock(*m_map_mutex);
+    report(vdbgpu_add_artificial_areas(
+      m_map, (int32_t)counts.size(), counts.data(), xyz.data(), negative_height, positive_height));
+    m_vdb_grid->markStale();
+  }
+
+  /*!
+   * \brief Adds the walls of one polygon (reference :1198-1207).
+   */
+  void addArtificialPolygon(const std::vector<Eigen::Matrix<double, 4, 1> >& polygon,
+                            const double negative_height,
+                            const double positive_height)
+  {
+    std::vector<double> walls;
+    for (size_t i = 0; i < polygon.size(); i++)
+    {
+      const auto& a = polygon[i];
+      const auto& b = polygon[(i + 1) % polygon.size()];
+      walls.insert(walls.end(), {a.x(), a.y(), a.z(), b.x(), b.y(), b.z()});
+    }
+    report(vdbgpu_add_artificial_walls(
+      m_map, (int32_t)(walls.size() / 6), walls.data(), negative_height, positive_height));
+  }
+
+  /*!
+   * \brief Adds one wall (reference :1217-1236).
+   */
+  void addArtificialWall(const Eigen::Matrix<double, 4, 1>& start,
+                         const Eigen::Matrix<double, 4, 1>& end,
+                         const double negative_height,
+                         const double positive_height)
+  {
+    const double w[6] = {start.x(), start.y(), start.z(), end.x(), end.y(), end.z()};
+    report(vdbgpu_add_artificial_walls(m_map, 1, w, negative_height, positive_height));
+  }
+
   virtual void setConfig(const TConfig& config)
   {
     if (config.max_range < 0.0)
@@ -338,6 +577,58 @@ public:
   vdbgpu_map* handle() const { return m_map; }
 
 protected:
+  bool editPoints(const PointCloudT& cloud, int32_t occupied)
+  {
+    const float* xyz = cloud.points.empty() ? nullptr : &cloud.points[0].x;
+    const int32_t rc = vdbgpu_set_points(
+      m_map, xyz, static_cast<int64_t>(cloud.points.size()), static_cast<int64_t>(sizeof(PointT)), occupied);
+    m_vdb_grid->markStale();
+    return report(rc) == VDBGPU_OK;
+  }
+
+  std::array<int32_t, 6> indexBox(const Eigen::Matrix<double, 3, 1>& min_boundary,
+                                  const Eigen::Matrix<double, 3, 1>& max_boundary,
+                                  const Eigen::Matrix<double, 4, 4>& tf) const
+  {
+    const double mn[3] = {min_boundary.x(), min_boundary.y(), min_boundary.z()};
+    const double mx[3] = {max_boundary.x(), max_boundary.y(), max_boundary.z()};
+    double t[16];
+    for (int r = 0; r < 4; ++r)
+    {
+      for (int c = 0; c < 4; ++c)
+      {
+        t[4 * r + c] = tf(r, c);
+      }
+    }
+    std::array<int32_t, 6> out{};
+    report(vdbgpu_index_bbox(m_resolution, mn, mx, t, out.data()));
+    return out;
+  }
+
+  void exportSection(const std::array<int32_t, 6>& box,
+                     bool full_grid,
+                     std::vector<int32_t>& o,
+                     std::vector<float>& v,
+                     std::vector<uint64_t>& a) const
+  {
+    std::shared_lock map_lock(*m_map_mutex);
+    const int64_t n = vdbgpu_export_section(m_map, box.data(), full_grid ? 1 : 0, nullptr, nullptr, nullptr, 0);
+    if (n <= 0)
+    {
+      return;
+    }
+    o.resize(3 * n);
+    v.resize(512 * n);
+    a.resize(8 * n);
+    const int64_t got = vdbgpu_export_section(m_map, box.data(), full_grid ? 1 : 0, o.data(), v.data(), a.data(), n);
+    if (got != n)
+    {
+      o.clear();
+      v.clear();
+      a.clear();
+    }
+  }
+
   int32_t accumulate(const std::string& source_id,
                      const PointCloudT& cloud,
                      const Eigen::Matrix<double, 3, 1>& origin)

@@ -10,6 +10,7 @@
 #ifndef VDB_MAPPING_COMPAT_HPP
 #define VDB_MAPPING_COMPAT_HPP
 
+#include <climits>
 #include <cmath>
 #include <cstdint>
 #include <memory>
@@ -66,6 +67,36 @@ class Matrix<double, 3, 1> {
  private:
   double v_[3];
 };
+template <>
+class Matrix<double, 4, 1> {
+ public:
+  Matrix() : v_{0, 0, 0, 0} {}
+  Matrix(double x, double y, double z, double w) : v_{x, y, z, w} {}
+  double x() const { return v_[0]; }
+  double y() const { return v_[1]; }
+  double z() const { return v_[2]; }
+  double w() const { return v_[3]; }
+  double& operator()(int i) { return v_[i]; }
+  double operator()(int i) const { return v_[i]; }
+
+ private:
+  double v_[4];
+};
+template <>
+class Matrix<double, 4, 4> {
+ public:
+  Matrix() : m_{} {}
+  static Matrix Identity() {
+    Matrix r;
+    for (int i = 0; i < 4; ++i) r.m_[i][i] = 1.0;
+    return r;
+  }
+  double& operator()(int r, int c) { return m_[r][c]; }
+  double operator()(int r, int c) const { return m_[r][c]; }
+
+ private:
+  double m_[4][4];
+};
 }  // namespace Eigen
 #endif
 
@@ -109,6 +140,31 @@ class Vec3 {
 };
 }  // namespace math
 using Vec3d = math::Vec3<double>;
+class CoordBBox {
+ public:
+  CoordBBox() : min_(INT32_MAX, INT32_MAX, INT32_MAX), max_(INT32_MIN, INT32_MIN, INT32_MIN) {}
+  CoordBBox(const Coord& mn, const Coord& mx) : min_(mn), max_(mx) {}
+  const Coord& min() const { return min_; }
+  const Coord& max() const { return max_; }
+  bool isInside(const Coord& c) const {
+    for (int a = 0; a < 3; ++a)
+      if (c[a] < min_[a] || c[a] > max_[a]) return false;
+    return true;
+  }
+
+ private:
+  Coord min_, max_;
+};
+class BBoxd {
+ public:
+  BBoxd() = default;
+  BBoxd(const Vec3d& mn, const Vec3d& mx) : min_(mn), max_(mx) {}
+  const Vec3d& min() const { return min_; }
+  const Vec3d& max() const { return max_; }
+
+ private:
+  Vec3d min_, max_;
+};
 }  // namespace openvdb
 #endif
 

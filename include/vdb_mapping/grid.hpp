@@ -18,8 +18,12 @@
 
 #include <cstdint>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <mutex>
+#include <stdexcept>
+#include <string>
+#include <type_traits>
 #include <unordered_map>
 #include <vector>
 
@@ -157,6 +161,26 @@ public:
   }
 #endif
 
+  // Detached grids (map sections): leaves set directly, Vec3d metadata.
+  static Ptr create(double voxel_size) { return std::make_shared<FloatGrid>(voxel_size); }
+  void insertLeaf(const HostLeaf& leaf)
+  {
+    std::lock_guard<std::mutex> lk(m_mu);
+    m_leaves[host_leaf_key(leaf.origin[0], leaf.origin[1], leaf.origin[2])] = leaf;
+  }
+  void insertMeta(const std::string& name, const openvdb::Vec3d& value) { m_meta[name] = value; }
+  template <typename T>
+  T metaValue(const std::string& name) const
+  {
+    static_assert(std::is_same<T, openvdb::Vec3d>::value, "grid metadata holds Vec3d values");
+    auto it = m_meta.find(name);
+    if (it == m_meta.end())
+    {
+      throw std::runtime_error("no grid metadata named " + name);
+    }
+    return it->second;
+  }
+
   // Live binding to a device map (used by VDBMapping).
   void attach(vdbgpu_map* map)
   {
@@ -209,6 +233,7 @@ private:
   }
 
   double m_voxel_size;
+  std::map<std::string, openvdb::Vec3d> m_meta;
   vdbgpu_map* m_map = nullptr;
   mutable std::mutex m_mu;
   mutable bool m_stale = false;
@@ -282,6 +307,18 @@ public:
 
   Accessor getAccessor() { return Accessor(this); }
   bool empty() const { return m_leaves.empty(); }
+  void insertMeta(const std::string& name, const openvdb::Vec3d& value) { m_meta[name] = value; }
+  template <typename T>
+  T metaValue(const std::string& name) const
+  {
+    static_assert(std::is_same<T, openvdb::Vec3d>::value, "grid metadata holds Vec3d values");
+    auto it = m_meta.find(name);
+    if (it == m_meta.end())
+    {
+      throw std::runtime_error("no grid metadata named " + name);
+    }
+    return it->second;
+  }
   openvdb::Index64 activeVoxelCount() const
   {
     openvdb::Index64 n = 0;
@@ -320,6 +357,7 @@ private:
     return it->second;
   }
   std::unordered_map<uint64_t, Leaf> m_leaves;
+  std::map<std::string, openvdb::Vec3d> m_meta;
 };
 
 } // namespace vdb_mapping

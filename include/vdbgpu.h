@@ -282,6 +282,12 @@ int32_t vdbgpu_set_points(vdbgpu_map* map, const float* xyz, int64_t n, int64_t 
 int32_t vdbgpu_add_artificial_areas(vdbgpu_map* map, int32_t n_polygons, const int32_t* counts,
                                     const double* xyz, double negative_height,
                                     double positive_height);
+/* addArtificialWall (VM:1217-1236) for n_walls walls, xyz = {start x, y, z,
+ * end x, y, z} per wall, each cast for the height layers as above, added to
+ * the artificial grid without a restore (addArtificialPolygon is the walls
+ * i -> (i + 1) mod size of a polygon). A rejected call changes nothing. */
+int32_t vdbgpu_add_artificial_walls(vdbgpu_map* map, int32_t n_walls, const double* xyz,
+                                    double negative_height, double positive_height);
 /* restoreMapIntegrity (VM:1152-1166): every artificial voxel gets
  * active = value > thres_max (setNodeState, OCC:130-134); the grid is cleared. */
 int32_t vdbgpu_restore_map_integrity(vdbgpu_map* map);

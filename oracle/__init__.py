@@ -89,6 +89,7 @@ def lib():
         L.orc_set_points.argtypes = [P, P, I64, I64, I32]
         L.orc_add_artificial_areas.argtypes = [P, I32, P, P, D, D]
         L.orc_restore_map_integrity.argtypes = [P]
+        L.orc_add_artificial_walls.argtypes = [P, I32, P, D, D]
         L.orc_export_artificial.restype = I64
         L.orc_export_artificial.argtypes = [P, P, P, I64]
         L.orc_export_section.restype = I64
@@ -230,6 +231,19 @@ class OracleMap:
         counts, xyz = polygons_arg(polygons)
         return lib().orc_add_artificial_areas(self.h, len(counts), _ptr(counts), _ptr(xyz),
                                               negative_height, positive_height)
+
+    def add_artificial_wall(self, start, end, negative_height: float, positive_height: float) -> int:
+        return self.add_artificial_walls([(start, end)], negative_height, positive_height)
+
+    def add_artificial_walls(self, walls, negative_height: float, positive_height: float) -> int:
+        w = np.ascontiguousarray([[*np.asarray(a, float)[:3], *np.asarray(b, float)[:3]] for a, b in walls],
+                                 dtype=np.float64).reshape(-1, 6)
+        return lib().orc_add_artificial_walls(self.h, w.shape[0], _ptr(w), negative_height, positive_height)
+
+    def add_artificial_polygon(self, polygon, negative_height: float, positive_height: float) -> int:
+        p = [np.asarray(v, float)[:3] for v in polygon]
+        walls = [(p[i], p[(i + 1) % len(p)]) for i in range(len(p))]
+        return self.add_artificial_walls(walls, negative_height, positive_height)
 
     def restore_map_integrity(self) -> int:
         return lib().orc_restore_map_integrity(self.h)

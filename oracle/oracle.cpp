@@ -918,42 +918,33 @@ int orc_restore_map_integrity(void* h) {
   return ORC_OK;
 }
 
-// addArtificialAreas (VM:1175-1190) -> addArtificialPolygon (VM:1198-1207)
-// -> addArtificialWall (VM:1217-1236): restore first, then for every polygon
-// edge (i, i+1 mod size) a castRayIntoGrid between the rounded edge ends
-// (VM:612-631) shifted by (0, 0, k) for k in [(int)(neg / res), (int)(pos /
-// res)), marking the artificial grid. Polygons are `counts[p]` consecutive
-// (x, y, z) vertices of `xyz`. Arguments are validated before any change.
-int orc_add_artificial_areas(void* h, int32_t n_polygons, const int32_t* counts, const double* xyz,
-                             double negative_height, double positive_height) {
-  Map& m = *(Map*)h;
-  if (n_polygons < 0 || (n_polygons && !counts)) return ORC_BAD_ARGUMENT;
-  const double qn = negative_height / m.res, qp = positive_height / m.res;
-  if (!(qn > -2147483649.0 && qn < 2147483648.0) || !(qp > -2147483649.0 && qp < 2147483648.0))
-    return ORC_BAD_ARGUMENT;
-  const int32_t k0 = (int32_t)qn, k1 = (int32_t)qp;
-  std::vector<std::array<int32_t, 6>> walls;
-  int64_t v0 = 0;
-  for (int32_t p = 0; p < n_polygons; ++p) {
-    const int32_t cnt = counts[p];
-    if (cnt < 0 || (cnt && !xyz)) return ORC_BAD_ARGUMENT;
-    for (int32_t i = 0; i < cnt; ++i) {
-      std::array<int32_t, 6> w;
-      world_to_index(m.res, m.inv, xyz + 3 * (v0 + i), w.data());
-      world_to_index(m.res, m.inv, xyz + 3 * (v0 + (i + 1) % cnt), w.data() + 3);
-      if (k0 < k1) {
-        for (int j = 0; j < 6; ++j) {
-          const int64_t lo = (int64_t)w[j] + ((j % 3 == 2) ? k0 : 0);
-          const int64_t hi = (int64_t)w[j] + ((j % 3 == 2) ? k1 - 1 : 0);
-          if (w[j] == INT32_MIN || lo < -kCoordLimit || hi >= kCoordLimit) return ORC_OUT_OF_RANGE;
-        }
-        walls.push_back(w);
-      }
-    }
-    v0 += cnt;
+namespace {
+
+// (int)(height / m_resolution) of addArtificialWall (VM:1227-1228).
+bool wall_layers(const Map& m, double neg, double pos, int32_t* k0, int32_t* k1) {
+  const double qn = neg / m.res, qp = pos / m.res;
+  if (!(qn > -2147483649.0 && qn < 2147483648.0) || !(qp > -2147483649.0 && qp < 2147483648.0)) return false;
+  *k0 = (int32_t)qn;
+  *k1 = (int32_t)qp;
+  return true;
+}
+
+// Rounded wall ends (VM:1222-1225, :612-631), checked for every layer.
+int wall_ends(const Map& m, const double* a, const double* b, int32_t k0, int32_t k1,
+              std::array<int32_t, 6>& w) {
+  world_to_index(m.res, m.inv, a, w.data());
+  world_to_index(m.res, m.inv, b, w.data() + 3);
+  for (int j = 0; j < 6; ++j) {
+    const int64_t lo = (int64_t)w[j] + ((j % 3 == 2) ? k0 : 0);
+    const int64_t hi = (int64_t)w[j] + ((j % 3 == 2) ? k1 - 1 : 0);
+    if (w[j] == INT32_MIN || lo < -kCoordLimit || hi >= kCoordLimit) return ORC_OUT_OF_RANGE;
   }
-  restore_integrity(m);
-  m.art_present = true;
+  return ORC_OK;
+}
+
+// castRayIntoGrid of every layer of the walls into the artificial grid
+// (VM:1230-1235).
+void cast_walls(Map& m, const std::vector<std::array<int32_t, 6>>& walls, int32_t k0, int32_t k1) {
   for (const auto& w : walls) {
     for (int32_t k = k0; k < k1; ++k) {
       const int32_t s[3] = {w[0], w[1], w[2] + k}, e[3] = {w[3], w[4], w[5] + k};
@@ -964,6 +955,57 @@ int orc_add_artificial_areas(void* h, int32_t n_polygons, const int32_t* counts,
       });
     }
   }
+}
+
+}  // namespace
+
+// addArtificialAreas (VM:1175-1190) -> addArtificialPolygon (VM:1198-1207)
+// -> addArtificialWall (VM:1217-1236): restore first, then for every polygon
+// edge (i, i+1 mod size) a castRayIntoGrid between the rounded edge ends
+// (VM:612-631) shifted by (0, 0, k) for k in [(int)(neg / res), (int)(pos /
+// res)), marking the artificial grid. Polygons are `counts[p]` consecutive
+// (x, y, z) vertices of `xyz`. Arguments are validated before any change.
+int orc_add_artificial_areas(void* h, int32_t n_polygons, const int32_t* counts, const double* xyz,
+                             double negative_height, double positive_height) {
+  Map& m = *(Map*)h;
+  if (n_polygons < 0 || (n_polygons && !counts)) return ORC_BAD_ARGUMENT;
+  int32_t k0, k1;
+  if (!wall_layers(m, negative_height, positive_height, &k0, &k1)) return ORC_BAD_ARGUMENT;
+  std::vector<std::array<int32_t, 6>> walls;
+  int64_t v0 = 0;
+  for (int32_t p = 0; p < n_polygons; ++p) {
+    const int32_t cnt = counts[p];
+    if (cnt < 0 || (cnt && !xyz)) return ORC_BAD_ARGUMENT;
+    for (int32_t i = 0; i < cnt && k0 < k1; ++i) {
+      std::array<int32_t, 6> w;
+      const int st = wall_ends(m, xyz + 3 * (v0 + i), xyz + 3 * (v0 + (i + 1) % cnt), k0, k1, w);
+      if (st) return st;
+      walls.push_back(w);
+    }
+    v0 += cnt;
+  }
+  restore_integrity(m);
+  m.art_present = true;
+  cast_walls(m, walls, k0, k1);
+  return ORC_OK;
+}
+
+// addArtificialWall (VM:1217-1236) for n walls (start xyz, end xyz), added to
+// the artificial grid without a restore.
+int orc_add_artificial_walls(void* h, int32_t n_walls, const double* xyz, double negative_height,
+                             double positive_height) {
+  Map& m = *(Map*)h;
+  if (n_walls < 0 || (n_walls && !xyz)) return ORC_BAD_ARGUMENT;
+  int32_t k0, k1;
+  if (!wall_layers(m, negative_height, positive_height, &k0, &k1)) return ORC_BAD_ARGUMENT;
+  std::vector<std::array<int32_t, 6>> walls;
+  for (int32_t i = 0; i < n_walls && k0 < k1; ++i) {
+    std::array<int32_t, 6> w;
+    const int st = wall_ends(m, xyz + 6 * i, xyz + 6 * i + 3, k0, k1, w);
+    if (st) return st;
+    walls.push_back(w);
+  }
+  cast_walls(m, walls, k0, k1);
   return ORC_OK;
 }
 

@@ -96,6 +96,9 @@ int orc_set_points(void* h, const float* xyz, int64_t n, int64_t stride_bytes, i
 /* addArtificialAreas (VM:1175-1236): polygons of counts[p] (x, y, z) vertices. */
 int orc_add_artificial_areas(void* h, int32_t n_polygons, const int32_t* counts,
                              const double* xyz, double negative_height, double positive_height);
+/* addArtificialWall (VM:1217-1236) for n walls (start xyz, end xyz each), no restore. */
+int orc_add_artificial_walls(void* h, int32_t n_walls, const double* xyz, double negative_height,
+                             double positive_height);
 /* restoreMapIntegrity (VM:1152-1166). */
 int orc_restore_map_integrity(void* h);
 /* The artificial grid: origins and voxel masks per leaf, sorted by origin. */

@@ -193,6 +193,101 @@ static void add_data_to_accumulate_test()
   a.addDataToAccumulate(cloud, origin, "missing");  // prints, does nothing
 }
 
+static void point_edits_test()
+{
+  // addPointsToGrid / removePointsFromGrid (reference :413-447)
+  OccupancyVDBMapping map(0.1);
+  Config conf = make_config(10);
+  map.setConfig(conf);
+  OccupancyVDBMapping::PointCloudT::Ptr cloud(new OccupancyVDBMapping::PointCloudT);
+  cloud->points.emplace_back(0.05f, 0.05f, 0.05f);
+  cloud->points.emplace_back(1.23f, -0.47f, 0.31f);
+  CHECK(map.addPointsToGrid(cloud));
+  auto acc           = map.getGrid()->getAccessor();
+  const float c_max  = static_cast<float>(std::log(0.99) - std::log(0.01));
+  const openvdb::Coord a(0, 0, 0), b(12, -5, 3);
+  CHECK(acc.getValue(a) == c_max && acc.isValueOn(a));
+  CHECK(acc.getValue(b) == c_max && acc.isValueOn(b));
+  cloud->points.pop_back();
+  CHECK(map.removePointsFromGrid(cloud));
+  CHECK(acc.getValue(a) == -c_max && !acc.isValueOn(a));
+  CHECK(acc.getValue(b) == c_max && acc.isValueOn(b));
+}
+
+static void map_section_test()
+{
+  // getMapSectionGrid -> applyMapSectionGrid into an empty map copies the
+  // voxels inside the box; the update-grid section carries the active states
+  OccupancyVDBMapping src(1), dst(1), upd(1);
+  Config conf = make_config(30);
+  for (auto* m : {&src, &dst, &upd})
+  {
+    m->setConfig(conf);
+    m->addInputSource("test", conf.max_range, 0);
+  }
+  OccupancyVDBMapping::PointCloudT::Ptr cloud(new OccupancyVDBMapping::PointCloudT);
+  for (int k = -6; k <= 6; ++k) cloud->points.emplace_back(12, k, k / 2);
+  Eigen::Matrix<double, 3, 1> origin(0.5, 0.5, 0.5);
+  src.insertPointCloud(cloud, origin, "test");
+  const Eigen::Matrix<double, 3, 1> lo(2, -3, -2), hi(12.5, 3, 2);
+  const Eigen::Matrix<double, 4, 4> tf = Eigen::Matrix<double, 4, 4>::Identity();
+  const openvdb::CoordBBox box = src.createIndexBoundingBox(lo, hi, tf);
+  CHECK(box.min() == openvdb::Coord(2, -3, -2) && box.max() == openvdb::Coord(12, 3, 2));
+  auto section = src.getMapSectionGrid(lo, hi, tf, true);
+  CHECK(section->metaValue<openvdb::Vec3d>("bb_max").x() == 12.0);
+  dst.applyMapSectionGrid(section);
+  auto sa = src.getGrid()->getAccessor();
+  auto da = dst.getGrid()->getAccessor();
+  int inside = 0, equal = 0, nonzero = 0;
+  for (int x = 0; x <= 14; ++x)
+    for (int y = -5; y <= 5; ++y)
+      for (int z = -4; z <= 4; ++z)
+      {
+        const openvdb::Coord c(x, y, z);
+        if (!box.isInside(c)) continue;
+        ++inside;
+        equal += sa.getValue(c) == da.getValue(c) && sa.isValueOn(c) == da.isValueOn(c);
+        nonzero += da.getValue(c) != 0.0f;
+      }
+  CHECK(inside == equal && nonzero > 20);
+  auto us = src.getMapSectionUpdateGrid(lo, hi, tf, false);
+  CHECK(us->activeVoxelCount() == 7);  // the hit voxels (12, k, k / 2) with |k| <= 3
+  upd.applyMapSectionUpdateGrid(us);
+  CHECK(upd.getGrid()->activeVoxelCount() == 7);
+  CHECK(upd.getGrid()->getAccessor().isValueOn(openvdb::Coord(12, 6, 3)) == false);  // outside the box
+  dst.applyMapSectionGrid(section, true);  // smoothing refused: prints, changes nothing
+}
+
+static void artificial_areas_test()
+{
+  // addArtificialAreas: the walls' voxels are forced active after every
+  // non-empty update; restoreMapIntegrity puts back value > t_max
+  OccupancyVDBMapping map(1);
+  Config conf = make_config(30);
+  map.setConfig(conf);
+  map.addInputSource("test", conf.max_range, 0);
+  std::vector<std::vector<Eigen::Matrix<double, 4, 1> > > areas(1);
+  areas[0].emplace_back(5.5, -5.5, 0.5, 1.0);
+  areas[0].emplace_back(5.5, 5.5, 0.5, 1.0);
+  areas[0].emplace_back(-5.5, 5.5, 0.5, 1.0);
+  map.addArtificialAreas(areas, 0.0, 2.0);
+  auto acc = map.getGrid()->getAccessor();
+  const openvdb::Coord wall(6, 1, 2);  // on the first wall (x = 6), layer 1
+  CHECK(!acc.isValueOn(wall));         // nothing until the next update
+  OccupancyVDBMapping::PointCloudT::Ptr cloud(new OccupancyVDBMapping::PointCloudT);
+  cloud->points.emplace_back(9.5f, 0.5f, 1.5f);  // its ray passes the wall voxel (a miss)
+  Eigen::Matrix<double, 3, 1> origin(0.5, 0.5, 1.5);
+  map.insertPointCloud(cloud, origin, "test");
+  CHECK(acc.isValueOn(wall) && acc.getValue(wall) == logodds(conf.prob_miss));
+  CHECK(acc.isValueOn(openvdb::Coord(0, 6, 2)));  // second wall, layer 1
+  map.restoreMapIntegrity();
+  CHECK(!acc.isValueOn(wall));
+  CHECK(!acc.isValueOn(openvdb::Coord(0, 6, 2)));
+  map.addArtificialWall(areas[0][0], areas[0][1], 0.0, 2.0);
+  map.insertPointCloud(cloud, origin, "test");
+  CHECK(acc.isValueOn(wall));
+}
+
 int main()
 {
   struct T
@@ -208,6 +303,9 @@ int main()
     {"RaycastPointCloud", raycast_point_cloud_test},
     {"Raytrace", raytrace_test},
     {"AddDataToAccumulate", add_data_to_accumulate_test},
+    {"PointEdits", point_edits_test},
+    {"MapSection", map_section_test},
+    {"ArtificialAreas", artificial_areas_test},
   };
   for (const T& t : tests)
   {

@@ -250,3 +250,25 @@ def test_sections_between_gpu_maps_outdoor():
     g2, o2 = pair(w.resolution, max_range=w.max_range, **W.PROBS)
     both(g2, o2, "apply_section", so, sv, sa)
     same_map(g2, o2)
+
+
+@pytest.mark.parametrize("traversal", [0, 1])
+def test_walls_and_polygons_append(traversal):
+    g, o, rng = scanned(traversal)
+    both(g, o, "add_artificial_areas", POLYS[:1], -0.2, 0.2)
+    both(g, o, "add_artificial_wall", [2.05, 2.0, -0.3], [-2.0, 2.55, -0.3], -0.1, 0.3)
+    same_art(g, o)
+    orig = rng.uniform(-0.5, 0.5, 3)
+    both(g, o, "insert", "test", (orig + rng.normal(0, 1.0, (2500, 3))).astype(np.float32), orig)
+    same_map(g, o)
+    both(g, o, "add_artificial_polygon", POLYS[0] * 1.7, 0.0, 0.1)
+    walls = [(rng.uniform(-3, 3, 3), rng.uniform(-3, 3, 3)) for _ in range(40)]
+    both(g, o, "add_artificial_walls", walls, -0.25, 0.15)
+    same_art(g, o)
+    both(g, o, "insert", "test", (orig + rng.normal(0, 1.0, (2500, 3))).astype(np.float32), orig)
+    same_map(g, o)
+    bad = [(np.array([0.0, 0.0, 0.0]), np.array([np.nan, 0.0, 0.0]))]
+    assert g.add_artificial_walls(bad, 0.0, 0.3) == o.add_artificial_walls(bad, 0.0, 0.3) == V.OUT_OF_RANGE
+    same_art(g, o)
+    both(g, o, "restore_map_integrity")
+    same_map(g, o)

@@ -352,3 +352,31 @@ def test_index_bbox_identity_and_rotation():
     assert O.index_bbox(0.25, [-1, -1, -1], [1, 1, 1], tr).tolist() == [37, -18, -1, 45, -10, 7]
     # corners are rounded to float first: 0.1f * 10 > 1 -> index 1
     assert O.index_bbox(0.1, [0.1, 0.1, 0.1], [0.1, 0.1, 0.1]).tolist() == [1, 1, 1, 1, 1, 1]
+
+
+def test_walls_and_polygons_append_without_restore():
+    m = scanned_map()
+    assert m.add_artificial_areas(POLYS[:1], -0.2, 0.2) == O.OK
+    first = art_voxels(m)
+    a, b = np.array([2.05, 2.0, -0.3]), np.array([-2.0, 2.55, -0.3])
+    assert m.add_artificial_wall(a, b, -0.1, 0.3) == O.OK
+    wall = set()
+    for k in range(int(-0.1 / 0.1), int(0.3 / 0.1)):
+        dz = np.array([0, 0, k], np.int32)
+        wall |= {tuple(int(x) for x in c)
+                 for c in O.cast_ray(O.world_to_index(0.1, a) + dz, O.world_to_index(0.1, b) + dz)}
+    assert art_voxels(m) == first | wall
+    assert m.add_artificial_polygon(POLYS[0], 0.0, 0.1) == O.OK
+    assert art_voxels(m) == first | wall | expected_walls(POLYS[:1], 0.0, 0.1, 0.1)
+
+
+def test_wall_direction_is_kept():
+    """castRayIntoGrid(s, e) and (e, s) can take different voxels on exact
+    ties; a wall is cast from its start vertex."""
+    m1, m2 = mk(1.0), mk(1.0)
+    a, b = np.array([0.5, 0.5, 0.5]), np.array([3.5, 1.5, 0.5])
+    assert m1.add_artificial_wall(a, b, 0.0, 1.0) == O.OK
+    assert m2.add_artificial_wall(b, a, 0.0, 1.0) == O.OK
+    s, e = O.world_to_index(1.0, a), O.world_to_index(1.0, b)
+    assert art_voxels(m1) == {tuple(int(x) for x in c) for c in O.cast_ray(s, e)}
+    assert art_voxels(m2) == {tuple(int(x) for x in c) for c in O.cast_ray(e, s)}

@@ -37,7 +37,7 @@ ABI_SYMBOLS = [
     "vdbgpu_async_flush", "vdbgpu_async_counters", "vdbgpu_async_stop", "vdbgpu_server_create", "vdbgpu_server_submit", "vdbgpu_server_drain",
     "vdbgpu_server_records", "vdbgpu_server_map", "vdbgpu_server_device_bytes",
     "vdbgpu_server_destroy", "vdbgpu_set_points", "vdbgpu_add_artificial_areas",
-    "vdbgpu_restore_map_integrity", "vdbgpu_export_artificial", "vdbgpu_export_section",
+    "vdbgpu_restore_map_integrity", "vdbgpu_export_artificial", "vdbgpu_add_artificial_walls", "vdbgpu_export_section",
     "vdbgpu_apply_section", "vdbgpu_apply_section_update", "vdbgpu_index_bbox",
 ]
 
@@ -182,6 +182,8 @@ def lib():
         L.vdbgpu_set_points.argtypes = [P, P, I64, I64, I32]
         L.vdbgpu_add_artificial_areas.restype = I32
         L.vdbgpu_add_artificial_areas.argtypes = [P, I32, P, P, D, D]
+        L.vdbgpu_add_artificial_walls.restype = I32
+        L.vdbgpu_add_artificial_walls.argtypes = [P, I32, P, D, D]
         L.vdbgpu_restore_map_integrity.restype = I32
         L.vdbgpu_restore_map_integrity.argtypes = [P]
         L.vdbgpu_export_artificial.restype = I64
@@ -408,6 +410,26 @@ class GpuMap:
         rc = lib().vdbgpu_add_artificial_areas(self.h, len(counts), _ptr(counts), _ptr(xyz),
                                                negative_height, positive_height)
         return self._check(rc, check)
+
+    def add_artificial_walls(self, walls, negative_height: float, positive_height: float,
+                             check=False) -> int:
+        """addArtificialWall for each (start, end) pair, without a restore."""
+        w = np.ascontiguousarray([[*np.asarray(a, float)[:3], *np.asarray(b, float)[:3]] for a, b in walls],
+                                 dtype=np.float64).reshape(-1, 6)
+        rc = lib().vdbgpu_add_artificial_walls(self.h, w.shape[0], _ptr(w), negative_height,
+                                               positive_height)
+        return self._check(rc, check)
+
+    def add_artificial_wall(self, start, end, negative_height: float, positive_height: float,
+                            check=False) -> int:
+        return self.add_artificial_walls([(start, end)], negative_height, positive_height, check)
+
+    def add_artificial_polygon(self, polygon, negative_height: float, positive_height: float,
+                               check=False) -> int:
+        """addArtificialPolygon: walls i -> (i + 1) mod size, without a restore."""
+        p = [np.asarray(v, float)[:3] for v in polygon]
+        walls = [(p[i], p[(i + 1) % len(p)]) for i in range(len(p))]
+        return self.add_artificial_walls(walls, negative_height, positive_height, check)
 
     def restore_map_integrity(self, check=False) -> int:
         return self._check(lib().vdbgpu_restore_map_integrity(self.h), check)

@@ -51,25 +51,32 @@ __global__ __launch_bounds__(256) void k_mark_points(const float* __restrict__ p
   }
 }
 
-// One lane per (wall, layer): wall w joins vertices walls[2w] and walls[2w+1]
-// of `verts` (x, y, z doubles), layer k in [k0, k1) shifts both rounded ends
-// by (0, 0, k). A zero-length wall marks nothing (castRayIntoGrid's
-// e == s guard); otherwise the walk visits 1 + |dx| + |dy| + |dz| voxels in
-// exact DDA order (integer crossing comparisons, the crossing table for
-// equal-time ties).
+// One lane per (wall, layer): wall w = {a, b, k0, k1} joins vertices a and b
+// of `verts` (x, y, z doubles); its layers k in [k0, k1) are lanes
+// prefix[w] .. prefix[w + 1] - 1 and shift both rounded ends by (0, 0, k). A
+// zero-length wall marks nothing (castRayIntoGrid's e == s guard); otherwise
+// the walk visits 1 + |dx| + |dy| + |dz| voxels in exact DDA order (integer
+// crossing comparisons, the crossing table for equal-time ties).
 __global__ __launch_bounds__(64) void k_mark_walls(const double* __restrict__ verts,
-                                                   const int* __restrict__ walls, uint32_t n_walls,
-                                                   int k0, int k1, double res, double inv,
-                                                   const double* __restrict__ tbl, uint32_t tbl_n,
-                                                   UpdView u, Status* st) {
-  const uint32_t layers = (uint32_t)((int64_t)k1 - k0);  // 1 .. 2^24 (checked by the host)
-  const uint64_t total = (uint64_t)n_walls * layers;
+                                                   const int4* __restrict__ walls,
+                                                   const uint64_t* __restrict__ prefix, uint32_t n_walls,
+                                                   double res, double inv, const double* __restrict__ tbl,
+                                                   uint32_t tbl_n, UpdView u, Status* st) {
+  const uint64_t total = prefix[n_walls];
   const uint64_t G = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += G) {
-    const uint32_t w = (uint32_t)(t / layers);
-    const int k = k0 + (int)(t % layers);
-    const double* a = verts + 3 * walls[2 * w];
-    const double* b = verts + 3 * walls[2 * w + 1];
+    uint32_t lo = 0, hi = n_walls - 1;  // last wall with prefix <= t
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi + 1) >> 1;
+      if (prefix[mid] <= t)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    const int4 wr = walls[lo];
+    const int k = wr.z + (int)(t - prefix[lo]);
+    const double* a = verts + 3 * (int64_t)wr.x;
+    const double* b = verts + 3 * (int64_t)wr.y;
     int s[3], e[3];
     bool ok = true;
 #pragma unroll
@@ -80,7 +87,7 @@ __global__ __launch_bounds__(64) void k_mark_walls(const double* __restrict__ ve
     const long long sz = (long long)s[2] + k, ez = (long long)e[2] + k;
 #pragma unroll
     for (int j = 0; j < 2; ++j)
-      ok = ok && s[j] > -kCoordLimit - 1 && s[j] < kCoordLimit && e[j] > -kCoordLimit - 1 && e[j] < kCoordLimit;
+      ok = ok && s[j] >= -kCoordLimit && s[j] < kCoordLimit && e[j] >= -kCoordLimit && e[j] < kCoordLimit;
     ok = ok && s[2] != INT_MIN && e[2] != INT_MIN && sz >= -kCoordLimit && sz < kCoordLimit &&
          ez >= -kCoordLimit && ez < kCoordLimit;
     if (!ok) {
@@ -476,12 +483,12 @@ hipError_t launch_mark_points(hipStream_t s, const float* pts, int64_t stride_by
   return hipGetLastError();
 }
 
-hipError_t launch_mark_walls(hipStream_t s, const double* verts, const int* walls, uint32_t n_walls, int k0,
-                             int k1, double res, double inv, const double* tbl, uint32_t tbl_n, UpdView u,
-                             Status* st) {
-  const uint64_t total = (uint64_t)n_walls * (uint32_t)((int64_t)k1 - k0);
+hipError_t launch_mark_walls(hipStream_t s, const double* verts, const int4* walls, const uint64_t* prefix,
+                             uint32_t n_walls, uint64_t total, double res, double inv, const double* tbl,
+                             uint32_t tbl_n, UpdView u, Status* st) {
+  if (total == 0 || n_walls == 0) return hipSuccess;
   const int grid = (int)std::min<uint64_t>(16384, (total + 63) / 64);
-  hipLaunchKernelGGL(k_mark_walls, dim3(grid), dim3(64), 0, s, verts, walls, n_walls, k0, k1, res, inv, tbl,
+  hipLaunchKernelGGL(k_mark_walls, dim3(grid), dim3(64), 0, s, verts, walls, prefix, n_walls, res, inv, tbl,
                      tbl_n, u, st);
   return hipGetLastError();
 }

@@ -83,9 +83,9 @@ hipError_t launch_raytrace(hipStream_t s, int grid, RtView rt, MapView mp, doubl
 
 hipError_t launch_mark_points(hipStream_t s, const float* pts, int64_t stride_bytes, uint32_t n, double inv,
                               UpdView u, Status* st);
-hipError_t launch_mark_walls(hipStream_t s, const double* verts, const int* walls, uint32_t n_walls, int k0,
-                             int k1, double res, double inv, const double* tbl, uint32_t tbl_n, UpdView u,
-                             Status* st);
+hipError_t launch_mark_walls(hipStream_t s, const double* verts, const int4* walls, const uint64_t* prefix,
+                             uint32_t n_walls, uint64_t total, double res, double inv, const double* tbl,
+                             uint32_t tbl_n, UpdView u, Status* st);
 hipError_t launch_set_voxels(hipStream_t s, UpdView u, const uint32_t* list, const uint32_t* eidx,
                              uint32_t count, MapView mp, int op, float val, float t_max, bool clear);
 hipError_t launch_section_pick(hipStream_t s, MapView mp, uint32_t leaf_count, const int* bbox, bool full,

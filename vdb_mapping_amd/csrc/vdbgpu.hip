@@ -168,6 +168,8 @@ struct vdbgpu_map {
   uint32_t art_eidx_cap = 0;
   bool art_present = false;    // m_artificial_areas_present (VM:1529)
   bool art_leaves = false;     // every artificial leaf exists in the map
+  std::vector<double> art_verts;  // walls added since the last restore: vertices (x, y, z)
+  std::vector<int> art_walls;     // and {a, b, k0, k1} per wall
 
   // status: [0] scan, [1] map (leaf_count), [2] auxiliary, [3] cumulative counters
   Status* d_st = nullptr;
@@ -1115,6 +1117,8 @@ int restore_integrity(vdbgpu_map* m) {
   m->art_count = 0;
   m->art_present = false;
   m->art_leaves = false;
+  m->art_verts.clear();
+  m->art_walls.clear();
   return VDBGPU_OK;
 }
 
@@ -1809,69 +1813,133 @@ int32_t vdbgpu_set_points(vdbgpu_map* m, const float* xyz, int64_t n, int64_t st
   return r;
 }
 
-int32_t vdbgpu_add_artificial_areas(vdbgpu_map* m, int32_t n_polygons, const int32_t* counts,
-                                    const double* xyz, double negative_height, double positive_height) {
-  if (!m) return VDBGPU_BAD_ARGUMENT;
-  Guard g(m);
-  if (n_polygons < 0 || (n_polygons && !counts)) return fail(m, VDBGPU_BAD_ARGUMENT, "bad polygon arguments");
+}  // extern "C"
+
+namespace {
+
+// Layer range [(int)(negative_height / res), (int)(positive_height / res))
+// of addArtificialWall (VM:1227-1228).
+int wall_layers(vdbgpu_map* m, double negative_height, double positive_height, int* k0, int* k1) {
   const double qn = negative_height / m->res, qp = positive_height / m->res;
   if (!(qn > -2147483649.0 && qn < 2147483648.0) || !(qp > -2147483649.0 && qp < 2147483648.0))
     return fail(m, VDBGPU_BAD_ARGUMENT, "height / resolution is outside the int range");
-  const int k0 = (int)qn, k1 = (int)qp;  // (int)(height / m_resolution), VM:1227-1228
-  std::vector<int> walls;
-  int64_t nv = 0;
+  *k0 = (int)qn;
+  *k1 = (int)qp;
+  if ((int64_t)*k1 - *k0 > (1 << 24)) return fail(m, VDBGPU_BAD_ARGUMENT, "more than 2^24 height layers");
+  return VDBGPU_OK;
+}
+
+// Cast the walls {a, b, k0, k1} over `verts` into art_tmp (empty before);
+// *count = its listed leaves. On error art_tmp is empty again.
+int build_art(vdbgpu_map* m, const std::vector<double>& verts, const std::vector<int>& walls, uint32_t* count) {
+  *count = 0;
+  const uint32_t n_walls = (uint32_t)(walls.size() / 4);
+  std::vector<uint64_t> prefix(n_walls + 1, 0);
   double ext = 0;
-  for (int32_t p = 0; p < n_polygons; ++p) {
-    const int32_t cnt = counts[p];
-    if (cnt < 0 || (cnt && !xyz)) return fail(m, VDBGPU_BAD_ARGUMENT, "bad polygon vertex count");
-    for (int32_t i = 0; i < cnt; ++i) {
-      const int64_t a = nv + i, b = nv + (i + 1) % cnt;
-      walls.push_back((int)a);
-      walls.push_back((int)b);
-      for (int c = 0; c < 3; ++c) {
-        const double d = std::fabs(xyz[3 * b + c] - xyz[3 * a + c]) * m->inv;
-        if (d > ext || std::isnan(d)) ext = std::isnan(d) ? 0 : d;
-      }
+  for (uint32_t w = 0; w < n_walls; ++w) {
+    prefix[w + 1] = prefix[w] + (uint64_t)std::max(0, walls[4 * w + 3] - walls[4 * w + 2]);
+    for (int c = 0; c < 3; ++c) {
+      const double d = std::fabs(verts[3 * (size_t)walls[4 * w + 1] + c] - verts[3 * (size_t)walls[4 * w] + c]) * m->inv;
+      if (d > ext) ext = d;  // NaN compares false
     }
-    nv += cnt;
-    if (nv > (1 << 28)) return fail(m, VDBGPU_BAD_ARGUMENT, "too many polygon vertices");
   }
-  const uint32_t n_walls = (uint32_t)(walls.size() / 2);
-  const bool marks = n_walls > 0 && k0 < k1;
-  if (marks && (int64_t)k1 - k0 > (1 << 24)) return fail(m, VDBGPU_BAD_ARGUMENT, "more than 2^24 height layers");
+  if (prefix[n_walls] == 0) return VDBGPU_OK;
   int r;
-  CK(hipStreamSynchronize(m->cast));
-  uint32_t count = 0;
-  if (marks) {
-    if ((r = ensure_table(m, (uint32_t)std::min<double>(ext + 2.0, (double)kMaxTableN), m->stream))) return r;
-    double* d_verts = nullptr;
-    int* d_walls = nullptr;
-    if ((r = upload(m, &d_verts, xyz, (size_t)nv * 3)) || (r = upload(m, &d_walls, walls.data(), walls.size())))
-      return r;
-    r = mark_set(m, m->art_tmp, kMinUpdateCap, [&]() -> int {
-      CK(launch_mark_walls(m->stream, d_verts, d_walls, n_walls, k0, k1, m->res, m->inv, m->tbl, m->tbl_n,
-                           upd_view(m->art_tmp), &m->d_st[2]));
-      return VDBGPU_OK;
-    }, &count);
-    dfree(m, &d_verts, (size_t)nv * 3);
-    dfree(m, &d_walls, walls.size());
-    if (r) return r;  // the live artificial grid and the map are unchanged
-  }
-  // restoreMapIntegrity first (VM:1181), then the new grid replaces the old
-  if ((r = restore_integrity(m))) {
-    (void)clear_set(m, m->art_tmp);
+  if ((r = ensure_table(m, (uint32_t)std::min<double>(ext + 2.0, (double)kMaxTableN), m->stream))) return r;
+  double* d_verts = nullptr;
+  int4* d_walls = nullptr;
+  uint64_t* d_prefix = nullptr;
+  if ((r = upload(m, &d_verts, verts.data(), verts.size())) ||
+      (r = upload(m, &d_walls, reinterpret_cast<const int4*>(walls.data()), n_walls)) ||
+      (r = upload(m, &d_prefix, prefix.data(), prefix.size())))
     return r;
-  }
+  r = mark_set(m, m->art_tmp, kMinUpdateCap, [&]() -> int {
+    CK(launch_mark_walls(m->stream, d_verts, d_walls, d_prefix, n_walls, prefix[n_walls], m->res, m->inv, m->tbl,
+                         m->tbl_n, upd_view(m->art_tmp), &m->d_st[2]));
+    return VDBGPU_OK;
+  }, count);
+  dfree(m, &d_verts, verts.size());
+  dfree(m, &d_walls, n_walls);
+  dfree(m, &d_prefix, prefix.size());
+  return r;
+}
+
+// The freshly built art_tmp becomes the artificial grid (the old one, already
+// cleared or about to be, becomes art_tmp).
+int install_art(vdbgpu_map* m, uint32_t count, std::vector<double>&& verts, std::vector<int>&& walls) {
   std::swap(m->art, m->art_tmp);
+  int r;
+  if ((r = clear_set(m, m->art_tmp))) return r;
   m->art_count = count;
-  m->art_present = true;
   m->art_leaves = false;
+  m->art_verts = std::move(verts);
+  m->art_walls = std::move(walls);
   if (count > m->art_eidx_cap) {
     dfree(m, &m->art_eidx, m->art_eidx_cap);
     m->art_eidx_cap = (uint32_t)next_pow2(count);
     CK(dalloc(m, &m->art_eidx, m->art_eidx_cap));
   }
   return VDBGPU_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t vdbgpu_add_artificial_areas(vdbgpu_map* m, int32_t n_polygons, const int32_t* counts,
+                                    const double* xyz, double negative_height, double positive_height) {
+  if (!m) return VDBGPU_BAD_ARGUMENT;
+  Guard g(m);
+  if (n_polygons < 0 || (n_polygons && !counts)) return fail(m, VDBGPU_BAD_ARGUMENT, "bad polygon arguments");
+  int k0, k1, r;
+  if ((r = wall_layers(m, negative_height, positive_height, &k0, &k1))) return r;
+  // addArtificialPolygon (VM:1198-1207): walls i -> (i + 1) mod size
+  std::vector<double> verts;
+  std::vector<int> walls;
+  int64_t nv = 0;
+  for (int32_t p = 0; p < n_polygons; ++p) {
+    const int32_t cnt = counts[p];
+    if (cnt < 0 || (cnt && !xyz)) return fail(m, VDBGPU_BAD_ARGUMENT, "bad polygon vertex count");
+    if (nv + cnt > (1 << 28)) return fail(m, VDBGPU_BAD_ARGUMENT, "too many polygon vertices");
+    for (int32_t i = 0; i < cnt && k0 < k1; ++i) {
+      walls.insert(walls.end(), {(int)(nv + i), (int)(nv + (i + 1) % cnt), k0, k1});
+    }
+    nv += cnt;
+  }
+  verts.assign(xyz ? xyz : nullptr, xyz ? xyz + 3 * nv : nullptr);
+  CK(hipStreamSynchronize(m->cast));
+  uint32_t count = 0;
+  if ((r = build_art(m, verts, walls, &count))) return r;  // the live grid and the map are unchanged
+  // restoreMapIntegrity first (VM:1181), then the new grid replaces the old
+  if ((r = restore_integrity(m))) {
+    (void)clear_set(m, m->art_tmp);
+    return r;
+  }
+  if ((r = install_art(m, count, std::move(verts), std::move(walls)))) return r;
+  m->art_present = true;
+  return VDBGPU_OK;
+}
+
+int32_t vdbgpu_add_artificial_walls(vdbgpu_map* m, int32_t n_walls, const double* xyz, double negative_height,
+                                    double positive_height) {
+  if (!m) return VDBGPU_BAD_ARGUMENT;
+  Guard g(m);
+  if (n_walls < 0 || (n_walls && !xyz) || n_walls > (1 << 26)) return fail(m, VDBGPU_BAD_ARGUMENT, "bad wall arguments");
+  int k0, k1, r;
+  if ((r = wall_layers(m, negative_height, positive_height, &k0, &k1))) return r;
+  std::vector<double> verts = m->art_verts;
+  std::vector<int> walls = m->art_walls;
+  if (verts.size() / 3 + 2 * (size_t)n_walls > (1u << 28)) return fail(m, VDBGPU_BAD_ARGUMENT, "too many walls");
+  for (int32_t w = 0; w < n_walls && k0 < k1; ++w) {
+    const int a = (int)(verts.size() / 3);
+    verts.insert(verts.end(), xyz + 6 * (size_t)w, xyz + 6 * (size_t)w + 6);
+    walls.insert(walls.end(), {a, a + 1, k0, k1});
+  }
+  CK(hipStreamSynchronize(m->cast));
+  uint32_t count = 0;
+  // the grid is rebuilt from every wall since the last restore (marks are an OR)
+  if ((r = build_art(m, verts, walls, &count))) return r;
+  return install_art(m, count, std::move(verts), std::move(walls));
 }
 
 int32_t vdbgpu_restore_map_integrity(vdbgpu_map* m) {
