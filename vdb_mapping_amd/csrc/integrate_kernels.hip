@@ -614,8 +614,72 @@ __device__ __forceinline__ void update_voxel(float& x, uint32_t& ab, int k, bool
 // updates voxels 4l .. 4l+3 with one float4 load and store and takes its
 // four visited / hit / active bits straight from the word (no cross-lane
 // mask transposes); the new active word is OR-reduced over the sixteen
-// lanes. Words with no visited voxel are skipped. Optionally clears the
-// update-set slots afterwards; counts the updated voxels.
+// lanes. Words with no visited voxel are skipped. Each thread carries
+// kApplyWords words per iteration, their loads issued together (list / index,
+// masks, values: three dependent steps each): the kernel is bound by the
+// bytes in flight, not by instruction issue. Optionally clears the update-set slots afterwards; counts the
+// updated voxels.
+struct ApplyWord {
+  uint32_t e, idx, slot;
+  int j;
+  uint64_t vis, hit;
+};
+
+__device__ __forceinline__ void apply_stage1(ApplyWord& w, uint64_t t, uint64_t total,
+                                             const uint32_t* __restrict__ list,
+                                             const uint32_t* __restrict__ eidx) {
+  w.vis = w.hit = 0;
+  w.slot = kNone;
+  w.idx = kNone;
+  w.e = 0;
+  w.j = 0;
+  if (t >= total) return;
+  const uint32_t ws = (uint32_t)(t >> 4);
+  w.e = ws >> 3;
+  w.j = (int)(ws & 7);
+  w.slot = list[w.e];
+  w.idx = eidx[w.e];
+}
+
+__device__ __forceinline__ void apply_stage2(ApplyWord& w, UpdView u) {
+  if (w.slot == kNone) return;
+  const unsigned long long* m = &u.masks[(size_t)w.slot * 16];
+  w.vis = m[w.j];
+  w.hit = m[8 + w.j];
+}
+
+// Clear the word's update-set slot (after every load of the iteration).
+__device__ __forceinline__ void apply_clear(const ApplyWord& w, UpdView u, int l) {
+  if (w.slot == kNone || l != 0) return;
+  unsigned long long* m = &u.masks[(size_t)w.slot * 16];
+  if (w.vis) {
+    m[w.j] = 0;
+    m[8 + w.j] = 0;
+  }
+  if (w.j == 0) u.keys[w.slot] = kEmpty;
+}
+
+// Update the lane's four voxels of word w (values x, active word aw) and store
+// them; `live`: the word has visited voxels and a map leaf.
+__device__ __forceinline__ void apply_finish(const ApplyWord& w, float4 x, uint64_t aw, bool live, MapView mp,
+                                             int l, float l_hit, float l_miss, float t_min, float t_max,
+                                             float c_max, float c_min) {
+  const uint32_t vb = (uint32_t)(w.vis >> (4 * l)) & 15u, hb = (uint32_t)(w.hit >> (4 * l)) & 15u;
+  uint32_t ab = (uint32_t)(aw >> (4 * l)) & 15u;
+  update_voxel(x.x, ab, 0, vb & 1, hb & 1, l_hit, l_miss, t_min, t_max, c_max, c_min);
+  update_voxel(x.y, ab, 1, vb & 2, hb & 2, l_hit, l_miss, t_min, t_max, c_max, c_min);
+  update_voxel(x.z, ab, 2, vb & 4, hb & 4, l_hit, l_miss, t_min, t_max, c_max, c_min);
+  update_voxel(x.w, ab, 3, vb & 8, hb & 8, l_hit, l_miss, t_min, t_max, c_max, c_min);
+  // the 16 bytes: unvisited voxels of it are written back unchanged
+  if (live && vb) reinterpret_cast<float4*>(mp.values)[(size_t)w.idx * 128 + w.j * 16 + l] = x;
+  uint64_t na = (uint64_t)ab << (4 * l);
+#pragma unroll
+  for (int o = 1; o < 16; o <<= 1) na |= __shfl_xor(na, o, 16);
+  if (live && l == 0) mp.active[(size_t)w.idx * 8 + w.j] = na;
+}
+
+constexpr int kApplyWords = 2;  // words in flight per thread
+
 __global__ __launch_bounds__(256) void k_apply(UpdView u, const uint32_t* __restrict__ list,
                                                const uint32_t* __restrict__ eidx, uint32_t count,
                                                MapView mp, float l_hit, float l_miss, float t_min,
@@ -625,43 +689,40 @@ __global__ __launch_bounds__(256) void k_apply(UpdView u, const uint32_t* __rest
   const uint64_t total = (uint64_t)count * 8 * 16;  // (record, word, lane) triples
   const int l = lane_id() & 15;
   unsigned long long vox = 0;
-  for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += G) {
-    // the sixteen lanes of a word group take the same branches below
-    const uint32_t ws = (uint32_t)(t >> 4);
-    const uint32_t e = ws >> 3;
-    const int j = (int)(ws & 7);
-    const uint32_t slot = list ? list[e] : (u.keys[e] != kEmpty ? e : kNone);
-    if (slot == kNone) continue;
-    unsigned long long* w = &u.masks[(size_t)slot * 16];
-    const uint64_t vis = w[j];
-    const uint64_t hit = w[8 + j];
-    if (clear && l == 0) {
-      if (vis) {
-        w[j] = 0;
-        w[8 + j] = 0;
-      }
-      if (j == 0) u.keys[slot] = kEmpty;
-    }
-    if (vis == 0) continue;
-    if (l == 0) vox += __popcll(vis);
-    const uint32_t idx = eidx[e];
-    if (idx == kNone) continue;
-    float4* px = reinterpret_cast<float4*>(mp.values + (size_t)idx * 512 + j * 64) + l;
-    unsigned long long* pa = mp.active + (size_t)idx * 8 + j;
-    const uint64_t aw = *pa;
-    const uint32_t vb = (uint32_t)(vis >> (4 * l)) & 15u, hb = (uint32_t)(hit >> (4 * l)) & 15u;
-    uint32_t ab = (uint32_t)(aw >> (4 * l)) & 15u;
-    // only float4s holding a visited voxel are read and written (partial rows)
-    float4 x = vb ? *px : make_float4(0.f, 0.f, 0.f, 0.f);
-    update_voxel(x.x, ab, 0, vb & 1, hb & 1, l_hit, l_miss, t_min, t_max, c_max, c_min);
-    update_voxel(x.y, ab, 1, vb & 2, hb & 2, l_hit, l_miss, t_min, t_max, c_max, c_min);
-    update_voxel(x.z, ab, 2, vb & 4, hb & 4, l_hit, l_miss, t_min, t_max, c_max, c_min);
-    update_voxel(x.w, ab, 3, vb & 8, hb & 8, l_hit, l_miss, t_min, t_max, c_max, c_min);
-    if (vb) *px = x;  // the 16 bytes: unvisited voxels of it are written back unchanged
-    uint64_t na = (uint64_t)ab << (4 * l);
+  // the sixteen lanes of a word group always hold the same word (G is a
+  // multiple of 16), so their loads and the active-word reduction agree
+  for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += kApplyWords * G) {
+    ApplyWord w[kApplyWords];
 #pragma unroll
-    for (int o = 1; o < 16; o <<= 1) na |= __shfl_xor(na, o, 16);
-    if (l == 0) *pa = na;
+    for (int k = 0; k < kApplyWords; ++k) apply_stage1(w[k], t + k * G, total, list, eidx);
+#pragma unroll
+    for (int k = 0; k < kApplyWords; ++k) apply_stage2(w[k], u);
+    // only float4s holding a visited voxel are read and written (partial rows)
+    float4 x[kApplyWords];
+    uint64_t aw[kApplyWords];
+    bool live[kApplyWords];
+#pragma unroll
+    for (int k = 0; k < kApplyWords; ++k) {
+      live[k] = w[k].vis != 0 && w[k].idx != kNone;
+      x[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+      aw[k] = 0;
+    }
+#pragma unroll
+    for (int k = 0; k < kApplyWords; ++k)
+      if (live[k] && ((w[k].vis >> (4 * l)) & 15u))
+        x[k] = reinterpret_cast<const float4*>(mp.values)[(size_t)w[k].idx * 128 + w[k].j * 16 + l];
+#pragma unroll
+    for (int k = 0; k < kApplyWords; ++k)
+      if (live[k]) aw[k] = mp.active[(size_t)w[k].idx * 8 + w[k].j];
+    if (clear) {
+#pragma unroll
+      for (int k = 0; k < kApplyWords; ++k) apply_clear(w[k], u, l);
+    }
+#pragma unroll
+    for (int k = 0; k < kApplyWords; ++k) {
+      if (l == 0) vox += __popcll(w[k].vis);
+      apply_finish(w[k], x[k], aw[k], live[k], mp, l, l_hit, l_miss, t_min, t_max, c_max, c_min);
+    }
   }
   __shared__ unsigned long long s_vox[4];
 #pragma unroll

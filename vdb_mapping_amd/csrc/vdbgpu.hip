@@ -806,7 +806,7 @@ int run_brick(vdbgpu_map* m, Source& src, const float* xyz, int64_t n, int64_t s
     }
     {
       Timer t(m, kWalk, cs);
-      CK(launch_brick_walk(cs, m->num_cu * 16, m->items, m->segs, m->rayinfo, m->d_scans, m->tbl, bv,
+      CK(launch_brick_walk(cs, m->num_cu * 32, m->items, m->segs, m->rayinfo, m->d_scans, m->tbl, bv,
                            m->shared_bits, rec, m->leaf_mask, m->rec_cap, &m->d_st[0]));
     }
     {
@@ -863,8 +863,10 @@ int run_brick(vdbgpu_map* m, Source& src, const float* xyz, int64_t n, int64_t s
   create_flags(m, &ch, &cm);
   const UpdView rec{m->rkeys[X], m->rmasks[X], 0};
   const int g1 = std::min<int>(4096, (int)((recs + 255) / 256));
-  // 128 lanes per leaf; six workgroups per CU leave wave slots for the next scan's cast
-  const int g2 = std::min<int>(m->num_cu * 6, (int)((recs + 1) / 2));
+  // 128 lanes per leaf; three workgroups per CU (each lane keeps two words in
+  // flight) leave the CUs to the next scan's cast: fewer apply waves slow the
+  // apply less than they speed up the overlapped cast (measured 3-6 per CU)
+  const int g2 = std::min<int>(m->num_cu * 3, (int)((recs + 1) / 2));
   {
     Timer t(m, kCompact);
     CK(launch_map_insert(m->stream, g1, rec, m->rlist[X], recs, map_view(m), m->eidx, ch, cm, m->noop4096,
