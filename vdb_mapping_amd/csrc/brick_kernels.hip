@@ -364,14 +364,67 @@ __global__ __launch_bounds__(256) void k_seg_fill(const int4* __restrict__ rayin
   }
 }
 
-__global__ __launch_bounds__(256) void k_brick_walk(const BrickItem* __restrict__ items,
-                                                    const uint4* __restrict__ segs,
-                                                    const int4* __restrict__ rayinfo,
-                                                    const ScanInfo* __restrict__ scans,
-                                                    const double* __restrict__ tbl, BrickView bv,
-                                                    unsigned long long* __restrict__ shared_bits,
-                                                    UpdView rec, unsigned long long* __restrict__ leaf_mask,
-                                                    uint32_t rec_cap, Status* st) {
+// Walk one segment of the item (record sr, its ray's rayinfo r) into the
+// brick bitmap `bits`. The path is monotone along every axis, so it stays
+// inside the brick when its first and last voxels do: both are checked, and
+// the LDS word index is masked into the brick so no access can leave it.
+__device__ __forceinline__ void walk_segment(const uint4 sr, const int4 r, const ScanInfo* __restrict__ scans,
+                                             const double* __restrict__ tbl, int ox, int oy, int oz,
+                                             unsigned long long* bits, Status* st) {
+  const uint32_t sc = ((uint32_t)r.w >> kScanShift) & kScanMask;
+  const int sx = scans[sc].s[0], sy = scans[sc].s[1], sz = scans[sc].s[2];
+  const uint32_t n[3] = {(uint32_t)abs(r.x - sx), (uint32_t)abs(r.y - sy), (uint32_t)abs(r.z - sz)};
+  const int stx = r.x > sx ? 1 : (r.x < sx ? -1 : 0);
+  const int sty = r.y > sy ? 1 : (r.y < sy ? -1 : 0);
+  const int stz = r.z > sz ? 1 : (r.z < sz ? -1 : 0);
+  const uint32_t c[3] = {sr.y & 0x3FFFu, sr.z, sr.w};
+  const uint32_t m = (sr.y >> 16) & 0x7Fu;
+  int lx = sx + stx * (int)c[0] - ox, ly = sy + sty * (int)c[1] - oy, lz = sz + stz * (int)c[2] - oz;
+  const bool ok0 = (unsigned)lx < (unsigned)kBrickDim && (unsigned)ly < (unsigned)kBrickDim &&
+                   (unsigned)lz < (unsigned)kBrickDim;
+  int cw = -1;
+  uint64_t cb = 0, lastb = 0;
+  IntDda d;
+  d.init(n, c);
+  for (uint32_t k = 0;;) {
+    const int w = (((lx >> 3) & 3) << 7) | (((ly >> 3) & 3) << 5) | (((lz >> 3) & 3) << 3) | (lx & 7);
+    lastb = 1ull << (((ly & 7) << 3) | (lz & 7));
+    if (w != cw) {
+      if (cb) atomicOr(&bits[cw], (unsigned long long)cb);
+      cw = w;
+      cb = 0;
+    }
+    cb |= lastb;
+    if (++k >= m) break;  // m == 0: the endpoint of a zero-length ray
+    const int a = d.next_axis(tbl);
+    d.advance(a);
+    lx += a == 0 ? stx : 0;
+    ly += a == 1 ? sty : 0;
+    lz += a == 2 ? stz : 0;
+  }
+  if (cb) atomicOr(&bits[cw], (unsigned long long)cb);
+  const bool ok1 = (unsigned)lx < (unsigned)kBrickDim && (unsigned)ly < (unsigned)kBrickDim &&
+                   (unsigned)lz < (unsigned)kBrickDim;
+  if (!(ok0 && ok1)) {
+    atomicOr(&st->error, kErrBrick);  // the scan is rejected
+  } else if (sr.y & 0x80000000u) {
+    // setValueOn(end, true) (VDBMapping.hpp:533-536): the last visit is the end voxel
+    atomicOr(&bits[512 + cw], (unsigned long long)lastb);
+  }
+}
+
+constexpr int kWalkThreads = 256;
+constexpr int kSegsPerThread = kItemSegs / kWalkThreads;  // an item's segments, per thread
+static_assert(kSegsPerThread <= 4, "segment lengths are packed one byte per segment");
+
+__global__ __launch_bounds__(kWalkThreads, 8) void k_brick_walk(const BrickItem* __restrict__ items,
+                                                             const uint4* __restrict__ segs,
+                                                             const int4* __restrict__ rayinfo,
+                                                             const ScanInfo* __restrict__ scans,
+                                                             const double* __restrict__ tbl, BrickView bv,
+                                                             unsigned long long* __restrict__ shared_bits,
+                                                             UpdView rec, unsigned long long* __restrict__ leaf_mask,
+                                                             uint32_t rec_cap, Status* st) {
   __shared__ unsigned long long bits[kBrickWords];
   __shared__ uint32_t s_pos[64];
   __shared__ uint32_t s_hist[128];
@@ -383,96 +436,64 @@ __global__ __launch_bounds__(256) void k_brick_walk(const BrickItem* __restrict_
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(&st->overflow, kOvfRecords);
     return;
   }
+  const uint32_t t = threadIdx.x;
   // static assignment: item it -> workgroup it % grid (no shared queue counter)
   for (uint32_t it = blockIdx.x; it < n_items; it += gridDim.x) {
     const BrickItem item = items[it];
     int b[3];
     brick_coords(bv.keys[item.slot], b);
     const int ox = b[0] * kBrickDim, oy = b[1] * kBrickDim, oz = b[2] * kBrickDim;
-    for (int w = threadIdx.x; w < kBrickWords; w += blockDim.x) bits[w] = 0;
-    if (threadIdx.x < 128) s_hist[threadIdx.x] = 0;
-    __syncthreads();
+    for (int w = t; w < kBrickWords; w += kWalkThreads) bits[w] = 0;
+    if (t < 128) s_hist[t] = 0;
     // Counting sort of the item's segments by visit count, longest first, so
-    // the lanes of a wave walk segments of about the same length.
+    // the lanes of a wave walk segments of about the same length. Segment k
+    // belongs to thread k % 256; its length is loaded once, for both passes.
     const uint32_t nseg = item.seg1 - item.seg0;  // <= kItemSegs
-    for (uint32_t k = threadIdx.x; k < nseg; k += blockDim.x)
-      atomicAdd(&s_hist[127 - ((segs[item.seg0 + k].y >> 16) & 0x7Fu)], 1u);
+    uint32_t lens = 0;  // 7-bit lengths of the thread's segments, one byte each
+#pragma unroll
+    for (int r = 0; r < kSegsPerThread; ++r) {
+      const uint32_t k = t + r * kWalkThreads;
+      if (k < nseg) lens |= ((segs[item.seg0 + k].y >> 16) & 0x7Fu) << (8 * r);
+    }
     __syncthreads();
-    if (threadIdx.x < 64) {  // exclusive scan of the 128 bins, two per lane
-      const uint32_t h0 = s_hist[2 * threadIdx.x], h1 = s_hist[2 * threadIdx.x + 1];
+#pragma unroll
+    for (int r = 0; r < kSegsPerThread; ++r)
+      if (t + r * kWalkThreads < nseg) atomicAdd(&s_hist[127 - ((lens >> (8 * r)) & 0xFFu)], 1u);
+    __syncthreads();
+    if (t < 64) {  // exclusive scan of the 128 bins, two per lane
+      const uint32_t h0 = s_hist[2 * t], h1 = s_hist[2 * t + 1];
       uint32_t v = h0 + h1;
 #pragma unroll
       for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t t = __shfl_up(v, o, 64);
-        if ((int)threadIdx.x >= o) v += t;
+        const uint32_t u = __shfl_up(v, o, 64);
+        if ((int)t >= o) v += u;
       }
       const uint32_t ex = v - h0 - h1;
-      s_hist[2 * threadIdx.x] = ex;
-      s_hist[2 * threadIdx.x + 1] = ex + h0;
+      s_hist[2 * t] = ex;
+      s_hist[2 * t + 1] = ex + h0;
     }
     __syncthreads();
-    for (uint32_t k = threadIdx.x; k < nseg; k += blockDim.x) {
-      const uint32_t pos = atomicAdd(&s_hist[127 - ((segs[item.seg0 + k].y >> 16) & 0x7Fu)], 1u);
-      s_order[pos] = (uint16_t)k;
+#pragma unroll
+    for (int r = 0; r < kSegsPerThread; ++r) {
+      const uint32_t k = t + r * kWalkThreads;
+      if (k < nseg) s_order[atomicAdd(&s_hist[127 - ((lens >> (8 * r)) & 0xFFu)], 1u)] = (uint16_t)k;
     }
     __syncthreads();
-    for (uint32_t k = threadIdx.x; k < nseg; k += blockDim.x) {
+#pragma unroll 1
+    for (uint32_t k = t; k < nseg; k += kWalkThreads) {
       const uint4 sr = segs[item.seg0 + s_order[k]];
-      const int4 r = rayinfo[sr.x];
-      const uint32_t sc = ((uint32_t)r.w >> kScanShift) & kScanMask;
-      const int sx = scans[sc].s[0], sy = scans[sc].s[1], sz = scans[sc].s[2];
-      const uint32_t n[3] = {(uint32_t)abs(r.x - sx), (uint32_t)abs(r.y - sy), (uint32_t)abs(r.z - sz)};
-      const int stx = r.x > sx ? 1 : (r.x < sx ? -1 : 0);
-      const int sty = r.y > sy ? 1 : (r.y < sy ? -1 : 0);
-      const int stz = r.z > sz ? 1 : (r.z < sz ? -1 : 0);
-      const uint32_t c[3] = {sr.y & 0x3FFFu, sr.z, sr.w};
-      const uint32_t m = (sr.y >> 16) & 0x7Fu;
-      int lx = sx + stx * (int)c[0] - ox, ly = sy + sty * (int)c[1] - oy, lz = sz + stz * (int)c[2] - oz;
-      // The path is monotone along every axis, so it stays inside the brick
-      // when its first and last voxels do: both are checked, and the LDS
-      // word index is masked into the brick so no access can leave it.
-      const bool ok0 = (unsigned)lx < (unsigned)kBrickDim && (unsigned)ly < (unsigned)kBrickDim &&
-                       (unsigned)lz < (unsigned)kBrickDim;
-      int cw = -1;
-      uint64_t cb = 0, lastb = 0;
-      IntDda d;
-      d.init(n, c);
-      for (uint32_t k = 0;;) {
-        const int w = (((lx >> 3) & 3) << 7) | (((ly >> 3) & 3) << 5) | (((lz >> 3) & 3) << 3) | (lx & 7);
-        lastb = 1ull << (((ly & 7) << 3) | (lz & 7));
-        if (w != cw) {
-          if (cb) atomicOr(&bits[cw], (unsigned long long)cb);
-          cw = w;
-          cb = 0;
-        }
-        cb |= lastb;
-        if (++k >= m) break;  // m == 0: the endpoint of a zero-length ray
-        const int a = d.next_axis(tbl);
-        d.advance(a);
-        lx += a == 0 ? stx : 0;
-        ly += a == 1 ? sty : 0;
-        lz += a == 2 ? stz : 0;
-      }
-      if (cb) atomicOr(&bits[cw], (unsigned long long)cb);
-      const bool ok1 = (unsigned)lx < (unsigned)kBrickDim && (unsigned)ly < (unsigned)kBrickDim &&
-                       (unsigned)lz < (unsigned)kBrickDim;
-      if (!(ok0 && ok1)) {
-        atomicOr(&st->error, kErrBrick);  // the scan is rejected
-      } else if (sr.y & 0x80000000u) {
-        // setValueOn(end, true) (VDBMapping.hpp:533-536): the last visit is the end voxel
-        atomicOr(&bits[512 + cw], (unsigned long long)lastb);
-      }
+      walk_segment(sr, rayinfo[sr.x], scans, tbl, ox, oy, oz, bits, st);
     }
     __syncthreads();
     if (item.shared == kNone) {
       emit_leaves(bits, ox, oy, oz, rec, it * 64, s_pos, &leaf_mask[it]);
     } else {
       unsigned long long* dst = shared_bits + (size_t)item.shared * kBrickWords;
-      for (int w = threadIdx.x; w < kBrickWords; w += blockDim.x) {
+      for (int w = t; w < kBrickWords; w += kWalkThreads) {
         const uint64_t v = bits[w];
         if (v) atomicOr(&dst[w], (unsigned long long)v);
       }
-      if (threadIdx.x == 0) leaf_mask[it] = 0;  // its records come from k_split_emit
+      if (t == 0) leaf_mask[it] = 0;  // its records come from k_split_emit
     }
     __syncthreads();
   }
@@ -582,7 +603,7 @@ hipError_t launch_brick_walk(hipStream_t s, int grid, const BrickItem* items, co
                              const int4* rayinfo, const ScanInfo* scans, const double* tbl,
                              BrickView bv, unsigned long long* shared_bits, UpdView rec,
                              unsigned long long* leaf_mask, uint32_t rec_cap, Status* st) {
-  hipLaunchKernelGGL(k_brick_walk, dim3(grid), dim3(256), 0, s, items, segs, rayinfo, scans, tbl, bv,
+  hipLaunchKernelGGL(k_brick_walk, dim3(grid), dim3(kWalkThreads), 0, s, items, segs, rayinfo, scans, tbl, bv,
                      shared_bits, rec, leaf_mask, rec_cap, st);
   return hipGetLastError();
 }
