@@ -765,6 +765,26 @@ __global__ __launch_bounds__(256) void k_rehash_map(MapView mp, uint32_t leaf_co
   }
 }
 
+// Per-scan set-up of a single-scan cast in one launch: the scan record and
+// the size block (by value, so no host-to-device copy is queued behind the
+// previous scan's map update), a zeroed scan status, and optionally the
+// brick table cleared (keys EMPTY, counts and cursors 0).
+__global__ void k_stage_scan(ScanInfo sc, BatchCtl ctl, ScanInfo* __restrict__ d_scans,
+                             BatchCtl* __restrict__ d_ctl, Status* st, unsigned long long* bkeys,
+                             unsigned long long* bcnt, uint64_t nb) {
+  const uint64_t i0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i0 == 0) {
+    d_scans[0] = sc;
+    *d_ctl = ctl;
+  }
+  if (i0 < sizeof(Status) / 4) reinterpret_cast<uint32_t*>(st)[i0] = 0u;
+  const uint64_t G = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = i0; i < nb; i += G) {
+    bkeys[i] = kEmpty;
+    bcnt[i] = 0;
+  }
+}
+
 __global__ void k_fill_u64(unsigned long long* p, uint64_t n, unsigned long long v) {
   const uint64_t G = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += G) p[i] = v;
@@ -893,6 +913,15 @@ hipError_t launch_rehash_update(hipStream_t s, int grid, UpdView from, uint32_t 
 
 hipError_t launch_rehash_map(hipStream_t s, int grid, MapView mp, uint32_t leaf_count) {
   hipLaunchKernelGGL(k_rehash_map, dim3(grid), dim3(256), 0, s, mp, leaf_count);
+  return hipGetLastError();
+}
+
+hipError_t launch_stage_scan(hipStream_t s, const ScanInfo& sc, const BatchCtl& ctl, ScanInfo* d_scans,
+                             BatchCtl* d_ctl, Status* st, unsigned long long* bkeys, unsigned long long* bcnt,
+                             uint64_t nb) {
+  const uint64_t blocks = (nb + 255) / 256;
+  const int grid = (int)(blocks < 1 ? 1 : (blocks < 4096 ? blocks : 4096));
+  hipLaunchKernelGGL(k_stage_scan, dim3(grid), dim3(256), 0, s, sc, ctl, d_scans, d_ctl, st, bkeys, bcnt, nb);
   return hipGetLastError();
 }
 

@@ -50,6 +50,9 @@ hipError_t launch_rehash_update(hipStream_t s, int grid, UpdView from, uint32_t 
                                 UpdView to, Status* st);
 hipError_t launch_rehash_map(hipStream_t s, int grid, MapView mp, uint32_t leaf_count);
 hipError_t launch_fill_u64(hipStream_t s, unsigned long long* p, uint64_t n, uint64_t v);
+hipError_t launch_stage_scan(hipStream_t s, const ScanInfo& sc, const BatchCtl& ctl, ScanInfo* d_scans,
+                             BatchCtl* d_ctl, Status* st, unsigned long long* bkeys, unsigned long long* bcnt,
+                             uint64_t nb);
 hipError_t launch_dirty_list(hipStream_t s, int grid, uint8_t* dirty, uint32_t leaf_count,
                              uint32_t* list, Status* st);
 hipError_t launch_gather_leaves(hipStream_t s, int grid, const uint32_t* list, uint32_t count,

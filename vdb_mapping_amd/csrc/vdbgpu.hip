@@ -483,7 +483,7 @@ int launch_fold_stage(vdbgpu_map* m, Source& src, bool from_groups) {
 // visit bound of the scans in *visits_bound.
 int stage_scans(vdbgpu_map* m, hipStream_t stream, const float* xyz, int64_t stride, bool on_device,
                 const std::vector<ScanInfo>& scans, uint32_t n_total, bool batched,
-                uint64_t* visits_bound) {
+                uint64_t* visits_bound, bool clear_bricks = false) {
   int r;
   if ((r = ensure_rays(m, n_total))) return r;
   uint32_t ext_max = 1;
@@ -509,11 +509,24 @@ int stage_scans(vdbgpu_map* m, hipStream_t stream, const float* xyz, int64_t str
     pts = m->d_points;
   }
   const uint32_t nb = (n_total + kPrepBlock - 1) / kPrepBlock;
-  std::copy(scans.begin(), scans.end(), m->h_scans);
-  *m->h_ctl = BatchCtl{n_total, nb, (uint32_t)scans.size(), batched ? 1u : 0u};
-  CK(hipMemcpyAsync(m->d_scans, m->h_scans, scans.size() * sizeof(ScanInfo), hipMemcpyHostToDevice, stream));
-  CK(hipMemcpyAsync(m->d_ctl, m->h_ctl, sizeof(BatchCtl), hipMemcpyHostToDevice, stream));
-  CK(hipMemsetAsync(&m->d_st[0], 0, sizeof(Status), stream));
+  const BatchCtl ctl{n_total, nb, (uint32_t)scans.size(), batched ? 1u : 0u};
+  if (scans.size() == 1) {
+    // one launch: scan record and sizes by value, status zeroed, brick table cleared
+    CK(launch_stage_scan(stream, scans[0], ctl, m->d_scans, m->d_ctl, &m->d_st[0],
+                         clear_bricks ? m->bkeys : nullptr,
+                         clear_bricks ? reinterpret_cast<unsigned long long*>(m->bcnt) : nullptr,
+                         clear_bricks ? m->bcap : 0));
+  } else {
+    std::copy(scans.begin(), scans.end(), m->h_scans);
+    *m->h_ctl = ctl;
+    CK(hipMemcpyAsync(m->d_scans, m->h_scans, scans.size() * sizeof(ScanInfo), hipMemcpyHostToDevice, stream));
+    CK(hipMemcpyAsync(m->d_ctl, m->h_ctl, sizeof(BatchCtl), hipMemcpyHostToDevice, stream));
+    CK(hipMemsetAsync(&m->d_st[0], 0, sizeof(Status), stream));
+    if (clear_bricks) {
+      CK(launch_fill_u64(stream, m->bkeys, m->bcap, kEmpty));
+      CK(launch_fill_u64(stream, reinterpret_cast<unsigned long long*>(m->bcnt), m->bcap, 0));
+    }
+  }
   const bool vec4 = stride == 16 && (reinterpret_cast<uintptr_t>(pts) & 15) == 0;
   Timer t(m, kPrep, stream);
   CK(launch_prep(stream, (int)nb, pts, stride, vec4, m->d_scans, m->d_ctl, m->res, m->inv, m->tbl_n,
@@ -777,16 +790,18 @@ int run_brick(vdbgpu_map* m, Source& src, const float* xyz, int64_t n, int64_t s
   sc.ray0 = 0;
   sc.nrays = (uint32_t)n;
   int r;
-  uint64_t vb = 0;
   hipStream_t cs = m->cast;
-  if ((r = stage_scans(m, cs, xyz, stride, on_device, {sc}, (uint32_t)n, false, &vb))) return r;
-  if (m->bcap == 0) {
+  if (m->bcap == 0) {  // first brick-path scan: tables sized from the scan's visit bound
+    const double ext = std::min<double>(std::ceil(sc.R * m->inv) + 2.0, (double)kMaxTableN);
+    const uint64_t vb = (uint64_t)n * (1 + 3 * (uint64_t)ext);
     const uint64_t segs0 = std::min<uint64_t>(vb, 16ull * (uint64_t)n);
     if ((r = ensure_bricks(m, (uint32_t)std::max<uint64_t>(4096, next_pow2((uint64_t)n / 2)),
                            (uint32_t)std::max<uint64_t>(1u << 16, std::min<uint64_t>(segs0, 0xC0000000ull)), 256,
                            (uint32_t)std::max<uint64_t>(1u << 16, std::min<uint64_t>(8ull * n, 0xC0000000ull)))))
       return r;
   }
+  uint64_t vb = 0;
+  if ((r = stage_scans(m, cs, xyz, stride, on_device, {sc}, (uint32_t)n, false, &vb, true))) return r;
   const int nb = (int)((n + kPrepBlock - 1) / kPrepBlock);
   // record set X: free once the map stream has applied the scan that last used it
   const int X = m->rec_cur;
@@ -794,8 +809,10 @@ int run_brick(vdbgpu_map* m, Source& src, const float* xyz, int64_t n, int64_t s
   for (int attempt = 0;; ++attempt) {
     const BrickView bv{m->bkeys, m->bcnt, m->bcnt + m->bcap, m->bcap - 1};
     const UpdView rec{m->rkeys[X], m->rmasks[X], 0};
-    CK(launch_fill_u64(cs, m->bkeys, m->bcap, kEmpty));
-    CK(launch_fill_u64(cs, reinterpret_cast<unsigned long long*>(m->bcnt), m->bcap, 0));
+    if (attempt > 0) {  // the first attempt's table was cleared with the staging
+      CK(launch_fill_u64(cs, m->bkeys, m->bcap, kEmpty));
+      CK(launch_fill_u64(cs, reinterpret_cast<unsigned long long*>(m->bcnt), m->bcap, 0));
+    }
     {
       Timer t(m, kTraverse, cs);
       CK(launch_seg_count(cs, nb, m->rayinfo, m->d_scans, m->d_ctl, m->tbl, bv, &m->d_st[0]));
