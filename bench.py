@@ -15,6 +15,8 @@ Other modes (--mode, not the default line) measure the north-star layers:
   mixed  64 scans of 10k-300k points with random poses and ranges: one batch vs per-scan
   rgbd   200 RGB-D frames at 0.02 m, then 1M raytrace queries (device arrays)
   serve  16 robot streams, Poisson arrivals at 10 Hz each, one map per stream
+  fast   the outdoor stream in fast mode (castRayIntoGridFast): only voxels already
+         active in the map are re-marked along each ray, so the update follows the map
 """
 from __future__ import annotations
 
@@ -42,7 +44,8 @@ def parse():
     p.add_argument("--no-profiling", action="store_true", help="do not time kernels with HIP events")
     p.add_argument("--tuning", type=str, default="",
                    help="block,steps_per_lane,waves_per_cu,load_factor[,traversal]")
-    p.add_argument("--mode", choices=("plain", "batch", "mixed", "rgbd", "serve", "copy"), default="plain")
+    p.add_argument("--mode", choices=("plain", "batch", "mixed", "rgbd", "serve", "copy", "fast"),
+                   default="plain")
     p.add_argument("--copy-mib", type=int, default=1024, help="copy mode: bytes per copy (MiB)")
     p.add_argument("--batch", type=int, default=8, help="scans per batch (batch mode)")
     p.add_argument("--autotune", action="store_true", help="mixed mode: autotune first")
@@ -132,6 +135,45 @@ def mode_batch(a, V, W):
                       "speedup_vs_plain": round(out["plain"] / out["batch"], 3), "batch_scans": a.batch,
                       "scans": a.scans,
                       "same_leaf_count": out["plain_leaves"] == out["batch_leaves"] == out["fold_leaves"],
+                      "config": {"workload": "outdoor_stream_262k_0.05m", "input": "device-resident"}}),
+          flush=True)
+
+
+def mode_fast(a, V, W):
+    """Outdoor stream in fast mode vs the normal path, each from an empty map:
+    every scan inserted back to back; the first scan of a fast-mode map marks
+    endpoints only (the map is empty), later scans walk the topology snapshot
+    of the previous integrate."""
+    w = W.outdoor_stream_workload(a.scans)
+    scans = [w.scan(i) for i in range(a.scans)]
+    dev, offs = upload_scans(V, scans)
+    n_pts = scans[0].points.shape[0]
+    rays = sum(int((~np.isnan(s.points[:, :3]).any(1)).sum()) for s in scans)
+    out = {}
+    for path in ("normal", "fast"):
+        g = V.GpuMap(w.resolution)
+        g.set_config(V.make_config(max_range=w.max_range, fast_mode=(path == "fast"), **w.probs))
+        g.add_input_source("lidar", w.max_range)
+        for j in range(min(5, a.scans)):  # warm: size every table, then start from an empty map
+            g.insert_device("lidar", dev.address + offs[j], n_pts, 16, scans[j].origin, check=True)
+        g.reset()
+        g.synchronize()
+        v0 = g.stats()["voxel_updates"]
+        t0 = time.perf_counter()
+        for j, s in enumerate(scans):
+            g.insert_device("lidar", dev.address + offs[j], n_pts, 16, s.origin, check=True)
+        g.synchronize()
+        out[path] = time.perf_counter() - t0
+        out[path + "_voxels"] = int(g.stats()["voxel_updates"] - v0)
+        out[path + "_leaves"] = g.leaf_count()
+        g.close()
+    print(json.dumps({"metric": "fast-mode integration of the outdoor stream", "value": round(rays / out["fast"], 1),
+                      "unit": "rays/s", "normal_rays_per_s": round(rays / out["normal"], 1),
+                      "fast_ms_per_scan": round(out["fast"] * 1e3 / a.scans, 3),
+                      "normal_ms_per_scan": round(out["normal"] * 1e3 / a.scans, 3),
+                      "fast_voxel_updates": out["fast_voxels"], "normal_voxel_updates": out["normal_voxels"],
+                      "fast_map_leaves": out["fast_leaves"], "normal_map_leaves": out["normal_leaves"],
+                      "scans": a.scans,
                       "config": {"workload": "outdoor_stream_262k_0.05m", "input": "device-resident"}}),
           flush=True)
 
@@ -301,7 +343,7 @@ def main():
         sys.exit(1)
     if a.mode != "plain":
         {"batch": mode_batch, "mixed": mode_mixed, "rgbd": mode_rgbd, "serve": mode_serve,
-         "copy": mode_copy}[a.mode](a, V, W)
+         "copy": mode_copy, "fast": mode_fast}[a.mode](a, V, W)
         return
     w = W.outdoor_stream_workload(a.scans)
     t_gen = time.perf_counter()
