@@ -24,8 +24,13 @@
 // / UpdateGrid :811-1085, without morphological smoothing) run in the library
 // too (vdbgpu_set_points, vdbgpu_*artificial*, vdbgpu_*section*).
 //
-// Not provided: persistence and PCD I/O (:193-307) and the byte-array wire
-// format (:1246-1338), which need OpenVDB's io::Stream; the morphology
+// PCD persistence (saveMapToPCD, loadMapFromPCD, createMapFromPointCloud
+// :218-256, :295-307, OccupancyVDBMapping.hpp:136-174) exports and loads through
+// the library (vdbgpu_export_active_points, vdbgpu_create_from_points).
+//
+// Not provided: .vdb persistence (saveMap / loadMap :193-211, :263-284) and
+// the byte-array wire format (:1246-1338), which need OpenVDB's io::File /
+// io::Stream; the morphology
 // operators (:1095-1140, OpenVDB tools); castRayIntoGrid (:550-566, a
 // per-ray host call) and updateMap (:731-792, reached through
 // integrateUpdate).
@@ -36,8 +41,12 @@
 #include <array>
 #include <cfloat>
 #include <cmath>
+#include <chrono>
 #include <cstring>
+#include <ctime>
+#include <iomanip>
 #include <iostream>
+#include <sstream>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -52,6 +61,7 @@
 #include "../vdbgpu.h"
 #include "vdb_mapping/compat.hpp"
 #include "vdb_mapping/grid.hpp"
+#include "vdb_mapping/pcd.hpp"
 
 namespace vdb_mapping {
 
@@ -327,6 +337,84 @@ public:
    * reference offers no way to wait for its threads).
    */
   void flushAccumulation() { report(vdbgpu_async_flush(m_map)); }
+
+  /*!
+   * \brief Writes the active voxel centres to
+   * <map_directory_path><date>_active_values_map.pcd (reference :218-256),
+   * in PCL's ASCII PCD layout.
+   */
+  bool saveMapToPCD()
+  {
+    auto timestamp     = std::chrono::system_clock::now();
+    std::time_t now_tt = std::chrono::system_clock::to_time_t(timestamp);
+    std::tm tm         = *std::localtime(&now_tt);
+    std::stringstream sstime;
+    sstime << std::put_time(&tm, "%Y-%m-%d_%H-%M-%S");
+    std::string pcd_path = m_map_directory_path + sstime.str() + "_active_values_map.pcd";
+    std::vector<float> xyz;
+    {
+      std::shared_lock map_lock(*m_map_mutex);
+      const int64_t n = vdbgpu_export_active_points(m_map, nullptr, 0);
+      if (n > 0)
+      {
+        xyz.resize(3 * n);
+        if (vdbgpu_export_active_points(m_map, xyz.data(), n) != n)
+        {
+          xyz.clear();
+        }
+      }
+    }
+    if (!pcd::writeAscii(pcd_path, xyz))
+    {
+      std::cerr << "Could not write PCD file." << std::endl;
+      return false;
+    }
+    std::cout << "Wrote pcd to: " << pcd_path << std::endl;
+    return true;
+  }
+
+  /*!
+   * \brief Loads a PCD file into the map (reference :295-307): ASCII or binary
+   * (uncompressed) files with float x, y, z fields.
+   */
+  bool loadMapFromPCD(const std::string& file_path, const bool set_background, const bool clear_map)
+  {
+    PointCloudT::Ptr cloud(new PointCloudT);
+    std::vector<float> xyz;
+    if (!pcd::read(file_path, xyz))
+    {
+      std::cerr << "Could not open PCD file" << std::endl;
+      return false;
+    }
+    for (size_t i = 0; i < xyz.size() / 3; ++i)
+    {
+      cloud->points.emplace_back(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]);
+    }
+    std::unique_lock map_lock(*m_map_mutex);
+    createMapFromPointCloud(cloud, set_background, clear_map);
+    map_lock.unlock();
+    return true;
+  }
+
+  /*!
+   * \brief Builds the map from points (reference OccupancyVDBMapping.hpp:136-174):
+   * every point's voxel occupied (max log-odds, active), optionally the
+   * inactive voxels of the active bounding box set to min log-odds. The final
+   * pruneGrid of the reference is not applied (uniform leaves stay leaves).
+   */
+  virtual void createMapFromPointCloud(const PointCloudT::Ptr& cloud,
+                                       const bool set_background,
+                                       const bool clear_map)
+  {
+    const float* xyz = cloud->points.empty() ? nullptr : &cloud->points[0].x;
+    report(vdbgpu_create_from_points(m_map,
+                                     xyz,
+                                     static_cast<int64_t>(cloud->points.size()),
+                                     static_cast<int64_t>(sizeof(PointT)),
+                                     set_background ? 1 : 0,
+                                     clear_map ? 1 : 0));
+    m_vdb_grid->markStale();
+  }
 
   /*!
    * \brief Sets the voxel of every point free (min log-odds, inactive),

@@ -321,6 +321,23 @@ int32_t vdbgpu_apply_section(vdbgpu_map* map, int64_t n, const int32_t* origins,
  * value 0). */
 int32_t vdbgpu_apply_section_update(vdbgpu_map* map, const int32_t bbox[6], int64_t n,
                                     const int32_t* origins, const uint64_t* masks);
+/* ---- persistence ---------------------------------------------------------------
+ * The point list of saveMapToPCD (VM:218-256): every active voxel at
+ * indexToWorld(ijk) + 0.5 resolution, as float xyz, in OpenVDB's cbeginValueOn
+ * order (root children by origin, 128^3 nodes and leaves by offset, voxels by
+ * offset). A null output returns the count, a short cap -count, an error -1. */
+int64_t vdbgpu_export_active_points(vdbgpu_map* map, float* xyz, int64_t cap);
+/* createMapFromPointCloud (OCC:136-174), the map half of loadMapFromPCD: with
+ * clear_map the grid is emptied (update sets kept); every point's voxel
+ * floor(p / resolution) becomes (max log-odds, active); with set_background
+ * every inactive voxel of the active-voxel bounding box gets min log-odds.
+ * The reference's final pruneGrid (uniform leaves collapsed into tiles) is
+ * not applied: such leaves stay leaves with the same voxel values and states.
+ * A NaN or out-of-range point rejects the call before any change; a
+ * background box of more than 2^27 leaves returns VDBGPU_UNSUPPORTED. */
+int32_t vdbgpu_create_from_points(vdbgpu_map* map, const float* xyz, int64_t n, int64_t stride_bytes,
+                                  int32_t set_background, int32_t clear_map);
+
 /* createIndexBoundingBox (VM:811-874): tf is the row-major 4x4 map-to-reference
  * transform; out = {min x, y, z, max x, y, z}. */
 int32_t vdbgpu_index_bbox(double resolution, const double min_boundary[3],

@@ -97,6 +97,9 @@ def lib():
         L.orc_apply_section.argtypes = [P, I64, P, P, P]
         L.orc_apply_section_update.argtypes = [P, P, I64, P, P]
         L.orc_index_bbox.argtypes = [D, P, P, P, P]
+        L.orc_export_active_points.restype = I64
+        L.orc_export_active_points.argtypes = [P, P, I64]
+        L.orc_create_from_points.argtypes = [P, P, I64, I64, I32, I32]
         _lib = L
     return _lib
 
@@ -277,6 +280,17 @@ class OracleMap:
         o = np.ascontiguousarray(origins, dtype=np.int32).reshape(-1, 3)
         m = np.ascontiguousarray(masks, dtype=np.uint64).reshape(-1, 8)
         return lib().orc_apply_section_update(self.h, _ptr(b), o.shape[0], _ptr(o), _ptr(m))
+
+    def export_active_points(self) -> np.ndarray:
+        n = lib().orc_export_active_points(self.h, None, 0)
+        out = np.zeros((n, 3), np.float32)
+        assert lib().orc_export_active_points(self.h, _ptr(out), n) == n
+        return out
+
+    def create_map_from_points(self, points, set_background: bool, clear_map: bool) -> int:
+        pts, stride = self._pts(points)
+        return lib().orc_create_from_points(self.h, _ptr(pts), pts.shape[0], stride, int(bool(set_background)),
+                                            int(bool(clear_map)))
 
     def raytrace(self, origins, dirs, lengths):
         o = np.ascontiguousarray(origins, dtype=np.float64).reshape(-1, 3)

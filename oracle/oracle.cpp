@@ -1226,4 +1226,117 @@ void orc_index_bbox(double res, const double mn[3], const double mx[3], const do
   }
 }
 
+// ------------------------------------------------------------ persistence --
+
+namespace {
+
+// Position of a leaf in OpenVDB's tree iteration order: root children by
+// origin (x, then y, then z), then the 128^3 nodes and the leaves inside
+// each by offset (x-major).
+std::array<int32_t, 9> tree_order(uint64_t k) {
+  int32_t o[3];
+  key_origin(k, o);
+  return {o[0] >> 12, o[1] >> 12, o[2] >> 12, o[0] >> 7, o[1] >> 7, o[2] >> 7, o[0] >> 3, o[1] >> 3, o[2] >> 3};
+}
+
+}  // namespace
+
+// The point list of saveMapToPCD (VM:218-256): every active voxel in
+// cbeginValueOn order (tree order, then voxel offset within the leaf), at
+// indexToWorld(ijk) + 0.5 res (ScaleMap::applyMap multiplies by res [U1]),
+// rounded to float. A null output returns the count; a short cap -count.
+int64_t orc_export_active_points(void* h, float* xyz, int64_t cap) {
+  Map& m = *(Map*)h;
+  std::vector<std::pair<std::array<int32_t, 9>, uint64_t>> order;
+  int64_t n = 0;
+  for (auto& kv : m.leaves) {
+    order.emplace_back(tree_order(kv.first), kv.first);
+    for (int w = 0; w < 8; ++w) n += __builtin_popcountll(kv.second.act[w]);
+  }
+  if (!xyz) return n;
+  if (n > cap) return -n;
+  std::sort(order.begin(), order.end());
+  int64_t i = 0;
+  for (auto& ok : order) {
+    const Leaf& l = m.leaves.at(ok.second);
+    int32_t o[3];
+    key_origin(ok.second, o);
+    for (int v = 0; v < 512; ++v) {
+      if (!((l.act[v >> 6] >> (v & 63)) & 1)) continue;
+      const int32_t c[3] = {o[0] + (v >> 6), o[1] + ((v >> 3) & 7), o[2] + (v & 7)};
+      for (int a = 0; a < 3; ++a) {
+        double w = (double)c[a] * m.res;
+        w += 0.5 * m.res;
+        xyz[3 * i + a] = (float)w;
+      }
+      ++i;
+    }
+  }
+  return n;
+}
+
+// createMapFromPointCloud (OCC:136-174) without its final pruneGrid: clear
+// the grid (not the update grids) if asked, setValueOn(floor(worldToIndex(p)),
+// max log-odds) per point [U1], then with set_background every voxel of the
+// active-voxel bounding box that is not active gets setValueOff(min log-odds)
+// (a missing leaf is created when that value differs from the background 0).
+// NaN or out-of-range points reject the call before any change.
+int orc_create_from_points(void* h, const float* xyz, int64_t n, int64_t stride, int32_t set_background,
+                           int32_t clear_map) {
+  Map& m = *(Map*)h;
+  if (n < 0 || stride < 12 || (stride & 3) || (n && !xyz)) return ORC_BAD_ARGUMENT;
+  const char* base = (const char*)xyz;
+  for (int64_t i = 0; i < n; ++i) {
+    const float* pt = (const float*)(base + i * stride);
+    for (int a = 0; a < 3; ++a) {
+      const double f = std::floor((double)pt[a] * m.inv);
+      if (!(f >= -(double)kCoordLimit && f < (double)kCoordLimit)) return ORC_OUT_OF_RANGE;
+    }
+  }
+  if (clear_map) {
+    m.leaves.clear();
+    m.n4096.clear();
+  }
+  for (int64_t i = 0; i < n; ++i) {
+    const float* pt = (const float*)(base + i * stride);
+    int32_t c[3];
+    for (int a = 0; a < 3; ++a) c[a] = (int32_t)std::floor((double)pt[a] * m.inv);
+    m.n4096.insert(key4096(c[0], c[1], c[2]));
+    Leaf& leaf = touch_leaf(m, leaf_key(c[0], c[1], c[2]));
+    const int off = leaf_offset(c[0], c[1], c[2]);
+    leaf.v[off] = m.c_max;
+    leaf.act[off >> 6] |= 1ull << (off & 63);
+  }
+  if (!set_background) return ORC_OK;
+  int32_t lo[3] = {INT32_MAX, INT32_MAX, INT32_MAX}, hi[3] = {INT32_MIN, INT32_MIN, INT32_MIN};
+  for (auto& kv : m.leaves) {
+    int32_t o[3];
+    key_origin(kv.first, o);
+    for (int v = 0; v < 512; ++v) {
+      if (!((kv.second.act[v >> 6] >> (v & 63)) & 1)) continue;
+      const int32_t c[3] = {o[0] + (v >> 6), o[1] + ((v >> 3) & 7), o[2] + (v & 7)};
+      for (int a = 0; a < 3; ++a) {
+        lo[a] = std::min(lo[a], c[a]);
+        hi[a] = std::max(hi[a], c[a]);
+      }
+    }
+  }
+  if (lo[0] > hi[0]) return ORC_OK;  // no active voxel: an empty box
+  for (int32_t x = lo[0]; x <= hi[0]; ++x)
+    for (int32_t y = lo[1]; y <= hi[1]; ++y)
+      for (int32_t z = lo[2]; z <= hi[2]; ++z) {
+        const uint64_t k = leaf_key(x, y, z);
+        const int off = leaf_offset(x, y, z);
+        auto it = m.leaves.find(k);
+        if (it != m.leaves.end()) {
+          if ((it->second.act[off >> 6] >> (off & 63)) & 1) continue;
+          it->second.v[off] = m.c_min;
+        } else if (m.c_min != 0.0f) {
+          m.n4096.insert(key4096(x, y, z));
+          touch_leaf(m, k).v[off] = m.c_min;
+        }
+      }
+  return ORC_OK;
+}
+
 }  // extern "C"

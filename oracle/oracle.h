@@ -117,6 +117,12 @@ int orc_apply_section_update(void* h, const int32_t bbox[6], int64_t n, const in
 void orc_index_bbox(double resolution, const double min_boundary[3],
                     const double max_boundary[3], const double tf[16], int32_t out[6]);
 
+/* saveMapToPCD's point list (VM:218-256): active voxel centres in tree order, as float xyz. */
+int64_t orc_export_active_points(void* h, float* xyz, int64_t cap);
+/* createMapFromPointCloud (OCC:136-174) without the final pruneGrid. */
+int orc_create_from_points(void* h, const float* xyz, int64_t n, int64_t stride_bytes,
+                           int32_t set_background, int32_t clear_map);
+
 #ifdef __cplusplus
 }
 #endif

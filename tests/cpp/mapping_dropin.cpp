@@ -2,8 +2,13 @@
 // the drop-in headers (include/vdb_mapping/), plus checks of the live grid,
 // raycastPointCloud and raytrace. Plain asserts: googletest is not required.
 // Needs a gfx950 GPU; prints one line per test and exits non-zero on failure.
+#include <dirent.h>
+#include <stdlib.h>
+#include <unistd.h>
+
 #include <cmath>
 #include <cstdio>
+#include <fstream>
 #include <vdb_mapping/OccupancyVDBMapping.hpp>
 
 using namespace vdb_mapping;
@@ -288,6 +293,70 @@ static void artificial_areas_test()
   CHECK(acc.isValueOn(wall));
 }
 
+static void pcd_persistence_test()
+{
+  // saveMapToPCD -> loadMapFromPCD(clear) reproduces the active voxels
+  // (reference :218-256, :295-307); a binary PCD file loads as well
+  char dir_tmpl[] = "/tmp/vdbgpu_pcd_XXXXXX";
+  const char* dir = mkdtemp(dir_tmpl);
+  CHECK(dir != nullptr);
+  if (!dir) return;
+  OccupancyVDBMapping a(0.1), b(0.1);
+  Config conf             = make_config(10);
+  conf.map_directory_path = std::string(dir) + "/";
+  a.setConfig(conf);
+  b.setConfig(conf);
+  OccupancyVDBMapping::PointCloudT::Ptr cloud(new OccupancyVDBMapping::PointCloudT);
+  for (int k = 0; k < 50; ++k) cloud->points.emplace_back(0.1f * k - 2.45f, 0.03f * k + 0.005f, -0.02f * k - 0.005f);
+  CHECK(a.addPointsToGrid(cloud));
+  CHECK(a.saveMapToPCD());
+  std::string file;
+  if (DIR* d = opendir(dir))
+  {
+    while (dirent* e = readdir(d))
+    {
+      const std::string name = e->d_name;
+      if (name.size() > 4 && name.find("_active_values_map.pcd") != std::string::npos) file = std::string(dir) + "/" + name;
+    }
+    closedir(d);
+  }
+  CHECK(!file.empty());
+  CHECK(b.loadMapFromPCD(file, false, true));
+  CHECK(b.getGrid()->activeVoxelCount() == a.getGrid()->activeVoxelCount());
+  CHECK(b.getGrid()->activeVoxelCount() == 50);
+  auto aa = a.getGrid()->getAccessor();
+  auto ba = b.getGrid()->getAccessor();
+  int same = 0;
+  a.getGrid()->forEachLeaf([&](const HostLeaf& l) {
+    for (int n = 0; n < 512; ++n)
+    {
+      if (!((l.mask[n >> 6] >> (n & 63)) & 1)) continue;
+      const openvdb::Coord c(l.origin[0] + (n >> 6), l.origin[1] + ((n >> 3) & 7), l.origin[2] + (n & 7));
+      same += ba.isValueOn(c) && ba.getValue(c) == aa.getValue(c);
+    }
+  });
+  CHECK(same == 50);
+  // binary PCD with an extra field: x y z intensity
+  const std::string bin = std::string(dir) + "/bin.pcd";
+  {
+    std::ofstream f(bin, std::ios::binary);
+    f << "# .PCD v0.7\nVERSION 0.7\nFIELDS x y z intensity\nSIZE 4 4 4 4\nTYPE F F F F\nCOUNT 1 1 1 1\n"
+         "WIDTH 2\nHEIGHT 1\nVIEWPOINT 0 0 0 1 0 0 0\nPOINTS 2\nDATA binary\n";
+    const float pts[8] = {0.05f, 0.05f, 0.05f, 9.0f, 1.25f, -0.35f, 0.45f, 7.0f};
+    f.write(reinterpret_cast<const char*>(pts), sizeof pts);
+  }
+  OccupancyVDBMapping c(0.1);
+  c.setConfig(conf);
+  CHECK(c.loadMapFromPCD(bin, true, true));
+  auto ca = c.getGrid()->getAccessor();
+  CHECK(ca.isValueOn(openvdb::Coord(0, 0, 0)) && ca.isValueOn(openvdb::Coord(12, -4, 4)));
+  CHECK(!ca.isValueOn(openvdb::Coord(5, -2, 2)) && ca.getValue(openvdb::Coord(5, -2, 2)) < -4.5f);  // background
+  CHECK(!c.loadMapFromPCD(std::string(dir) + "/missing.pcd", false, false));
+  std::remove(file.c_str());
+  std::remove(bin.c_str());
+  rmdir(dir);
+}
+
 int main()
 {
   struct T
@@ -306,6 +375,7 @@ int main()
     {"PointEdits", point_edits_test},
     {"MapSection", map_section_test},
     {"ArtificialAreas", artificial_areas_test},
+    {"PcdPersistence", pcd_persistence_test},
   };
   for (const T& t : tests)
   {

@@ -272,3 +272,41 @@ def test_walls_and_polygons_append(traversal):
     same_art(g, o)
     both(g, o, "restore_map_integrity")
     same_map(g, o)
+
+
+def test_active_points_export():
+    g, o, rng = scanned(0)
+    both(g, o, "add_points_to_grid", rng.uniform(-3, 3, (2000, 3)).astype(np.float32))
+    both(g, o, "add_points_to_grid", np.array([[-410.0, 0.3, 7.2], [409.7, -0.2, -6.1]], np.float32))
+    gp, op = g.export_active_points(), o.export_active_points()
+    assert gp.shape == op.shape and gp.shape[0] > 1900
+    assert np.array_equal(gp.view(np.uint32), op.view(np.uint32))
+
+
+@pytest.mark.parametrize("set_background,clear_map", [(False, False), (True, False), (True, True), (False, True)])
+def test_create_map_from_points(set_background, clear_map):
+    g, o, rng = scanned(0)
+    pts = rng.uniform(-1.5, 2.0, (5000, 3)).astype(np.float32)
+    both(g, o, "create_map_from_points", pts, set_background, clear_map)
+    same_map(g, o)
+    assert g.empty() == o.empty()
+    # the map keeps integrating afterwards
+    orig = rng.uniform(-0.5, 0.5, 3)
+    both(g, o, "insert", "test", (orig + rng.normal(0, 1.0, (2500, 3))).astype(np.float32), orig)
+    same_map(g, o)
+    bad = np.array([[0.1, 0.2, 0.3], [np.nan, 0, 0]], np.float32)
+    assert g.create_map_from_points(bad, True, True) == o.create_map_from_points(bad, True, True) == V.OUT_OF_RANGE
+    same_map(g, o)
+
+
+def test_save_load_round_trip_through_points():
+    """saveMapToPCD's points loaded back with createMapFromPointCloud give
+    the same active voxels (no background, cleared map)."""
+    g, o, rng = scanned(0)
+    both(g, o, "add_points_to_grid", rng.uniform(-3, 3, (3000, 3)).astype(np.float32))
+    pts = g.export_active_points()
+    g2, o2 = pair(0.1, max_range=5.0, **W.PROBS)
+    both(g2, o2, "create_map_from_points", pts, False, True)
+    same_map(g2, o2)
+    act = lambda m: {tuple(r) for r in (m.export_active_points() * 1).tolist()}
+    assert act(g2) == act(g)

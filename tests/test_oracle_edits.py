@@ -380,3 +380,77 @@ def test_wall_direction_is_kept():
     s, e = O.world_to_index(1.0, a), O.world_to_index(1.0, b)
     assert art_voxels(m1) == {tuple(int(x) for x in c) for c in O.cast_ray(s, e)}
     assert art_voxels(m2) == {tuple(int(x) for x in c) for c in O.cast_ray(e, s)}
+
+
+# ------------------------------------------------------------ persistence --
+
+def tree_key(c):
+    return (c[0] >> 12, c[1] >> 12, c[2] >> 12, c[0] >> 7, c[1] >> 7, c[2] >> 7, c[0] >> 3, c[1] >> 3, c[2] >> 3,
+            ((c[0] & 7) << 6) | ((c[1] & 7) << 3) | (c[2] & 7))
+
+
+def test_active_points_closed_form_and_tree_order():
+    m = scanned_map()
+    m.add_points_to_grid(np.random.default_rng(8).uniform(-3, 3, (400, 3)).astype(np.float32))
+    m.add_points_to_grid(np.array([[-410.0, 0.3, 7.2], [409.7, -0.2, -6.1]], np.float32))  # other root children
+    act = sorted((c for c, (b, a) in as_dict(m).items() if a), key=tree_key)
+    want = np.array([[np.float32(ci * 0.1 + 0.05) for ci in c] for c in act], np.float32)
+    got = m.export_active_points()
+    assert got.shape == want.shape and got.shape[0] > 400
+    assert np.array_equal(got, want)
+
+
+def test_create_map_from_points():
+    rng = np.random.default_rng(12)
+    pts = rng.uniform(-0.9, 1.3, (300, 3)).astype(np.float32)
+    cells = {vox(p, 0.1) for p in pts}
+    m = mk()
+    assert m.create_map_from_points(pts, False, False) == O.OK
+    d = as_dict(m)
+    assert {c for c, (b, a) in d.items() if a} == cells
+    assert all(d[c] == (f32bits(C_MAX), True) for c in cells)
+    assert leaf_set(m) == {tuple((x // 8) * 8 for x in c) for c in cells}
+    m2 = mk()
+    assert m2.create_map_from_points(pts, True, False) == O.OK
+    lo = [min(c[a] for c in cells) for a in range(3)]
+    hi = [max(c[a] for c in cells) for a in range(3)]
+    d2 = as_dict(m2)
+    for c, st in d2.items():
+        inside = all(lo[a] <= c[a] <= hi[a] for a in range(3))
+        if c in cells:
+            assert st == (f32bits(C_MAX), True)
+        elif inside:
+            assert st == (f32bits(C_MIN), False)
+        else:
+            assert st == (0, False)
+    leaves = {(x, y, z) for x in range((lo[0] // 8) * 8, hi[0] + 1, 8) for y in range((lo[1] // 8) * 8, hi[1] + 1, 8)
+              for z in range((lo[2] // 8) * 8, hi[2] + 1, 8)}
+    assert leaf_set(m2) == leaves
+
+
+def test_create_map_from_points_into_a_scanned_map():
+    m = scanned_map()
+    before = as_dict(m)
+    act0 = {c for c, (b, a) in before.items() if a}
+    pts = np.array([[0.31, -0.22, 0.05], [0.75, 0.12, 0.33]], np.float32)
+    assert m.create_map_from_points(pts, True, False) == O.OK
+    new = {vox(p, 0.1) for p in pts}
+    allact = act0 | new
+    lo = [min(c[a] for c in allact) for a in range(3)]
+    hi = [max(c[a] for c in allact) for a in range(3)]
+    for c, st in as_dict(m).items():
+        inside = all(lo[a] <= c[a] <= hi[a] for a in range(3))
+        if c in new:
+            assert st == (f32bits(C_MAX), True)
+        elif c in act0:
+            assert st == before[c]
+        elif inside:
+            assert st == (f32bits(C_MIN), False)
+        else:
+            assert st == before.get(c, (0, False))
+    m3 = scanned_map()
+    assert m3.create_map_from_points(pts, False, True) == O.OK  # clear first
+    assert {c for c, (b, a) in as_dict(m3).items() if a} == new
+    bad = np.array([[np.nan, 0, 0]], np.float32)
+    assert m3.create_map_from_points(bad, True, True) == O.OUT_OF_RANGE
+    assert {c for c, (b, a) in as_dict(m3).items() if a} == new  # unchanged

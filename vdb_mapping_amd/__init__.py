@@ -39,6 +39,7 @@ ABI_SYMBOLS = [
     "vdbgpu_server_destroy", "vdbgpu_set_points", "vdbgpu_add_artificial_areas",
     "vdbgpu_restore_map_integrity", "vdbgpu_export_artificial", "vdbgpu_add_artificial_walls", "vdbgpu_export_section",
     "vdbgpu_apply_section", "vdbgpu_apply_section_update", "vdbgpu_index_bbox",
+    "vdbgpu_export_active_points", "vdbgpu_create_from_points",
 ]
 
 
@@ -194,6 +195,10 @@ def lib():
         L.vdbgpu_apply_section.argtypes = [P, I64, P, P, P]
         L.vdbgpu_apply_section_update.restype = I32
         L.vdbgpu_apply_section_update.argtypes = [P, P, I64, P, P]
+        L.vdbgpu_export_active_points.restype = I64
+        L.vdbgpu_export_active_points.argtypes = [P, P, I64]
+        L.vdbgpu_create_from_points.restype = I32
+        L.vdbgpu_create_from_points.argtypes = [P, P, I64, I64, I32, I32]
         L.vdbgpu_index_bbox.restype = I32
         L.vdbgpu_index_bbox.argtypes = [D, P, P, P, P]
         _lib = L
@@ -473,6 +478,23 @@ class GpuMap:
         m = np.ascontiguousarray(masks, dtype=np.uint64).reshape(-1, 8)
         return self._check(lib().vdbgpu_apply_section_update(self.h, _ptr(b), o.shape[0], _ptr(o),
                                                              _ptr(m)), check)
+
+    # persistence (the map side of saveMapToPCD / loadMapFromPCD)
+    def export_active_points(self) -> np.ndarray:
+        """Active voxel centres (n, 3) float32 in OpenVDB's value-on order."""
+        n = lib().vdbgpu_export_active_points(self.h, None, 0)
+        if n < 0:
+            raise VdbGpuError(f"export_active_points failed: {self.last_error()}")
+        out = np.zeros((n, 3), np.float32)
+        if lib().vdbgpu_export_active_points(self.h, _ptr(out), n) != n:
+            raise VdbGpuError(f"export_active_points failed: {self.last_error()}")
+        return out
+
+    def create_map_from_points(self, points, set_background: bool, clear_map: bool, check=False) -> int:
+        pts, stride = self._pts(points)
+        rc = lib().vdbgpu_create_from_points(self.h, _ptr(pts), pts.shape[0], stride,
+                                             int(bool(set_background)), int(bool(clear_map)))
+        return self._check(rc, check)
 
     def raytrace(self, origins, dirs, lengths):
         o = np.ascontiguousarray(origins, dtype=np.float64).reshape(-1, 3)

@@ -474,6 +474,85 @@ __global__ __launch_bounds__(256) void k_section_prep(const int* __restrict__ or
   }
 }
 
+// evalActiveVoxelBoundingBox: min / max voxel coordinate of the active
+// voxels, folded into box[0..2] (atomicMin) and box[3..5] (atomicMax). One
+// lane per (leaf, word).
+__global__ __launch_bounds__(256) void k_active_bbox(MapView mp, uint32_t leaf_count, int* box) {
+  const uint64_t G = (uint64_t)gridDim.x * blockDim.x;
+  int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
+  for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < (uint64_t)leaf_count * 8; t += G) {
+    const uint32_t L = (uint32_t)(t >> 3);
+    const int j = (int)(t & 7);
+    const uint64_t a = mp.active[(size_t)L * 8 + j];
+    if (!a) continue;
+    int o[3];
+    key_origin(mp.leaf_keys[L], o);
+    uint32_t rows = 0, cols = 0;  // y rows and z columns present in the word
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      if ((a >> (8 * k)) & 0xFFu) rows |= 1u << k;
+      cols |= (uint32_t)(a >> (8 * k)) & 0xFFu;
+    }
+    lo[0] = min(lo[0], o[0] + j);
+    hi[0] = max(hi[0], o[0] + j);
+    lo[1] = min(lo[1], o[1] + __builtin_ctz(rows));
+    hi[1] = max(hi[1], o[1] + 31 - __builtin_clz(rows));
+    lo[2] = min(lo[2], o[2] + __builtin_ctz(cols));
+    hi[2] = max(hi[2], o[2] + 31 - __builtin_clz(cols));
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = min(lo[a], __shfl_xor(lo[a], o, 64));
+      hi[a] = max(hi[a], __shfl_xor(hi[a], o, 64));
+    }
+  }
+  if (lane_id() == 0 && lo[0] != INT_MAX) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      atomicMin(&box[a], lo[a]);
+      atomicMax(&box[3 + a], hi[a]);
+    }
+  }
+}
+
+// Keys of every leaf overlapping the box (leaf-grid origin lo, extents dims,
+// x-major), each with the same creation flag.
+__global__ __launch_bounds__(256) void k_box_keys(int lx, int ly, int lz, uint32_t dx, uint32_t dy, uint32_t dz,
+                                                  unsigned long long* __restrict__ keys, uint8_t* __restrict__ need,
+                                                  uint8_t need_v) {
+  const uint64_t n = (uint64_t)dx * dy * dz;
+  const uint64_t G = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += G) {
+    const uint32_t z = (uint32_t)(i % dz), y = (uint32_t)((i / dz) % dy), x = (uint32_t)(i / ((uint64_t)dz * dy));
+    keys[i] = leaf_key((lx + (int)x) * 8, (ly + (int)y) * 8, (lz + (int)z) * 8);
+    need[i] = need_v;
+  }
+}
+
+// The background of createMapFromPointCloud: inside the box, every inactive
+// voxel of the leaves eidx[i] gets value v (its state stays off). One lane
+// per (leaf, word).
+__global__ __launch_bounds__(256) void k_box_fill(const unsigned long long* __restrict__ keys,
+                                                  const uint32_t* __restrict__ eidx, uint32_t n, const int* box,
+                                                  MapView mp, float v) {
+  const uint64_t G = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < (uint64_t)n * 8; t += G) {
+    const uint32_t i = (uint32_t)(t >> 3);
+    const int j = (int)(t & 7);
+    const uint32_t idx = eidx[i];
+    if (idx == kNone) continue;
+    int o[3];
+    key_origin(keys[i], o);
+    const uint64_t m = box_word(o, box, j) & ~(uint64_t)mp.active[(size_t)idx * 8 + j];
+    if (!m) continue;
+    float* pv = mp.values + (size_t)idx * 512 + j * 64;
+    for (uint64_t b = m; b; b &= b - 1) pv[__builtin_ctzll(b)] = v;
+    mp.dirty[idx] = 1;
+  }
+}
+
 // ------------------------------------------------------------- launchers --
 
 hipError_t launch_mark_points(hipStream_t s, const float* pts, int64_t stride_bytes, uint32_t n, double inv,
@@ -568,6 +647,31 @@ hipError_t launch_section_prep(hipStream_t s, const int* origins, const float* v
   const int grid = (int)std::min<uint32_t>(8192, (n + 3) / 4);
   hipLaunchKernelGGL(k_section_prep, dim3(grid), dim3(256), 0, s, origins, vals, act, n, keys, need, first,
                      sleaf, sleaf_mask, s128, s128_mask, n128, s4096, s4096_mask, n4096, counters);
+  return hipGetLastError();
+}
+
+hipError_t launch_active_bbox(hipStream_t s, MapView mp, uint32_t leaf_count, int* box) {
+  if (leaf_count == 0) return hipSuccess;
+  const int grid = (int)std::min<uint64_t>(4096, ((uint64_t)leaf_count * 8 + 255) / 256);
+  hipLaunchKernelGGL(k_active_bbox, dim3(grid), dim3(256), 0, s, mp, leaf_count, box);
+  return hipGetLastError();
+}
+
+hipError_t launch_box_keys(hipStream_t s, const int lo[3], const uint32_t dims[3], unsigned long long* keys,
+                           uint8_t* need, bool need_v) {
+  const uint64_t n = (uint64_t)dims[0] * dims[1] * dims[2];
+  if (n == 0) return hipSuccess;
+  const int grid = (int)std::min<uint64_t>(8192, (n + 255) / 256);
+  hipLaunchKernelGGL(k_box_keys, dim3(grid), dim3(256), 0, s, lo[0], lo[1], lo[2], dims[0], dims[1], dims[2], keys,
+                     need, (uint8_t)(need_v ? 1 : 0));
+  return hipGetLastError();
+}
+
+hipError_t launch_box_fill(hipStream_t s, const unsigned long long* keys, const uint32_t* eidx, uint32_t n,
+                           const int* box, MapView mp, float v) {
+  if (n == 0) return hipSuccess;
+  const int grid = (int)std::min<uint64_t>(8192, ((uint64_t)n * 8 + 255) / 256);
+  hipLaunchKernelGGL(k_box_fill, dim3(grid), dim3(256), 0, s, keys, eidx, n, box, mp, v);
   return hipGetLastError();
 }
 

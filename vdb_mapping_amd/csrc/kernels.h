@@ -113,4 +113,10 @@ hipError_t launch_section_prep(hipStream_t s, const int* origins, const float* v
 hipError_t launch_section_or(hipStream_t s, const unsigned long long* act, const uint32_t* eidx, uint32_t n,
                              MapView mp);
 
+hipError_t launch_active_bbox(hipStream_t s, MapView mp, uint32_t leaf_count, int* box);
+hipError_t launch_box_keys(hipStream_t s, const int lo[3], const uint32_t dims[3], unsigned long long* keys,
+                           uint8_t* need, bool need_v);
+hipError_t launch_box_fill(hipStream_t s, const unsigned long long* keys, const uint32_t* eidx, uint32_t n,
+                           const int* box, MapView mp, float v);
+
 }  // namespace vdbgpu

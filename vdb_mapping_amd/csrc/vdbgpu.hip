@@ -428,6 +428,7 @@ int ensure_map(vdbgpu_map* m, uint64_t need) {
 
 void create_flags(const vdbgpu_map* m, bool* ch, bool* cm);
 int activate_artificial(vdbgpu_map* m);
+int clear_grid(vdbgpu_map* m);
 
 int ensure_groups(vdbgpu_map* m, uint32_t cap) {
   if (cap <= m->gcap) return VDBGPU_OK;
@@ -1520,14 +1521,8 @@ int32_t vdbgpu_insert_batch(vdbgpu_map* m, const char* id, int32_t nscans, const
 int32_t vdbgpu_reset(vdbgpu_map* m) {
   if (!m) return VDBGPU_BAD_ARGUMENT;
   Guard g(m);
-  if (m->lcap) {
-    CK(hipMemsetAsync(m->values, 0, (size_t)m->lcap * 512 * sizeof(float), m->stream));
-    CK(hipMemsetAsync(m->active, 0, (size_t)m->lcap * 8 * sizeof(unsigned long long), m->stream));
-    CK(hipMemsetAsync(m->dirty, 0, m->lcap, m->stream));
-  }
-  if (m->mcap) CK(launch_fill_u64(m->stream, m->mkeys, m->mcap, kEmpty));
-  CK(launch_fill_u64(m->stream, m->noop4096, vdbgpu_map::kNoopCap, kEmpty));
-  CK(hipMemsetAsync(&m->d_st[1], 0, sizeof(Status), m->stream));
+  int r;
+  if ((r = clear_grid(m))) return r;
   for (auto& kv : m->sources) {
     Source& s = kv.second;
     if (s.cap) {
@@ -1536,11 +1531,7 @@ int32_t vdbgpu_reset(vdbgpu_map* m) {
     }
     s.pending = 0;
   }
-  m->leaf_count_ub = 0;
-  m->win_a = m->win_b = 0;
-  m->touched = false;
-  m->snap_dirty = true;
-  m->art_leaves = false;  // resetMap keeps the artificial grid (VM:174-186), not its leaves
+  // resetMap keeps the artificial grid (VM:174-186), not its leaves
   CK(hipStreamSynchronize(m->stream));
   return VDBGPU_OK;
 }
@@ -1799,39 +1790,181 @@ const char* vdbgpu_last_error(vdbgpu_map* m) { return m ? m->last_error.c_str() 
 
 // ------------------------------------------------------------ map edits --
 
+}  // extern "C"
+
+namespace {
+
+// Mark the voxels floor(p * 1/res) of n host points in the edit set (a NaN
+// or out-of-range point clears the set and returns its status); *count =
+// the set's leaves.
+int mark_points(vdbgpu_map* m, const float* xyz, int64_t n, int64_t stride, uint32_t* count) {
+  *count = 0;
+  if (n == 0) return VDBGPU_OK;
+  const size_t bytes = (size_t)(n - 1) * stride + 12;
+  float* pts = nullptr;
+  CK(dalloc(m, &pts, (bytes + 3) / 4));
+  CK(hipMemcpyAsync(pts, xyz, bytes, hipMemcpyHostToDevice, m->stream));
+  // at most n leaves; large edits start smaller and grow on overflow
+  const uint32_t cap0 = (uint32_t)std::max<uint64_t>(kMinUpdateCap, next_pow2(std::min<uint64_t>(2 * (uint64_t)n, 1u << 22)));
+  const int r = mark_set(m, m->edit, cap0, [&]() -> int {
+    CK(launch_mark_points(m->stream, pts, stride, (uint32_t)n, m->inv, upd_view(m->edit), &m->d_st[2]));
+    return VDBGPU_OK;
+  }, count);
+  dfree(m, &pts, (bytes + 3) / 4);  // hipFree waits for the kernels that read it
+  return r;
+}
+
+// setNodeToOccupied / setNodeToFree on the marked voxels; clears the edit set.
+int apply_points(vdbgpu_map* m, uint32_t count, bool occupied) {
+  const float val = occupied ? m->lo[4] : m->lo[5];
+  const bool need = occupied || val != 0.0f;  // does the op change a background voxel?
+  int r;
+  if (!(r = ensure_map(m, m->leaf_count_ub + count)) && !(r = ensure_eidx(m, count)) &&
+      !(r = apply_edit(m, m->edit, count, m->eidx, occupied ? kEditOccupied : kEditFree, val, need, true))) {
+    m->leaf_count_ub += count;
+    m->win_b += count;
+    return VDBGPU_OK;
+  }
+  (void)clear_set(m, m->edit);  // no stale marks for the next call
+  return r;
+}
+
+// The grid emptied (leaf pool, leaf hash, root children); update sets and
+// the artificial grid are kept.
+int clear_grid(vdbgpu_map* m) {
+  if (m->lcap) {
+    CK(hipMemsetAsync(m->values, 0, (size_t)m->lcap * 512 * sizeof(float), m->stream));
+    CK(hipMemsetAsync(m->active, 0, (size_t)m->lcap * 8 * sizeof(unsigned long long), m->stream));
+    CK(hipMemsetAsync(m->dirty, 0, m->lcap, m->stream));
+  }
+  if (m->mcap) CK(launch_fill_u64(m->stream, m->mkeys, m->mcap, kEmpty));
+  CK(launch_fill_u64(m->stream, m->noop4096, vdbgpu_map::kNoopCap, kEmpty));
+  CK(hipMemsetAsync(&m->d_st[1], 0, sizeof(Status), m->stream));
+  m->leaf_count_ub = 0;
+  m->win_a = m->win_b = 0;
+  m->touched = false;
+  m->snap_dirty = true;
+  m->art_leaves = false;
+  return VDBGPU_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 int32_t vdbgpu_set_points(vdbgpu_map* m, const float* xyz, int64_t n, int64_t stride, int32_t occupied) {
   if (!m) return VDBGPU_BAD_ARGUMENT;
   Guard g(m);
   if (n < 0 || stride < 12 || (stride & 3) || n > (int64_t)kMaxRays || (n && !xyz))
     return fail(m, VDBGPU_BAD_ARGUMENT, "bad point buffer arguments");
-  if (n == 0) return VDBGPU_OK;
-  int r;
-  const size_t bytes = (size_t)(n - 1) * stride + 12;
-  float* pts = nullptr;
-  CK(dalloc(m, &pts, (bytes + 3) / 4));
-  CK(hipMemcpyAsync(pts, xyz, bytes, hipMemcpyHostToDevice, m->stream));
   uint32_t count = 0;
-  // at most n leaves; large edits start smaller and grow on overflow
-  const uint32_t cap0 = (uint32_t)std::max<uint64_t>(kMinUpdateCap, next_pow2(std::min<uint64_t>(2 * (uint64_t)n, 1u << 22)));
-  r = mark_set(m, m->edit, cap0, [&]() -> int {
-    CK(launch_mark_points(m->stream, pts, stride, (uint32_t)n, m->inv, upd_view(m->edit), &m->d_st[2]));
-    return VDBGPU_OK;
-  }, &count);
-  if (r == VDBGPU_OK) {
-    const float val = occupied ? m->lo[4] : m->lo[5];
-    const bool need = occupied || val != 0.0f;  // does the op change a background voxel?
-    if (!(r = ensure_map(m, m->leaf_count_ub + count)) && !(r = ensure_eidx(m, count)) &&
-        !(r = apply_edit(m, m->edit, count, m->eidx, occupied ? kEditOccupied : kEditFree, val, need, true))) {
-      m->leaf_count_ub += count;
-      m->win_b += count;
-    } else {
-      (void)clear_set(m, m->edit);  // no stale marks for the next call
-    }
-  }
-  dfree(m, &pts, (bytes + 3) / 4);  // hipFree waits for the kernels that read it
-  return r;
+  int r;
+  if ((r = mark_points(m, xyz, n, stride, &count))) return r;
+  return count ? apply_points(m, count, occupied != 0) : VDBGPU_OK;
 }
 
+int32_t vdbgpu_create_from_points(vdbgpu_map* m, const float* xyz, int64_t n, int64_t stride,
+                                  int32_t set_background, int32_t clear_map) {
+  if (!m) return VDBGPU_BAD_ARGUMENT;
+  Guard g(m);
+  if (n < 0 || stride < 12 || (stride & 3) || n > (int64_t)kMaxRays || (n && !xyz))
+    return fail(m, VDBGPU_BAD_ARGUMENT, "bad point buffer arguments");
+  CK(hipStreamSynchronize(m->cast));
+  uint32_t count = 0;
+  int r;
+  if ((r = mark_points(m, xyz, n, stride, &count))) return r;  // validated before any change
+  if (clear_map && (r = clear_grid(m))) return r;
+  // setValueOn(ijk, max log-odds): the same voxel state as setNodeToOccupied
+  if (count && (r = apply_points(m, count, true))) return r;
+  if (!set_background) return VDBGPU_OK;
+  if ((r = read_status(m))) return r;
+  const uint32_t L = m->h_st[1].leaf_count;
+  if (L == 0) return VDBGPU_OK;
+  int* d_box = nullptr;
+  const int init[6] = {INT_MAX, INT_MAX, INT_MAX, INT_MIN, INT_MIN, INT_MIN};
+  if ((r = upload(m, &d_box, init, 6))) return r;
+  CK(launch_active_bbox(m->stream, map_view(m), L, d_box));
+  int box[6];
+  CK(hipMemcpyAsync(box, d_box, sizeof box, hipMemcpyDeviceToHost, m->stream));
+  CK(hipStreamSynchronize(m->stream));
+  if (box[0] > box[3]) {  // no active voxel: an empty box
+    dfree(m, &d_box, 6);
+    return VDBGPU_OK;
+  }
+  const int lo[3] = {box[0] >> 3, box[1] >> 3, box[2] >> 3};
+  const uint32_t dims[3] = {(uint32_t)((box[3] >> 3) - lo[0] + 1), (uint32_t)((box[4] >> 3) - lo[1] + 1),
+                            (uint32_t)((box[5] >> 3) - lo[2] + 1)};
+  const uint64_t nb = (uint64_t)dims[0] * dims[1] * dims[2];
+  if (nb > (1ull << 27)) {
+    dfree(m, &d_box, 6);
+    return fail(m, VDBGPU_UNSUPPORTED, "background box holds more than 2^27 leaves");
+  }
+  const uint32_t N = (uint32_t)nb;
+  const bool need = m->lo[5] != 0.0f;  // setValueOff(min log-odds) changes a background voxel
+  unsigned long long* keys = nullptr;
+  uint8_t* needs = nullptr;
+  uint32_t* eidx = nullptr;
+  if ((r = ensure_map(m, m->leaf_count_ub + N))) return r;
+  CK(dalloc(m, &keys, N));
+  CK(dalloc(m, &needs, N));
+  CK(dalloc(m, &eidx, N));
+  CK(launch_box_keys(m->stream, lo, dims, keys, needs, need));
+  CK(launch_upsert_leaves(m->stream, keys, needs, N, map_view(m), eidx, nullptr, &m->d_st[1]));
+  CK(launch_box_fill(m->stream, keys, eidx, N, d_box, map_view(m), m->lo[5]));
+  m->leaf_count_ub += N;
+  m->win_b += N;
+  m->snap_dirty = true;
+  if ((r = read_status(m))) return r;
+  m->touched = m->touched || m->h_st[1].leaf_count > 0;
+  dfree(m, &keys, N);
+  dfree(m, &needs, N);
+  dfree(m, &eidx, N);
+  dfree(m, &d_box, 6);
+  return VDBGPU_OK;
+}
+
+int64_t vdbgpu_export_active_points(vdbgpu_map* m, float* xyz, int64_t cap) {
+  if (!m) return -1;
+  Guard g(m);
+  if (hipStreamSynchronize(m->cast) != hipSuccess || read_status(m)) return -1;
+  const uint32_t n = m->h_st[1].leaf_count;
+  std::vector<unsigned long long> keys(n), act((size_t)n * 8);
+  if (n && (hipMemcpy(keys.data(), m->leaf_keys, n * sizeof(unsigned long long), hipMemcpyDeviceToHost) ||
+            hipMemcpy(act.data(), m->active, (size_t)n * 64, hipMemcpyDeviceToHost))) {
+    m->last_error = "export copy failed";
+    return -1;
+  }
+  int64_t total = 0;
+  for (uint64_t w : act) total += __builtin_popcountll(w);
+  if (!xyz) return total;
+  if (total > cap) return -total;
+  // cbeginValueOn order: root children by origin, then the 128^3 nodes and
+  // the leaves inside each by offset (x-major), voxels by offset
+  std::vector<std::pair<std::array<int, 9>, uint32_t>> order(n);
+  for (uint32_t i = 0; i < n; ++i) {
+    int o[3];
+    key_origin(keys[i], o);
+    order[i] = {{o[0] >> 12, o[1] >> 12, o[2] >> 12, o[0] >> 7, o[1] >> 7, o[2] >> 7, o[0] >> 3, o[1] >> 3, o[2] >> 3},
+                i};
+  }
+  std::sort(order.begin(), order.end());
+  int64_t k = 0;
+  for (const auto& oi : order) {
+    int o[3];
+    key_origin(keys[oi.second], o);
+    for (int v = 0; v < 512; ++v) {
+      if (!((act[(size_t)oi.second * 8 + (v >> 6)] >> (v & 63)) & 1)) continue;
+      const int c[3] = {o[0] + (v >> 6), o[1] + ((v >> 3) & 7), o[2] + (v & 7)};
+      for (int a = 0; a < 3; ++a) {
+        double w = (double)c[a] * m->res;  // indexToWorld (ScaleMap::applyMap)
+        w += 0.5 * m->res;
+        xyz[3 * k + a] = (float)w;
+      }
+      ++k;
+    }
+  }
+  return total;
+}
 }  // extern "C"
 
 namespace {
