@@ -21,8 +21,10 @@
 // areas (restoreMapIntegrity, addArtificialAreas / Polygon / Wall
 // :1152-1236) and map sections (createWorldBoundingBox /
 // createIndexBoundingBox, getMapSectionGrid / UpdateGrid, applyMapSectionGrid
-// / UpdateGrid :811-1085, without morphological smoothing) run in the library
-// too (vdbgpu_set_points, vdbgpu_*artificial*, vdbgpu_*section*).
+// / UpdateGrid :811-1085) run in the library too (vdbgpu_set_points,
+// vdbgpu_*artificial*, vdbgpu_*section*), and so do the morphology members
+// (morphologicalClose / Open / Dilate / ErodeMap :1095-1148, vdbgpu_morphology)
+// that smooth a section before it is applied.
 //
 // PCD persistence (saveMapToPCD, loadMapFromPCD, createMapFromPointCloud
 // :218-256, :295-307, OccupancyVDBMapping.hpp:136-174) exports and loads through
@@ -30,10 +32,8 @@
 //
 // Not provided: .vdb persistence (saveMap / loadMap :193-211, :263-284) and
 // the byte-array wire format (:1246-1338), which need OpenVDB's io::File /
-// io::Stream; the morphology
-// operators (:1095-1140, OpenVDB tools); castRayIntoGrid (:550-566, a
-// per-ray host call) and updateMap (:731-792, reached through
-// integrateUpdate).
+// io::Stream; castRayIntoGrid (:550-566, a per-ray host call) and updateMap
+// (:731-792, reached through integrateUpdate).
 #ifndef VDB_MAPPING_VDB_MAPPING_H_INCLUDED
 #define VDB_MAPPING_VDB_MAPPING_H_INCLUDED
 
@@ -524,19 +524,17 @@ public:
   }
 
   /*!
-   * \brief Writes a section grid into the map (reference :1022-1050).
-   * Morphological smoothing (smooth_map) is not offered: such a call is
-   * refused with a message and changes nothing.
+   * \brief Writes a section grid into the map (reference :1022-1050). With
+   * smooth_map the section is closed first (morphologicalCloseMap, which
+   * changes the caller's section grid as the reference's does).
    */
   void applyMapSectionGrid(const typename GridT::Ptr section,
                            bool smooth_map          = false,
                            int smoothing_iterations = 2)
   {
-    (void)smoothing_iterations;
     if (smooth_map)
     {
-      std::cerr << "applyMapSectionGrid: morphological smoothing is not supported" << std::endl;
-      return;
+      morphologicalCloseMap<GridT>(section, smoothing_iterations);
     }
     std::vector<int32_t> o;
     std::vector<float> v;
@@ -553,17 +551,15 @@ public:
 
   /*!
    * \brief Replaces the active states inside the section's box by the
-   * section's (reference :1058-1085). Smoothing is not offered (see above).
+   * section's (reference :1058-1085), closed first with smooth_map.
    */
   void applyMapSectionUpdateGrid(const typename UpdateGridT::Ptr section,
                                  bool smooth_map          = false,
                                  int smoothing_iterations = 2)
   {
-    (void)smoothing_iterations;
     if (smooth_map)
     {
-      std::cerr << "applyMapSectionUpdateGrid: morphological smoothing is not supported" << std::endl;
-      return;
+      morphologicalCloseMap<UpdateGridT>(section, smoothing_iterations);
     }
     const openvdb::Vec3d mn = section->template metaValue<openvdb::Vec3d>("bb_min");
     const openvdb::Vec3d mx = section->template metaValue<openvdb::Vec3d>("bb_max");
@@ -579,6 +575,47 @@ public:
     std::unique_lock map_lock(*m_map_mutex);
     report(vdbgpu_apply_section_update(m_map, box, (int64_t)(o.size() / 3), o.data(), a.data()));
     m_vdb_grid->markStale();
+  }
+
+  /*!
+   * \brief Morphological closing (reference :1095-1099): dilation, then
+   * erosion, `iterations` 26-neighbour steps each, on the GPU.
+   */
+  template <typename TGrid>
+  void morphologicalCloseMap(typename TGrid::Ptr grid, int iterations)
+  {
+    morphology(*grid, 2, iterations);
+  }
+
+  /*!
+   * \brief Morphological opening (reference :1109-1113): erosion, then dilation.
+   */
+  template <typename TGrid>
+  void morphologicalOpenMap(typename TGrid::Ptr grid, int iterations)
+  {
+    morphology(*grid, 3, iterations);
+  }
+
+  /*!
+   * \brief dilateActiveValues with NN_FACE_EDGE_VERTEX (reference :1123-1130):
+   * each step activates every voxel with an active 26-neighbour; a missing leaf
+   * is created with the background value.
+   */
+  template <typename TGrid>
+  void morphologicalDilateMap(typename TGrid::Ptr grid, int iterations)
+  {
+    morphology(*grid, 0, iterations);
+  }
+
+  /*!
+   * \brief erodeActiveValues with NN_FACE_EDGE_VERTEX (reference :1139-1146):
+   * each step keeps an active voxel only if all 26 neighbours are active;
+   * leaves stay, emptied or not.
+   */
+  template <typename TGrid>
+  void morphologicalErodeMap(typename TGrid::Ptr grid, int iterations)
+  {
+    morphology(*grid, 1, iterations);
   }
 
   /*!
@@ -691,6 +728,38 @@ protected:
     std::array<int32_t, 6> out{};
     report(vdbgpu_index_bbox(m_resolution, mn, mx, t, out.data()));
     return out;
+  }
+
+  // vdbgpu_morphology over the leaves of a detached grid (a section); the
+  // new masks are written back, created leaves take the background value.
+  template <typename TGridObj>
+  void morphology(TGridObj& grid, int32_t op, int iterations)
+  {
+    if (grid.attached())
+    {
+      std::cerr << "vdbgpu: morphology of the live map grid is not supported" << std::endl;
+      return;
+    }
+    std::vector<int32_t> o;
+    std::vector<uint64_t> a;
+    grid.forEachLeaf([&](const auto& l) {
+      o.insert(o.end(), l.origin, l.origin + 3);
+      const uint64_t* w = leafActiveWords(l);
+      a.insert(a.end(), w, w + 8);
+    });
+    const int64_t n = static_cast<int64_t>(o.size() / 3);
+    const int64_t k = vdbgpu_morphology(m_map, op, iterations, n, o.data(), a.data(), nullptr, nullptr, 0);
+    std::vector<int32_t> ro(3 * std::max<int64_t>(k, 0));
+    std::vector<uint64_t> rm(8 * std::max<int64_t>(k, 0));
+    if (k < 0 || vdbgpu_morphology(m_map, op, iterations, n, o.data(), a.data(), ro.data(), rm.data(), k) != k)
+    {
+      std::cerr << "vdbgpu: morphology failed: " << vdbgpu_last_error(m_map) << std::endl;
+      return;
+    }
+    for (int64_t i = 0; i < k; ++i)
+    {
+      grid.setLeafMask(&ro[3 * i], &rm[8 * i]);
+    }
   }
 
   void exportSection(const std::array<int32_t, 6>& box,

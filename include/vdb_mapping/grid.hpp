@@ -180,6 +180,21 @@ public:
     }
     return it->second;
   }
+  // Active mask of the leaf at origin; a missing leaf is created with the
+  // background value 0 in every voxel (detached grids only).
+  void setLeafMask(const int32_t origin[3], const uint64_t* mask)
+  {
+    std::lock_guard<std::mutex> lk(m_mu);
+    auto it = m_leaves.find(host_leaf_key(origin[0], origin[1], origin[2]));
+    if (it == m_leaves.end())
+    {
+      HostLeaf l;
+      std::memset(&l, 0, sizeof l);
+      std::memcpy(l.origin, origin, sizeof l.origin);
+      it = m_leaves.emplace(host_leaf_key(origin[0], origin[1], origin[2]), l).first;
+    }
+    std::memcpy(it->second.mask, mask, sizeof it->second.mask);
+  }
 
   // Live binding to a device map (used by VDBMapping).
   void attach(vdbgpu_map* map)
@@ -197,6 +212,11 @@ public:
   {
     std::lock_guard<std::mutex> lk(m_mu);
     m_stale = true;
+  }
+  bool attached() const
+  {
+    std::lock_guard<std::mutex> lk(m_mu);
+    return m_map != nullptr;
   }
 
 private:
@@ -339,6 +359,14 @@ public:
       f(kv.second);
     }
   }
+  // Active mask of the leaf at origin; a missing leaf is created with every
+  // value false (the background).
+  void setLeafMask(const int32_t origin[3], const uint64_t* active)
+  {
+    Leaf& l = touch(openvdb::Coord(origin[0], origin[1], origin[2]));
+    std::memcpy(l.active, active, sizeof l.active);
+  }
+  bool attached() const { return false; }
 
 private:
   Leaf& touch(const openvdb::Coord& c)
@@ -359,6 +387,10 @@ private:
   std::unordered_map<uint64_t, Leaf> m_leaves;
   std::map<std::string, openvdb::Vec3d> m_meta;
 };
+
+// Active-mask words of a leaf of either grid type.
+inline const uint64_t* leafActiveWords(const HostLeaf& l) { return l.mask; }
+inline const uint64_t* leafActiveWords(const BoolGrid::Leaf& l) { return l.active; }
 
 } // namespace vdb_mapping
 

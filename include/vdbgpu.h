@@ -307,20 +307,39 @@ int64_t vdbgpu_export_artificial(vdbgpu_map* map, int32_t* origins, uint64_t* ma
  * -count, an error -1. */
 int64_t vdbgpu_export_section(vdbgpu_map* map, const int32_t bbox[6], int32_t full_grid,
                               int32_t* origins, float* values, uint64_t* masks, int64_t cap);
-/* applyMapSectionGrid without smoothing (VM:1022-1050): every voxel of every
+/* applyMapSectionGrid (VM:1022-1050): every voxel of every
  * section leaf is written to the map with its value and state, and the
  * inactive tiles of the section grid's internal nodes write (0, inactive) at
  * their origins, as the reference's cbeginValueAll loop does. Origins must be
- * distinct multiples of 8. smooth_map = true (OpenVDB morphology) is not
- * offered. */
+ * distinct multiples of 8. For smooth_map = true the caller closes the
+ * section first (vdbgpu_morphology, op 2). */
 int32_t vdbgpu_apply_section(vdbgpu_map* map, int64_t n, const int32_t* origins,
                              const float* values, const uint64_t* masks);
-/* applyMapSectionUpdateGrid without smoothing (VM:1058-1085): active map
+/* applyMapSectionUpdateGrid (VM:1058-1085), smoothing as above: active map
  * voxels inside bbox (the section's bb_min / bb_max) are deactivated, then
  * every active section voxel is activated (a missing leaf is created with
  * value 0). */
 int32_t vdbgpu_apply_section_update(vdbgpu_map* map, const int32_t bbox[6], int64_t n,
                                     const int32_t* origins, const uint64_t* masks);
+/* The public morphology members (VM:1095-1148) on a leaf set, e.g. a section
+ * grid's leaves: op 0 morphologicalDilateMap, 1 morphologicalErodeMap,
+ * 2 morphologicalCloseMap (dilate, then erode), 3 morphologicalOpenMap
+ * (erode, then dilate), each `iterations` steps of tools::dilateActiveValues /
+ * erodeActiveValues with 26-neighbour connectivity (NN_FACE_EDGE_VERTEX): a
+ * dilation step activates every voxel with an active neighbour, an erosion
+ * step keeps an active voxel only if all 26 neighbours are active (a voxel in
+ * no leaf is inactive). n leaves (origins distinct multiples of 8, 8 active-mask
+ * words each) in; out: the n input leaves in input order with their new masks,
+ * then the leaves dilation created (sorted by origin; a created leaf takes the
+ * grid's background value, an emptied leaf stays). Values are not touched.
+ * Null outputs return the count, a short cap -count, an error -1
+ * (vdbgpu_last_error); iterations <= 0 return the input unchanged. Runs on
+ * the map's device and stream; the map itself is not read or changed. This
+ * is what applyMapSectionGrid / UpdateGrid run first when smooth_map is set
+ * (morphologicalCloseMap on the section, VM:1026-1029, :1062-1065). */
+int64_t vdbgpu_morphology(vdbgpu_map* map, int32_t op, int32_t iterations, int64_t n,
+                          const int32_t* origins, const uint64_t* masks, int32_t* out_origins,
+                          uint64_t* out_masks, int64_t cap);
 /* ---- persistence ---------------------------------------------------------------
  * The point list of saveMapToPCD (VM:218-256): every active voxel at
  * indexToWorld(ijk) + 0.5 resolution, as float xyz, in OpenVDB's cbeginValueOn

@@ -100,6 +100,8 @@ def lib():
         L.orc_export_active_points.restype = I64
         L.orc_export_active_points.argtypes = [P, P, I64]
         L.orc_create_from_points.argtypes = [P, P, I64, I64, I32, I32]
+        L.orc_morphology.restype = I64
+        L.orc_morphology.argtypes = [I32, I32, I64, P, P, P, P, I64]
         _lib = L
     return _lib
 
@@ -132,6 +134,25 @@ def index_bbox(res: float, min_boundary, max_boundary, tf=None) -> np.ndarray:
     out = np.zeros(6, np.int32)
     lib().orc_index_bbox(res, _ptr(mn), _ptr(mx), _ptr(t), _ptr(out))
     return out
+
+
+MORPH_OPS = {"dilate": 0, "erode": 1, "close": 2, "open": 3}
+
+
+def morphology(op: str, iterations: int, origins, masks):
+    """Morphology of a leaf set (VM:1095-1148): op in MORPH_OPS, 26-neighbour
+    steps. Returns (origins (k, 3) int32, masks (k, 8) uint64): the input
+    leaves in input order, then the leaves dilation created, sorted by origin."""
+    o = np.ascontiguousarray(origins, dtype=np.int32).reshape(-1, 3)
+    m = np.ascontiguousarray(masks, dtype=np.uint64).reshape(-1, 8)
+    n = len(o)
+    k = lib().orc_morphology(MORPH_OPS[op], iterations, n, _ptr(o), _ptr(m), None, None, 0)
+    if k < 0:
+        raise ValueError("bad morphology input")
+    oo = np.zeros((max(k, 1), 3), np.int32)
+    om = np.zeros((max(k, 1), 8), np.uint64)
+    lib().orc_morphology(MORPH_OPS[op], iterations, n, _ptr(o), _ptr(m), _ptr(oo), _ptr(om), k)
+    return oo[:k], om[:k]
 
 
 def polygons_arg(polygons):

@@ -28,6 +28,10 @@
 //      value per tile, at the tile's origin) besides the leaf voxels.
 //   U13 pcl::transformPointCloud with a double matrix computes each output
 //      coordinate as ((m0 p0 + m1 p1) + m2 p2) + m3 in double, rounded to float.
+//   U14 tools::dilateActiveValues / erodeActiveValues (NN_FACE_EDGE_VERTEX,
+//      EXPAND_TILES, no pruning): a voxel outside every leaf is inactive; a
+//      dilation step creates a leaf exactly where it activates a voxel in none;
+//      erosion keeps every leaf, emptied or not.
 //
 // Compile with -ffp-contract=off (no FMA contraction): the reference is built
 // for baseline x86-64 (Release, no -march flags, VM CMakeLists.txt:8-11),
@@ -1337,6 +1341,124 @@ int orc_create_from_points(void* h, const float* xyz, int64_t n, int64_t stride,
         }
       }
   return ORC_OK;
+}
+
+}  // extern "C"
+
+// Morphology of a leaf set (VM:1095-1148: morphologicalDilateMap and
+// morphologicalErodeMap, composed by morphologicalCloseMap / OpenMap), voxel by
+// voxel as the textbook operators with a 3x3x3 structuring element [U14].
+namespace {
+
+struct MorphSet {
+  std::unordered_map<uint64_t, std::array<uint64_t, 8>> leaves;
+  std::vector<uint64_t> created;  // leaves dilation added, in creation order
+};
+
+bool morph_on(const MorphSet& s, int32_t x, int32_t y, int32_t z) {
+  auto it = s.leaves.find(leaf_key(x, y, z));
+  if (it == s.leaves.end()) return false;
+  const int off = leaf_offset(x, y, z);
+  return (it->second[off >> 6] >> (off & 63)) & 1;
+}
+
+template <class F>
+void morph_for_each_on(const MorphSet& s, F&& f) {
+  for (const auto& kv : s.leaves) {
+    int32_t o[3];
+    key_origin(kv.first, o);
+    for (int v = 0; v < 512; ++v)
+      if ((kv.second[v >> 6] >> (v & 63)) & 1) f(o[0] + (v >> 6), o[1] + ((v >> 3) & 7), o[2] + (v & 7));
+  }
+}
+
+// One step of dilateActiveValues: every voxel with an active 26-neighbour (or
+// itself active) becomes active; a missing leaf is created on first activation.
+void morph_dilate1(MorphSet& s) {
+  MorphSet next = s;
+  morph_for_each_on(s, [&](int32_t x, int32_t y, int32_t z) {
+    for (int dx = -1; dx <= 1; ++dx)
+      for (int dy = -1; dy <= 1; ++dy)
+        for (int dz = -1; dz <= 1; ++dz) {
+          const int32_t c[3] = {x + dx, y + dy, z + dz};
+          const uint64_t k = leaf_key(c[0], c[1], c[2]);
+          auto it = next.leaves.find(k);
+          if (it == next.leaves.end()) {
+            it = next.leaves.emplace(k, std::array<uint64_t, 8>{}).first;
+            next.created.push_back(k);
+          }
+          const int off = leaf_offset(c[0], c[1], c[2]);
+          it->second[off >> 6] |= 1ull << (off & 63);
+        }
+  });
+  s = std::move(next);
+}
+
+// One step of erodeActiveValues: an active voxel stays active only if all 26
+// neighbours are active (a voxel in no leaf counts as inactive).
+void morph_erode1(MorphSet& s) {
+  MorphSet next = s;
+  morph_for_each_on(s, [&](int32_t x, int32_t y, int32_t z) {
+    bool all = true;
+    for (int dx = -1; dx <= 1 && all; ++dx)
+      for (int dy = -1; dy <= 1 && all; ++dy)
+        for (int dz = -1; dz <= 1 && all; ++dz) all = morph_on(s, x + dx, y + dy, z + dz);
+    if (!all) {
+      const int off = leaf_offset(x, y, z);
+      next.leaves[leaf_key(x, y, z)][off >> 6] &= ~(1ull << (off & 63));
+    }
+  });
+  s = std::move(next);
+}
+
+}  // namespace
+
+extern "C" {
+
+int64_t orc_morphology(int32_t op, int32_t iterations, int64_t n, const int32_t* origins,
+                       const uint64_t* masks, int32_t* out_origins, uint64_t* out_masks, int64_t cap) {
+  if (op < 0 || op > 3 || n < 0 || (n && (!origins || !masks))) return -1;
+  MorphSet s;
+  std::vector<uint64_t> input((size_t)n);
+  // dilation must not carry a leaf out of the key range: 8 voxels per 8 steps
+  const int64_t margin = op == 1 || iterations <= 0 ? 0 : 8 * (((int64_t)iterations + 7) / 8);
+  for (int64_t i = 0; i < n; ++i) {
+    const int32_t* o = origins + 3 * i;
+    if ((o[0] | o[1] | o[2]) & 7) return -1;
+    for (int a = 0; a < 3; ++a)
+      if (o[a] - margin < -kCoordLimit || o[a] + 8 + margin > kCoordLimit) return -1;
+    input[(size_t)i] = leaf_key(o[0], o[1], o[2]);
+    std::array<uint64_t, 8> w;
+    std::memcpy(w.data(), masks + 8 * i, 64);
+    if (!s.leaves.emplace(input[(size_t)i], w).second) return -1;  // repeated origin
+  }
+  const bool dilate_first = op == 0 || op == 2;
+  const int phases = op >= 2 ? 2 : 1;
+  for (int p = 0; p < phases; ++p) {
+    const bool dilate = (p == 0) == dilate_first;
+    for (int32_t it = 0; it < iterations; ++it) dilate ? morph_dilate1(s) : morph_erode1(s);
+  }
+  std::vector<std::array<int32_t, 3>> fresh;
+  for (uint64_t k : s.created) {
+    std::array<int32_t, 3> o;
+    key_origin(k, o.data());
+    fresh.push_back(o);
+  }
+  std::sort(fresh.begin(), fresh.end());
+  const int64_t total = n + (int64_t)fresh.size();
+  if (!out_origins && !out_masks) return total;
+  if (total > cap) return -total;
+  for (int64_t i = 0; i < total; ++i) {
+    int32_t o[3];
+    if (i < n) {
+      key_origin(input[(size_t)i], o);
+    } else {
+      std::memcpy(o, fresh[(size_t)(i - n)].data(), 12);
+    }
+    if (out_origins) std::memcpy(out_origins + 3 * i, o, 12);
+    if (out_masks) std::memcpy(out_masks + 8 * i, s.leaves[leaf_key(o[0], o[1], o[2])].data(), 64);
+  }
+  return total;
 }
 
 }  // extern "C"

@@ -18,7 +18,9 @@
  * forms U1-U3, the double tie-break between axes of different extent (U5),
  * raytrace clip/HDDA details (U6, U7, U9, U10), the fast-mode hits walk
  * (U11), leaf creation by no-op updates (U8), the section-apply visit of
- * internal-node tiles (U12) and the bounding-box corner transform (U13).
+ * internal-node tiles (U12), the bounding-box corner transform (U13) and the
+ * leaf set of the morphology operators (U14; their voxel states are pinned
+ * against scipy.ndimage's binary dilation and erosion).
  */
 #ifndef VDB_ORACLE_H
 #define VDB_ORACLE_H
@@ -122,6 +124,15 @@ int64_t orc_export_active_points(void* h, float* xyz, int64_t cap);
 /* createMapFromPointCloud (OCC:136-174) without the final pruneGrid. */
 int orc_create_from_points(void* h, const float* xyz, int64_t n, int64_t stride_bytes,
                            int32_t set_background, int32_t clear_map);
+
+/* The morphology members of VM:1095-1148 on a leaf set (U14): op 0 dilate,
+ * 1 erode, 2 close (dilate, then erode), 3 open (erode, then dilate), each by
+ * `iterations` 26-neighbour steps. Output: the n input leaves in input order
+ * with their new masks, then the leaves dilation created, sorted by origin.
+ * Null outputs return the count, a short cap -count, bad input -1. */
+int64_t orc_morphology(int32_t op, int32_t iterations, int64_t n, const int32_t* origins,
+                       const uint64_t* masks, int32_t* out_origins, uint64_t* out_masks,
+                       int64_t cap);
 
 #ifdef __cplusplus
 }

@@ -260,7 +260,71 @@ static void map_section_test()
   upd.applyMapSectionUpdateGrid(us);
   CHECK(upd.getGrid()->activeVoxelCount() == 7);
   CHECK(upd.getGrid()->getAccessor().isValueOn(openvdb::Coord(12, 6, 3)) == false);  // outside the box
-  dst.applyMapSectionGrid(section, true);  // smoothing refused: prints, changes nothing
+  // smooth_map closes the section first (in place, as the reference does):
+  // the same as closing it by hand and applying it unsmoothed
+  OccupancyVDBMapping sm(1), sm2(1);
+  for (auto* m : {&sm, &sm2})
+  {
+    m->setConfig(conf);
+    m->addInputSource("test", conf.max_range, 0);
+  }
+  auto us2 = src.getMapSectionUpdateGrid(lo, hi, tf, false);
+  auto us3 = src.getMapSectionUpdateGrid(lo, hi, tf, false);
+  sm.applyMapSectionUpdateGrid(us2, true, 1);
+  sm2.morphologicalCloseMap<OccupancyVDBMapping::UpdateGridT>(us3, 1);
+  sm2.applyMapSectionUpdateGrid(us3);
+  CHECK(us2->activeVoxelCount() == us3->activeVoxelCount() && us2->activeVoxelCount() >= 7);
+  CHECK(sm.getGrid()->activeVoxelCount() == us2->activeVoxelCount());
+  CHECK(sm2.getGrid()->activeVoxelCount() == us2->activeVoxelCount());
+  auto fs = src.getMapSectionGrid(lo, hi, tf, true);
+  const openvdb::Index64 before = fs->activeVoxelCount();
+  dst.applyMapSectionGrid(fs, true);
+  CHECK(fs->activeVoxelCount() >= before);
+}
+
+static void morphology_test()
+{
+  // closed forms on one voxel: a 3^3 / 5^3 cube after 1 / 2 dilation steps,
+  // back to one voxel after erosion; opening removes it; created leaves of a
+  // float grid hold the background value
+  OccupancyVDBMapping map(1);
+  map.setConfig(make_config(30));
+  auto g = OccupancyVDBMapping::UpdateGridT::create(false);
+  g->getAccessor().setValueOn(openvdb::Coord(0, 0, 0), true);
+  map.morphologicalDilateMap<OccupancyVDBMapping::UpdateGridT>(g, 1);
+  CHECK(g->activeVoxelCount() == 27);
+  CHECK(g->getAccessor().isValueOn(openvdb::Coord(-1, -1, -1)));
+  map.morphologicalErodeMap<OccupancyVDBMapping::UpdateGridT>(g, 1);
+  CHECK(g->activeVoxelCount() == 1 && g->getAccessor().isValueOn(openvdb::Coord(0, 0, 0)));
+  map.morphologicalDilateMap<OccupancyVDBMapping::UpdateGridT>(g, 2);
+  CHECK(g->activeVoxelCount() == 125);
+  map.morphologicalCloseMap<OccupancyVDBMapping::UpdateGridT>(g, 2);
+  CHECK(g->activeVoxelCount() == 125);
+  map.morphologicalOpenMap<OccupancyVDBMapping::UpdateGridT>(g, 2);  // a 5^3 cube survives a 5^3 opening
+  CHECK(g->activeVoxelCount() == 125);
+  map.morphologicalOpenMap<OccupancyVDBMapping::UpdateGridT>(g, 3);
+  CHECK(g->activeVoxelCount() == 0);
+  // closing fills a one-voxel gap
+  auto gap = OccupancyVDBMapping::UpdateGridT::create(false);
+  gap->getAccessor().setValueOn(openvdb::Coord(7, 0, 0), true);
+  gap->getAccessor().setValueOn(openvdb::Coord(9, 0, 0), true);
+  map.morphologicalCloseMap<OccupancyVDBMapping::UpdateGridT>(gap, 1);
+  CHECK(gap->activeVoxelCount() == 3 && gap->getAccessor().isValueOn(openvdb::Coord(8, 0, 0)));
+
+  auto f = OccupancyVDBMapping::GridT::create(1.0);
+  HostLeaf l;
+  std::memset(&l, 0, sizeof l);
+  l.values[0] = 2.5f;
+  l.values[1] = -1.0f;  // inactive voxel (0, 0, 1) keeps its value when activated
+  l.mask[0]   = 1;
+  f->insertLeaf(l);
+  map.morphologicalDilateMap<OccupancyVDBMapping::GridT>(f, 1);
+  CHECK(f->activeVoxelCount() == 27 && f->leafCount() == 8);
+  auto fa = f->getAccessor();
+  CHECK(fa.getValue(openvdb::Coord(0, 0, 0)) == 2.5f && fa.getValue(openvdb::Coord(0, 0, 1)) == -1.0f);
+  CHECK(fa.isValueOn(openvdb::Coord(-1, 0, 0)) && fa.getValue(openvdb::Coord(-1, 0, 0)) == 0.0f);
+  map.morphologicalErodeMap<OccupancyVDBMapping::GridT>(f, 1);
+  CHECK(f->activeVoxelCount() == 1 && f->leafCount() == 8);  // emptied leaves stay
 }
 
 static void artificial_areas_test()
@@ -374,6 +438,7 @@ int main()
     {"AddDataToAccumulate", add_data_to_accumulate_test},
     {"PointEdits", point_edits_test},
     {"MapSection", map_section_test},
+    {"Morphology", morphology_test},
     {"ArtificialAreas", artificial_areas_test},
     {"PcdPersistence", pcd_persistence_test},
   };

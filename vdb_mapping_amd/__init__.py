@@ -39,7 +39,7 @@ ABI_SYMBOLS = [
     "vdbgpu_server_destroy", "vdbgpu_set_points", "vdbgpu_add_artificial_areas",
     "vdbgpu_restore_map_integrity", "vdbgpu_export_artificial", "vdbgpu_add_artificial_walls", "vdbgpu_export_section",
     "vdbgpu_apply_section", "vdbgpu_apply_section_update", "vdbgpu_index_bbox",
-    "vdbgpu_export_active_points", "vdbgpu_create_from_points",
+    "vdbgpu_export_active_points", "vdbgpu_create_from_points", "vdbgpu_morphology",
 ]
 
 
@@ -199,6 +199,8 @@ def lib():
         L.vdbgpu_export_active_points.argtypes = [P, P, I64]
         L.vdbgpu_create_from_points.restype = I32
         L.vdbgpu_create_from_points.argtypes = [P, P, I64, I64, I32, I32]
+        L.vdbgpu_morphology.restype = I64
+        L.vdbgpu_morphology.argtypes = [P, I32, I32, I64, P, P, P, P, I64]
         L.vdbgpu_index_bbox.restype = I32
         L.vdbgpu_index_bbox.argtypes = [D, P, P, P, P]
         _lib = L
@@ -495,6 +497,25 @@ class GpuMap:
         rc = lib().vdbgpu_create_from_points(self.h, _ptr(pts), pts.shape[0], stride,
                                              int(bool(set_background)), int(bool(clear_map)))
         return self._check(rc, check)
+
+    # morphology of a leaf set (VM:1095-1148), on this map's device
+    MORPH_OPS = {"dilate": 0, "erode": 1, "close": 2, "open": 3}
+
+    def morphology(self, op: str, iterations: int, origins, masks):
+        """(origins (k, 3) int32, masks (k, 8) uint64) after op: the input
+        leaves in input order, then the leaves dilation created, by origin."""
+        o = np.ascontiguousarray(origins, dtype=np.int32).reshape(-1, 3)
+        m = np.ascontiguousarray(masks, dtype=np.uint64).reshape(-1, 8)
+        code = self.MORPH_OPS[op]
+        k = lib().vdbgpu_morphology(self.h, code, iterations, o.shape[0], _ptr(o), _ptr(m), None, None, 0)
+        if k < 0:
+            raise VdbGpuError(f"morphology failed: {self.last_error()}")
+        oo = np.zeros((max(k, 1), 3), np.int32)
+        om = np.zeros((max(k, 1), 8), np.uint64)
+        if lib().vdbgpu_morphology(self.h, code, iterations, o.shape[0], _ptr(o), _ptr(m), _ptr(oo), _ptr(om),
+                                   k) != k:
+            raise VdbGpuError(f"morphology failed: {self.last_error()}")
+        return oo[:k], om[:k]
 
     def raytrace(self, origins, dirs, lengths):
         o = np.ascontiguousarray(origins, dtype=np.float64).reshape(-1, 3)

@@ -119,4 +119,21 @@ hipError_t launch_box_keys(hipStream_t s, const int lo[3], const uint32_t dims[3
 hipError_t launch_box_fill(hipStream_t s, const unsigned long long* keys, const uint32_t* eidx, uint32_t n,
                            const int* box, MapView mp, float v);
 
+// Leaf table of the morphology operators (morph_kernels.hip): per leaf index
+// its origin (3 ints) and 27 neighbour indices; a hash of leaf key -> index.
+struct MorphView {
+  int* org;
+  unsigned long long* keys;  // hash slots, kEmpty = free
+  uint32_t* hidx;            // hash slot -> leaf index
+  uint32_t hmask;
+  uint32_t* nbr;
+};
+hipError_t launch_morph_seed(hipStream_t s, MorphView mv, uint32_t count);
+hipError_t launch_morph_grow(hipStream_t s, MorphView mv, unsigned long long* masks, uint8_t* keep, uint32_t count,
+                             uint32_t cap, unsigned int* added);
+hipError_t launch_morph_nbrs(hipStream_t s, MorphView mv, uint32_t count);
+hipError_t launch_morph_step(hipStream_t s, const unsigned long long* in, unsigned long long* out,
+                             const uint32_t* nbr, uint32_t count, bool erode);
+hipError_t launch_morph_keep(hipStream_t s, const unsigned long long* masks, uint8_t* keep, uint32_t count);
+
 }  // namespace vdbgpu

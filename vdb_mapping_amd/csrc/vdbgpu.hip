@@ -2296,6 +2296,190 @@ int32_t vdbgpu_apply_section_update(vdbgpu_map* m, const int32_t bbox[6], int64_
   return VDBGPU_OK;
 }
 
+// ------------------------------------------------------------ morphology --
+
+}  // extern "C"
+
+namespace {
+
+constexpr uint64_t kMorphMaxLeaves = 1ull << 26;
+
+// Device leaf table of one morphology call (morph_kernels.hip): entries
+// 0 .. count-1 by index (origin, mask ping-pong pair, keep flag, neighbour
+// indices) and the key hash over them.
+struct MorphTable {
+  vdbgpu_map* m;
+  uint32_t count = 0, ecap = 0, hcap = 0;
+  int* org = nullptr;
+  unsigned long long *ma = nullptr, *mb = nullptr, *keys = nullptr;
+  uint32_t *hidx = nullptr, *nbr = nullptr;
+  uint8_t* keep = nullptr;
+  unsigned int* added = nullptr;
+  explicit MorphTable(vdbgpu_map* mm) : m(mm) {}
+  ~MorphTable() {
+    (void)hipStreamSynchronize(m->stream);
+    free_entries();
+    dfree(m, &keys, hcap);
+    dfree(m, &hidx, hcap);
+    dfree(m, &added, 1);
+  }
+  void free_entries() {
+    dfree(m, &org, (size_t)ecap * 3);
+    dfree(m, &ma, (size_t)ecap * 8);
+    dfree(m, &mb, (size_t)ecap * 8);
+    dfree(m, &nbr, (size_t)ecap * 27);
+    dfree(m, &keep, ecap);
+  }
+  MorphView view() const { return MorphView{org, keys, hidx, hcap - 1, nbr}; }
+
+  // Room for `need` entries; the first `count` (origin, mask, keep) move along.
+  int reserve(uint32_t need) {
+    if (need <= ecap) return VDBGPU_OK;
+    int* o2 = nullptr;
+    unsigned long long *a2 = nullptr, *b2 = nullptr;
+    uint32_t* n2 = nullptr;
+    uint8_t* k2 = nullptr;
+    CK(dalloc(m, &o2, (size_t)need * 3));
+    CK(dalloc(m, &a2, (size_t)need * 8));
+    CK(dalloc(m, &b2, (size_t)need * 8));
+    CK(dalloc(m, &n2, (size_t)need * 27));
+    CK(dalloc(m, &k2, need));
+    if (count) {
+      CK(hipMemcpyAsync(o2, org, (size_t)count * 12, hipMemcpyDeviceToDevice, m->stream));
+      CK(hipMemcpyAsync(a2, ma, (size_t)count * 64, hipMemcpyDeviceToDevice, m->stream));
+      CK(hipMemcpyAsync(k2, keep, count, hipMemcpyDeviceToDevice, m->stream));
+    }
+    CK(hipStreamSynchronize(m->stream));
+    free_entries();
+    org = o2;
+    ma = a2;
+    mb = b2;
+    nbr = n2;
+    keep = k2;
+    ecap = need;
+    return VDBGPU_OK;
+  }
+
+  // Hash of the `count` entries, sized for ecap of them (at most half full).
+  int rehash() {
+    const uint32_t want = (uint32_t)std::max<uint64_t>(1024, next_pow2(2 * (uint64_t)ecap));
+    if (want != hcap) {
+      CK(hipStreamSynchronize(m->stream));
+      dfree(m, &keys, hcap);
+      dfree(m, &hidx, hcap);
+      CK(dalloc(m, &keys, want));
+      CK(dalloc(m, &hidx, want));
+      hcap = want;
+    }
+    CK(launch_fill_u64(m->stream, keys, hcap, kEmpty));
+    CK(launch_morph_seed(m->stream, view(), count));
+    return VDBGPU_OK;
+  }
+
+  // `iterations` dilation steps: before each run of up to 8 steps the table
+  // takes in the 26 neighbours of every leaf with an active voxel.
+  int dilate(int32_t iterations) {
+    for (int32_t left = iterations; left > 0;) {
+      const uint64_t bound = (uint64_t)count * 27;
+      if (bound > kMorphMaxLeaves)
+        return fail(m, VDBGPU_UNSUPPORTED, "morphology would exceed 2^26 leaves");
+      int r;
+      if ((r = reserve((uint32_t)bound)) || (r = rehash())) return r;
+      CK(hipMemsetAsync(added, 0, sizeof(unsigned int), m->stream));
+      CK(launch_morph_grow(m->stream, view(), ma, keep, count, ecap, added));
+      unsigned int h_added = 0;
+      CK(hipMemcpyAsync(&h_added, added, sizeof h_added, hipMemcpyDeviceToHost, m->stream));
+      CK(hipStreamSynchronize(m->stream));
+      count += h_added;
+      CK(launch_morph_nbrs(m->stream, view(), count));
+      const int32_t steps = std::min<int32_t>(8, left);
+      for (int32_t i = 0; i < steps; ++i) {
+        CK(launch_morph_step(m->stream, ma, mb, nbr, count, false));
+        std::swap(ma, mb);
+      }
+      left -= steps;
+    }
+    CK(launch_morph_keep(m->stream, ma, keep, count));
+    return VDBGPU_OK;
+  }
+
+  int erode(int32_t iterations, bool nbrs_current) {
+    if (!nbrs_current) {
+      int r;
+      if ((r = rehash())) return r;
+      CK(launch_morph_nbrs(m->stream, view(), count));
+    }
+    for (int32_t i = 0; i < iterations; ++i) {
+      CK(launch_morph_step(m->stream, ma, mb, nbr, count, true));
+      std::swap(ma, mb);
+    }
+    return VDBGPU_OK;
+  }
+};
+
+}  // namespace
+
+extern "C" {
+
+int64_t vdbgpu_morphology(vdbgpu_map* m, int32_t op, int32_t iterations, int64_t n, const int32_t* origins,
+                          const uint64_t* masks, int32_t* out_origins, uint64_t* out_masks, int64_t cap) {
+  if (!m) return -1;
+  Guard g(m);
+  if (op < 0 || op > 3 || n < 0 || (uint64_t)n > kMorphMaxLeaves || (n && (!origins || !masks))) {
+    fail(m, VDBGPU_BAD_ARGUMENT, "bad morphology arguments");
+    return -1;
+  }
+  if (check_origins(m, n, origins)) return -1;
+  // dilation moves the leaf set by at most one leaf per 8 steps; it must stay
+  // inside the leaf key range
+  const int64_t margin = op == 1 || iterations <= 0 ? 0 : 8 * (((int64_t)iterations + 7) / 8);
+  for (int64_t i = 0; i < 3 * n; ++i)
+    if (origins[i] - margin < -kCoordLimit || origins[i] + 8 + margin > kCoordLimit) {
+      fail(m, VDBGPU_OUT_OF_RANGE, "dilation would leave the voxel range [-2^23, 2^23)");
+      return -1;
+    }
+  std::vector<int32_t> ro(origins, origins + 3 * n);
+  std::vector<uint64_t> rm(masks, masks + 8 * n);
+  if (n && iterations > 0) {
+    MorphTable t(m);
+    const uint32_t N = (uint32_t)n;
+    if (t.reserve(N) || dalloc(m, &t.added, 1) != hipSuccess) return -1;
+    CKN(hipMemcpyAsync(t.org, origins, (size_t)N * 12, hipMemcpyHostToDevice, m->stream));
+    CKN(hipMemcpyAsync(t.ma, masks, (size_t)N * 64, hipMemcpyHostToDevice, m->stream));
+    CKN(hipMemsetAsync(t.keep, 0, N, m->stream));
+    t.count = N;
+    const bool dilate_first = op == 0 || op == 2;
+    for (int p = 0; p < (op >= 2 ? 2 : 1); ++p) {
+      const bool dilate = (p == 0) == dilate_first;
+      if (dilate ? t.dilate(iterations) : t.erode(iterations, p == 1)) return -1;
+    }
+    const uint32_t C = t.count;
+    std::vector<int32_t> o((size_t)C * 3);
+    std::vector<uint64_t> a((size_t)C * 8);
+    std::vector<uint8_t> k(C);
+    CKN(hipMemcpyAsync(o.data(), t.org, (size_t)C * 12, hipMemcpyDeviceToHost, m->stream));
+    CKN(hipMemcpyAsync(a.data(), t.ma, (size_t)C * 64, hipMemcpyDeviceToHost, m->stream));
+    CKN(hipMemcpyAsync(k.data(), t.keep, C, hipMemcpyDeviceToHost, m->stream));
+    CKN(hipStreamSynchronize(m->stream));
+    std::memcpy(rm.data(), a.data(), (size_t)N * 64);
+    // the leaves dilation created, sorted by origin
+    std::vector<std::pair<std::array<int32_t, 3>, uint32_t>> fresh;
+    for (uint32_t i = N; i < C; ++i)
+      if (k[i]) fresh.push_back({{o[3 * (size_t)i], o[3 * (size_t)i + 1], o[3 * (size_t)i + 2]}, i});
+    std::sort(fresh.begin(), fresh.end());
+    for (const auto& f : fresh) {
+      ro.insert(ro.end(), f.first.begin(), f.first.end());
+      rm.insert(rm.end(), a.begin() + (size_t)f.second * 8, a.begin() + (size_t)f.second * 8 + 8);
+    }
+  }
+  const int64_t total = (int64_t)(ro.size() / 3);
+  if (!out_origins && !out_masks) return total;
+  if (total > cap) return -total;
+  if (out_origins) std::memcpy(out_origins, ro.data(), ro.size() * sizeof(int32_t));
+  if (out_masks) std::memcpy(out_masks, rm.data(), rm.size() * sizeof(uint64_t));
+  return total;
+}
+
 // createIndexBoundingBox (VM:811-874): corners of [min, max] as float, each
 // transformed by the row-major 4x4 matrix in double (((m0 p0 + m1 p1) + m2 p2)
 // + m3, rounded to float: pcl::transformPointCloud), float min / max,
