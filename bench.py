@@ -43,7 +43,7 @@ def parse():
     p.add_argument("--cpu-scans", type=int, default=2, help="scans timed on the CPU oracle (0 = skip)")
     p.add_argument("--no-profiling", action="store_true", help="do not time kernels with HIP events")
     p.add_argument("--tuning", type=str, default="",
-                   help="block,steps_per_lane,waves_per_cu,load_factor[,traversal]")
+                   help="block,steps_per_lane,waves_per_cu,load_factor[,traversal[,defer]]")
     p.add_argument("--mode", choices=("plain", "batch", "mixed", "rgbd", "serve", "copy", "fast"),
                    default="plain")
     p.add_argument("--copy-mib", type=int, default=1024, help="copy mode: bytes per copy (MiB)")
@@ -367,7 +367,9 @@ def main():
         parts = a.tuning.split(",")
         b, spl, wpc, lf = parts[:4]
         trav = int(parts[4]) if len(parts) > 4 else 0
-        assert g.set_tuning(int(b), int(spl), int(wpc), float(lf), trav) == V.OK
+        defer = int(parts[5]) if len(parts) > 5 else 8
+        assert g.set_tuning(int(b), int(spl), int(wpc), float(lf), trav, defer) == V.OK
+    defer = g.get_tuning()["defer"]
 
     def step(i):
         j = i % a.scans
@@ -401,14 +403,15 @@ def main():
     # Roofline of the dominant kernel (largest time per scan), with its
     # algorithmic bytes: the voxel walk moves 12 B of point data per ray and
     # touches one 8-byte key per voxel visit; the log-odds apply reads a
-    # key/flag and reads and writes a float per unique voxel (16 B).
+    # key/flag and reads and writes a float per voxel update (16 B). With
+    # deferred scans the apply is one group pass per `defer` scans.
     candidates = []
     if d["n_walk"] > 0 and d["ms_walk"] > 0:
         candidates.append(("k_brick_walk", d["ms_walk"], d["n_walk"], 12.0 * rays + 8.0 * visits))
     elif d["n_traverse"] > 0 and d["ms_traverse"] > 0:
         candidates.append(("k_traverse", d["ms_traverse"], d["n_traverse"], 12.0 * rays + 8.0 * visits))
     if d["n_apply"] > 0 and d["ms_apply"] > 0:
-        candidates.append(("k_apply", d["ms_apply"], d["n_apply"], 16.0 * upd))
+        candidates.append(("k_apply_groups" if defer > 1 else "k_apply", d["ms_apply"], d["n_apply"], 16.0 * upd))
     rooflines = []
     for name, ms, n, alg_bytes in candidates:
         achieved = alg_bytes / (ms * 1e-3) / 1e9
@@ -421,7 +424,7 @@ def main():
                           "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": name,
                           "launches": n, "avg_launch_ms": round(ms / n, 4),
                           "alg_bytes_per_launch": round(alg_bytes / n)})
-    roofline = max(rooflines, key=lambda r: r["avg_launch_ms"]) if rooflines else None
+    roofline = max(rooflines, key=lambda r: r["avg_launch_ms"] * r["launches"]) if rooflines else None
 
     cpu = None
     checked = None
@@ -457,7 +460,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": "outdoor_stream_262k_0.05m", "scans": a.scans,
                    "points_per_scan": n_pts, "resolution": w.resolution, "max_range": w.max_range,
-                   "input": "device-resident", "path": "plain per-scan insertPointCloud"},
+                   "input": "device-resident", "path": "plain per-scan insertPointCloud",
+                   "defer": defer},
         "voxel_updates_per_s": round(upd / elapsed, 1),
         "visits_per_s": round(visits / elapsed, 1),
         "p50_scan_ms": round(float(np.percentile(lat_ms, 50)), 4),  # synchronous, one scan at a time

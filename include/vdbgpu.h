@@ -76,7 +76,10 @@ typedef struct vdbgpu_tuning {
   int32_t waves_per_cu;   /* resident waves per CU requested for the traversal grid */
   double load_factor;     /* update-set hash target load (0.3..0.9) */
   int32_t traversal;      /* plain inserts: 0 = brick walk (default), 1 = update-set hash traversal */
-  int32_t _pad;
+  int32_t defer;          /* brick-walk inserts: the map update of up to this many consecutive scans (1..8,
+                           * default 8) runs as one pass that reads and writes each leaf once and applies
+                           * the scans in insert order; any other call on the map (reads, edits, syncs)
+                           * applies the waiting scans first. 0 or 1: every scan is applied on its own. */
 } vdbgpu_tuning;
 
 typedef struct vdbgpu_map vdbgpu_map;

@@ -81,7 +81,7 @@ class Stats(ctypes.Structure):
 class Tuning(ctypes.Structure):
     _fields_ = [("block", ctypes.c_int32), ("steps_per_lane", ctypes.c_int32),
                 ("waves_per_cu", ctypes.c_int32), ("load_factor", ctypes.c_double),
-                ("traversal", ctypes.c_int32), ("_pad", ctypes.c_int32)]
+                ("traversal", ctypes.c_int32), ("defer", ctypes.c_int32)]
 
 
 class RequestRecord(ctypes.Structure):
@@ -563,9 +563,10 @@ class GpuMap:
         lib().vdbgpu_set_profiling(self.h, int(on))
 
     def set_tuning(self, block=256, steps_per_lane=32, waves_per_cu=16, load_factor=0.5,
-                   traversal=0) -> int:
-        """traversal: 0 = brick walk for plain inserts, 1 = update-set hash traversal."""
-        t = Tuning(block, steps_per_lane, waves_per_cu, load_factor, traversal, 0)
+                   traversal=0, defer=8) -> int:
+        """traversal: 0 = brick walk for plain inserts, 1 = update-set hash traversal;
+        defer: brick-walk scans whose map update runs as one pass (1 = every scan alone)."""
+        t = Tuning(block, steps_per_lane, waves_per_cu, load_factor, traversal, defer)
         return lib().vdbgpu_set_tuning(self.h, ctypes.byref(t))
 
     def get_tuning(self) -> dict:

@@ -25,6 +25,8 @@
 //   k_split_emit    leaf records of split bricks (positions after the items)
 //   k_mask_sums, k_mask_list  dense list of the filled record positions from
 //                   the per-item leaf masks (scan, no shared counters)
+//   k_rec_gather    the listed records copied densely into the scan's record
+//                   set (read by the map stream, possibly several scans later)
 //
 // Segment record (uint4): ray index; crossing count c0 (bits 0-13) | visits
 // m (bits 16-22) | end-voxel hit flag (bit 31); c1; c2. The walk of a
@@ -568,7 +570,52 @@ __global__ __launch_bounds__(256) void k_mask_list(const unsigned long long* __r
   for (uint64_t r = m; r; r &= r - 1) list[pos++] = g * 64 + (uint32_t)__builtin_ctzll(r);
 }
 
+// Dense copy of the listed records (key and 16 mask words each, sixteen lanes
+// per record) into the record set the map stream reads; the positional set is
+// then free for the next scan's walk. Copies nothing when the count exceeds
+// dcap (the host grows the set and runs it again).
+__global__ __launch_bounds__(256) void k_rec_gather(const uint32_t* __restrict__ list, UpdView sparse, UpdView dense,
+                                                    uint32_t dcap, const Status* st) {
+  if (st->error | st->overflow) return;
+  const uint32_t n = st->records;
+  if (n > dcap) return;
+  const int l = (int)(threadIdx.x & 15);
+  const uint64_t G = ((uint64_t)gridDim.x * blockDim.x) >> 4;
+  for (uint64_t i = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 4; i < n; i += G) {
+    const uint32_t pos = list[i];
+    dense.masks[i * 16 + l] = sparse.masks[(size_t)pos * 16 + l];
+    if (l == 0) dense.keys[i] = sparse.keys[pos];
+  }
+}
+
+// The map's leaf count into host-visible memory (one lane; no copy engine).
+__global__ void k_snap_count(const Status* st, uint32_t* host_dst) {
+  if (threadIdx.x == 0) __hip_atomic_store(host_dst, st->leaf_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+__global__ __launch_bounds__(256) void k_iota_u32(uint32_t* p, uint32_t n) {
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  if (i < n) p[i] = i;
+}
+
 // ---------------------------------------------------------------- launchers --
+
+hipError_t launch_rec_gather(hipStream_t s, int grid, const uint32_t* list, UpdView sparse, UpdView dense,
+                             uint32_t dcap, const Status* st) {
+  hipLaunchKernelGGL(k_rec_gather, dim3(grid), dim3(256), 0, s, list, sparse, dense, dcap, st);
+  return hipGetLastError();
+}
+
+hipError_t launch_snap_count(hipStream_t s, const Status* st, uint32_t* host_dst) {
+  hipLaunchKernelGGL(k_snap_count, dim3(1), dim3(64), 0, s, st, host_dst);
+  return hipGetLastError();
+}
+
+hipError_t launch_iota_u32(hipStream_t s, uint32_t* p, uint32_t n) {
+  if (!n) return hipSuccess;
+  hipLaunchKernelGGL(k_iota_u32, dim3((n + 255) / 256), dim3(256), 0, s, p, n);
+  return hipGetLastError();
+}
 
 hipError_t launch_seg_count(hipStream_t s, int grid, const int4* rayinfo, const ScanInfo* scans,
                             const BatchCtl* ctl, const double* tbl, BrickView bv, Status* st) {

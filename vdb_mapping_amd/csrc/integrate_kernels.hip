@@ -736,6 +736,159 @@ __global__ __launch_bounds__(256) void k_apply(UpdView u, const uint32_t* __rest
   }
 }
 
+// ---- deferred scans: several brick-path scans applied in one pass ----------
+// The records of up to kDeferMax consecutive scans (scan t of the group,
+// t = 0 .. nscan-1 in insert order) are grouped by map leaf; each group's leaf
+// is then read and written once while every scan's updates are applied to it
+// in scan order, in registers (the per-voxel sequence of clamped adds of
+// updateMap, VDBMapping.hpp:764-783, scan after scan). gslot (one entry per
+// map leaf, kNone outside a grouping) names the record that opened the group.
+
+// Lane per record of scan t: the first record of each map leaf opens a group.
+__global__ __launch_bounds__(256) void k_group_open(const uint32_t* __restrict__ eidx, uint32_t count, uint32_t t,
+                                                    uint32_t* gslot, uint32_t* rgrp, uint32_t rgrp_stride,
+                                                    uint32_t* glist, uint32_t* grec, unsigned int* ngroups) {
+  __shared__ uint32_t s_wave[4];
+  __shared__ uint32_t s_base;
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  const uint32_t e = i < count ? eidx[i] : kNone;
+  bool opener = false;
+  if (e != kNone) opener = atomicCAS(&gslot[e], kNone, (t << kGroupScanShift) | i) == kNone;
+  const uint32_t g = block_append(opener, ngroups, s_wave, &s_base);
+  if (opener) {
+    glist[g] = e;
+    rgrp[(size_t)t * rgrp_stride + i] = g;
+#pragma unroll
+    for (int k = 0; k < kDeferMax; ++k) grec[(size_t)g * kDeferMax + k] = kNone;
+  }
+}
+
+// Lane per record of scan t: enter the record's position in its group.
+__global__ __launch_bounds__(256) void k_group_fill(const uint32_t* __restrict__ eidx,
+                                                    const uint32_t* __restrict__ list, uint32_t count, uint32_t t,
+                                                    const uint32_t* __restrict__ gslot,
+                                                    const uint32_t* __restrict__ rgrp, uint32_t rgrp_stride,
+                                                    uint32_t* grec) {
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  if (i >= count) return;
+  const uint32_t e = eidx[i];
+  if (e == kNone) return;
+  const uint32_t code = gslot[e];
+  const uint32_t g = rgrp[(size_t)(code >> kGroupScanShift) * rgrp_stride + (code & kGroupRecMask)];
+  grec[(size_t)g * kDeferMax + t] = list[i];
+}
+
+// Group apply: sixteen lanes own one 64-voxel word of a group's leaf, as in
+// k_apply; the word's values and active bits are loaded once, every scan's
+// visited / hit bits of the word are applied in scan order, and the word is
+// stored once. A lane keeps only its own four bits per scan (4 x kDeferMax
+// bits of visited and of hit flags), so each thread carries kGroupWords words
+// per iteration with their three dependent load steps (group, masks, values)
+// issued together. Frees the group's gslot entry.
+constexpr int kGroupWords = 4;
+static_assert(kDeferMax == 8, "a group's record positions are read as two uint4");
+
+struct GroupWord {
+  uint32_t e, g;
+  int j;
+  uint32_t slot[kDeferMax];
+  uint32_t vb, hb;  // this lane's 4 bits of scan k at bits 4k .. 4k+3
+  uint64_t any;     // visited voxels of the word over all scans
+};
+
+__global__ __launch_bounds__(256) void k_apply_groups(GroupScans gs, const uint32_t* __restrict__ glist,
+                                                      const uint32_t* __restrict__ grec,
+                                                      const unsigned int* __restrict__ ngroups, uint32_t* gslot,
+                                                      MapView mp, float l_hit, float l_miss, float t_min,
+                                                      float t_max, float c_max, float c_min, Status* st) {
+  const uint64_t G = (uint64_t)gridDim.x * blockDim.x;
+  const uint64_t total = (uint64_t)(*ngroups) * 8 * 16;  // (group, word, lane) triples
+  const int l = lane_id() & 15;
+  unsigned long long vox = 0;
+  // the sixteen lanes of a word group always hold the same word (G is a
+  // multiple of 16), so their loads and the active-word reduction agree
+  for (uint64_t t0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t0 < total; t0 += kGroupWords * G) {
+    GroupWord w[kGroupWords];
+#pragma unroll
+    for (int q = 0; q < kGroupWords; ++q) {  // step 1: the group
+      const uint64_t t = t0 + q * G;
+      w[q].e = kNone;
+      w[q].vb = w[q].hb = 0;
+      w[q].any = 0;
+      w[q].j = (int)((t >> 4) & 7);
+      w[q].g = (uint32_t)(t >> 7);
+      if (t < total) {
+        w[q].e = glist[w[q].g];
+        const uint4* r = reinterpret_cast<const uint4*>(grec + (size_t)w[q].g * kDeferMax);
+        const uint4 a = r[0], b = r[1];
+        w[q].slot[0] = a.x, w[q].slot[1] = a.y, w[q].slot[2] = a.z, w[q].slot[3] = a.w;
+        w[q].slot[4] = b.x, w[q].slot[5] = b.y, w[q].slot[6] = b.z, w[q].slot[7] = b.w;
+      } else {
+#pragma unroll
+        for (int k = 0; k < kDeferMax; ++k) w[q].slot[k] = kNone;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < kGroupWords; ++q) {  // step 2: every scan's masks of the word
+#pragma unroll
+      for (int k = 0; k < kDeferMax; ++k) {
+        if (k < gs.nscan && w[q].slot[k] != kNone) {
+          const unsigned long long* m = gs.masks[k] + (size_t)w[q].slot[k] * 16;
+          const uint64_t v = m[w[q].j], h = m[8 + w[q].j];
+          w[q].any |= v;
+          w[q].vb |= ((uint32_t)(v >> (4 * l)) & 15u) << (4 * k);
+          w[q].hb |= ((uint32_t)(h >> (4 * l)) & 15u) << (4 * k);
+          if (l == 0) vox += __popcll(v);
+        }
+      }
+    }
+    float4 x[kGroupWords];
+    uint64_t aw[kGroupWords];
+#pragma unroll
+    for (int q = 0; q < kGroupWords; ++q) {  // step 3: the leaf word, once
+      x[q] = make_float4(0.f, 0.f, 0.f, 0.f);
+      aw[q] = 0;
+      if (w[q].any) {
+        uint32_t mine = 0;
+#pragma unroll
+        for (int k = 0; k < kDeferMax; ++k) mine |= (w[q].vb >> (4 * k)) & 15u;
+        if (mine) x[q] = reinterpret_cast<const float4*>(mp.values)[(size_t)w[q].e * 128 + w[q].j * 16 + l];
+        aw[q] = mp.active[(size_t)w[q].e * 8 + w[q].j];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < kGroupWords; ++q) {
+      if (l == 0 && w[q].j == 0 && w[q].e != kNone) gslot[w[q].e] = kNone;
+      if (!w[q].any) continue;  // the sixteen lanes of the word agree
+      uint32_t ab = (uint32_t)(aw[q] >> (4 * l)) & 15u, mine = 0;
+#pragma unroll
+      for (int k = 0; k < kDeferMax; ++k) {
+        const uint32_t vb = (w[q].vb >> (4 * k)) & 15u, hb = (w[q].hb >> (4 * k)) & 15u;
+        mine |= vb;
+        update_voxel(x[q].x, ab, 0, vb & 1, hb & 1, l_hit, l_miss, t_min, t_max, c_max, c_min);
+        update_voxel(x[q].y, ab, 1, vb & 2, hb & 2, l_hit, l_miss, t_min, t_max, c_max, c_min);
+        update_voxel(x[q].z, ab, 2, vb & 4, hb & 4, l_hit, l_miss, t_min, t_max, c_max, c_min);
+        update_voxel(x[q].w, ab, 3, vb & 8, hb & 8, l_hit, l_miss, t_min, t_max, c_max, c_min);
+      }
+      if (mine) reinterpret_cast<float4*>(mp.values)[(size_t)w[q].e * 128 + w[q].j * 16 + l] = x[q];
+      uint64_t na = (uint64_t)ab << (4 * l);
+#pragma unroll
+      for (int o = 1; o < 16; o <<= 1) na |= __shfl_xor(na, o, 16);
+      if (l == 0) mp.active[(size_t)w[q].e * 8 + w[q].j] = na;
+    }
+  }
+  __shared__ unsigned long long s_vox[4];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) vox += __shfl_xor(vox, o, 64);
+  if (lane_id() == 0) s_vox[threadIdx.x >> 6] = vox;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long tv = 0;
+    for (int k = 0; k < (int)(blockDim.x >> 6); ++k) tv += s_vox[k];
+    if (tv) atomicAdd(&st->voxels, tv);
+  }
+}
+
 // Move every entry of an update set into a larger one.
 __global__ __launch_bounds__(256) void k_rehash_update(UpdView from, uint32_t from_cap, UpdView to,
                                                        Status* st) {
@@ -902,6 +1055,31 @@ hipError_t launch_apply(hipStream_t s, int grid, UpdView u, const uint32_t* list
                         Status* st, bool clear) {
   hipLaunchKernelGGL(k_apply, dim3(grid), dim3(256), 0, s, u, list, eidx, count, mp, lo[0], lo[1],
                      lo[2], lo[3], lo[4], lo[5], st, (uint32_t)clear);
+  return hipGetLastError();
+}
+
+hipError_t launch_group_open(hipStream_t s, const uint32_t* eidx, uint32_t count, uint32_t t, uint32_t* gslot,
+                             uint32_t* rgrp, uint32_t rgrp_stride, uint32_t* glist, uint32_t* grec,
+                             unsigned int* ngroups) {
+  if (!count) return hipSuccess;
+  hipLaunchKernelGGL(k_group_open, dim3((count + 255) / 256), dim3(256), 0, s, eidx, count, t, gslot, rgrp,
+                     rgrp_stride, glist, grec, ngroups);
+  return hipGetLastError();
+}
+
+hipError_t launch_group_fill(hipStream_t s, const uint32_t* eidx, const uint32_t* list, uint32_t count, uint32_t t,
+                             const uint32_t* gslot, const uint32_t* rgrp, uint32_t rgrp_stride, uint32_t* grec) {
+  if (!count) return hipSuccess;
+  hipLaunchKernelGGL(k_group_fill, dim3((count + 255) / 256), dim3(256), 0, s, eidx, list, count, t, gslot, rgrp,
+                     rgrp_stride, grec);
+  return hipGetLastError();
+}
+
+hipError_t launch_apply_groups(hipStream_t s, int grid, const GroupScans& gs, const uint32_t* glist,
+                               const uint32_t* grec, const unsigned int* ngroups, uint32_t* gslot, MapView mp,
+                               const float lo[6], Status* st) {
+  hipLaunchKernelGGL(k_apply_groups, dim3(grid), dim3(256), 0, s, gs, glist, grec, ngroups, gslot, mp, lo[0],
+                     lo[1], lo[2], lo[3], lo[4], lo[5], st);
   return hipGetLastError();
 }
 

@@ -43,6 +43,26 @@ hipError_t launch_map_insert(hipStream_t s, int grid, UpdView u, const uint32_t*
                              uint32_t count, MapView mp, uint32_t* eidx, bool create_hit,
                              bool create_miss, unsigned long long* noop_set, uint32_t noop_mask,
                              Status* st, Status* stc);
+hipError_t launch_rec_gather(hipStream_t s, int grid, const uint32_t* list, UpdView sparse, UpdView dense,
+                             uint32_t dcap, const Status* st);
+hipError_t launch_iota_u32(hipStream_t s, uint32_t* p, uint32_t n);
+hipError_t launch_snap_count(hipStream_t s, const Status* st, uint32_t* host_dst);
+// Deferred brick-path scans applied as one group pass (integrate_kernels.hip).
+constexpr int kDeferMax = 8;                       // scans per group pass
+constexpr uint32_t kGroupScanShift = 29;           // gslot code: scan << 29 | record
+constexpr uint32_t kGroupRecMask = (1u << kGroupScanShift) - 1;
+struct GroupScans {
+  const unsigned long long* masks[kDeferMax];  // record masks of scan t (16 words per position)
+  int nscan;
+};
+hipError_t launch_group_open(hipStream_t s, const uint32_t* eidx, uint32_t count, uint32_t t, uint32_t* gslot,
+                             uint32_t* rgrp, uint32_t rgrp_stride, uint32_t* glist, uint32_t* grec,
+                             unsigned int* ngroups);
+hipError_t launch_group_fill(hipStream_t s, const uint32_t* eidx, const uint32_t* list, uint32_t count, uint32_t t,
+                             const uint32_t* gslot, const uint32_t* rgrp, uint32_t rgrp_stride, uint32_t* grec);
+hipError_t launch_apply_groups(hipStream_t s, int grid, const GroupScans& gs, const uint32_t* glist,
+                               const uint32_t* grec, const unsigned int* ngroups, uint32_t* gslot, MapView mp,
+                               const float lo[6], Status* st);
 hipError_t launch_apply(hipStream_t s, int grid, UpdView u, const uint32_t* list,
                         const uint32_t* eidx, uint32_t count, MapView mp, const float lo[6],
                         Status* st, bool clear);

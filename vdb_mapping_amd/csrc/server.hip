@@ -157,6 +157,14 @@ int32_t vdbgpu_server_create(int32_t n_maps, double resolution, const vdbgpu_con
     int32_t rc = vdbgpu_create(resolution, &s->lanes[i].map);
     if (rc == VDBGPU_OK) rc = vdbgpu_set_config(s->lanes[i].map, config);
     if (rc == VDBGPU_OK) rc = vdbgpu_add_source(s->lanes[i].map, "stream", max_range, 0.0);
+    if (rc == VDBGPU_OK) {
+      // every request is synchronised on its own: no deferred scans, and two
+      // record sets per map within the HBM budget
+      vdbgpu_tuning t;
+      rc = vdbgpu_get_tuning(s->lanes[i].map, &t);
+      t.defer = 1;
+      if (rc == VDBGPU_OK) rc = vdbgpu_set_tuning(s->lanes[i].map, &t);
+    }
     if (rc != VDBGPU_OK) {
       for (int j = 0; j <= i; ++j) vdbgpu_destroy(s->lanes[j].map);
       delete s;

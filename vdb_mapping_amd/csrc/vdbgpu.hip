@@ -15,6 +15,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <deque>
 #include <map>
 #include <mutex>
 #include <string>
@@ -84,13 +85,12 @@ struct vdbgpu_map {
   uint8_t* dirty = nullptr;
   uint32_t lcap = 0;
   uint64_t leaf_count_ub = 0;  // upper bound of the device leaf count
-  // Records whose map insert may still be running on the map stream, for the
-  // leaf-count bound taken between full syncs: win_a = those enqueued after
-  // the previous brick-path apply up to and including the last one, win_b =
-  // those enqueued after the last one. The cast of a brick-path scan waits for
-  // the previous brick apply, so a leaf-count snapshot taken there misses at
-  // most win_a + win_b leaves.
-  uint64_t win_a = 0, win_b = 0;
+  // Leaves that map-stream work enqueued since the last full sync may create,
+  // as (sequence number, count) in enqueue order: a leaf-count snapshot taken
+  // on the cast stream after it waited for an event recorded at sequence q
+  // misses at most the entries after q.
+  std::deque<std::pair<uint64_t, uint64_t>> ledger;
+  uint64_t ledger_seq = 0;
   bool touched = false;        // any voxel modify since creation/reset
   bool snap_dirty = true;      // the map changed since the last topology snapshot
   unsigned long long* noop4096 = nullptr;  // root children of updates that created no leaf
@@ -145,13 +145,46 @@ struct vdbgpu_map {
   uint32_t* split_slot = nullptr;
   unsigned long long* shared_bits = nullptr;
   uint32_t split_cap = 0;
-  // update-leaf records, double-buffered: the cast of scan i+1 fills one set
-  // while the map stream still applies scan i from the other
-  unsigned long long* rkeys[2] = {nullptr, nullptr};
-  unsigned long long* rmasks[2] = {nullptr, nullptr};
-  uint32_t* rlist[2] = {nullptr, nullptr};  // filled record positions (dense)
-  hipEvent_t rec_free[2] = {nullptr, nullptr};  // recorded on the map stream after the last read
+  // update-leaf records: the walk fills positional records (64 positions per
+  // work item, cast stream only), listed and copied densely into one record
+  // set of a ring; the cast of scan i+1 fills one set while the map stream
+  // still applies earlier scans from the others (two sets, or 2 defer + 2
+  // when scans are applied in groups)
+  static constexpr int kRecSets = 2 * kDeferMax + 2;
+  unsigned long long* rkeys = nullptr;
+  unsigned long long* rmasks = nullptr;
+  uint32_t* rlist = nullptr;  // filled record positions (dense)
+  unsigned long long* dkeys[kRecSets] = {};   // record set: leaf keys
+  unsigned long long* dmasks[kRecSets] = {};  // record set: 16 mask words per record
+  uint32_t* reidx[kRecSets] = {};             // record set: map leaf index of each record
+  uint32_t dcap[kRecSets] = {};
+  uint32_t dense_hint = 0;     // record-set size for a first use (largest scan seen, +25%)
+  uint32_t* ident = nullptr;   // 0, 1, 2, ...: the list of a dense record set
+  uint32_t ident_cap = 0;
+  hipEvent_t rec_free[kRecSets] = {};  // recorded on the map stream after the last read
+  // leaf count after the map insert of each record set's scan (pinned, copied
+  // on the map stream before ins_done), for a tight leaf-count bound
+  hipEvent_t ins_done[kRecSets] = {};
+  uint32_t* h_lc = nullptr;  // pinned; written by the device through d_lc
+  uint32_t* d_lc = nullptr;
+  uint64_t ins_seq[kRecSets] = {};  // ledger_seq after that insert
+  std::deque<int> ins_fifo;         // sets with an insert snapshot in flight, oldest first
+  uint64_t rec_seq[kRecSets] = {};     // ledger_seq when rec_free was recorded
   int rec_cur = 0;
+  int rec_sets = kRecSets;  // 2 when tuning.defer <= 1, else 2 defer + 2
+  // deferred scans (tuning.defer > 1): record sets whose map insert ran and
+  // whose apply waits for the group pass, in insert order
+  int npend = 0;
+  int pend_set[kDeferMax] = {};
+  uint32_t pend_recs[kDeferMax] = {};
+  uint32_t* gslot = nullptr;  // per map leaf: kNone, or the record that opened its group
+  uint32_t gslot_cap = 0;
+  uint32_t* rgrp = nullptr;   // per (scan, record): the group it opened
+  uint64_t rgrp_cap = 0;
+  uint32_t* gl = nullptr;     // per group: map leaf index
+  uint32_t* grec = nullptr;   // per group: record position of each scan (kDeferMax)
+  uint64_t grp_cap = 0;
+  unsigned int* ngroups = nullptr;
   uint32_t rec_cap = 0;        // record positions: 64 per work item and per split brick
   unsigned long long* leaf_mask = nullptr;  // filled positions per 64-position group
   uint64_t* lbsum = nullptr;
@@ -175,7 +208,7 @@ struct vdbgpu_map {
   Status* d_st = nullptr;
   Status* h_st = nullptr;
 
-  vdbgpu_tuning tuning{256, 32, 16, 0.5, 0, 0};
+  vdbgpu_tuning tuning{256, 32, 16, 0.5, 0, kDeferMax};
   bool profiling = false;
   std::vector<TimedPair> timed;
   std::vector<hipEvent_t> event_pool;
@@ -275,12 +308,45 @@ int fail(vdbgpu_map* m, int code, const std::string& msg) {
   return code;
 }
 
+void ledger_add(vdbgpu_map* m, uint64_t leaves) {
+  if (leaves) m->ledger.emplace_back(++m->ledger_seq, leaves);
+}
+
+// Leaves of the ledger entries after sequence q.
+uint64_t ledger_since(const vdbgpu_map* m, uint64_t q) {
+  uint64_t n = 0;
+  for (auto it = m->ledger.rbegin(); it != m->ledger.rend() && it->first > q; ++it) n += it->second;
+  return n;
+}
+
+// Upper bound of the leaf count once every enqueued map insert has run, taken
+// while the map stream may still be busy: the leaf count `cast_lc` that the
+// cast stream read after waiting for an event at ledger position cast_q, or
+// the newest map-insert snapshot already complete (no wait), each plus the
+// leaves enqueued after it; the smaller bound is used. Both kinds of
+// positions only grow, so ledger entries before both are dropped.
+uint64_t leaf_bound(vdbgpu_map* m, uint32_t cast_lc, uint64_t cast_q) {
+  uint64_t best = (uint64_t)cast_lc + ledger_since(m, cast_q);
+  uint64_t q = cast_q;
+  for (int k = (int)m->ins_fifo.size() - 1; k >= 0; --k) {
+    const int Y = m->ins_fifo[k];
+    if (hipEventQuery(m->ins_done[Y]) != hipSuccess) continue;
+    best = std::min<uint64_t>(best, (uint64_t)m->h_lc[Y] + ledger_since(m, m->ins_seq[Y]));
+    q = std::min(q, m->ins_seq[Y]);
+    m->ins_fifo.erase(m->ins_fifo.begin(), m->ins_fifo.begin() + k);  // older snapshots are not needed
+    break;
+  }
+  while (!m->ledger.empty() && m->ledger.front().first <= q) m->ledger.pop_front();
+  return best;
+}
+
 int read_status(vdbgpu_map* m) {
   CK(hipMemcpyAsync(m->h_st, m->d_st, 4 * sizeof(Status), hipMemcpyDeviceToHost, m->stream));
   CK(hipStreamSynchronize(m->stream));
   collect_timing(m);
   m->leaf_count_ub = m->h_st[1].leaf_count;
-  m->win_a = m->win_b = 0;
+  m->ledger.clear();
+  m->ins_fifo.clear();
   if (m->h_st[1].map_full)
     return fail(m, VDBGPU_DEVICE_ERROR, "leaf pool overflow in a map insert (internal bound violated)");
   return VDBGPU_OK;
@@ -429,6 +495,7 @@ int ensure_map(vdbgpu_map* m, uint64_t need) {
 void create_flags(const vdbgpu_map* m, bool* ch, bool* cm);
 int activate_artificial(vdbgpu_map* m);
 int clear_grid(vdbgpu_map* m);
+int flush_pending(vdbgpu_map* m);
 
 int ensure_groups(vdbgpu_map* m, uint32_t cap) {
   if (cap <= m->gcap) return VDBGPU_OK;
@@ -748,18 +815,14 @@ int ensure_bricks(vdbgpu_map* m, uint32_t bcap, uint32_t seg_cap, uint32_t split
     CK(dalloc(m, &m->shared_bits, (size_t)split_cap * 1024));
     CK(launch_fill_u64(m->cast, m->shared_bits, (uint64_t)split_cap * 1024, 0));
   }
-  if (rec_cap > m->rec_cap) {
-    for (int i = 0; i < 2; ++i) {
-      dfree(m, &m->rkeys[i], m->rec_cap);
-      dfree(m, &m->rmasks[i], (size_t)m->rec_cap * 16);
-      dfree(m, &m->rlist[i], m->rec_cap);
-    }
+  if (rec_cap > m->rec_cap) {  // positional records: read by the cast stream only
+    dfree(m, &m->rkeys, m->rec_cap);
+    dfree(m, &m->rmasks, (size_t)m->rec_cap * 16);
+    dfree(m, &m->rlist, m->rec_cap);
     m->rec_cap = rec_cap;
-    for (int i = 0; i < 2; ++i) {
-      CK(dalloc(m, &m->rkeys[i], rec_cap));
-      CK(dalloc(m, &m->rmasks[i], (size_t)rec_cap * 16));
-      CK(dalloc(m, &m->rlist[i], rec_cap));
-    }
+    CK(dalloc(m, &m->rkeys, rec_cap));
+    CK(dalloc(m, &m->rmasks, (size_t)rec_cap * 16));
+    CK(dalloc(m, &m->rlist, rec_cap));
   }
   const uint32_t groups = (m->item_cap + m->split_cap + 255) & ~255u;
   if (groups > m->group_cap) {
@@ -777,6 +840,108 @@ int ensure_bricks(vdbgpu_map* m, uint32_t bcap, uint32_t seg_cap, uint32_t split
 uint32_t grow_cap(uint64_t need, uint32_t cur) {
   const uint64_t want = std::max<uint64_t>(need + need / 4, (uint64_t)cur * 2);
   return (uint32_t)std::min<uint64_t>(want, 0xC0000000ull);
+}
+
+void free_set(vdbgpu_map* m, int X) {
+  dfree(m, &m->dkeys[X], m->dcap[X]);
+  dfree(m, &m->dmasks[X], (size_t)m->dcap[X] * 16);
+  dfree(m, &m->reidx[X], m->dcap[X]);
+  m->dcap[X] = 0;
+}
+
+// Room for `cap` records in record set X (which no pending scan holds).
+int grow_set(vdbgpu_map* m, int X, uint32_t cap) {
+  m->dense_hint = std::max(m->dense_hint, cap);
+  free_set(m, X);
+  CK(dalloc(m, &m->dkeys[X], cap));
+  CK(dalloc(m, &m->dmasks[X], (size_t)cap * 16));
+  CK(dalloc(m, &m->reidx[X], cap));
+  m->dcap[X] = cap;
+  return VDBGPU_OK;
+}
+
+// Group pass of the deferred scans (insert order t = 0 .. npend-1): group
+// their records by map leaf, apply every scan to each leaf in one read and
+// write (k_apply_groups), activate the artificial voxels once (values do not
+// depend on active states, so once after the last scan equals once after each)
+// and free the record sets.
+int flush_pending(vdbgpu_map* m) {
+  const int P = m->npend;
+  if (P == 0) return VDBGPU_OK;
+  if (P == 1) {  // one scan: the per-scan apply (no grouping to pay for)
+    const int X = m->pend_set[0];
+    const uint32_t recs = m->pend_recs[0];
+    const UpdView rec{m->dkeys[X], m->dmasks[X], 0};
+    {
+      Timer t(m, kApply);
+      CK(launch_apply(m->stream, std::min<int>(m->num_cu * 3, (int)((recs + 1) / 2)), rec, m->ident, m->reidx[X],
+                      recs, map_view(m), m->lo, &m->d_st[3], false));
+    }
+    m->npend = 0;
+    int r;
+    if ((r = activate_artificial(m))) return r;
+    CK(hipEventRecord(m->rec_free[X], m->stream));
+    m->rec_seq[X] = m->ledger_seq;
+    return VDBGPU_OK;
+  }
+  uint64_t total = 0;
+  uint32_t stride = 0;
+  for (int p = 0; p < P; ++p) {
+    total += m->pend_recs[p];
+    stride = std::max(stride, m->pend_recs[p]);
+  }
+  const bool grow_slots = m->gslot_cap < m->lcap, grow_rgrp = (uint64_t)kDeferMax * stride > m->rgrp_cap,
+             grow_grp = total > m->grp_cap;
+  if (grow_slots || grow_rgrp || grow_grp) CK(hipStreamSynchronize(m->stream));  // earlier passes done
+  if (grow_slots) {
+    dfree(m, &m->gslot, m->gslot_cap);
+    m->gslot_cap = m->lcap;
+    CK(dalloc(m, &m->gslot, m->gslot_cap));
+    CK(hipMemsetAsync(m->gslot, 0xFF, (size_t)m->gslot_cap * 4, m->stream));  // kNone
+  }
+  if (grow_rgrp) {
+    dfree(m, &m->rgrp, m->rgrp_cap);
+    m->rgrp_cap = (uint64_t)kDeferMax * next_pow2(stride);
+    CK(dalloc(m, &m->rgrp, m->rgrp_cap));
+  }
+  if (grow_grp) {
+    dfree(m, &m->gl, m->grp_cap);
+    dfree(m, &m->grec, m->grp_cap * kDeferMax);
+    m->grp_cap = std::max<uint64_t>(total, m->grp_cap * 2);
+    CK(dalloc(m, &m->gl, m->grp_cap));
+    CK(dalloc(m, &m->grec, m->grp_cap * kDeferMax));
+  }
+  if (!m->ngroups) CK(dalloc(m, &m->ngroups, 1));
+  const uint32_t rs = (uint32_t)(m->rgrp_cap / kDeferMax);
+  CK(hipMemsetAsync(m->ngroups, 0, sizeof(unsigned int), m->stream));
+  GroupScans gs{};
+  gs.nscan = P;
+  {
+    Timer t(m, kCompact);
+    for (int p = 0; p < P; ++p)
+      CK(launch_group_open(m->stream, m->reidx[m->pend_set[p]], m->pend_recs[p], (uint32_t)p, m->gslot, m->rgrp,
+                           rs, m->gl, m->grec, m->ngroups));
+    for (int p = 0; p < P; ++p) {
+      const int X = m->pend_set[p];
+      CK(launch_group_fill(m->stream, m->reidx[X], m->ident, m->pend_recs[p], (uint32_t)p, m->gslot, m->rgrp,
+                           rs, m->grec));
+      gs.masks[p] = m->dmasks[X];
+    }
+  }
+  {
+    Timer t(m, kApply);
+    CK(launch_apply_groups(m->stream, m->num_cu * 8, gs, m->gl, m->grec, m->ngroups, m->gslot, map_view(m), m->lo,
+                           &m->d_st[3]));
+  }
+  m->npend = 0;
+  int r;
+  if ((r = activate_artificial(m))) return r;
+  for (int p = 0; p < P; ++p) {
+    const int X = m->pend_set[p];
+    CK(hipEventRecord(m->rec_free[X], m->stream));
+    m->rec_seq[X] = m->ledger_seq;
+  }
+  return VDBGPU_OK;
 }
 
 // insertPointCloud of one scan when no source holds pending updates: the
@@ -807,9 +972,13 @@ int run_brick(vdbgpu_map* m, Source& src, const float* xyz, int64_t n, int64_t s
   // record set X: free once the map stream has applied the scan that last used it
   const int X = m->rec_cur;
   CK(hipStreamWaitEvent(cs, m->rec_free[X], 0));
+  if (m->dcap[X] < m->dense_hint) {  // a first use of set X: size it like the others
+    CK(hipStreamSynchronize(cs));    // (the map stream is done with it)
+    if ((r = grow_set(m, X, m->dense_hint))) return r;
+  }
   for (int attempt = 0;; ++attempt) {
     const BrickView bv{m->bkeys, m->bcnt, m->bcnt + m->bcap, m->bcap - 1};
-    const UpdView rec{m->rkeys[X], m->rmasks[X], 0};
+    const UpdView rec{m->rkeys, m->rmasks, 0};
     if (attempt > 0) {  // the first attempt's table was cleared with the staging
       CK(launch_fill_u64(cs, m->bkeys, m->bcap, kEmpty));
       CK(launch_fill_u64(cs, reinterpret_cast<unsigned long long*>(m->bcnt), m->bcap, 0));
@@ -831,15 +1000,18 @@ int run_brick(vdbgpu_map* m, Source& src, const float* xyz, int64_t n, int64_t s
       Timer t(m, kCompact, cs);
       CK(launch_split_emit(cs, m->num_cu, m->split_slot, m->shared_bits, bv, rec, m->leaf_mask,
                            m->rec_cap, &m->d_st[0]));
-      CK(launch_rec_list(cs, m->leaf_mask, m->lbsum, m->lboff, m->group_cap / 256, m->rlist[X],
-                         &m->d_st[0]));
+      CK(launch_rec_list(cs, m->leaf_mask, m->lbsum, m->lboff, m->group_cap / 256, m->rlist, &m->d_st[0]));
+      CK(launch_rec_gather(cs, m->num_cu * 8, m->rlist, rec, UpdView{m->dkeys[X], m->dmasks[X], 0}, m->dcap[X],
+                           &m->d_st[0]));
     }
+    // a full group of deferred scans: enqueued now, while the cast runs
+    if (attempt == 0 && m->npend >= m->tuning.defer && (r = flush_pending(m))) return r;
     // the scan's status and a snapshot of the leaf count: the map stream keeps
     // applying the previous scan meanwhile
     CK(hipMemcpyAsync(m->h_st, m->d_st, 2 * sizeof(Status), hipMemcpyDeviceToHost, cs));
     CK(hipStreamSynchronize(cs));
     collect_timing(m);
-    m->leaf_count_ub = (uint64_t)m->h_st[1].leaf_count + m->win_a + m->win_b;
+    m->leaf_count_ub = leaf_bound(m, m->h_st[1].leaf_count, m->rec_seq[X]);
     if (m->h_st[1].map_full)
       return fail(m, VDBGPU_DEVICE_ERROR, "leaf pool overflow in a map insert (internal bound violated)");
     const Status st = m->h_st[0];
@@ -859,49 +1031,75 @@ int run_brick(vdbgpu_map* m, Source& src, const float* xyz, int64_t n, int64_t s
     CK(hipMemsetAsync(&m->d_st[0].segs, 0, 6 * sizeof(unsigned int), cs));  // segs .. pad
   }
   const Status& st = m->h_st[0];
-  const uint32_t recs = st.records;  // listed in m->rlist[X]
+  const uint32_t recs = st.records;  // listed in m->rlist, copied to set X when it had room
   m->stats.scans += 1;
   m->stats.rays += st.rays;
   m->stats.visits += st.total;
-  m->rec_cur ^= 1;
+  m->rec_cur = (X + 1) % m->rec_sets;
   if (recs == 0) {
     CK(hipEventRecord(m->rec_free[X], m->stream));  // orders the next scan's snapshot as usual
-    m->win_a = m->win_b;
-    m->win_b = 0;
+    m->rec_seq[X] = m->ledger_seq;
     return VDBGPU_OK;
   }
   const uint32_t art_new = m->art_leaves ? 0 : m->art_count;  // leaves the activation may create
   if ((r = ensure_map(m, m->leaf_count_ub + recs + art_new))) return r;
-  if (recs > m->eidx_cap) {
-    dfree(m, &m->eidx, m->eidx_cap);
-    m->eidx_cap = (uint32_t)next_pow2(recs);
-    CK(dalloc(m, &m->eidx, m->eidx_cap));
+  if (recs > m->dcap[X]) {  // set X (no pending scan holds it) grows; the copy runs again
+    if ((r = grow_set(m, X, (uint32_t)std::min<uint64_t>(recs + (uint64_t)recs / 4, 0xC0000000ull)))) return r;
+    CK(launch_rec_gather(cs, m->num_cu * 8, m->rlist, UpdView{m->rkeys, m->rmasks, 0},
+                         UpdView{m->dkeys[X], m->dmasks[X], 0}, m->dcap[X], &m->d_st[0]));
+    CK(hipStreamSynchronize(cs));
+  }
+  if (recs > m->ident_cap) {
+    dfree(m, &m->ident, m->ident_cap);
+    m->ident_cap = m->dcap[X];
+    CK(dalloc(m, &m->ident, m->ident_cap));
+    CK(launch_iota_u32(m->stream, m->ident, m->ident_cap));
   }
   bool ch, cm;
   create_flags(m, &ch, &cm);
-  const UpdView rec{m->rkeys[X], m->rmasks[X], 0};
+  const UpdView rec{m->dkeys[X], m->dmasks[X], 0};
   const int g1 = std::min<int>(4096, (int)((recs + 255) / 256));
+  {
+    Timer t(m, kCompact);
+    CK(launch_map_insert(m->stream, g1, rec, m->ident, recs, map_view(m), m->reidx[X], ch, cm, m->noop4096,
+                         vdbgpu_map::kNoopCap - 1, &m->d_st[1], &m->d_st[3]));
+  }
+  m->leaf_count_ub += recs;
+  ledger_add(m, recs);
+  CK(launch_snap_count(m->stream, &m->d_st[1], &m->d_lc[X]));
+  CK(hipEventRecord(m->ins_done[X], m->stream));
+  m->ins_seq[X] = m->ledger_seq;
+  for (auto it = m->ins_fifo.begin(); it != m->ins_fifo.end(); ++it)
+    if (*it == X) {  // its earlier snapshot slot is being rewritten
+      m->ins_fifo.erase(it);
+      break;
+    }
+  m->ins_fifo.push_back(X);
+  m->touched = true;
+  m->snap_dirty = true;
+  if (m->rec_sets > 2 && recs <= kGroupRecMask) {
+    // applied with the next group pass, when defer scans are waiting or the
+    // map is read or changed otherwise
+    // a full group is enqueued by the next insert while its cast runs (or by
+    // any other call first)
+    if (m->npend >= kDeferMax && (r = flush_pending(m))) return r;
+    m->pend_set[m->npend] = X;
+    m->pend_recs[m->npend] = recs;
+    ++m->npend;
+    return VDBGPU_OK;
+  }
+  if ((r = flush_pending(m))) return r;  // earlier scans first
   // 128 lanes per leaf; three workgroups per CU (each lane keeps two words in
   // flight) leave the CUs to the next scan's cast: fewer apply waves slow the
   // apply less than they speed up the overlapped cast (measured 3-6 per CU)
   const int g2 = std::min<int>(m->num_cu * 3, (int)((recs + 1) / 2));
   {
-    Timer t(m, kCompact);
-    CK(launch_map_insert(m->stream, g1, rec, m->rlist[X], recs, map_view(m), m->eidx, ch, cm, m->noop4096,
-                         vdbgpu_map::kNoopCap - 1, &m->d_st[1], &m->d_st[3]));
-  }
-  {
     Timer t(m, kApply);
-    CK(launch_apply(m->stream, g2, rec, m->rlist[X], m->eidx, recs, map_view(m), m->lo, &m->d_st[3], false));
+    CK(launch_apply(m->stream, g2, rec, m->ident, m->reidx[X], recs, map_view(m), m->lo, &m->d_st[3], false));
   }
-  m->leaf_count_ub += recs;
-  m->win_b += recs;
-  if ((r = activate_artificial(m))) return r;  // counts its new leaves into win_b
+  if ((r = activate_artificial(m))) return r;  // enters its new leaves in the ledger
   CK(hipEventRecord(m->rec_free[X], m->stream));
-  m->win_a = m->win_b;
-  m->win_b = 0;
-  m->touched = true;
-  m->snap_dirty = true;
+  m->rec_seq[X] = m->ledger_seq;
   return VDBGPU_OK;
 }
 
@@ -993,6 +1191,12 @@ int refresh_snapshot(vdbgpu_map* m) {
 int do_integrate(vdbgpu_map* m) {
   bool ch, cm;
   create_flags(m, &ch, &cm);
+  for (auto& kv : m->sources)
+    if (kv.second.pending) {
+      const int r = flush_pending(m);  // deferred brick-path scans come first
+      if (r) return r;
+      break;
+    }
   bool applied = false;
   for (auto& kv : m->sources) {
     Source& s = kv.second;
@@ -1017,7 +1221,7 @@ int do_integrate(vdbgpu_map* m) {
                       &m->d_st[3], true));
     }
     m->leaf_count_ub += s.pending;
-    m->win_b += s.pending;
+    ledger_add(m, s.pending);
     m->touched = true;
     m->snap_dirty = true;
     s.pending = 0;
@@ -1116,7 +1320,7 @@ int activate_artificial(vdbgpu_map* m) {
   if (fresh && (r = ensure_map(m, m->leaf_count_ub + fresh))) return r;
   if ((r = apply_edit(m, m->art, m->art_count, m->art_eidx, kEditActivate, 0.0f, true, false))) return r;
   m->leaf_count_ub += fresh;
-  m->win_b += fresh;
+  ledger_add(m, fresh);
   m->art_leaves = true;
   return VDBGPU_OK;
 }
@@ -1131,7 +1335,7 @@ int restore_integrity(vdbgpu_map* m) {
     if ((r = apply_edit(m, m->art, m->art_count, m->art_eidx, kEditState, 0.0f, need, true))) return r;
     if (need) {
       m->leaf_count_ub += m->art_count;
-      m->win_b += m->art_count;
+      ledger_add(m, m->art_count);
     }
   }
   m->art_count = 0;
@@ -1167,10 +1371,17 @@ int check_origins(vdbgpu_map* m, int64_t n, const int32_t* origins) {
   return VDBGPU_OK;
 }
 
+// Lock of one ABI call. Every call but a plain insert first applies the
+// deferred scans, so whatever it reads or changes sees the map of every
+// earlier insert (a failure there is left in last_error and shows again at
+// the call's own status read).
 struct Guard {
   vdbgpu_map* m;
   std::lock_guard<std::mutex> lk;
-  explicit Guard(vdbgpu_map* mm) : m(mm), lk(mm->mu) { (void)hipSetDevice(m->device); }
+  explicit Guard(vdbgpu_map* mm, bool keep_deferred = false) : m(mm), lk(mm->mu) {
+    (void)hipSetDevice(m->device);
+    if (!keep_deferred) (void)flush_pending(m);
+  }
 };
 
 int64_t export_sorted(vdbgpu_map* m, std::vector<unsigned long long>& keys, std::vector<float>& vals,
@@ -1226,6 +1437,16 @@ int32_t vdbgpu_device_count(void) {
   return n;
 }
 
+static bool create_rec_events(vdbgpu_map* m) {
+  for (int i = 0; i < vdbgpu_map::kRecSets; ++i)
+    if (hipEventCreateWithFlags(&m->rec_free[i], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&m->ins_done[i], hipEventDisableTiming) != hipSuccess)
+      return false;
+  return hipHostMalloc((void**)&m->h_lc, vdbgpu_map::kRecSets * sizeof(uint32_t), hipHostMallocDefault) ==
+             hipSuccess &&
+         hipHostGetDevicePointer((void**)&m->d_lc, m->h_lc, 0) == hipSuccess;
+}
+
 int32_t vdbgpu_create(double resolution, vdbgpu_map** out) {
   if (!out) return VDBGPU_BAD_ARGUMENT;
   *out = nullptr;
@@ -1244,8 +1465,7 @@ int32_t vdbgpu_create(double resolution, vdbgpu_map** out) {
     m->num_cu = cu;
   if (hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) != hipSuccess ||
       hipStreamCreateWithFlags(&m->cast, hipStreamNonBlocking) != hipSuccess ||
-      hipEventCreateWithFlags(&m->rec_free[0], hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&m->rec_free[1], hipEventDisableTiming) != hipSuccess ||
+      !create_rec_events(m) ||
       hipMalloc((void**)&m->d_st, 4 * sizeof(Status)) != hipSuccess ||
       hipHostMalloc((void**)&m->h_st, 4 * sizeof(Status), hipHostMallocDefault) != hipSuccess ||
       hipMemsetAsync(m->d_st, 0, 4 * sizeof(Status), m->stream) != hipSuccess) {
@@ -1298,12 +1518,21 @@ void vdbgpu_destroy(vdbgpu_map* m) {
   dfree(m, &m->items, m->item_cap);
   dfree(m, &m->split_slot, m->split_cap);
   dfree(m, &m->shared_bits, (size_t)m->split_cap * 1024);
-  for (int i = 0; i < 2; ++i) {
-    dfree(m, &m->rkeys[i], m->rec_cap);
-    dfree(m, &m->rmasks[i], (size_t)m->rec_cap * 16);
-    dfree(m, &m->rlist[i], m->rec_cap);
+  dfree(m, &m->rkeys, m->rec_cap);
+  dfree(m, &m->rmasks, (size_t)m->rec_cap * 16);
+  dfree(m, &m->rlist, m->rec_cap);
+  for (int i = 0; i < vdbgpu_map::kRecSets; ++i) {
+    free_set(m, i);
     if (m->rec_free[i]) (void)hipEventDestroy(m->rec_free[i]);
+    if (m->ins_done[i]) (void)hipEventDestroy(m->ins_done[i]);
   }
+  if (m->h_lc) (void)hipHostFree(m->h_lc);
+  dfree(m, &m->ident, m->ident_cap);
+  dfree(m, &m->gslot, m->gslot_cap);
+  dfree(m, &m->rgrp, m->rgrp_cap);
+  dfree(m, &m->gl, m->grp_cap);
+  dfree(m, &m->grec, m->grp_cap * kDeferMax);
+  dfree(m, &m->ngroups, 1);
   dfree(m, &m->leaf_mask, m->group_cap);
   dfree(m, &m->lbsum, m->group_cap / 256);
   dfree(m, &m->lboff, m->group_cap / 256);
@@ -1413,7 +1642,7 @@ int32_t vdbgpu_integrate(vdbgpu_map* m) {
 int32_t vdbgpu_insert(vdbgpu_map* m, const char* id, const float* xyz, int64_t n, int64_t stride,
                       const double origin[3], int32_t on_device) {
   if (!m || !id) return VDBGPU_BAD_ARGUMENT;
-  Guard g(m);
+  Guard g(m, true);  // a brick-path scan joins the deferred ones
   int status = VDBGPU_OK;
   auto it = m->sources.find(id);
   if (it == m->sources.end())
@@ -1772,9 +2001,18 @@ int32_t vdbgpu_set_tuning(vdbgpu_map* m, const vdbgpu_tuning* t) {
   if (!m || !t) return VDBGPU_BAD_ARGUMENT;
   if (t->block < 64 || t->block > kTraverseMaxBlock || (t->block & 63) || t->steps_per_lane < 1 ||
       t->steps_per_lane > 1 << 20 || t->waves_per_cu < 1 || t->waves_per_cu > 64 ||
-      !(t->load_factor >= 0.2 && t->load_factor <= 0.95) || t->traversal < 0 || t->traversal > 1)
+      !(t->load_factor >= 0.2 && t->load_factor <= 0.95) || t->traversal < 0 || t->traversal > 1 ||
+      t->defer < 0 || t->defer > kDeferMax)
     return VDBGPU_BAD_ARGUMENT;
-  Guard g(m);
+  Guard g(m);  // applies the scans deferred under the old setting
+  const int sets = t->defer > 1 ? t->defer + 2 : 2;
+  if (sets != m->rec_sets) {  // a new ring of record sets
+    CK(hipStreamSynchronize(m->cast));
+    CK(hipStreamSynchronize(m->stream));
+    for (int i = 0; i < vdbgpu_map::kRecSets; ++i) free_set(m, i);
+    m->rec_sets = sets;
+    m->rec_cur = 0;
+  }
   m->tuning = *t;
   return VDBGPU_OK;
 }
@@ -1822,7 +2060,7 @@ int apply_points(vdbgpu_map* m, uint32_t count, bool occupied) {
   if (!(r = ensure_map(m, m->leaf_count_ub + count)) && !(r = ensure_eidx(m, count)) &&
       !(r = apply_edit(m, m->edit, count, m->eidx, occupied ? kEditOccupied : kEditFree, val, need, true))) {
     m->leaf_count_ub += count;
-    m->win_b += count;
+    ledger_add(m, count);
     return VDBGPU_OK;
   }
   (void)clear_set(m, m->edit);  // no stale marks for the next call
@@ -1841,7 +2079,8 @@ int clear_grid(vdbgpu_map* m) {
   CK(launch_fill_u64(m->stream, m->noop4096, vdbgpu_map::kNoopCap, kEmpty));
   CK(hipMemsetAsync(&m->d_st[1], 0, sizeof(Status), m->stream));
   m->leaf_count_ub = 0;
-  m->win_a = m->win_b = 0;
+  m->ledger.clear();
+  m->ins_fifo.clear();
   m->touched = false;
   m->snap_dirty = true;
   m->art_leaves = false;
@@ -1912,7 +2151,7 @@ int32_t vdbgpu_create_from_points(vdbgpu_map* m, const float* xyz, int64_t n, in
   CK(launch_upsert_leaves(m->stream, keys, needs, N, map_view(m), eidx, nullptr, &m->d_st[1]));
   CK(launch_box_fill(m->stream, keys, eidx, N, d_box, map_view(m), m->lo[5]));
   m->leaf_count_ub += N;
-  m->win_b += N;
+  ledger_add(m, N);
   m->snap_dirty = true;
   if ((r = read_status(m))) return r;
   m->touched = m->touched || m->h_st[1].leaf_count > 0;
@@ -2192,7 +2431,7 @@ int section_upsert(vdbgpu_map* m, const unsigned long long* keys, const uint8_t*
   CK(dalloc(m, eidx_out, n));
   CK(launch_upsert_leaves(m->stream, keys, need, n, map_view(m), *eidx_out, first, &m->d_st[1]));
   m->leaf_count_ub += n;
-  m->win_b += n;
+  ledger_add(m, n);
   m->snap_dirty = true;
   return VDBGPU_OK;
 }
