@@ -903,7 +903,8 @@ __global__ __launch_bounds__(256) void k_rehash_update(UpdView from, uint32_t fr
 }
 
 // Rebuild the map leaf hash from the leaf pool's key list.
-__global__ __launch_bounds__(256) void k_rehash_map(MapView mp, uint32_t leaf_count) {
+__global__ __launch_bounds__(256) void k_rehash_map(MapView mp, uint32_t leaf_count, const Status* stc) {
+  if (stc) leaf_count = stc->leaf_count;  // the count at this point of the stream
   const uint32_t G = gridDim.x * blockDim.x;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < leaf_count; i += G) {
     const uint64_t key = mp.leaf_keys[i];
@@ -1089,8 +1090,8 @@ hipError_t launch_rehash_update(hipStream_t s, int grid, UpdView from, uint32_t 
   return hipGetLastError();
 }
 
-hipError_t launch_rehash_map(hipStream_t s, int grid, MapView mp, uint32_t leaf_count) {
-  hipLaunchKernelGGL(k_rehash_map, dim3(grid), dim3(256), 0, s, mp, leaf_count);
+hipError_t launch_rehash_map(hipStream_t s, int grid, MapView mp, uint32_t leaf_count, const Status* stc) {
+  hipLaunchKernelGGL(k_rehash_map, dim3(grid), dim3(256), 0, s, mp, leaf_count, stc);
   return hipGetLastError();
 }
 

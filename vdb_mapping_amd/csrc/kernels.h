@@ -68,7 +68,8 @@ hipError_t launch_apply(hipStream_t s, int grid, UpdView u, const uint32_t* list
                         Status* st, bool clear);
 hipError_t launch_rehash_update(hipStream_t s, int grid, UpdView from, uint32_t from_cap,
                                 UpdView to, Status* st);
-hipError_t launch_rehash_map(hipStream_t s, int grid, MapView mp, uint32_t leaf_count);
+// stc non-null: the leaf count is read from stc->leaf_count on the device.
+hipError_t launch_rehash_map(hipStream_t s, int grid, MapView mp, uint32_t leaf_count, const Status* stc);
 hipError_t launch_fill_u64(hipStream_t s, unsigned long long* p, uint64_t n, uint64_t v);
 hipError_t launch_stage_scan(hipStream_t s, const ScanInfo& sc, const BatchCtl& ctl, ScanInfo* d_scans,
                              BatchCtl* d_ctl, Status* st, unsigned long long* bkeys, unsigned long long* bcnt,

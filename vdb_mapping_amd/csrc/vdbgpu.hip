@@ -60,6 +60,14 @@ uint64_t next_pow2(uint64_t v) {
 
 }  // namespace
 
+// A device array that grows in place: an address range reserved once and
+// backed by physical chunks mapped at its end (HIP virtual memory).
+struct GrowArray {
+  char* base = nullptr;
+  size_t reserved = 0, mapped = 0;
+  std::vector<std::pair<hipMemGenericAllocationHandle_t, size_t>> chunks;
+};
+
 struct vdbgpu_map {
   std::mutex mu;
   int device = 0;
@@ -84,6 +92,13 @@ struct vdbgpu_map {
   unsigned long long* leaf_keys = nullptr;
   uint8_t* dirty = nullptr;
   uint32_t lcap = 0;
+  // the four leaf-pool arrays in reserved address ranges that grow in place
+  // (physical chunks mapped at the end; vmm = 1), or plain allocations that
+  // grow by copy (vmm = -1); 0 until the first leaf
+  int vmm = 0;
+  GrowArray g_values, g_active, g_keys, g_dirty;
+  uint64_t vmm_leaves = 0;  // leaves the reserved ranges can hold
+  size_t vmm_gran = 0;
   uint64_t leaf_count_ub = 0;  // upper bound of the device leaf count
   // Leaves that map-stream work enqueued since the last full sync may create,
   // as (sequence number, count) in enqueue order: a leaf-count snapshot taken
@@ -442,6 +457,86 @@ int ensure_table(vdbgpu_map* m, uint32_t need, hipStream_t stream) {
   return VDBGPU_OK;
 }
 
+void ga_release(vdbgpu_map* m, GrowArray& g) {
+  size_t off = 0;
+  for (auto& c : g.chunks) {
+    (void)hipMemUnmap(g.base + off, c.second);
+    (void)hipMemRelease(c.first);
+    off += c.second;
+    m->device_bytes -= c.second;
+  }
+  g.chunks.clear();
+  if (g.base) (void)hipMemAddressFree(g.base, g.reserved);
+  g = GrowArray{};
+}
+
+// Map physical memory behind g up to at least `bytes` (a multiple of gran),
+// zero-filled on the map stream. Access is granted over the whole mapped
+// range (the driver refuses some ranges that start inside a reservation).
+// False, with the new chunk released, when the driver refuses.
+bool ga_grow(vdbgpu_map* m, GrowArray& g, size_t bytes, size_t gran) {
+  bytes = (bytes + gran - 1) / gran * gran;
+  if (bytes <= g.mapped) return true;
+  if (bytes > g.reserved) return false;
+  const size_t sz = bytes - g.mapped;
+  hipMemAllocationProp prop{};
+  prop.type = hipMemAllocationTypePinned;
+  prop.location.type = hipMemLocationTypeDevice;
+  prop.location.id = m->device;
+  hipMemGenericAllocationHandle_t h;
+  if (hipMemCreate(&h, sz, &prop, 0) != hipSuccess) return false;
+  if (hipMemMap(g.base + g.mapped, sz, 0, h, 0) != hipSuccess) {
+    (void)hipMemRelease(h);
+    return false;
+  }
+  hipMemAccessDesc acc{};
+  acc.location = prop.location;
+  acc.flags = hipMemAccessFlagsProtReadWrite;
+  if (hipMemSetAccess(g.base, bytes, &acc, 1) != hipSuccess ||
+      hipMemsetAsync(g.base + g.mapped, 0, sz, m->stream) != hipSuccess) {
+    (void)hipMemUnmap(g.base + g.mapped, sz);
+    (void)hipMemRelease(h);
+    return false;
+  }
+  g.chunks.push_back({h, sz});
+  g.mapped = bytes;
+  m->device_bytes += sz;
+  return true;
+}
+
+// Reserve the four pool ranges for as many leaves as the device's memory
+// could hold; vmm = 1 on success, -1 (plain allocations) otherwise.
+void vmm_init(vdbgpu_map* m) {
+  m->vmm = -1;
+  int ok = 0;
+  if (hipDeviceGetAttribute(&ok, hipDeviceAttributeVirtualMemoryManagementSupported, m->device) != hipSuccess || !ok)
+    return;
+  hipMemAllocationProp prop{};
+  prop.type = hipMemAllocationTypePinned;
+  prop.location.type = hipMemLocationTypeDevice;
+  prop.location.id = m->device;
+  size_t gran = 0, free_b = 0, total_b = 0;
+  if (hipMemGetAllocationGranularity(&gran, &prop, hipMemAllocationGranularityMinimum) != hipSuccess || !gran ||
+      hipMemGetInfo(&free_b, &total_b) != hipSuccess)
+    return;
+  const uint64_t leaves = std::min<uint64_t>(0xF0000000ull, total_b / 2048 + 1);
+  GrowArray* arrs[4] = {&m->g_values, &m->g_active, &m->g_keys, &m->g_dirty};
+  const size_t elem[4] = {2048, 64, 8, 1};
+  for (int i = 0; i < 4; ++i) {
+    const size_t bytes = (leaves * elem[i] + gran - 1) / gran * gran;
+    void* p = nullptr;
+    if (hipMemAddressReserve(&p, bytes, 0, nullptr, 0) != hipSuccess) {
+      for (int j = 0; j < i; ++j) ga_release(m, *arrs[j]);
+      return;
+    }
+    arrs[i]->base = static_cast<char*>(p);
+    arrs[i]->reserved = bytes;
+  }
+  m->vmm_gran = gran;
+  m->vmm_leaves = leaves;
+  m->vmm = 1;
+}
+
 // Leaf pool and leaf hash large enough for `need` leaves (hash load <= 0.5).
 int ensure_map(vdbgpu_map* m, uint64_t need) {
   if (need > 0xF0000000ull) {
@@ -450,31 +545,58 @@ int ensure_map(vdbgpu_map* m, uint64_t need) {
   }
   if (need > m->lcap) {
     const uint32_t cap = (uint32_t)std::max<uint64_t>(need, std::max<uint64_t>(4096, m->lcap + m->lcap / 2));
-    float* v = nullptr;
-    unsigned long long *a = nullptr, *k = nullptr;
-    uint8_t* d = nullptr;
-    CK(dalloc(m, &v, (size_t)cap * 512));
-    CK(dalloc(m, &a, (size_t)cap * 8));
-    CK(dalloc(m, &k, cap));
-    CK(dalloc(m, &d, cap));
-    CK(hipMemsetAsync(v, 0, (size_t)cap * 512 * sizeof(float), m->stream));
-    CK(hipMemsetAsync(a, 0, (size_t)cap * 8 * sizeof(unsigned long long), m->stream));
-    CK(hipMemsetAsync(d, 0, cap, m->stream));
-    if (m->lcap) {
-      CK(hipMemcpyAsync(v, m->values, (size_t)m->lcap * 512 * sizeof(float), hipMemcpyDeviceToDevice, m->stream));
-      CK(hipMemcpyAsync(a, m->active, (size_t)m->lcap * 8 * sizeof(unsigned long long), hipMemcpyDeviceToDevice, m->stream));
-      CK(hipMemcpyAsync(k, m->leaf_keys, (size_t)m->lcap * sizeof(unsigned long long), hipMemcpyDeviceToDevice, m->stream));
-      CK(hipMemcpyAsync(d, m->dirty, m->lcap, hipMemcpyDeviceToDevice, m->stream));
+    if (m->vmm == 0) vmm_init(m);
+    const size_t g = m->vmm_gran;
+    const bool in_place = m->vmm == 1 && cap <= m->vmm_leaves && ga_grow(m, m->g_values, (size_t)cap * 2048, g) &&
+                          ga_grow(m, m->g_active, (size_t)cap * 64, g) && ga_grow(m, m->g_keys, (size_t)cap * 8, g) &&
+                          ga_grow(m, m->g_dirty, (size_t)cap, g);
+    if (in_place) {  // grows in place: the pointers stay, nothing is copied
+      m->values = reinterpret_cast<float*>(m->g_values.base);
+      m->active = reinterpret_cast<unsigned long long*>(m->g_active.base);
+      m->leaf_keys = reinterpret_cast<unsigned long long*>(m->g_keys.base);
+      m->dirty = reinterpret_cast<uint8_t*>(m->g_dirty.base);
+      // every array holds at least cap leaves (each was mapped to cap or more)
+      m->lcap = (uint32_t)std::min<uint64_t>(
+          std::min(m->g_values.mapped / 2048, m->g_active.mapped / 64),
+          std::min<uint64_t>(std::min<uint64_t>(m->g_keys.mapped / 8, m->g_dirty.mapped), 0xF0000000ull));
+    } else {
+      float* v = nullptr;
+      unsigned long long *a = nullptr, *k = nullptr;
+      uint8_t* d = nullptr;
+      CK(dalloc(m, &v, (size_t)cap * 512));
+      CK(dalloc(m, &a, (size_t)cap * 8));
+      CK(dalloc(m, &k, cap));
+      CK(dalloc(m, &d, cap));
+      CK(hipMemsetAsync(v, 0, (size_t)cap * 512 * sizeof(float), m->stream));
+      CK(hipMemsetAsync(a, 0, (size_t)cap * 8 * sizeof(unsigned long long), m->stream));
+      CK(hipMemsetAsync(d, 0, cap, m->stream));
+      if (m->lcap) {
+        CK(hipMemcpyAsync(v, m->values, (size_t)m->lcap * 512 * sizeof(float), hipMemcpyDeviceToDevice, m->stream));
+        CK(hipMemcpyAsync(a, m->active, (size_t)m->lcap * 8 * sizeof(unsigned long long), hipMemcpyDeviceToDevice,
+                          m->stream));
+        CK(hipMemcpyAsync(k, m->leaf_keys, (size_t)m->lcap * sizeof(unsigned long long), hipMemcpyDeviceToDevice,
+                          m->stream));
+        CK(hipMemcpyAsync(d, m->dirty, m->lcap, hipMemcpyDeviceToDevice, m->stream));
+      }
+      if (m->vmm == 1) {  // the reserved ranges could not grow: plain allocations from now on
+        CK(hipStreamSynchronize(m->stream));
+        ga_release(m, m->g_values);
+        ga_release(m, m->g_active);
+        ga_release(m, m->g_keys);
+        ga_release(m, m->g_dirty);
+        m->vmm = -1;
+      } else {
+        dfree(m, &m->values, (size_t)m->lcap * 512);
+        dfree(m, &m->active, (size_t)m->lcap * 8);
+        dfree(m, &m->leaf_keys, m->lcap);
+        dfree(m, &m->dirty, m->lcap);
+      }
+      m->values = v;
+      m->active = a;
+      m->leaf_keys = k;
+      m->dirty = d;
+      m->lcap = cap;
     }
-    dfree(m, &m->values, (size_t)m->lcap * 512);
-    dfree(m, &m->active, (size_t)m->lcap * 8);
-    dfree(m, &m->leaf_keys, m->lcap);
-    dfree(m, &m->dirty, m->lcap);
-    m->values = v;
-    m->active = a;
-    m->leaf_keys = k;
-    m->dirty = d;
-    m->lcap = cap;
   }
   if (2 * need > m->mcap) {
     const uint32_t cap = (uint32_t)std::min<uint64_t>(1ull << 31, std::max<uint64_t>(8192, next_pow2(4 * need)));
@@ -484,10 +606,9 @@ int ensure_map(vdbgpu_map* m, uint64_t need) {
     CK(dalloc(m, &m->mkeys, cap));
     CK(dalloc(m, &m->midx, cap));
     CK(launch_fill_u64(m->stream, m->mkeys, cap, kEmpty));
-    const int rs = read_status(m);  // exact leaf count for the rehash bound
-    if (rs) return rs;
-    CK(launch_rehash_map(m->stream, std::min<int>(4096, (int)((m->h_st[1].leaf_count + 255) / 256) + 1),
-                         map_view(m), m->h_st[1].leaf_count));
+    // the leaf keys of the pool, as many as the map holds when the rehash runs
+    CK(launch_rehash_map(m->stream, std::min<int>(4096, (int)((std::min<uint64_t>(need, m->lcap) + 255) / 256) + 1),
+                         map_view(m), 0, &m->d_st[1]));
   }
   return VDBGPU_OK;
 }
@@ -1499,10 +1620,17 @@ void vdbgpu_destroy(vdbgpu_map* m) {
   dfree(m, &m->art_eidx, m->art_eidx_cap);
   dfree(m, &m->mkeys, m->mcap);
   dfree(m, &m->midx, m->mcap);
-  dfree(m, &m->values, (size_t)m->lcap * 512);
-  dfree(m, &m->active, (size_t)m->lcap * 8);
-  dfree(m, &m->leaf_keys, m->lcap);
-  dfree(m, &m->dirty, m->lcap);
+  if (m->vmm == 1) {
+    ga_release(m, m->g_values);
+    ga_release(m, m->g_active);
+    ga_release(m, m->g_keys);
+    ga_release(m, m->g_dirty);
+  } else {
+    dfree(m, &m->values, (size_t)m->lcap * 512);
+    dfree(m, &m->active, (size_t)m->lcap * 8);
+    dfree(m, &m->leaf_keys, m->lcap);
+    dfree(m, &m->dirty, m->lcap);
+  }
   dfree(m, &m->d_points, m->points_bytes / 4);
   const uint32_t nb = (m->ray_cap + kPrepBlock - 1) / kPrepBlock;
   dfree(m, &m->rayinfo, m->ray_cap);
