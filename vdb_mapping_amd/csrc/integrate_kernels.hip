@@ -863,6 +863,7 @@ __global__ __launch_bounds__(256) void k_apply_groups(GroupScans gs, const uint3
       uint32_t ab = (uint32_t)(aw[q] >> (4 * l)) & 15u, mine = 0;
 #pragma unroll
       for (int k = 0; k < kDeferMax; ++k) {
+        if (w[q].slot[k] == kNone) continue;  // the scan has no record of this leaf (same for the whole wave)
         const uint32_t vb = (w[q].vb >> (4 * k)) & 15u, hb = (w[q].hb >> (4 * k)) & 15u;
         mine |= vb;
         update_voxel(x[q].x, ab, 0, vb & 1, hb & 1, l_hit, l_miss, t_min, t_max, c_max, c_min);

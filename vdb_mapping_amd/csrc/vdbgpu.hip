@@ -190,6 +190,12 @@ struct vdbgpu_map {
   // deferred scans (tuning.defer > 1): record sets whose map insert ran and
   // whose apply waits for the group pass, in insert order
   int npend = 0;
+  // a scan whose cast is done and whose map-stream work (map insert, then the
+  // apply or a place in the group) is enqueued by the next insert while that
+  // insert's cast runs, or by any other call first
+  bool staged = false;
+  int staged_set = 0;
+  uint32_t staged_recs = 0;
   int pend_set[kDeferMax] = {};
   uint32_t pend_recs[kDeferMax] = {};
   uint32_t* gslot = nullptr;  // per map leaf: kNone, or the record that opened its group
@@ -617,6 +623,7 @@ void create_flags(const vdbgpu_map* m, bool* ch, bool* cm);
 int activate_artificial(vdbgpu_map* m);
 int clear_grid(vdbgpu_map* m);
 int flush_pending(vdbgpu_map* m);
+int commit_staged(vdbgpu_map* m);
 
 int ensure_groups(vdbgpu_map* m, uint32_t cap) {
   if (cap <= m->gcap) return VDBGPU_OK;
@@ -987,6 +994,10 @@ int grow_set(vdbgpu_map* m, int X, uint32_t cap) {
 // depend on active states, so once after the last scan equals once after each)
 // and free the record sets.
 int flush_pending(vdbgpu_map* m) {
+  if (m->staged) {
+    const int r = commit_staged(m);
+    if (r) return r;
+  }
   const int P = m->npend;
   if (P == 0) return VDBGPU_OK;
   if (P == 1) {  // one scan: the per-scan apply (no grouping to pay for)
@@ -1125,8 +1136,10 @@ int run_brick(vdbgpu_map* m, Source& src, const float* xyz, int64_t n, int64_t s
       CK(launch_rec_gather(cs, m->num_cu * 8, m->rlist, rec, UpdView{m->dkeys[X], m->dmasks[X], 0}, m->dcap[X],
                            &m->d_st[0]));
     }
-    // a full group of deferred scans: enqueued now, while the cast runs
-    if (attempt == 0 && m->npend >= m->tuning.defer && (r = flush_pending(m))) return r;
+    // the previous scan's map-stream work and a full group of deferred scans:
+    // enqueued now, while this cast runs
+    if (attempt == 0 && ((r = commit_staged(m)) || (m->npend >= m->tuning.defer && (r = flush_pending(m)))))
+      return r;
     // the scan's status and a snapshot of the leaf count: the map stream keeps
     // applying the previous scan meanwhile
     CK(hipMemcpyAsync(m->h_st, m->d_st, 2 * sizeof(Status), hipMemcpyDeviceToHost, cs));
@@ -1162,14 +1175,29 @@ int run_brick(vdbgpu_map* m, Source& src, const float* xyz, int64_t n, int64_t s
     m->rec_seq[X] = m->ledger_seq;
     return VDBGPU_OK;
   }
-  const uint32_t art_new = m->art_leaves ? 0 : m->art_count;  // leaves the activation may create
-  if ((r = ensure_map(m, m->leaf_count_ub + recs + art_new))) return r;
   if (recs > m->dcap[X]) {  // set X (no pending scan holds it) grows; the copy runs again
     if ((r = grow_set(m, X, (uint32_t)std::min<uint64_t>(recs + (uint64_t)recs / 4, 0xC0000000ull)))) return r;
     CK(launch_rec_gather(cs, m->num_cu * 8, m->rlist, UpdView{m->rkeys, m->rmasks, 0},
                          UpdView{m->dkeys[X], m->dmasks[X], 0}, m->dcap[X], &m->d_st[0]));
     CK(hipStreamSynchronize(cs));
   }
+  m->staged = true;
+  m->staged_set = X;
+  m->staged_recs = recs;
+  return VDBGPU_OK;
+}
+
+// The map-stream work of the staged scan (set X, recs records): map insert
+// and leaf-count snapshot, then a place among the deferred scans, or (one
+// scan at a time) the apply.
+int commit_staged(vdbgpu_map* m) {
+  if (!m->staged) return VDBGPU_OK;
+  m->staged = false;
+  const int X = m->staged_set;
+  const uint32_t recs = m->staged_recs;
+  int r;
+  const uint32_t art_new = m->art_leaves ? 0 : m->art_count;  // leaves the activation may create
+  if ((r = ensure_map(m, m->leaf_count_ub + recs + art_new))) return r;
   if (recs > m->ident_cap) {
     dfree(m, &m->ident, m->ident_cap);
     m->ident_cap = m->dcap[X];
@@ -1199,10 +1227,9 @@ int run_brick(vdbgpu_map* m, Source& src, const float* xyz, int64_t n, int64_t s
   m->touched = true;
   m->snap_dirty = true;
   if (m->rec_sets > 2 && recs <= kGroupRecMask) {
-    // applied with the next group pass, when defer scans are waiting or the
-    // map is read or changed otherwise
-    // a full group is enqueued by the next insert while its cast runs (or by
-    // any other call first)
+    // applied with the next group pass: when defer scans are waiting (the
+    // next insert enqueues it while its cast runs) or the map is read or
+    // changed otherwise
     if (m->npend >= kDeferMax && (r = flush_pending(m))) return r;
     m->pend_set[m->npend] = X;
     m->pend_recs[m->npend] = recs;
