@@ -144,7 +144,8 @@ int32_t vdbgpu_insert_batch(vdbgpu_map* map, const char* source_id, int32_t nsca
 
 /* resetMap VM:174-186: empty map and empty update sets. */
 int32_t vdbgpu_reset(vdbgpu_map* map);
-/* GridT::empty() as used at VM:494 and VM:1438: 1 if no voxel was ever modified. */
+/* GridT::empty() as used at VM:494 and VM:1438: 1 if no voxel was ever modified, 0 if one
+ * was, a status code (> 1) on error. */
 int32_t vdbgpu_empty(vdbgpu_map* map);
 
 /* getGrid VM:799, as a leaf dump in openvdb::FloatTree leaf layout: for leaf

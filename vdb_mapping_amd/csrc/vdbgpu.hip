@@ -1524,14 +1524,15 @@ int check_origins(vdbgpu_map* m, int64_t n, const int32_t* origins) {
 
 // Lock of one ABI call. Every call but a plain insert first applies the
 // deferred scans, so whatever it reads or changes sees the map of every
-// earlier insert (a failure there is left in last_error and shows again at
-// the call's own status read).
+// earlier insert; when that fails the call returns the failure (last_error
+// says what it was) and does nothing else.
 struct Guard {
   vdbgpu_map* m;
   std::lock_guard<std::mutex> lk;
+  int rc = VDBGPU_OK;  // status of applying the deferred scans: the call returns it when not OK
   explicit Guard(vdbgpu_map* mm, bool keep_deferred = false) : m(mm), lk(mm->mu) {
     (void)hipSetDevice(m->device);
-    if (!keep_deferred) (void)flush_pending(m);
+    if (!keep_deferred) rc = flush_pending(m);
   }
 };
 
@@ -1729,6 +1730,7 @@ void vdbgpu_destroy(vdbgpu_map* m) {
 int32_t vdbgpu_set_config(vdbgpu_map* m, const vdbgpu_config* c) {
   if (!m || !c) return VDBGPU_BAD_ARGUMENT;
   Guard g(m);
+  if (g.rc) return g.rc;
   int status = VDBGPU_OK;
   if (c->max_range < 0.0) {
     status = VDBGPU_BAD_MAX_RANGE;
@@ -1752,6 +1754,7 @@ int32_t vdbgpu_set_config(vdbgpu_map* m, const vdbgpu_config* c) {
 int32_t vdbgpu_logodds(vdbgpu_map* m, float out[6]) {
   if (!m || !out) return VDBGPU_BAD_ARGUMENT;
   Guard g(m);
+  if (g.rc) return g.rc;
   std::memcpy(out, m->lo, sizeof m->lo);
   return VDBGPU_OK;
 }
@@ -1760,6 +1763,7 @@ int32_t vdbgpu_add_source(vdbgpu_map* m, const char* id, double max_range, doubl
   (void)max_rate;
   if (!m || !id) return VDBGPU_BAD_ARGUMENT;
   Guard g(m);
+  if (g.rc) return g.rc;
   Source& s = m->sources[id];
   s.id = id;
   s.max_range = (max_range == 0) ? m->max_range : max_range;
@@ -1774,6 +1778,7 @@ int32_t vdbgpu_add_source(vdbgpu_map* m, const char* id, double max_range, doubl
 int32_t vdbgpu_remove_source(vdbgpu_map* m, const char* id) {
   if (!m || !id) return VDBGPU_BAD_ARGUMENT;
   Guard g(m);
+  if (g.rc) return g.rc;
   auto it = m->sources.find(id);
   if (it == m->sources.end()) return fail(m, VDBGPU_UNKNOWN_SOURCE, std::string("no source ") + id);
   free_update(m, it->second);
@@ -1785,6 +1790,7 @@ int32_t vdbgpu_accumulate(vdbgpu_map* m, const char* id, const float* xyz, int64
                           const double origin[3], int32_t on_device) {
   if (!m || !id) return VDBGPU_BAD_ARGUMENT;
   Guard g(m);
+  if (g.rc) return g.rc;
   auto it = m->sources.find(id);
   if (it == m->sources.end()) return fail(m, VDBGPU_UNKNOWN_SOURCE, std::string("Tried to accumulate update for ") + id + ". Source not available");
   if (!(it->second.max_range > 0)) return VDBGPU_OK;
@@ -1794,6 +1800,7 @@ int32_t vdbgpu_accumulate(vdbgpu_map* m, const char* id, const float* xyz, int64
 int32_t vdbgpu_integrate(vdbgpu_map* m) {
   if (!m) return VDBGPU_BAD_ARGUMENT;
   Guard g(m);
+  if (g.rc) return g.rc;
   return do_integrate(m);
 }
 
@@ -1816,6 +1823,7 @@ int32_t vdbgpu_insert_batch(vdbgpu_map* m, const char* id, int32_t nscans, const
                             const double* ranges, int32_t on_device) {
   if (!m || !id || nscans < 0 || (nscans && (!counts || !origins))) return VDBGPU_BAD_ARGUMENT;
   Guard g(m);
+  if (g.rc) return g.rc;
   auto it = m->sources.find(id);
   bool any_range = false;
   if (it != m->sources.end())
@@ -1908,6 +1916,7 @@ int32_t vdbgpu_insert_batch(vdbgpu_map* m, const char* id, int32_t nscans, const
 int32_t vdbgpu_reset(vdbgpu_map* m) {
   if (!m) return VDBGPU_BAD_ARGUMENT;
   Guard g(m);
+  if (g.rc) return g.rc;
   int r;
   if ((r = clear_grid(m))) return r;
   for (auto& kv : m->sources) {
@@ -1926,12 +1935,14 @@ int32_t vdbgpu_reset(vdbgpu_map* m) {
 int32_t vdbgpu_empty(vdbgpu_map* m) {
   if (!m) return VDBGPU_BAD_ARGUMENT;
   Guard g(m);
+  if (g.rc) return g.rc;
   return m->touched ? 0 : 1;
 }
 
 int64_t vdbgpu_leaf_count(vdbgpu_map* m) {
   if (!m) return -1;
   Guard g(m);
+  if (g.rc) return -1;
   if (read_status(m)) return -1;
   return m->h_st[1].leaf_count;
 }
@@ -1940,6 +1951,7 @@ int64_t vdbgpu_export_leaves(vdbgpu_map* m, int32_t* origins, float* values, uin
                              int64_t cap) {
   if (!m) return -1;
   Guard g(m);
+  if (g.rc) return -1;
   if (read_status(m)) return -1;
   const uint32_t n = m->h_st[1].leaf_count;
   if (!origins && !values && !masks) return n;
@@ -1961,6 +1973,7 @@ int64_t vdbgpu_export_dirty_leaves(vdbgpu_map* m, int32_t* origins, float* value
                                    int64_t cap) {
   if (!m) return -1;
   Guard g(m);
+  if (g.rc) return -1;
   if (read_status(m)) return -1;
   const uint32_t lc = m->h_st[1].leaf_count;
   if (lc == 0) return 0;
@@ -2004,6 +2017,7 @@ int64_t vdbgpu_export_dirty_leaves(vdbgpu_map* m, int32_t* origins, float* value
 int64_t vdbgpu_update_leaf_count(vdbgpu_map* m, const char* id) {
   if (!m || !id) return -1;
   Guard g(m);
+  if (g.rc) return -1;
   auto it = m->sources.find(id);
   if (it == m->sources.end()) return -1;
   return it->second.pending;
@@ -2013,6 +2027,7 @@ int64_t vdbgpu_export_update(vdbgpu_map* m, const char* id, int32_t* origins, ui
                              uint64_t* hit, int64_t cap) {
   if (!m || !id) return -1;
   Guard g(m);
+  if (g.rc) return -1;
   auto it = m->sources.find(id);
   if (it == m->sources.end()) return -1;
   Source& s = it->second;
@@ -2057,6 +2072,7 @@ int32_t vdbgpu_raytrace(vdbgpu_map* m, int64_t n, const double* o, const double*
                         const double* len, uint8_t* succ, double* ends) {
   if (!m || n < 0 || (n && (!o || !d || !len || !succ || !ends))) return VDBGPU_BAD_ARGUMENT;
   Guard g(m);
+  if (g.rc) return g.rc;
   if (!m->has_isect)
     return fail(m, VDBGPU_NO_INTERSECTOR, "raytrace needs an integrate in fast mode on a non-empty map");
   if (n == 0) return VDBGPU_OK;
@@ -2085,6 +2101,7 @@ int32_t vdbgpu_raytrace_device(vdbgpu_map* m, int64_t n, const double* o, const 
                                const double* len, uint8_t* succ, double* ends) {
   if (!m || n < 0 || (n && (!o || !d || !len || !succ || !ends))) return VDBGPU_BAD_ARGUMENT;
   Guard g(m);
+  if (g.rc) return g.rc;
   if (!m->has_isect)
     return fail(m, VDBGPU_NO_INTERSECTOR, "raytrace needs an integrate in fast mode on a non-empty map");
   if (n == 0) return VDBGPU_OK;
@@ -2130,6 +2147,7 @@ int32_t vdbgpu_device_copy(void* dst, const void* src, int64_t bytes) {
 int32_t vdbgpu_synchronize(vdbgpu_map* m) {
   if (!m) return VDBGPU_BAD_ARGUMENT;
   Guard g(m);
+  if (g.rc) return g.rc;
   CK(hipStreamSynchronize(m->cast));
   return read_status(m);
 }
@@ -2137,6 +2155,7 @@ int32_t vdbgpu_synchronize(vdbgpu_map* m) {
 int32_t vdbgpu_get_stats(vdbgpu_map* m, vdbgpu_stats* out) {
   if (!m || !out) return VDBGPU_BAD_ARGUMENT;
   Guard g(m);
+  if (g.rc) return g.rc;
   int r = read_status(m);
   if (r) return r;
   m->stats.leaf_count = m->h_st[1].leaf_count;
@@ -2151,6 +2170,7 @@ int32_t vdbgpu_get_stats(vdbgpu_map* m, vdbgpu_stats* out) {
 int32_t vdbgpu_set_profiling(vdbgpu_map* m, int32_t on) {
   if (!m) return VDBGPU_BAD_ARGUMENT;
   Guard g(m);
+  if (g.rc) return g.rc;
   m->profiling = on != 0;
   return VDBGPU_OK;
 }
@@ -2163,6 +2183,7 @@ int32_t vdbgpu_set_tuning(vdbgpu_map* m, const vdbgpu_tuning* t) {
       t->defer < 0 || t->defer > kDeferMax)
     return VDBGPU_BAD_ARGUMENT;
   Guard g(m);  // applies the scans deferred under the old setting
+  if (g.rc) return g.rc;
   const int sets = t->defer > 1 ? t->defer + 2 : 2;
   if (sets != m->rec_sets) {  // a new ring of record sets
     CK(hipStreamSynchronize(m->cast));
@@ -2178,6 +2199,7 @@ int32_t vdbgpu_set_tuning(vdbgpu_map* m, const vdbgpu_tuning* t) {
 int32_t vdbgpu_get_tuning(vdbgpu_map* m, vdbgpu_tuning* out) {
   if (!m || !out) return VDBGPU_BAD_ARGUMENT;
   Guard g(m);
+  if (g.rc) return g.rc;
   *out = m->tuning;
   return VDBGPU_OK;
 }
@@ -2252,6 +2274,7 @@ extern "C" {
 int32_t vdbgpu_set_points(vdbgpu_map* m, const float* xyz, int64_t n, int64_t stride, int32_t occupied) {
   if (!m) return VDBGPU_BAD_ARGUMENT;
   Guard g(m);
+  if (g.rc) return g.rc;
   if (n < 0 || stride < 12 || (stride & 3) || n > (int64_t)kMaxRays || (n && !xyz))
     return fail(m, VDBGPU_BAD_ARGUMENT, "bad point buffer arguments");
   uint32_t count = 0;
@@ -2264,6 +2287,7 @@ int32_t vdbgpu_create_from_points(vdbgpu_map* m, const float* xyz, int64_t n, in
                                   int32_t set_background, int32_t clear_map) {
   if (!m) return VDBGPU_BAD_ARGUMENT;
   Guard g(m);
+  if (g.rc) return g.rc;
   if (n < 0 || stride < 12 || (stride & 3) || n > (int64_t)kMaxRays || (n && !xyz))
     return fail(m, VDBGPU_BAD_ARGUMENT, "bad point buffer arguments");
   CK(hipStreamSynchronize(m->cast));
@@ -2323,6 +2347,7 @@ int32_t vdbgpu_create_from_points(vdbgpu_map* m, const float* xyz, int64_t n, in
 int64_t vdbgpu_export_active_points(vdbgpu_map* m, float* xyz, int64_t cap) {
   if (!m) return -1;
   Guard g(m);
+  if (g.rc) return -1;
   if (hipStreamSynchronize(m->cast) != hipSuccess || read_status(m)) return -1;
   const uint32_t n = m->h_st[1].leaf_count;
   std::vector<unsigned long long> keys(n), act((size_t)n * 8);
@@ -2439,6 +2464,7 @@ int32_t vdbgpu_add_artificial_areas(vdbgpu_map* m, int32_t n_polygons, const int
                                     const double* xyz, double negative_height, double positive_height) {
   if (!m) return VDBGPU_BAD_ARGUMENT;
   Guard g(m);
+  if (g.rc) return g.rc;
   if (n_polygons < 0 || (n_polygons && !counts)) return fail(m, VDBGPU_BAD_ARGUMENT, "bad polygon arguments");
   int k0, k1, r;
   if ((r = wall_layers(m, negative_height, positive_height, &k0, &k1))) return r;
@@ -2473,6 +2499,7 @@ int32_t vdbgpu_add_artificial_walls(vdbgpu_map* m, int32_t n_walls, const double
                                     double positive_height) {
   if (!m) return VDBGPU_BAD_ARGUMENT;
   Guard g(m);
+  if (g.rc) return g.rc;
   if (n_walls < 0 || (n_walls && !xyz) || n_walls > (1 << 26)) return fail(m, VDBGPU_BAD_ARGUMENT, "bad wall arguments");
   int k0, k1, r;
   if ((r = wall_layers(m, negative_height, positive_height, &k0, &k1))) return r;
@@ -2494,6 +2521,7 @@ int32_t vdbgpu_add_artificial_walls(vdbgpu_map* m, int32_t n_walls, const double
 int32_t vdbgpu_restore_map_integrity(vdbgpu_map* m) {
   if (!m) return VDBGPU_BAD_ARGUMENT;
   Guard g(m);
+  if (g.rc) return g.rc;
   return restore_integrity(m);
 }
 
@@ -2510,6 +2538,7 @@ int32_t vdbgpu_restore_map_integrity(vdbgpu_map* m) {
 int64_t vdbgpu_export_artificial(vdbgpu_map* m, int32_t* origins, uint64_t* masks, int64_t cap) {
   if (!m) return -1;
   Guard g(m);
+  if (g.rc) return -1;
   const uint32_t n = m->art_count;
   std::vector<unsigned long long> keys(n), act((size_t)n * 8);
   std::vector<float> none;
@@ -2537,6 +2566,7 @@ int64_t vdbgpu_export_section(vdbgpu_map* m, const int32_t bbox[6], int32_t full
                               float* values, uint64_t* masks, int64_t cap) {
   if (!m || !bbox) return -1;
   Guard g(m);
+  if (g.rc) return -1;
   CKN(hipStreamSynchronize(m->cast));
   if (read_status(m)) return -1;
   const uint32_t L = m->h_st[1].leaf_count;
@@ -2600,6 +2630,7 @@ int32_t vdbgpu_apply_section(vdbgpu_map* m, int64_t n, const int32_t* origins, c
                              const uint64_t* masks) {
   if (!m) return VDBGPU_BAD_ARGUMENT;
   Guard g(m);
+  if (g.rc) return g.rc;
   if (n < 0 || n >= (1 << 28) || (n && (!origins || !values || !masks)))
     return fail(m, VDBGPU_BAD_ARGUMENT, "bad section arguments");
   int r;
@@ -2652,6 +2683,7 @@ int32_t vdbgpu_apply_section_update(vdbgpu_map* m, const int32_t bbox[6], int64_
                                     const uint64_t* masks) {
   if (!m) return VDBGPU_BAD_ARGUMENT;
   Guard g(m);
+  if (g.rc) return g.rc;
   if (!bbox || n < 0 || n >= (1 << 28) || (n && (!origins || !masks)))
     return fail(m, VDBGPU_BAD_ARGUMENT, "bad section arguments");
   int r;
@@ -2822,6 +2854,7 @@ int64_t vdbgpu_morphology(vdbgpu_map* m, int32_t op, int32_t iterations, int64_t
                           const uint64_t* masks, int32_t* out_origins, uint64_t* out_masks, int64_t cap) {
   if (!m) return -1;
   Guard g(m);
+  if (g.rc) return -1;
   if (op < 0 || op > 3 || n < 0 || (uint64_t)n > kMorphMaxLeaves || (n && (!origins || !masks))) {
     fail(m, VDBGPU_BAD_ARGUMENT, "bad morphology arguments");
     return -1;
