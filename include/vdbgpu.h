@@ -69,6 +69,11 @@ typedef struct vdbgpu_stats {
   uint64_t n_walk;
 } vdbgpu_stats;
 
+/* The leaf pool of a map grows in place in address ranges reserved at its first leaf (HIP
+ * virtual memory); with the environment variable VDBGPU_PLAIN_POOL=1 set when a map gets its
+ * first leaf, or where the driver refuses a mapping, it uses plain allocations that grow by copy.
+ * Results are the same either way. */
+
 /* Launch parameters of the integration kernels (see vdbgpu_set_tuning). */
 typedef struct vdbgpu_tuning {
   int32_t block;          /* threads per workgroup of the traversal kernel (64, 128, 192, 256) */

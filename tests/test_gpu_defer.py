@@ -166,3 +166,21 @@ def test_outdoor_stream_deferred_equals_per_scan():
     assert np.array_equal(ka, kb) and np.array_equal(va.view(np.uint32), vb.view(np.uint32))
     assert np.array_equal(ma, mb)
     assert a.stats()["voxel_updates"] == b.stats()["voxel_updates"]
+
+
+def test_plain_leaf_pool_equals_reserved_ranges(monkeypatch):
+    """The leaf pool grown by copy (VDBGPU_PLAIN_POOL=1, the fallback when the
+    driver refuses a mapping) and grown in place give the same map through
+    several pool growths."""
+    ss = scans(9, 16, pts=6000, spread=2.5)
+    maps = []
+    for plain in ("1", "0"):
+        monkeypatch.setenv("VDBGPU_PLAIN_POOL", plain)
+        g, o = deferred_pair(8, res=0.05)
+        for p, orig in ss:
+            both(g, o, "insert", "test", p, orig)
+        same_map(g, o)
+        maps.append(g)
+    a, b = (m.export_leaves() for m in maps)
+    assert a[0].shape[0] > 4096  # past the first pool size
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1].view(np.uint32), b[1].view(np.uint32))

@@ -14,6 +14,7 @@
 #include <array>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -514,6 +515,8 @@ bool ga_grow(vdbgpu_map* m, GrowArray& g, size_t bytes, size_t gran) {
 // could hold; vmm = 1 on success, -1 (plain allocations) otherwise.
 void vmm_init(vdbgpu_map* m) {
   m->vmm = -1;
+  const char* plain = std::getenv("VDBGPU_PLAIN_POOL");  // "1": plain allocations (the fallback)
+  if (plain && plain[0] == '1') return;
   int ok = 0;
   if (hipDeviceGetAttribute(&ok, hipDeviceAttributeVirtualMemoryManagementSupported, m->device) != hipSuccess || !ok)
     return;
