@@ -30,9 +30,12 @@
 // :218-256, :295-307, OccupancyVDBMapping.hpp:136-174) exports and loads through
 // the library (vdbgpu_export_active_points, vdbgpu_create_from_points).
 //
-// Not provided: .vdb persistence (saveMap / loadMap :193-211, :263-284) and
-// the byte-array wire format (:1246-1338), which need OpenVDB's io::File /
-// io::Stream; castRayIntoGrid (:550-566, a per-ray host call) and updateMap
+// .vdb persistence (saveMap / loadMap :193-211, :263-284) and the byte-array
+// wire format (gridToByteArray / byteArrayToGrid, compressString /
+// decompressByteArray :1246-1338) write and read OpenVDB's archive layout on
+// the host (vdb_io.hpp, zstd); link zlib and zstd.
+//
+// Not provided: castRayIntoGrid (:550-566, a per-ray host call) and updateMap
 // (:731-792, reached through integrateUpdate).
 #ifndef VDB_MAPPING_VDB_MAPPING_H_INCLUDED
 #define VDB_MAPPING_VDB_MAPPING_H_INCLUDED
@@ -44,6 +47,8 @@
 #include <chrono>
 #include <cstring>
 #include <ctime>
+#include <fstream>
+#include <iterator>
 #include <iomanip>
 #include <iostream>
 #include <sstream>
@@ -62,6 +67,21 @@
 #include "vdb_mapping/compat.hpp"
 #include "vdb_mapping/grid.hpp"
 #include "vdb_mapping/pcd.hpp"
+#include "vdb_mapping/vdb_io.hpp"
+
+#if __has_include(<zstd.h>)
+#include <zstd.h>
+#else
+// zstd's C API (libzstd), for the byte arrays of gridToByteArray
+extern "C" {
+size_t ZSTD_compressBound(size_t srcSize);
+size_t ZSTD_compress(void* dst, size_t dstCapacity, const void* src, size_t srcSize, int compressionLevel);
+unsigned long long ZSTD_getDecompressedSize(const void* src, size_t srcSize);
+size_t ZSTD_decompress(void* dst, size_t dstCapacity, const void* src, size_t compressedSize);
+unsigned ZSTD_isError(size_t code);
+const char* ZSTD_getErrorName(size_t code);
+}
+#endif
 
 namespace vdb_mapping {
 
@@ -339,6 +359,134 @@ public:
   void flushAccumulation() { report(vdbgpu_async_flush(m_map)); }
 
   /*!
+   * \brief Writes the map to <map_directory_path><date>_map.vdb (reference
+   * :193-211) in OpenVDB's archive layout (vdb_io.hpp; compatibility with
+   * OpenVDB itself is not checked by this build).
+   */
+  bool saveMap() const
+  {
+    auto timestamp     = std::chrono::system_clock::now();
+    std::time_t now_tt = std::chrono::system_clock::to_time_t(timestamp);
+    std::tm tm         = *std::localtime(&now_tt);
+    std::stringstream sstime;
+    sstime << std::put_time(&tm, "%Y-%m-%d_%H-%M-%S");
+    std::string map_name = m_map_directory_path + sstime.str() + "_map.vdb";
+    std::cout << map_name << std::endl;
+    std::shared_lock map_lock(*m_map_mutex);
+    const std::string bytes = vdbio::write_archive(toArchiveGrid(*m_vdb_grid), true);
+    map_lock.unlock();
+    std::ofstream out(map_name, std::ios::binary);
+    out.write(bytes.data(), (std::streamsize)bytes.size());
+    return (bool)out;
+  }
+
+  /*!
+   * \brief Loads a map from a .vdb file (reference :263-284): the last grid of
+   * the file becomes the map. Float 5-4-3 trees at the map's resolution only;
+   * like resetMap, pending update sets are dropped.
+   */
+  bool loadMap(const std::string& file_path)
+  {
+    std::ifstream in(file_path, std::ios::binary);
+    if (!in)
+    {
+      std::cerr << "Could not open " << file_path << std::endl;
+      return false;
+    }
+    const std::string bytes((std::istreambuf_iterator<char>(in)), std::istreambuf_iterator<char>());
+    std::vector<vdbio::Grid> grids;
+    std::string err;
+    if (!vdbio::read_archive(bytes, grids, err) || !grids.back().is_float ||
+        std::fabs(grids.back().voxel_size - m_resolution) > 1e-12 * m_resolution)
+    {
+      std::cerr << "loadMap: " << (err.empty() ? "not a float grid at the map's resolution" : err) << std::endl;
+      return false;
+    }
+    const vdbio::Grid& g = grids.back();
+    std::vector<int32_t> o;
+    std::vector<float> v;
+    std::vector<uint64_t> a;
+    for (const auto& l : g.leaves)
+    {
+      o.insert(o.end(), l.origin, l.origin + 3);
+      v.insert(v.end(), l.values, l.values + 512);
+      a.insert(a.end(), l.active, l.active + 8);
+    }
+    std::unique_lock map_lock(*m_map_mutex);
+    int32_t rc = report(vdbgpu_reset(m_map));
+    if (rc == VDBGPU_OK && !o.empty())
+    {
+      rc = report(vdbgpu_apply_section(m_map, (int64_t)(o.size() / 3), o.data(), v.data(), a.data()));
+    }
+    m_vdb_grid->markStale();
+    return rc == VDBGPU_OK;
+  }
+
+  /*!
+   * \brief zstd compression of a string at m_compression_level (reference
+   * :1246-1268); the uncompressed bytes when compression fails.
+   */
+  std::vector<uint8_t> compressString(const std::string& string) const
+  {
+    auto uncompressed = std::vector<uint8_t>(string.begin(), string.end());
+    size_t len        = ZSTD_compressBound(uncompressed.size());
+    std::vector<uint8_t> compressed(len);
+    const size_t ret =
+      ZSTD_compress(compressed.data(), len, uncompressed.data(), uncompressed.size(), (int)m_compression_level);
+    if (ZSTD_isError(ret))
+    {
+      std::cerr << "Compression using ZSTD failed: " << ZSTD_getErrorName(ret)
+                << " , sending uncompressed byte array" << std::endl;
+      return uncompressed;
+    }
+    compressed.resize(ret);
+    return compressed;
+  }
+
+  /*!
+   * \brief Decompresses a zstd byte array into a string (reference :1277-1297).
+   */
+  std::string decompressByteArray(const std::vector<uint8_t>& byte_array) const
+  {
+    const unsigned long long len = ZSTD_getDecompressedSize(byte_array.data(), byte_array.size());
+    std::vector<uint8_t> uncompressed((size_t)len);
+    const size_t size = ZSTD_decompress(uncompressed.data(), (size_t)len, byte_array.data(), byte_array.size());
+    if (ZSTD_isError(size))
+    {
+      std::cerr << "Could not decompress map using ZSTD failed: " << ZSTD_getErrorName(size) << std::endl;
+      return std::string();
+    }
+    return std::string(uncompressed.begin(), uncompressed.begin() + size);
+  }
+
+  /*!
+   * \brief A grid as a compressed byte array (reference :1310-1317): OpenVDB's
+   * stream layout (vdb_io.hpp), compressed with zstd.
+   */
+  template <typename TGrid>
+  std::vector<uint8_t> gridToByteArray(typename TGrid::Ptr grid)
+  {
+    return compressString(vdbio::write_archive(toArchiveGrid(*grid), false));
+  }
+
+  /*!
+   * \brief The grid of a byte array (reference :1327-1338); null when the
+   * array holds no grid of type TGrid.
+   */
+  template <typename TGrid>
+  typename TGrid::Ptr byteArrayToGrid(std::vector<uint8_t> byte_array)
+  {
+    std::vector<vdbio::Grid> grids;
+    std::string err;
+    if (!vdbio::read_archive(decompressByteArray(byte_array), grids, err))
+    {
+      std::cerr << "byteArrayToGrid: " << err << std::endl;
+      return nullptr;
+    }
+    return fromArchiveGrid<TGrid>(grids.front());
+  }
+
+  /*!
    * \brief Writes the active voxel centres to
    * <map_directory_path><date>_active_values_map.pcd (reference :218-256),
    * in PCL's ASCII PCD layout.
@@ -478,6 +626,7 @@ public:
     std::vector<uint64_t> a;
     exportSection(box, full_grid, o, v, a);
     typename UpdateGridT::Ptr grid = UpdateGridT::create(false);
+    grid->setVoxelSize(m_resolution);  // the section's linear transform (reference :928)
     auto acc                       = grid->getAccessor();
     for (size_t i = 0; i < o.size() / 3; ++i)
     {
@@ -730,6 +879,96 @@ protected:
     return out;
   }
 
+  static vdbio::Grid toArchiveGrid(const FloatGrid& grid)
+  {
+    vdbio::Grid g;
+    g.is_float   = true;
+    g.grid_class = "level set";
+    g.voxel_size = grid.voxelSize();
+    grid.forEachLeaf([&](const HostLeaf& l) {
+      vdbio::Leaf a;
+      std::memcpy(a.origin, l.origin, sizeof a.origin);
+      std::memcpy(a.active, l.mask, sizeof a.active);
+      std::memcpy(a.values, l.values, sizeof a.values);
+      std::memset(a.bits, 0, sizeof a.bits);
+      g.leaves.push_back(a);
+    });
+    for (const char* k : {"bb_min", "bb_max"})
+    {
+      if (grid.hasMeta(k))
+      {
+        const openvdb::Vec3d v = grid.template metaValue<openvdb::Vec3d>(k);
+        g.vec3d_meta[k]        = {v.x(), v.y(), v.z()};
+      }
+    }
+    return g;
+  }
+  static vdbio::Grid toArchiveGrid(const BoolGrid& grid)
+  {
+    vdbio::Grid g;
+    g.is_float   = false;
+    g.voxel_size = grid.voxelSize();
+    grid.forEachLeaf([&](const BoolGrid::Leaf& l) {
+      vdbio::Leaf a;
+      std::memcpy(a.origin, l.origin, sizeof a.origin);
+      std::memcpy(a.active, l.active, sizeof a.active);
+      std::memcpy(a.bits, l.value, sizeof a.bits);
+      std::memset(a.values, 0, sizeof a.values);
+      g.leaves.push_back(a);
+    });
+    for (const char* k : {"bb_min", "bb_max"})
+    {
+      if (grid.hasMeta(k))
+      {
+        const openvdb::Vec3d v = grid.template metaValue<openvdb::Vec3d>(k);
+        g.vec3d_meta[k]        = {v.x(), v.y(), v.z()};
+      }
+    }
+    return g;
+  }
+  template <typename TGrid>
+  static typename TGrid::Ptr fromArchiveGrid(const vdbio::Grid& g)
+  {
+    if (g.is_float != std::is_same<TGrid, FloatGrid>::value)
+    {
+      return nullptr;  // the reference's gridPtrCast gives null on a type mismatch
+    }
+    typename TGrid::Ptr out;
+    if constexpr (std::is_same<TGrid, FloatGrid>::value)
+    {
+      out = TGrid::create(g.voxel_size);
+    }
+    else
+    {
+      out = TGrid::create(false);
+      out->setVoxelSize(g.voxel_size);
+    }
+    for (const auto& kv : g.vec3d_meta)
+    {
+      out->insertMeta(kv.first, openvdb::Vec3d(kv.second[0], kv.second[1], kv.second[2]));
+    }
+    if constexpr (std::is_same<TGrid, FloatGrid>::value)
+    {
+      for (const auto& l : g.leaves)
+      {
+        HostLeaf h;
+        std::memcpy(h.origin, l.origin, sizeof h.origin);
+        std::memcpy(h.values, l.values, sizeof h.values);
+        std::memcpy(h.mask, l.active, sizeof h.mask);
+        out->insertLeaf(h);
+      }
+    }
+    else
+    {
+      auto acc = out->getAccessor();
+      for (const auto& l : g.leaves)
+      {
+        acc.mergeLeaf(l.origin, l.active, l.bits);
+      }
+    }
+    return out;
+  }
+
   // vdbgpu_morphology over the leaves of a detached grid (a section); the
   // new masks are written back, created leaves take the background value.
   template <typename TGridObj>
@@ -817,6 +1056,7 @@ protected:
   double m_resolution;
   bool m_fast_mode = false;
   std::string m_map_directory_path;
+  unsigned int m_compression_level = 1;  // zstd level of gridToByteArray (reference :1524)
   bool m_config_set;
   std::map<std::string, double> m_sources;
   std::map<std::string, double> m_rates;

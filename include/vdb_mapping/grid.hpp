@@ -180,6 +180,7 @@ public:
     }
     return it->second;
   }
+  bool hasMeta(const std::string& name) const { return m_meta.count(name) != 0; }
   // Active mask of the leaf at origin; a missing leaf is created with the
   // background value 0 in every voxel (detached grids only).
   void setLeafMask(const int32_t origin[3], const uint64_t* mask)
@@ -327,6 +328,10 @@ public:
 
   Accessor getAccessor() { return Accessor(this); }
   bool empty() const { return m_leaves.empty(); }
+  // the grid's linear transform (voxel size); 1 unless set
+  double voxelSize() const { return m_voxel_size; }
+  void setVoxelSize(double voxel_size) { m_voxel_size = voxel_size; }
+  bool hasMeta(const std::string& name) const { return m_meta.count(name) != 0; }
   void insertMeta(const std::string& name, const openvdb::Vec3d& value) { m_meta[name] = value; }
   template <typename T>
   T metaValue(const std::string& name) const
@@ -386,6 +391,7 @@ private:
   }
   std::unordered_map<uint64_t, Leaf> m_leaves;
   std::map<std::string, openvdb::Vec3d> m_meta;
+  double m_voxel_size = 1.0;
 };
 
 // Active-mask words of a leaf of either grid type.

@@ -9,6 +9,8 @@
 #include <cmath>
 #include <cstdio>
 #include <fstream>
+#include <map>
+#include <array>
 #include <vdb_mapping/OccupancyVDBMapping.hpp>
 
 using namespace vdb_mapping;
@@ -421,6 +423,111 @@ static void pcd_persistence_test()
   rmdir(dir);
 }
 
+// Every leaf of a equals a leaf of b (origin, values bit for bit, mask) and
+// the leaf counts agree.
+static bool same_leaves(const OccupancyVDBMapping::GridT& a, const OccupancyVDBMapping::GridT& b)
+{
+  std::map<std::array<int32_t, 3>, const HostLeaf*> idx;
+  b.forEachLeaf([&](const HostLeaf& l) { idx[{l.origin[0], l.origin[1], l.origin[2]}] = &l; });
+  size_t n = 0;
+  bool ok  = true;
+  a.forEachLeaf([&](const HostLeaf& l) {
+    ++n;
+    auto it = idx.find({l.origin[0], l.origin[1], l.origin[2]});
+    ok = ok && it != idx.end() && std::memcmp(l.values, it->second->values, sizeof l.values) == 0 &&
+         std::memcmp(l.mask, it->second->mask, sizeof l.mask) == 0;
+  });
+  return ok && n == idx.size();
+}
+
+static void vdb_persistence_test()
+{
+  // saveMap -> loadMap into a second map gives the same grid (reference
+  // :193-211, :263-284); byte arrays of section grids and update grids
+  // round-trip through gridToByteArray / byteArrayToGrid (:1246-1338)
+  char dir_tmpl[] = "/tmp/vdbgpu_vdb_XXXXXX";
+  const char* dir = mkdtemp(dir_tmpl);
+  CHECK(dir != nullptr);
+  if (!dir) return;
+  OccupancyVDBMapping a(0.1), b(0.1);
+  Config conf             = make_config(10);
+  conf.map_directory_path = std::string(dir) + "/";
+  a.setConfig(conf);
+  b.setConfig(conf);
+  a.addInputSource("test", conf.max_range, 0);
+  OccupancyVDBMapping::PointCloudT::Ptr cloud(new OccupancyVDBMapping::PointCloudT);
+  for (int k = 0; k < 400; ++k)
+    cloud->points.emplace_back(3.0f * std::cos(0.05f * k), 2.0f * std::sin(0.07f * k), 0.01f * k - 2.0f);
+  Eigen::Matrix<double, 3, 1> origin(0.05, -0.15, 0.25);
+  a.insertPointCloud(cloud, origin, "test");
+  a.insertPointCloud(cloud, origin, "test");
+  CHECK(a.saveMap());
+  std::string file;
+  if (DIR* d = opendir(dir))
+  {
+    while (dirent* e = readdir(d))
+    {
+      const std::string name = e->d_name;
+      if (name.find("_map.vdb") != std::string::npos) file = std::string(dir) + "/" + name;
+    }
+    closedir(d);
+  }
+  CHECK(!file.empty());
+  CHECK(b.loadMap(file));
+  CHECK(a.getGrid()->leafCount() > 100);
+  CHECK(same_leaves(*a.getGrid(), *b.getGrid()));
+  CHECK(!b.loadMap(std::string(dir) + "/missing.vdb"));
+  OccupancyVDBMapping c(0.2);  // another resolution: refused
+  c.setConfig(conf);
+  CHECK(!c.loadMap(file));
+  // byte arrays
+  const Eigen::Matrix<double, 3, 1> lo(-1.5, -1.0, -2.0), hi(2.5, 1.5, 1.0);
+  const Eigen::Matrix<double, 4, 4> tf = Eigen::Matrix<double, 4, 4>::Identity();
+  auto section = a.getMapSectionGrid(lo, hi, tf, true);
+  const std::vector<uint8_t> bytes = a.gridToByteArray<OccupancyVDBMapping::GridT>(section);
+  auto back = b.byteArrayToGrid<OccupancyVDBMapping::GridT>(bytes);
+  CHECK(back != nullptr);
+  if (back)
+  {
+    CHECK(same_leaves(*section, *back));
+    CHECK(back->metaValue<openvdb::Vec3d>("bb_max").x() == section->metaValue<openvdb::Vec3d>("bb_max").x());
+    CHECK(back->voxelSize() == 0.1);
+  }
+  CHECK(b.byteArrayToGrid<OccupancyVDBMapping::UpdateGridT>(bytes) == nullptr);  // a float grid
+  auto us       = a.getMapSectionUpdateGrid(lo, hi, tf, false);
+  auto us_bytes = a.gridToByteArray<OccupancyVDBMapping::UpdateGridT>(us);
+  auto us_back  = b.byteArrayToGrid<OccupancyVDBMapping::UpdateGridT>(us_bytes);
+  CHECK(us_back != nullptr && us_back->activeVoxelCount() == us->activeVoxelCount() && us->activeVoxelCount() > 0);
+  if (us_back)
+  {
+    int same = 0, total = 0;
+    auto ua = us->getAccessor();
+    auto ub = us_back->getAccessor();
+    for (int x = -20; x <= 30; ++x)
+      for (int y = -12; y <= 17; ++y)
+        for (int z = -22; z <= 12; ++z)
+        {
+          const openvdb::Coord q(x, y, z);
+          ++total;
+          same += ua.isValueOn(q) == ub.isValueOn(q) && ua.getValue(q) == ub.getValue(q);
+        }
+    CHECK(same == total);
+    // an update grid from a byte array applies like the original
+    OccupancyVDBMapping d(0.1), e(0.1);
+    for (auto* m : {&d, &e})
+    {
+      m->setConfig(conf);
+    }
+    d.applyMapSectionUpdateGrid(us);
+    e.applyMapSectionUpdateGrid(us_back);
+    CHECK(same_leaves(*d.getGrid(), *e.getGrid()));
+  }
+  const std::string text = "vdb byte array";
+  CHECK(a.decompressByteArray(a.compressString(text)) == text);
+  std::remove(file.c_str());
+  rmdir(dir);
+}
+
 int main()
 {
   struct T
@@ -441,6 +548,7 @@ int main()
     {"Morphology", morphology_test},
     {"ArtificialAreas", artificial_areas_test},
     {"PcdPersistence", pcd_persistence_test},
+    {"VdbPersistence", vdb_persistence_test},
   };
   for (const T& t : tests)
   {
