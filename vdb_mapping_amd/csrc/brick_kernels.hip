@@ -12,11 +12,13 @@
 // several workgroups share them.
 //
 //   k_seg_count     one lane per ray: enumerate its bricks, count segments
-//                   per brick (aggregated per workgroup in LDS)
+//                   per brick (aggregated per workgroup in an LDS table,
+//                   which is stored for the fill)
 //   k_brick_sums    per 256 brick slots: sums of segments and work items
 //   k_scan_u64      exclusive scan of those sums (one 256-thread workgroup)
 //   k_brick_offsets segment offset and work items of each brick
-//   k_seg_fill      enumerate again, write each segment record at its
+//   k_seg_fill      reserve the run of each entry of the workgroup's stored
+//                   table, enumerate again, write each segment record at its
 //                   brick's next position
 //   k_brick_walk    one workgroup per work item: walk the item's segments
 //                   into the LDS bitmap, write the brick's leaf records at
@@ -40,7 +42,7 @@ namespace vdbgpu {
 
 namespace {
 
-constexpr int kLdsBricks = 1024;  // per-workgroup brick aggregation table
+constexpr int kLdsBricks = kSegTab;  // per-workgroup brick aggregation table
 constexpr int kLdsBrickProbe = 32;
 constexpr uint32_t kMaxBrickProbe = 512;
 constexpr int kBrickWords = 1024;  // 64 leaves x (8 visited + 8 hit) words
@@ -167,7 +169,8 @@ __global__ __launch_bounds__(256) void k_seg_count(const int4* __restrict__ rayi
                                                    const ScanInfo* __restrict__ scans,
                                                    const BatchCtl* __restrict__ ctl,
                                                    const double* __restrict__ tbl, BrickView bv,
-                                                   Status* st) {
+                                                   unsigned long long* __restrict__ wkey,
+                                                   unsigned long long* __restrict__ wsc, Status* st) {
   __shared__ unsigned long long lkey[kLdsBricks];
   __shared__ uint32_t lcnt[kLdsBricks];
   __shared__ unsigned int s_stop;
@@ -197,11 +200,18 @@ __global__ __launch_bounds__(256) void k_seg_count(const int4* __restrict__ rayi
     }
   }
   __syncthreads();
+  // the table, as it stands, for the fill: a segment is counted here iff its
+  // brick is in the table (a key once in the table stays; one that found its
+  // probe run full never gets in), so the fill's lookups in the same table
+  // split the workgroup's segments exactly as the counts did
+  const size_t tb = (size_t)blockIdx.x * kLdsBricks;
   for (int e = threadIdx.x; e < kLdsBricks; e += blockDim.x) {
     const uint64_t key = lkey[e];
+    wkey[tb + e] = key;
     if (key == kEmpty) continue;
     const uint32_t gs = brick_slot(bv, key, true, &st->overflow);
     if (gs != kNone) atomicAdd(&bv.cnt[gs], lcnt[e]);
+    wsc[tb + e] = ((uint64_t)gs << 32) | lcnt[e];
   }
   nseg = wave_sum_u32(nseg);
   if (bw_lane() == 0 && nseg) atomicAdd(&st->segs, nseg);
@@ -301,47 +311,39 @@ __global__ __launch_bounds__(256) void k_seg_fill(const int4* __restrict__ rayin
                                                   const ScanInfo* __restrict__ scans,
                                                   const BatchCtl* __restrict__ ctl,
                                                   const double* __restrict__ tbl, BrickView bv,
+                                                  const unsigned long long* __restrict__ wkey,
+                                                  const unsigned long long* __restrict__ wsc,
                                                   uint4* __restrict__ segs, uint32_t seg_cap,
                                                   Status* st) {
   __shared__ unsigned long long lkey[kLdsBricks];
   __shared__ uint32_t lcnt[kLdsBricks];
   __shared__ uint32_t lbase[kLdsBricks];
   __shared__ unsigned int s_stop;
+  // stops when the count stopped: a count workgroup that stored no table saw
+  // the flags this one sees (nothing clears them between the two launches)
   if (stopped(st, &s_stop)) return;
+  // 1: the count's table of this workgroup (the same rays), one reservation
+  // per (workgroup, brick)
+  const size_t tb = (size_t)blockIdx.x * kLdsBricks;
   for (int e = threadIdx.x; e < kLdsBricks; e += blockDim.x) {
-    lkey[e] = kEmpty;
+    const uint64_t key = wkey[tb + e];
+    lkey[e] = key;
     lcnt[e] = 0;
+    if (key == kEmpty) continue;
+    const uint64_t sc = wsc[tb + e];
+    const uint32_t gs = (uint32_t)(sc >> 32);
+    if (gs == kNone) {
+      atomicOr(&st->error, kErrBrick);
+      lbase[e] = seg_cap;
+    } else {
+      lbase[e] = atomicAdd(&bv.cur[gs], (uint32_t)sc);
+    }
   }
   __syncthreads();
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   RayGeom g;
   const bool valid = i < ctl->n_rays && ray_geom(rayinfo, scans, i, g);
-  // 1: local counts of the bricks that fit the LDS table
-  if (valid) {
-    SegEnum en;
-    en.init(tbl, g.s, g.e, g.cnt);
-    Segment sg;
-    while (en.next(tbl, sg)) {
-      const int ls = lds_brick(lkey, brick_key(sg.bx, sg.by, sg.bz), true);
-      if (ls >= 0) atomicAdd(&lcnt[ls], 1u);
-    }
-  }
-  __syncthreads();
-  // 2: one reservation per (workgroup, brick)
-  for (int e = threadIdx.x; e < kLdsBricks; e += blockDim.x) {
-    const uint64_t key = lkey[e];
-    if (key == kEmpty) continue;
-    const uint32_t gs = brick_slot(bv, key, false, &st->overflow);
-    if (gs == kNone) {
-      atomicOr(&st->error, kErrBrick);
-      lbase[e] = seg_cap;
-    } else {
-      lbase[e] = atomicAdd(&bv.cur[gs], lcnt[e]);
-    }
-    lcnt[e] = 0;
-  }
-  __syncthreads();
-  // 3: write the records
+  // 2: write the records
   if (valid) {
     SegEnum en;
     en.init(tbl, g.s, g.e, g.cnt);
@@ -618,8 +620,9 @@ hipError_t launch_iota_u32(hipStream_t s, uint32_t* p, uint32_t n) {
 }
 
 hipError_t launch_seg_count(hipStream_t s, int grid, const int4* rayinfo, const ScanInfo* scans,
-                            const BatchCtl* ctl, const double* tbl, BrickView bv, Status* st) {
-  hipLaunchKernelGGL(k_seg_count, dim3(grid), dim3(256), 0, s, rayinfo, scans, ctl, tbl, bv, st);
+                            const BatchCtl* ctl, const double* tbl, BrickView bv, unsigned long long* wkey,
+                            unsigned long long* wsc, Status* st) {
+  hipLaunchKernelGGL(k_seg_count, dim3(grid), dim3(256), 0, s, rayinfo, scans, ctl, tbl, bv, wkey, wsc, st);
   return hipGetLastError();
 }
 
@@ -639,9 +642,10 @@ hipError_t launch_brick_offsets(hipStream_t s, BrickView bv, uint64_t* bsum, uin
 }
 
 hipError_t launch_seg_fill(hipStream_t s, int grid, const int4* rayinfo, const ScanInfo* scans,
-                           const BatchCtl* ctl, const double* tbl, BrickView bv, uint4* segs,
+                           const BatchCtl* ctl, const double* tbl, BrickView bv,
+                           const unsigned long long* wkey, const unsigned long long* wsc, uint4* segs,
                            uint32_t seg_cap, Status* st) {
-  hipLaunchKernelGGL(k_seg_fill, dim3(grid), dim3(256), 0, s, rayinfo, scans, ctl, tbl, bv, segs,
+  hipLaunchKernelGGL(k_seg_fill, dim3(grid), dim3(256), 0, s, rayinfo, scans, ctl, tbl, bv, wkey, wsc, segs,
                      seg_cap, st);
   return hipGetLastError();
 }

@@ -80,13 +80,19 @@ hipError_t launch_gather_leaves(hipStream_t s, int grid, const uint32_t* list, u
                                 MapView mp, unsigned long long* keys_out, float* vals_out,
                                 unsigned long long* act_out);
 
+// Entries of a workgroup's brick table in k_seg_count / k_seg_fill (LDS); the
+// count stores each workgroup's table (keys; brick slot << 32 | segments) at
+// workgroup * kSegTab of wkey / wsc, and the fill rereads it.
+constexpr int kSegTab = 1024;
 hipError_t launch_seg_count(hipStream_t s, int grid, const int4* rayinfo, const ScanInfo* scans,
-                            const BatchCtl* ctl, const double* tbl, BrickView bv, Status* st);
+                            const BatchCtl* ctl, const double* tbl, BrickView bv, unsigned long long* wkey,
+                            unsigned long long* wsc, Status* st);
 hipError_t launch_brick_offsets(hipStream_t s, BrickView bv, uint64_t* bsum, uint64_t* boff,
                                 BrickItem* items, uint32_t seg_cap, uint32_t item_cap,
                                 uint32_t* split_slot, uint32_t split_cap, Status* st);
 hipError_t launch_seg_fill(hipStream_t s, int grid, const int4* rayinfo, const ScanInfo* scans,
-                           const BatchCtl* ctl, const double* tbl, BrickView bv, uint4* segs,
+                           const BatchCtl* ctl, const double* tbl, BrickView bv,
+                           const unsigned long long* wkey, const unsigned long long* wsc, uint4* segs,
                            uint32_t seg_cap, Status* st);
 hipError_t launch_brick_walk(hipStream_t s, int grid, const BrickItem* items, const uint4* segs,
                              const int4* rayinfo, const ScanInfo* scans, const double* tbl,

@@ -160,6 +160,11 @@ struct vdbgpu_map {
   uint32_t item_cap = 0;
   uint32_t* split_slot = nullptr;
   unsigned long long* shared_bits = nullptr;
+  // per-workgroup brick tables of the segment count (kSegTab entries per
+  // workgroup), reread by the segment fill
+  unsigned long long* wkey = nullptr;
+  unsigned long long* wsc = nullptr;
+  uint64_t wtab_cap = 0;
   uint32_t split_cap = 0;
   // update-leaf records: the walk fills positional records (64 positions per
   // work item, cast stream only), listed and copied densely into one record
@@ -1104,9 +1109,16 @@ int run_brick(vdbgpu_map* m, Source& src, const float* xyz, int64_t n, int64_t s
                            (uint32_t)std::max<uint64_t>(1u << 16, std::min<uint64_t>(8ull * n, 0xC0000000ull)))))
       return r;
   }
+  const int nb = (int)((n + kPrepBlock - 1) / kPrepBlock);
+  if ((uint64_t)nb * kSegTab > m->wtab_cap) {  // (hipFree waits for the kernels using them)
+    dfree(m, &m->wkey, m->wtab_cap);
+    dfree(m, &m->wsc, m->wtab_cap);
+    m->wtab_cap = (uint64_t)nb * kSegTab;
+    CK(dalloc(m, &m->wkey, m->wtab_cap));
+    CK(dalloc(m, &m->wsc, m->wtab_cap));
+  }
   uint64_t vb = 0;
   if ((r = stage_scans(m, cs, xyz, stride, on_device, {sc}, (uint32_t)n, false, &vb, true))) return r;
-  const int nb = (int)((n + kPrepBlock - 1) / kPrepBlock);
   // record set X: free once the map stream has applied the scan that last used it
   const int X = m->rec_cur;
   CK(hipStreamWaitEvent(cs, m->rec_free[X], 0));
@@ -1123,11 +1135,11 @@ int run_brick(vdbgpu_map* m, Source& src, const float* xyz, int64_t n, int64_t s
     }
     {
       Timer t(m, kTraverse, cs);
-      CK(launch_seg_count(cs, nb, m->rayinfo, m->d_scans, m->d_ctl, m->tbl, bv, &m->d_st[0]));
+      CK(launch_seg_count(cs, nb, m->rayinfo, m->d_scans, m->d_ctl, m->tbl, bv, m->wkey, m->wsc, &m->d_st[0]));
       CK(launch_brick_offsets(cs, bv, m->bbsum, m->bboff, m->items, m->seg_cap, m->item_cap,
                               m->split_slot, m->split_cap, &m->d_st[0]));
-      CK(launch_seg_fill(cs, nb, m->rayinfo, m->d_scans, m->d_ctl, m->tbl, bv, m->segs, m->seg_cap,
-                         &m->d_st[0]));
+      CK(launch_seg_fill(cs, nb, m->rayinfo, m->d_scans, m->d_ctl, m->tbl, bv, m->wkey, m->wsc, m->segs,
+                         m->seg_cap, &m->d_st[0]));
     }
     {
       Timer t(m, kWalk, cs);
@@ -1680,6 +1692,8 @@ void vdbgpu_destroy(vdbgpu_map* m) {
   dfree(m, &m->items, m->item_cap);
   dfree(m, &m->split_slot, m->split_cap);
   dfree(m, &m->shared_bits, (size_t)m->split_cap * 1024);
+  dfree(m, &m->wkey, m->wtab_cap);
+  dfree(m, &m->wsc, m->wtab_cap);
   dfree(m, &m->rkeys, m->rec_cap);
   dfree(m, &m->rmasks, (size_t)m->rec_cap * 16);
   dfree(m, &m->rlist, m->rec_cap);
