@@ -1125,6 +1125,10 @@ int run_brick(vdbgpu_map* m, Source& src, const float* xyz, int64_t n, int64_t s
   if (m->dcap[X] < m->dense_hint) {  // a first use of set X: size it like the others
     CK(hipStreamSynchronize(cs));    // (the map stream is done with it)
     if ((r = grow_set(m, X, m->dense_hint))) return r;
+    // the ring's sets never used yet are sized now as well, so the next
+    // scans of the ring do not each stop the cast stream for an allocation
+    for (int y = 0; y < m->rec_sets; ++y)
+      if (m->dcap[y] == 0 && (r = grow_set(m, y, m->dense_hint))) return r;
   }
   for (int attempt = 0;; ++attempt) {
     const BrickView bv{m->bkeys, m->bcnt, m->bcnt + m->bcap, m->bcap - 1};
