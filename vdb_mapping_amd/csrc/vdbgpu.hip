@@ -1113,7 +1113,8 @@ int run_brick(vdbgpu_map* m, Source& src, const float* xyz, int64_t n, int64_t s
   if ((uint64_t)nb * kSegTab > m->wtab_cap) {  // (hipFree waits for the kernels using them)
     dfree(m, &m->wkey, m->wtab_cap);
     dfree(m, &m->wsc, m->wtab_cap);
-    m->wtab_cap = (uint64_t)nb * kSegTab;
+    // room for 262,144 rays at least, then powers of two: few growths
+    m->wtab_cap = std::max<uint64_t>(1024, next_pow2((uint64_t)nb)) * kSegTab;
     CK(dalloc(m, &m->wkey, m->wtab_cap));
     CK(dalloc(m, &m->wsc, m->wtab_cap));
   }
