@@ -1260,7 +1260,7 @@ int commit_staged(vdbgpu_map* m) {
     return VDBGPU_OK;
   }
   if ((r = flush_pending(m))) return r;  // earlier scans first
-  // 128 lanes per leaf; three workgroups per CU (each lane keeps two words in
+  // 128 lanes per leaf; three workgroups per CU (each lane keeps four words in
   // flight) leave the CUs to the next scan's cast: fewer apply waves slow the
   // apply less than they speed up the overlapped cast (measured 3-6 per CU)
   const int g2 = std::min<int>(m->num_cu * 3, (int)((recs + 1) / 2));
