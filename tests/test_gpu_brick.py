@@ -61,6 +61,24 @@ def test_outdoor_stream_insert_brick_equals_hash():
     assert gb.stats()["visits"] == gh.stats()["visits"]
 
 
+def test_scan_sizes_grow_and_shrink_the_cut_tables():
+    """Scans of 600,000, 100,000 and 1,200,000 rays (2,344, 391 and 4,688
+    workgroups of the brick cut, whose per-workgroup brick tables are stored by
+    the count and reread by the fill): the tables grow past their first size
+    and are reused by a smaller scan; brick path = hash path, bit for bit."""
+    gb, gh = two_paths(0.1, 10.0, **W.PROBS)
+    rng = np.random.default_rng(17)
+    for n in (600_000, 100_000, 1_200_000):
+        print(f"scan of {n} rays", flush=True)
+        orig = rng.uniform(-1.0, 1.0, 3)
+        pts = (orig + rng.normal(0.0, 3.0, (n, 3))).astype(np.float32)
+        for g in (gb, gh):
+            assert g.insert("s", pts, orig) == V.OK
+    assert exports_equal(gb, gh) > 10_000
+    assert gb.stats()["visits"] == gh.stats()["visits"]
+    assert gb.stats()["rays"] == 1_900_000
+
+
 @pytest.mark.parametrize("origin", [(0.25, 0.25, 0.25), (-31.75, 63.25, -0.75), (31.5, -32.5, 95.5)])
 def test_tie_rays_across_brick_faces(origin):
     """Rays with exact crossing ties (diagonals, rational multiples) from
