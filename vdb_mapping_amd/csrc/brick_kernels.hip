@@ -581,12 +581,33 @@ __global__ __launch_bounds__(256) void k_rec_gather(const uint32_t* __restrict__
   if (st->error | st->overflow) return;
   const uint32_t n = st->records;
   if (n > dcap) return;
-  const int l = (int)(threadIdx.x & 15);
-  const uint64_t G = ((uint64_t)gridDim.x * blockDim.x) >> 4;
-  for (uint64_t i = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 4; i < n; i += G) {
-    const uint32_t pos = list[i];
-    dense.masks[i * 16 + l] = sparse.masks[(size_t)pos * 16 + l];
-    if (l == 0) dense.keys[i] = sparse.keys[pos];
+  // eight lanes per record, 16 bytes each (the 128 mask bytes); each lane
+  // group keeps kGatherRecs records in flight (list, then record loads, then
+  // stores, issued together)
+  constexpr int kGatherRecs = 4;
+  const int l = (int)(threadIdx.x & 7);
+  const uint64_t G = ((uint64_t)gridDim.x * blockDim.x) >> 3;
+  const uint4* __restrict__ src = reinterpret_cast<const uint4*>(sparse.masks);
+  uint4* __restrict__ dst = reinterpret_cast<uint4*>(dense.masks);
+  for (uint64_t i0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 3; i0 < n; i0 += kGatherRecs * G) {
+    uint32_t pos[kGatherRecs];
+#pragma unroll
+    for (int r = 0; r < kGatherRecs; ++r) pos[r] = i0 + r * G < n ? list[i0 + r * G] : kNone;
+    uint4 v[kGatherRecs];
+    unsigned long long key[kGatherRecs];
+#pragma unroll
+    for (int r = 0; r < kGatherRecs; ++r) {
+      if (pos[r] == kNone) continue;
+      v[r] = src[(size_t)pos[r] * 8 + l];
+      if (l == 0) key[r] = sparse.keys[pos[r]];
+    }
+#pragma unroll
+    for (int r = 0; r < kGatherRecs; ++r) {
+      if (pos[r] == kNone) continue;
+      const uint64_t i = i0 + r * G;
+      dst[i * 8 + l] = v[r];
+      if (l == 0) dense.keys[i] = key[r];
+    }
   }
 }
 
