@@ -2206,7 +2206,7 @@ int32_t vdbgpu_set_tuning(vdbgpu_map* m, const vdbgpu_tuning* t) {
     return VDBGPU_BAD_ARGUMENT;
   Guard g(m);  // applies the scans deferred under the old setting
   if (g.rc) return g.rc;
-  const int sets = t->defer > 1 ? t->defer + 2 : 2;
+  const int sets = t->defer > 1 ? 2 * t->defer + 2 : 2;  // as at creation (kRecSets for the default 8)
   if (sets != m->rec_sets) {  // a new ring of record sets
     CK(hipStreamSynchronize(m->cast));
     CK(hipStreamSynchronize(m->stream));
