@@ -795,6 +795,25 @@ int64_t orc_cast_ray(const int32_t s[3], const int32_t e[3], int32_t* out, int64
   return n;
 }
 
+// castRayIntoGridFast (VM:577-602) on its own: the map voxels the fast-mode
+// cast from voxel s to voxel e marks, in visiting order (a voxel can repeat
+// when spans share it). -1 without an intersector.
+int64_t orc_cast_ray_fast(void* h, const int32_t s[3], const int32_t e[3], int32_t* out,
+                          int64_t cap) {
+  const Map& m = *(const Map*)h;
+  if (!m.isect) return -1;
+  int64_t n = 0;
+  cast_ray_fast(m, *m.isect, s, e, [&](int32_t x, int32_t y, int32_t z) {
+    if (n < cap) {
+      out[3 * n] = x;
+      out[3 * n + 1] = y;
+      out[3 * n + 2] = z;
+    }
+    ++n;
+  });
+  return n;
+}
+
 // raytrace (VM:675-721).
 int orc_raytrace(void* h, int64_t n, const double* origins, const double* dirs,
                  const double* lens, uint8_t* success, double* ends) {

@@ -13,11 +13,16 @@
  * oracle.cpp next to the code that uses them.
  *
  * Pinning (tests/test_oracle.py): the reference's known-answer tests, closed
- * forms, an exact-rational DDA walk and brute-force set checks. Parity
- * unpinned (no check here can compare them with OpenVDB itself): the rounding
- * forms U1-U3, the double tie-break between axes of different extent (U5),
- * raytrace clip/HDDA details (U6, U7, U9, U10), the fast-mode hits walk
- * (U11), leaf creation by no-op updates (U8), the section-apply visit of
+ * forms, an exact-rational DDA walk and brute-force set checks. raytrace and
+ * the fast-mode cast are checked against an exact-rational flat leaf-cell
+ * march (tests/test_oracle_query_exact.py) on random maps with inactive leaves
+ * inside and outside the active box, which also catches planted mistakes
+ * (clip skipped, first voxel tested, box pad off by one either way, a later
+ * run, spans dropped). Parity unpinned (no check here can compare them with
+ * OpenVDB itself): the rounding forms U1-U3, the double tie-break between axes
+ * of different extent (U5), the OpenVDB semantics restated for raytrace (U6,
+ * U7, U9, U10) and the fast-mode hits walk (U11), which the brute force
+ * restates in the same terms, leaf creation by no-op updates (U8), the section-apply visit of
  * internal-node tiles (U12), the bounding-box corner transform (U13) and the
  * leaf set of the morphology operators (U14; their voxel states are pinned
  * against scipy.ndimage's binary dilation and erosion).
@@ -86,6 +91,11 @@ float orc_get_value(void* h, const int32_t ijk[3], int32_t* active);
 /* Primitives, exposed so tests can pin them on their own. */
 void orc_world_to_index(double resolution, const double world[3], int32_t out[3]);
 int64_t orc_cast_ray(const int32_t s[3], const int32_t e[3], int32_t* out_xyz, int64_t cap);
+
+/* castRayIntoGridFast (VM:577-602): the map-active voxels a fast-mode cast from
+ * voxel s to voxel e marks, in visiting order; needs the intersector (else -1). */
+int64_t orc_cast_ray_fast(void* h, const int32_t s[3], const int32_t e[3], int32_t* out_xyz,
+                          int64_t cap);
 
 /* Batch raytrace (VM:675-721). Requires the intersector state the reference
  * builds in integrateUpdate when fast_mode is set (VM:1436-1449). */
