@@ -192,7 +192,7 @@ int32_t vdbgpu_device_free(void* ptr);
 int32_t vdbgpu_device_upload(void* dst, const void* src, int64_t bytes);
 int32_t vdbgpu_device_download(void* dst, const void* src, int64_t bytes);
 /* Device-to-device copy (the plain copy kernel the traffic counters are
- * calibrated against). */
+ * calibrated against); returns after the copy has completed. */
 int32_t vdbgpu_device_copy(void* dst, const void* src, int64_t bytes);
 
 /* raytrace with every array in device memory (same layout). */

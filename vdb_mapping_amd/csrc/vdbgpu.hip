@@ -2162,8 +2162,10 @@ int32_t vdbgpu_device_download(void* dst, const void* src, int64_t bytes) {
 
 int32_t vdbgpu_device_copy(void* dst, const void* src, int64_t bytes) {
   if (!dst || !src || bytes < 0) return VDBGPU_BAD_ARGUMENT;
-  return hipMemcpy(dst, src, (size_t)bytes, hipMemcpyDeviceToDevice) == hipSuccess ? VDBGPU_OK
-                                                                                   : VDBGPU_DEVICE_ERROR;
+  // a device-to-device hipMemcpy may return before the copy has run: wait for it,
+  // so that the call's duration is the copy's
+  if (hipMemcpy(dst, src, (size_t)bytes, hipMemcpyDeviceToDevice) != hipSuccess) return VDBGPU_DEVICE_ERROR;
+  return hipStreamSynchronize(nullptr) == hipSuccess ? VDBGPU_OK : VDBGPU_DEVICE_ERROR;
 }
 
 int32_t vdbgpu_synchronize(vdbgpu_map* m) {
