@@ -200,6 +200,22 @@ int32_t vdbgpu_raytrace_device(vdbgpu_map* map, int64_t n, const double* origins
                                const double* directions, const double* max_lengths,
                                uint8_t* success, double* end_points);
 
+/* raytrace (same arguments and results as vdbgpu_raytrace, host pointers) that
+ * allocates nothing: no hipMalloc / hipFree, no growth of the handle's buffers.
+ * No counterpart in the reference; for callers that cannot allocate (code under
+ * HIP graph capture, tight serving or planning loops, harnesses that forbid
+ * allocation). Queries, results and the rejected-query counter are staged in a
+ * static 32 MiB device array of the library, one per GPU, held until the
+ * process ends. Limits: at most 393216 queries per call (VDBGPU_BAD_ARGUMENT
+ * beyond); map updates still waiting to be applied (deferred brick-path scans)
+ * are refused with VDBGPU_BAD_ARGUMENT instead of applied, since applying them
+ * may allocate: call vdbgpu_synchronize first. Calls on one GPU share its
+ * array: the library serialises them within a process; calls from different
+ * processes on one GPU use different arrays. */
+int32_t vdbgpu_raytrace_noalloc(vdbgpu_map* map, int64_t n, const double* origins,
+                                const double* directions, const double* max_lengths,
+                                uint8_t* success, double* end_points);
+
 /* Wait for all work of the handle. */
 int32_t vdbgpu_synchronize(vdbgpu_map* map);
 /* Counters; profiling adds HIP-event timing of every kernel launch. */

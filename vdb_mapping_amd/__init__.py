@@ -20,6 +20,7 @@ INCLUDE_DIR = os.path.join(os.path.dirname(HERE), "include")
 OK, NOT_CONFIGURED, UNKNOWN_SOURCE, BAD_MAX_RANGE, BAD_PROB_MISS, BAD_PROB_HIT = 0, 1, 2, 3, 4, 5
 OUT_OF_RANGE, UNSUPPORTED, DEVICE_ERROR, BAD_ARGUMENT, NO_INTERSECTOR = 6, 7, 8, 9, 10
 OVER_BUDGET = 11
+NOALLOC_MAX_QUERIES = 393216  # vdbgpu_raytrace_noalloc: queries per call its static scratch holds
 
 # Every symbol include/vdbgpu.h declares.
 ABI_SYMBOLS = [
@@ -30,7 +31,7 @@ ABI_SYMBOLS = [
     "vdbgpu_empty",
     "vdbgpu_leaf_count", "vdbgpu_export_leaves", "vdbgpu_export_dirty_leaves",
     "vdbgpu_update_leaf_count", "vdbgpu_export_update", "vdbgpu_raytrace", "vdbgpu_raytrace_device",
-    "vdbgpu_synchronize",
+    "vdbgpu_raytrace_noalloc", "vdbgpu_synchronize",
     "vdbgpu_get_stats", "vdbgpu_set_profiling", "vdbgpu_set_tuning", "vdbgpu_get_tuning",
     "vdbgpu_last_error", "vdbgpu_device_alloc", "vdbgpu_device_free", "vdbgpu_device_upload",
     "vdbgpu_device_download", "vdbgpu_device_copy", "vdbgpu_async_start", "vdbgpu_async_submit",
@@ -137,7 +138,7 @@ def lib():
         L.vdbgpu_update_leaf_count.argtypes = [P, ctypes.c_char_p]
         L.vdbgpu_export_update.restype = I64
         L.vdbgpu_export_update.argtypes = [P, ctypes.c_char_p, P, P, P, I64]
-        for f in (L.vdbgpu_raytrace, L.vdbgpu_raytrace_device):
+        for f in (L.vdbgpu_raytrace, L.vdbgpu_raytrace_device, L.vdbgpu_raytrace_noalloc):
             f.restype = I32
             f.argtypes = [P, I64, P, P, P, P, P]
         L.vdbgpu_get_stats.restype = I32
@@ -525,6 +526,18 @@ class GpuMap:
         succ = np.zeros(n, np.uint8)
         ends = np.zeros((n, 3), np.float64)
         st = lib().vdbgpu_raytrace(self.h, n, _ptr(o), _ptr(d), _ptr(ln), _ptr(succ), _ptr(ends))
+        return st, succ.astype(bool), ends
+
+    def raytrace_noalloc(self, origins, dirs, lengths):
+        """raytrace through the allocation-free entry point (static device scratch,
+        at most NOALLOC_MAX_QUERIES queries, no map updates waiting)."""
+        o = np.ascontiguousarray(origins, dtype=np.float64).reshape(-1, 3)
+        d = np.ascontiguousarray(dirs, dtype=np.float64).reshape(-1, 3)
+        ln = np.ascontiguousarray(lengths, dtype=np.float64).reshape(-1)
+        n = o.shape[0]
+        succ = np.zeros(n, np.uint8)
+        ends = np.zeros((n, 3), np.float64)
+        st = lib().vdbgpu_raytrace_noalloc(self.h, n, _ptr(o), _ptr(d), _ptr(ln), _ptr(succ), _ptr(ends))
         return st, succ.astype(bool), ends
 
     def raytrace_device(self, n: int, d_origins: int, d_dirs: int, d_lens: int, d_success: int,

@@ -2137,6 +2137,94 @@ int32_t vdbgpu_raytrace_device(vdbgpu_map* m, int64_t n, const double* o, const 
   return raytrace_status(m);
 }
 
+}  // extern "C"
+
+// ------------------------------------------------- allocation-free raytrace --
+// vdbgpu_raytrace_noalloc stages the queries, the results and the rejected-query
+// counter in this static device array instead of the handle's growable query
+// buffers. Every GPU the library runs on holds its own copy until the process
+// ends. In host code `scratch` names a host-side stand-in, never the device
+// memory: its device address comes from hipGetSymbolAddress (per device, on the
+// first call there) and only that address is used.
+static __device__ __align__(256) unsigned char scratch[32u << 20];
+
+namespace {
+
+constexpr size_t kNoAllocScratchBytes = sizeof(scratch);
+constexpr int64_t kNoAllocMaxQueries = 393216;  // 81 bytes per query + 5 x 256 bytes of alignment
+static_assert(kNoAllocMaxQueries * 81 + 5 * 256 <= (int64_t)kNoAllocScratchBytes, "scratch too small");
+constexpr int kNoAllocMaxDevices = 64;
+
+struct NoAllocDevice {
+  std::once_flag once;
+  hipError_t err = hipSuccess;
+  unsigned char* ptr = nullptr;  // device address of this GPU's `scratch`
+  std::mutex mu;                 // calls on one GPU share its scratch: one at a time
+};
+NoAllocDevice g_noalloc[kNoAllocMaxDevices];
+
+inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+}  // namespace
+
+extern "C" {
+
+int32_t vdbgpu_raytrace_noalloc(vdbgpu_map* m, int64_t n, const double* o, const double* d,
+                                const double* len, uint8_t* succ, double* ends) {
+  if (!m || n < 0 || (n && (!o || !d || !len || !succ || !ends))) return VDBGPU_BAD_ARGUMENT;
+  Guard g(m, true);  // applying waiting scans could allocate: they are refused below instead
+  if (m->staged || m->npend)
+    return fail(m, VDBGPU_BAD_ARGUMENT,
+                "vdbgpu_raytrace_noalloc: map updates are still waiting to be applied (applying them may "
+                "allocate); call vdbgpu_synchronize first");
+  if (!m->has_isect)
+    return fail(m, VDBGPU_NO_INTERSECTOR, "raytrace needs an integrate in fast mode on a non-empty map");
+  if (n > kNoAllocMaxQueries)
+    return fail(m, VDBGPU_BAD_ARGUMENT,
+                "vdbgpu_raytrace_noalloc: at most 393216 queries per call (the library's static device "
+                "scratch); split the batch");
+  if (n == 0) return VDBGPU_OK;
+  if (m->device < 0 || m->device >= kNoAllocMaxDevices)
+    return fail(m, VDBGPU_UNSUPPORTED, "vdbgpu_raytrace_noalloc: device ordinal above 63");
+  NoAllocDevice& nd = g_noalloc[m->device];
+  // Guard made this handle's device current: the symbol address is that device's copy
+  std::call_once(nd.once, [&nd] { nd.err = hipGetSymbolAddress((void**)&nd.ptr, HIP_SYMBOL(scratch)); });
+  if (nd.err != hipSuccess || !nd.ptr)
+    return fail(m, VDBGPU_DEVICE_ERROR,
+                std::string("vdbgpu_raytrace_noalloc: hipGetSymbolAddress failed: ") + hipGetErrorString(nd.err));
+  std::lock_guard<std::mutex> lk(nd.mu);
+  size_t off = 0;
+  auto carve = [&](size_t bytes) {
+    unsigned char* p = nd.ptr + off;
+    off += align256(bytes);
+    return p;
+  };
+  unsigned int* err = (unsigned int*)carve(sizeof(unsigned long long));
+  double* q_o = (double*)carve((size_t)n * 24);
+  double* q_d = (double*)carve((size_t)n * 24);
+  double* q_l = (double*)carve((size_t)n * 8);
+  uint8_t* q_s = (uint8_t*)carve((size_t)n);
+  double* q_e = (double*)carve((size_t)n * 24);
+  if (off > kNoAllocScratchBytes) return fail(m, VDBGPU_BAD_ARGUMENT, "vdbgpu_raytrace_noalloc: scratch overrun");
+  // the counter keeps the last call's value: clear it with a kernel, not a memset node
+  CK(launch_zero_words(m->stream, (unsigned long long*)err, 1));
+  CK(hipMemcpyAsync(q_o, o, (size_t)n * 24, hipMemcpyHostToDevice, m->stream));
+  CK(hipMemcpyAsync(q_d, d, (size_t)n * 24, hipMemcpyHostToDevice, m->stream));
+  CK(hipMemcpyAsync(q_l, len, (size_t)n * 8, hipMemcpyHostToDevice, m->stream));
+  const int grid = (int)std::min<int64_t>((int64_t)m->num_cu * 16, (n + 255) / 256);
+  CK(launch_raytrace(m->stream, grid, m->rt, map_view(m), m->inv, m->res, n, q_o, q_d, q_l, q_s, q_e, err));
+  unsigned int h_err = 0;
+  CK(hipMemcpyAsync(succ, q_s, (size_t)n, hipMemcpyDeviceToHost, m->stream));
+  CK(hipMemcpyAsync(ends, q_e, (size_t)n * 24, hipMemcpyDeviceToHost, m->stream));
+  CK(hipMemcpyAsync(&h_err, err, sizeof h_err, hipMemcpyDeviceToHost, m->stream));
+  CK(hipStreamSynchronize(m->stream));
+  if (h_err)
+    return fail(m, VDBGPU_BAD_ARGUMENT,
+                "query rays with non-finite values or more than 2^24 voxel steps were rejected "
+                "(success = 0, end point = NaN)");
+  return VDBGPU_OK;
+}
+
 int32_t vdbgpu_device_alloc(int64_t bytes, void** out) {
   if (!out || bytes <= 0) return VDBGPU_BAD_ARGUMENT;
   *out = nullptr;
