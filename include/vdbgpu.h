@@ -67,6 +67,7 @@ typedef struct vdbgpu_stats {
   uint64_t n_prep, n_traverse, n_compact, n_apply, n_raytrace;      /* launches timed */
   double ms_walk;          /* brick path: the brick walk kernel (ms_traverse: the brick cut) */
   uint64_t n_walk;
+  uint64_t scratch_bytes;  /* part of device_bytes that vdbgpu_release_scratch would free */
 } vdbgpu_stats;
 
 /* The leaf pool of a map grows in place in address ranges reserved at its first leaf (HIP
@@ -216,6 +217,15 @@ int32_t vdbgpu_raytrace_noalloc(vdbgpu_map* map, int64_t n, const double* origin
                                 const double* directions, const double* max_lengths,
                                 uint8_t* success, double* end_points);
 
+/* Free the handle's per-scan working memory (point staging, per-ray arrays,
+ * brick tables, segments, records, record sets, fold and group tables, query
+ * staging, idle update sets) after applying any waiting scans; the map, the
+ * sources' pending updates, the raytrace snapshot and the artificial grid are
+ * kept, and the next call that needs a buffer allocates it again. No
+ * reference counterpart: the serving loop (vdbgpu_server_*) calls it to hand
+ * device memory to other maps while requests wait for HBM. */
+int32_t vdbgpu_release_scratch(vdbgpu_map* map);
+
 /* Wait for all work of the handle. */
 int32_t vdbgpu_synchronize(vdbgpu_map* map);
 /* Counters; profiling adds HIP-event timing of every kernel launch. */
@@ -232,10 +242,17 @@ const char* vdbgpu_last_error(vdbgpu_map* map);
  * scans). Here every submitted scan is integrated exactly once, in arrival
  * order per map: a worker per map takes all requests queued while its map was
  * busy and integrates them in one vdbgpu_insert_batch (same result as one by
- * one); different maps run on their own streams and overlap on the device. A
- * map that holds device memory gets no more work while the maps' summed
- * footprint exceeds hbm_budget_bytes (its requests fail with
- * VDBGPU_OVER_BUDGET); 0 = no budget. */
+ * one); different maps run on their own streams and overlap on the device.
+ * HBM admission (hbm_budget_bytes, 0 = no budget): a batch runs when the maps'
+ * device memory, the reservations of the batches in flight and its own
+ * projected need fit the budget (per-scan working memory per point of its
+ * largest scan plus map growth per point, from what finished batches needed;
+ * 2 KiB and 1 KiB to start); otherwise it waits, and lanes give back their
+ * per-scan working memory (vdbgpu_release_scratch) as batches finish. When
+ * nothing is in flight and nothing is left to free, it runs alone if the maps
+ * are within the budget; its requests fail with VDBGPU_OVER_BUDGET only when
+ * the maps alone fill the budget (maps never shrink; reset or destroy maps and
+ * resubmit). */
 typedef struct vdbgpu_server vdbgpu_server;
 
 typedef struct vdbgpu_request_record {
@@ -244,7 +261,7 @@ typedef struct vdbgpu_request_record {
   int32_t map_index;
   int32_t batch_size; /* requests integrated together with this one */
   int32_t status;
-  int32_t _pad;
+  int32_t mem_waits;  /* times its batch waited for device memory before admission */
   double latency_ms;  /* submit -> device work finished */
 } vdbgpu_request_record;
 

@@ -10,13 +10,17 @@
                       tie-heavy rays (update set and map), a fast-mode map
                       with raytrace queries, and the map edits.
   digests.json        SHA-256 digests of the oracle's map export after five
-                      32,768-ray room scans (res 0.1, R 15) and of the update
-                      set of the first scan's first 4,096 rays.
+                      32,768-ray room scans (res 0.1, R 15), of the update
+                      set of the first scan's first 4,096 rays, of the 16
+                      maps of the multi-robot serving test (0.05 m) and of
+                      the map after the first 24 outdoor-stream scans
+                      (262,144 rays each, 0.05 m).
 
 The fixtures are committed; the tests compare both the oracle (drift check)
 and the GPU library (parity without running the oracle) with them. Re-run
 this script only when the restated semantics change on purpose:
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py               # everything (minutes)
+    python tests/golden/make_golden.py serve outdoor # only those digests
 """
 from __future__ import annotations
 
@@ -126,7 +130,72 @@ def known_answers():
     return {"probs": DEFAULT_PROBS, "cases": cases}
 
 
+def multi_robot_digests():
+    """Per-map digests of the multi-robot serving test: 16 maps at 0.05 m, each
+    built by inserting its pool scans (workloads.multi_robot_bursts) in order."""
+    w = W.multi_robot_pool()
+    order = W.multi_robot_bursts()
+    per_map = {}
+    for mi, k in order:
+        per_map.setdefault(mi, []).append(k)
+    maps = []
+    for mi in sorted(per_map):
+        m = O.OracleMap(w.resolution)
+        m.set_config(O.make_config(max_range=w.max_range, **W.PROBS))
+        m.add_input_source("stream", w.max_range)
+        for k in per_map[mi]:
+            s = w.scan(k)
+            assert m.insert("stream", s.points, s.origin) == O.OK
+        o, v, a = m.export_leaves()
+        maps.append({"map": mi, "scans": per_map[mi], "leaves": int(o.shape[0]),
+                     "active_voxels": int(sum(bin(int(x)).count("1") for x in a.ravel())),
+                     "sha256": digest(o, v.view(np.uint32), a)})
+        print("multi-robot map", mi, maps[-1]["leaves"], "leaves", flush=True)
+    return {"workload": "multi-robot serving: 16 maps at 0.05 m, room scene (seed 1001), pool scans "
+                        "(20k-262k points) 3i x5, 3i+1, 3i+2 per map i, p 0.7/0.4/0.12/0.97, R 50",
+            "maps": maps}
+
+
+OUTDOOR_SCANS = 24
+
+
+def outdoor_digest():
+    """Map digest after the first OUTDOOR_SCANS scans of the outdoor stream (three
+    groups of eight consecutive scans, the default deferred group size)."""
+    w = W.outdoor_stream_workload(OUTDOOR_SCANS)
+    m = O.OracleMap(w.resolution)
+    m.set_config(O.make_config(max_range=w.max_range, **W.PROBS))
+    m.add_input_source("stream", w.max_range)
+    for i in range(OUTDOOR_SCANS):
+        s = w.scan(i)
+        assert m.insert("stream", s.points, s.origin) == O.OK
+        print("outdoor scan", i, flush=True)
+    o, v, a = m.export_leaves()
+    return {"workload": "outdoor stream, scans 0-23 inserted in order, 262,144 rays each, res 0.05, R 50, "
+                        "p 0.7/0.4/0.12/0.97",
+            "scans": OUTDOOR_SCANS, "leaves": int(o.shape[0]),
+            "active_voxels": int(sum(bin(int(x)).count("1") for x in a.ravel())),
+            "sha256": digest(o, v.view(np.uint32), a)}
+
+
+def update_digests(**entries):
+    path = os.path.join(HERE, "digests.json")
+    with open(path) as f:
+        res = json.load(f)
+    res.update(entries)
+    with open(path, "w") as f:
+        json.dump(res, f, indent=1)
+
+
 def main():
+    if len(sys.argv) > 1:  # only the named large digests: "serve", "outdoor"
+        O.build()
+        W.build()
+        if "serve" in sys.argv[1:]:
+            update_digests(multi_robot_serve=multi_robot_digests())
+        if "outdoor" in sys.argv[1:]:
+            update_digests(outdoor_stream_24=outdoor_digest())
+        return
     O.build()
     W.build()
     with open(os.path.join(HERE, "known_answers.json"), "w") as f:
@@ -192,6 +261,8 @@ def main():
                          "active_voxels": int(sum(bin(int(x)).count("1") for x in a.ravel())),
                          "sha256": digest(o, v.view(np.uint32), a)},
            "room_scan_first_4096_update": upd}
+    res["multi_robot_serve"] = multi_robot_digests()
+    res["outdoor_stream_24"] = outdoor_digest()
     with open(os.path.join(HERE, "digests.json"), "w") as f:
         json.dump(res, f, indent=1)
     print("wrote", sorted(os.listdir(HERE)))

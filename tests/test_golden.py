@@ -75,3 +75,16 @@ def test_edit_fixture():
     from golden_cases import check_edit_case, run_edit_case
     m = run_edit_case(O.OracleMap, O.make_config)
     check_edit_case(m, cases())
+
+
+def test_multi_robot_digest_first_map():
+    """Drift check of the multi-robot serving digests: the oracle rebuilds map 0."""
+    entry = digests()["multi_robot_serve"]["maps"][0]
+    w = W.multi_robot_pool()
+    m = make_oracle(w.resolution, w.max_range, W.PROBS)
+    for k in entry["scans"]:
+        s = w.scan(k)
+        assert m.insert("test", s.points, s.origin) == O.OK
+    o, v, a = m.export_leaves()
+    assert (o.shape[0], popcount(a)) == (entry["leaves"], entry["active_voxels"])
+    assert digest(o, v.view(np.uint32), a) == entry["sha256"]

@@ -10,6 +10,7 @@ import pytest
 
 import oracle as O
 import vdb_mapping_amd as V
+from golden_cases import digest, digests, popcount
 from parity import both, pair, same_map
 from vdb_mapping_amd import workloads as W
 
@@ -166,6 +167,24 @@ def test_outdoor_stream_deferred_equals_per_scan():
     assert np.array_equal(ka, kb) and np.array_equal(va.view(np.uint32), vb.view(np.uint32))
     assert np.array_equal(ma, mb)
     assert a.stats()["voxel_updates"] == b.stats()["voxel_updates"]
+
+
+def test_outdoor_stream_24_scans_matches_oracle_digest():
+    """The timed configuration (outdoor stream, 262,144-point scans at 0.05 m,
+    brick path, defer 8) over its first 24 scans, three full group passes: the
+    map equals the oracle's after the same scans (committed digest)."""
+    dg = digests()["outdoor_stream_24"]
+    w = W.outdoor_stream_workload(dg["scans"])
+    g = V.GpuMap(w.resolution)
+    assert g.set_config(V.make_config(max_range=w.max_range, **W.PROBS)) == V.OK
+    g.add_input_source("s", w.max_range)
+    assert g.get_tuning()["defer"] == 8 and g.get_tuning()["traversal"] == 0
+    for i in range(dg["scans"]):
+        s = w.scan(i)
+        assert g.insert("s", s.points, s.origin) == V.OK
+    o, v, m = g.export_leaves()
+    assert (o.shape[0], popcount(m)) == (dg["leaves"], dg["active_voxels"])
+    assert digest(o, v.view(np.uint32), m) == dg["sha256"]
 
 
 def test_plain_leaf_pool_equals_reserved_ranges(monkeypatch):

@@ -31,7 +31,7 @@ ABI_SYMBOLS = [
     "vdbgpu_empty",
     "vdbgpu_leaf_count", "vdbgpu_export_leaves", "vdbgpu_export_dirty_leaves",
     "vdbgpu_update_leaf_count", "vdbgpu_export_update", "vdbgpu_raytrace", "vdbgpu_raytrace_device",
-    "vdbgpu_raytrace_noalloc", "vdbgpu_synchronize",
+    "vdbgpu_raytrace_noalloc", "vdbgpu_release_scratch", "vdbgpu_synchronize",
     "vdbgpu_get_stats", "vdbgpu_set_profiling", "vdbgpu_set_tuning", "vdbgpu_get_tuning",
     "vdbgpu_last_error", "vdbgpu_device_alloc", "vdbgpu_device_free", "vdbgpu_device_upload",
     "vdbgpu_device_download", "vdbgpu_device_copy", "vdbgpu_async_start", "vdbgpu_async_submit",
@@ -73,7 +73,7 @@ class Stats(ctypes.Structure):
                                                "ms_raytrace")] + \
                [(n, ctypes.c_uint64) for n in ("n_prep", "n_traverse", "n_compact", "n_apply",
                                                "n_raytrace")] + \
-               [("ms_walk", ctypes.c_double), ("n_walk", ctypes.c_uint64)]
+               [("ms_walk", ctypes.c_double), ("n_walk", ctypes.c_uint64), ("scratch_bytes", ctypes.c_uint64)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
@@ -87,7 +87,7 @@ class Tuning(ctypes.Structure):
 
 class RequestRecord(ctypes.Structure):
     _fields_ = [("request_id", ctypes.c_int64), ("points", ctypes.c_int64), ("map_index", ctypes.c_int32),
-                ("batch_size", ctypes.c_int32), ("status", ctypes.c_int32), ("_pad", ctypes.c_int32),
+                ("batch_size", ctypes.c_int32), ("status", ctypes.c_int32), ("mem_waits", ctypes.c_int32),
                 ("latency_ms", ctypes.c_double)]
 
 
@@ -666,7 +666,7 @@ class Server:
         n = lib().vdbgpu_server_records(self.h, None, 0)
         arr = (RequestRecord * max(n, 1))()
         got = lib().vdbgpu_server_records(self.h, arr, n)
-        return [{k: getattr(arr[i], k) for k, _ in RequestRecord._fields_ if k != "_pad"} for i in range(got)]
+        return [{k: getattr(arr[i], k) for k, _ in RequestRecord._fields_} for i in range(got)]
 
     def map(self, i: int) -> "GpuMap":
         return GpuMap(self.resolution, _borrowed_handle=lib().vdbgpu_server_map(self.h, i))

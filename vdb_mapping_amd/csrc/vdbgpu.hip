@@ -1546,6 +1546,89 @@ int check_origins(vdbgpu_map* m, int64_t n, const int32_t* origins) {
 // deferred scans, so whatever it reads or changes sees the map of every
 // earlier insert; when that fails the call returns the failure (last_error
 // says what it was) and does nothing else.
+// The per-scan working memory of the insert, batch, edit and query paths:
+// point staging, per-ray arrays, brick tables, segments, work items, records,
+// record sets, group-pass and fold tables, query staging, and the update sets
+// of sources with nothing pending. Each buffer regrows on its next use, as
+// after vdbgpu_create. `release` frees them (the caller has waited for both
+// streams); otherwise their bytes are summed.
+uint64_t walk_scratch(vdbgpu_map* m, bool release) {
+  uint64_t bytes = 0;
+  auto one = [&](auto*& p, size_t count) {
+    if (!p) return;
+    bytes += count * sizeof(*p);
+    if (release) dfree(m, &p, count);
+  };
+  const uint32_t nb = (m->ray_cap + kPrepBlock - 1) / kPrepBlock;
+  one(m->d_points, m->points_bytes / 4);
+  one(m->rayinfo, m->ray_cap);
+  one(m->pre, m->ray_cap);
+  one(m->roff, m->ray_cap);
+  one(m->bsum, nb);
+  one(m->boff, nb);
+  one(m->first_ray, m->chunk_cap);
+  one(m->eidx, m->eidx_cap);
+  one(m->bkeys, m->bcap);
+  one(m->bcnt, (size_t)m->bcap * 2);
+  one(m->bbsum, m->bcap / 256);
+  one(m->bboff, m->bcap / 256);
+  one(m->segs, m->seg_cap);
+  one(m->items, m->item_cap);
+  one(m->split_slot, m->split_cap);
+  one(m->shared_bits, (size_t)m->split_cap * 1024);
+  one(m->wkey, m->wtab_cap);
+  one(m->wsc, m->wtab_cap);
+  one(m->rkeys, m->rec_cap);
+  one(m->rmasks, (size_t)m->rec_cap * 16);
+  one(m->rlist, m->rec_cap);
+  one(m->leaf_mask, m->group_cap);
+  one(m->lbsum, m->group_cap / 256);
+  one(m->lboff, m->group_cap / 256);
+  for (int X = 0; X < vdbgpu_map::kRecSets; ++X) {
+    one(m->dkeys[X], m->dcap[X]);
+    one(m->dmasks[X], (size_t)m->dcap[X] * 16);
+    one(m->reidx[X], m->dcap[X]);
+  }
+  one(m->ident, m->ident_cap);
+  one(m->gslot, m->gslot_cap);
+  one(m->rgrp, m->rgrp_cap);
+  one(m->gl, m->grp_cap);
+  one(m->grec, m->grp_cap * kDeferMax);
+  one(m->gkeys, m->gcap);
+  one(m->gscans, (size_t)m->gcap * 2);
+  one(m->gneed, m->gcap);
+  one(m->glist, m->gcap);
+  one(m->gidx, m->gcap);
+  one(m->q_in, (size_t)m->q_cap * 7);
+  one(m->q_succ, (size_t)m->q_cap);
+  one(m->q_end, (size_t)m->q_cap * 3);
+  auto upd = [&](Source& u) {
+    if (u.pending) return;
+    one(u.keys, u.cap);
+    one(u.masks, (size_t)u.cap * 16);
+    one(u.list, u.cap);
+    if (release) u.cap = 0;
+  };
+  for (auto& kv : m->sources) upd(kv.second);
+  upd(m->edit);
+  if (release) {
+    m->points_bytes = 0;
+    m->ray_cap = 0;
+    m->chunk_cap = 0;
+    m->eidx_cap = 0;
+    m->bcap = m->seg_cap = m->item_cap = m->split_cap = m->rec_cap = m->group_cap = 0;
+    m->wtab_cap = 0;
+    for (int X = 0; X < vdbgpu_map::kRecSets; ++X) m->dcap[X] = 0;
+    m->ident_cap = 0;
+    m->gslot_cap = 0;
+    m->rgrp_cap = 0;
+    m->grp_cap = 0;
+    m->gcap = 0;
+    m->q_cap = 0;
+  }
+  return bytes;
+}
+
 struct Guard {
   vdbgpu_map* m;
   std::lock_guard<std::mutex> lk;
@@ -2256,6 +2339,16 @@ int32_t vdbgpu_device_copy(void* dst, const void* src, int64_t bytes) {
   return hipStreamSynchronize(nullptr) == hipSuccess ? VDBGPU_OK : VDBGPU_DEVICE_ERROR;
 }
 
+int32_t vdbgpu_release_scratch(vdbgpu_map* m) {
+  if (!m) return VDBGPU_BAD_ARGUMENT;
+  Guard g(m);  // applies the waiting scans first: nothing in flight uses the scratch after the syncs
+  if (g.rc) return g.rc;
+  CK(hipStreamSynchronize(m->cast));
+  CK(hipStreamSynchronize(m->stream));
+  walk_scratch(m, true);
+  return VDBGPU_OK;
+}
+
 int32_t vdbgpu_synchronize(vdbgpu_map* m) {
   if (!m) return VDBGPU_BAD_ARGUMENT;
   Guard g(m);
@@ -2273,6 +2366,7 @@ int32_t vdbgpu_get_stats(vdbgpu_map* m, vdbgpu_stats* out) {
   m->stats.leaf_count = m->h_st[1].leaf_count;
   m->stats.voxel_updates = m->h_st[3].voxels;
   m->stats.device_bytes = m->device_bytes;
+  m->stats.scratch_bytes = walk_scratch(m, false);
   *out = m->stats;
   // batch folds count on the host, every k_map_insert on the device
   out->update_leaves = m->stats.update_leaves + m->h_st[3].entries;

@@ -232,6 +232,41 @@ def rgbd_workload(n_frames: int = 200) -> Workload:
     return wl
 
 
+def multi_robot_pool(n_scans: int = 48) -> Workload:
+    """The scans of the multi-robot serving workload: n_scans scans in the room
+    scene (seed 1001) with point counts log-uniform in [20k, 262,144] (64
+    rings, elevation [-22.5, 22.5] deg), random poses (x, y in [-8, 8] m, z in
+    [-0.5, 2] m, roll and pitch within 3 deg, yaw uniform); resolution 0.05 m,
+    max range 50 m. Robot streams draw their scans from this pool."""
+    k = 5
+    scene = room_scene(1001)
+    rng = np.random.default_rng(1000 + k)
+    counts, poses = [], []
+    for _ in range(n_scans):
+        counts.append(int(math.exp(rng.uniform(math.log(20_000), math.log(262_144)))))
+        poses.append([rng.uniform(-8, 8), rng.uniform(-8, 8), rng.uniform(-0.5, 2.0),
+                      math.radians(rng.uniform(-3, 3)), math.radians(rng.uniform(-3, 3)),
+                      rng.uniform(-math.pi, math.pi)])
+    wl = Workload("multi_robot_room_0.05m", 0.05, 50.0, n_scans=n_scans)
+    wl.counts, wl.poses = np.array(counts, np.int64), np.array(poses, np.float64)
+
+    def make(i):
+        n = int(wl.counts[i])
+        pts = lidar_scan(scene, wl.poses[i], 64, (n + 63) // 64, n, -22.5, 22.5, 0.02, scan_seed(k, i))
+        return Scan(pts, wl.poses[i, :3].copy(), 50.0)
+
+    wl._make = make
+    return wl
+
+
+def multi_robot_bursts(n_maps: int = 16, repeats: int = 5):
+    """Request order of the serving test: map i receives pool scan 3i `repeats`
+    times (a robot standing still: voxels hit that often turn active at the
+    workload's probabilities), then pool scans 3i + 1 and 3i + 2, all as one
+    burst per map. Returns [(map, pool index)] in submission order."""
+    return [(i, k) for i in range(n_maps) for k in [3 * i] * repeats + [3 * i + 1, 3 * i + 2]]
+
+
 def raytrace_queries(n: int, seed: int = 1004 * 1_000_003 + 999_999, half=3.5, zlo=0.3, zhi=2.5,
                      length=5.0):
     """n query rays: origins uniform in the RGB-D room interior, unit
