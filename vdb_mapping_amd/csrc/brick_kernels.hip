@@ -400,11 +400,15 @@ __device__ __forceinline__ void walk_segment(const uint4 sr, const int4 r, const
     }
     cb |= lastb;
     if (++k >= m) break;  // m == 0: the endpoint of a zero-length ray
-    const int a = d.next_axis(tbl);
-    d.advance(a);
-    lx += a == 0 ? stx : 0;
-    ly += a == 1 ? sty : 0;
-    lz += a == 2 ? stz : 0;
+    // the step without per-lane branches; the exact-tie rule only when a lane of
+    // the wave meets a tie (one A/B run on the outdoor stream: walk 1.04 -> 1.01 ms
+    // per scan beside the group pass, 136.1 -> 139.2 M rays/s)
+    bool az, ay;
+    d.next_axis_wave(tbl, az, ay);
+    d.advance_sel(az, ay);
+    lx += (az || ay) ? 0 : stx;
+    ly += (ay && !az) ? sty : 0;
+    lz += az ? stz : 0;
   }
   if (cb) atomicOr(&bits[cw], (unsigned long long)cb);
   const bool ok1 = (unsigned)lx < (unsigned)kBrickDim && (unsigned)ly < (unsigned)kBrickDim &&

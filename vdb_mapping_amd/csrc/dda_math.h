@@ -145,6 +145,35 @@ struct IntDda {
         dzm < 0 || (dzm == 0 && (yx ? tie_le(tbl, n2, c2, n1, c1) : tie_le(tbl, n2, c2, n0, c0)));
     return zm ? 2 : (yx ? 1 : 0);
   }
+#if defined(__HIPCC__)
+  // next_axis for a wave walking many rays: without an exact tie (dyx == 0 or
+  // the deciding z difference == 0) the axis follows from the signs alone; the
+  // tie rule (which may read the crossing table) runs only when some lane of
+  // the wave meets a tie, for all lanes at once (it is exact for every lane),
+  // so the common step has no per-lane branches.
+  // The step is returned as two flags: z (axis 2) or else y (axis 1) or else x.
+  __device__ __forceinline__ void next_axis_wave(const double* __restrict__ tbl, bool& z, bool& y) const {
+    y = dyx < 0;
+    const int dzm = y ? dzy : dzx;
+    z = dzm < 0;
+    if (__any(dyx == 0 || dzm == 0)) {
+      const int a = next_axis(tbl);
+      z = a == 2;
+      y = a == 1;
+    }
+  }
+  // advance by the flags of next_axis_wave, with selects instead of branches.
+  __device__ __forceinline__ void advance_sel(bool z, bool y) {
+    const bool x = !z && !y;
+    y = y && !z;
+    c0 += x;
+    c1 += y;
+    c2 += z;
+    dyx += z ? 0 : (y ? 2 * (int)n0 : -2 * (int)n1);
+    dzx += y ? 0 : (z ? 2 * (int)n0 : -2 * (int)n2);
+    dzy += x ? 0 : (z ? 2 * (int)n1 : -2 * (int)n2);
+  }
+#endif
   VDB_HD void advance(int a) {
     if (a == 0) {
       ++c0;

@@ -836,7 +836,8 @@ __global__ __launch_bounds__(256) void k_apply_groups(GroupScans gs, const uint3
     for (int q = 0; q < kGroupWords; ++q) {  // step 2: every scan's masks of the word
 #pragma unroll
       for (int k = 0; k < kDeferMax; ++k) {
-        if (k < gs.nscan && w[q].slot[k] != kNone) {
+        // a wave holds four words of one group: the test is uniform (a scalar branch)
+        if (k < gs.nscan && __builtin_amdgcn_readfirstlane(w[q].slot[k]) != kNone) {
           const unsigned long long* m = gs.masks[k] + (size_t)w[q].slot[k] * 16;
           const uint64_t v = m[w[q].j], h = m[8 + w[q].j];
           w[q].any |= v;
@@ -867,7 +868,8 @@ __global__ __launch_bounds__(256) void k_apply_groups(GroupScans gs, const uint3
       uint32_t ab = (uint32_t)(aw[q] >> (4 * l)) & 15u, mine = 0;
 #pragma unroll
       for (int k = 0; k < kDeferMax; ++k) {
-        if (w[q].slot[k] == kNone) continue;  // the scan has no record of this leaf (same for the whole wave)
+        // the scan has no record of this leaf (same for the whole wave: a scalar branch)
+        if (__builtin_amdgcn_readfirstlane(w[q].slot[k]) == kNone) continue;
         const uint32_t vb = (w[q].vb >> (4 * k)) & 15u, hb = (w[q].hb >> (4 * k)) & 15u;
         mine |= vb;
         update_voxel(x[q].x, ab, 0, vb & 1, hb & 1, l_hit, l_miss, t_min, t_max, c_max, c_min);
