@@ -41,6 +41,8 @@ def parse():
     p.add_argument("--warmup", type=int, default=5)
     p.add_argument("--scans", type=int, default=100, help="distinct scans in the stream")
     p.add_argument("--cpu-scans", type=int, default=2, help="scans timed on the CPU oracle (0 = skip)")
+    p.add_argument("--host-scans", type=int, default=24,
+                   help="scans streamed from host memory for the PCIe-inclusive rate (0 = skip)")
     p.add_argument("--latency-scans", type=int, default=50,
                    help="scans inserted one at a time after the timed region for p50/p99 (0 = skip)")
     p.add_argument("--no-profiling", action="store_true", help="do not time kernels with HIP events")
@@ -52,6 +54,7 @@ def parse():
     p.add_argument("--batch", type=int, default=8, help="scans per batch (batch mode)")
     p.add_argument("--autotune", action="store_true", help="mixed mode: autotune first")
     p.add_argument("--seconds", type=float, default=10.0, help="serve mode: arrival window")
+    p.add_argument("--rate", type=float, default=10.0, help="serve mode: Poisson arrivals per stream (Hz)")
     p.add_argument("--frames", type=int, default=200, help="rgbd mode: depth frames")
     p.add_argument("--queries", type=int, default=1_000_000, help="rgbd mode: raytrace queries")
     return p.parse_args()
@@ -267,20 +270,13 @@ def mode_rgbd(a, V, W):
 
 
 def mode_serve(a, V, W):
-    """16 streams, Poisson arrivals (10 Hz each) of 20k-262k-point room scans, one map each."""
-    import math
-    import threading
+    """16 streams, Poisson arrivals (10 Hz each) of 20k-262k-point room scans, one map each.
+    Afterwards every map is rebuilt by inserting its requests one by one on a fresh map
+    (alone) and compared bit for bit with the served map (in the mix)."""
+    w = W.multi_robot_pool(48)
+    pool = [w.scan(i) for i in range(w.n_scans)]
     rng = np.random.default_rng(1005)
-    scene = W.room_scene(1001)
-    pool = []
-    for i in range(48):
-        n = int(math.exp(rng.uniform(math.log(20_000), math.log(262_144))))
-        pose = np.array([rng.uniform(-8, 8), rng.uniform(-8, 8), rng.uniform(-0.5, 2.0),
-                         math.radians(rng.uniform(-3, 3)), math.radians(rng.uniform(-3, 3)),
-                         rng.uniform(-math.pi, math.pi)])
-        pts = W.lidar_scan(scene, pose, 64, (n + 63) // 64, n, -22.5, 22.5, 0.02, W.scan_seed(5, i))
-        pool.append((pts, pose[:3].copy()))
-    streams, rate = 16, 10.0
+    streams, rate = 16, a.rate
     arrivals = []
     for sidx in range(streams):
         t = rng.exponential(1 / rate)
@@ -288,11 +284,13 @@ def mode_serve(a, V, W):
             arrivals.append((t, sidx, int(rng.integers(0, len(pool)))))
             t += rng.exponential(1 / rate)
     arrivals.sort()
-    cfg = V.make_config(max_range=50.0, **W.PROBS)
-    srv = V.Server(streams, 0.05, cfg, 50.0, hbm_budget_bytes=200 << 30, max_batch=32)
+    cfg = V.make_config(max_range=w.max_range, **W.PROBS)
+    srv = V.Server(streams, w.resolution, cfg, w.max_range, hbm_budget_bytes=200 << 30, max_batch=32)
+    order = [[] for _ in range(streams)]
     # warm every map once (table sizing), then start the clock
     for sidx in range(streams):
-        srv.submit(sidx, pool[sidx % len(pool)][0], pool[sidx % len(pool)][1])
+        srv.submit(sidx, pool[sidx].points, pool[sidx].origin)
+        order[sidx].append(sidx)
     srv.drain()
     n_warm = len(srv.records())
     t0 = time.perf_counter()
@@ -300,20 +298,37 @@ def mode_serve(a, V, W):
         delay = t0 + t - time.perf_counter()
         if delay > 0:
             time.sleep(delay)
-        srv.submit(sidx, pool[k][0], pool[k][1])
+        srv.submit(sidx, pool[k].points, pool[k].origin)
+        order[sidx].append(k)
     srv.drain()
     elapsed = time.perf_counter() - t0
     recs = srv.records()[n_warm:]
     lat = np.array([r["latency_ms"] for r in recs])
     rays = sum(r["points"] for r in recs)
+    # alone vs in the mix
+    same = True
+    for sidx in range(streams):
+        ref = V.GpuMap(w.resolution)
+        ref.set_config(cfg)
+        ref.add_input_source("s", w.max_range)
+        for k in order[sidx]:
+            ref.insert("s", pool[k].points, pool[k].origin, check=True)
+        a_o, a_v, a_m = srv.map(sidx).export_leaves()
+        b_o, b_v, b_m = ref.export_leaves()
+        same &= bool(np.array_equal(a_o, b_o) and np.array_equal(a_v.view(np.uint32), b_v.view(np.uint32))
+                     and np.array_equal(a_m, b_m))
+        ref.close()
     print(json.dumps({"metric": "multi-robot serving: p50/p99 scan latency", "value": round(float(np.percentile(lat, 99)), 3),
                       "unit": "ms (p99)", "higher_is_better": False,
                       "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
                       "max_ms": round(float(lat.max()), 3), "requests": len(recs), "streams": streams,
                       "offered_hz": rate * streams, "rays_per_s": round(rays / elapsed, 1),
                       "mean_batch": round(float(np.mean([r["batch_size"] for r in recs])), 3),
+                      "max_batch": int(max(r["batch_size"] for r in recs)),
                       "all_ok": all(r["status"] == 0 for r in recs), "device_gb": round(srv.device_bytes() / 1e9, 2),
-                      "config": {"workload": "multi_robot_16_streams_0.05m", "seconds": a.seconds}}), flush=True)
+                      "maps_equal_sequential": same,
+                      "config": {"workload": "multi_robot_16_streams_0.05m", "seconds": a.seconds,
+                                 "rate_hz_per_stream": rate}}), flush=True)
     srv.close()
 
 
@@ -402,6 +417,24 @@ def main():
         lat.append(time.perf_counter() - ts)
     lat_ms = np.array(lat) * 1e3
 
+    # PCIe-inclusive rate: the next scans of the stream handed over from host
+    # memory (pageable arrays, as insertPointCloud receives a host cloud), the
+    # host-to-device copy of every cloud inside the timed region
+    host = None
+    if a.host_scans > 0:
+        g.synchronize()
+        sh0 = g.stats()
+        first = a.warmup + a.steps + min(a.steps, a.latency_scans)
+        th = time.perf_counter()
+        for k in range(a.host_scans):
+            j = (first + k) % a.scans
+            g.insert("lidar", scans[j].points, scans[j].origin, check=True)
+        g.synchronize()
+        th = time.perf_counter() - th
+        hr = g.stats()["rays"] - sh0["rays"]
+        host = {"rays_per_s": round(hr / th, 1), "ms_per_scan": round(th / a.host_scans * 1e3, 4),
+                "scans": a.host_scans, "bytes_per_scan": int(scans[0].points.nbytes)}
+
     # Roofline of the dominant kernel, with its
     # algorithmic bytes: the voxel walk moves 12 B of point data per ray and
     # touches one 8-byte key per voxel visit; the log-odds apply reads a
@@ -468,6 +501,8 @@ def main():
                    "defer": defer},
         "voxel_updates_per_s": round(upd / elapsed, 1),
         "visits_per_s": round(visits / elapsed, 1),
+        "timed_scans": a.steps,
+        "host_input": host,  # PCIe-inclusive (host clouds); `value` is device-resident input
         "p50_scan_ms": round(float(np.percentile(lat_ms, 50)), 4) if len(lat_ms) else None,  # synchronous, one scan at a time
         "p99_scan_ms": round(float(np.percentile(lat_ms, 99)), 4) if len(lat_ms) else None,
         "kernel_ms_per_step": {k: round(d["ms_" + k] / max(a.steps, 1), 4)

@@ -121,6 +121,10 @@ int32_t vdbgpu_add_source(vdbgpu_map* map, const char* source_id, double max_ran
  * cast into a private scratch source. */
 int32_t vdbgpu_remove_source(vdbgpu_map* map, const char* source_id);
 
+/* The source lookup of addDataToAccumulate (VM:359-364): VDBGPU_OK if the
+ * source was added, VDBGPU_UNKNOWN_SOURCE if not. Has no other effect. */
+int32_t vdbgpu_has_source(vdbgpu_map* map, const char* source_id);
+
 /* accumulateUpdate VM:316-346 -> raycastPointCloud VM:466-539: cast every
  * point of the cloud from `origin` into the source's update set. `xyz` holds
  * n points of 3 float32 at `stride_bytes` (16 for pcl::PointXYZ); it is a host

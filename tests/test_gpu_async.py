@@ -90,3 +90,29 @@ def test_destroy_stops_the_threads():
         assert g.add_data_to_accumulate("s", pts, org) == V.OK
     time.sleep(0.05)
     g.close()  # joins the worker and integration threads
+
+
+def test_has_source_is_a_pure_lookup():
+    """The source check of addDataToAccumulate (VDBMapping.hpp:359-364) neither
+    applies waiting scans nor touches the update sets."""
+    g, o = pair(0.1, max_range=6.0)
+    assert g.has_source("test") and not g.has_source("nope")
+    (pts, org), = clouds(7, 1)
+    assert g.accumulate("test", pts, org) == V.OK
+    before = g.export_update("test")
+    assert g.has_source("test")
+    after = g.export_update("test")
+    assert all(np.array_equal(a, b) for a, b in zip(before, after))
+
+
+def test_stopped_engine_takes_no_more_clouds():
+    g, o = pair(0.1, max_range=6.0)
+    (pts, org), (pts2, org2) = clouds(8, 2)
+    assert g.async_start(0.05) == V.OK
+    assert g.add_data_to_accumulate("test", pts, org) == V.OK
+    assert g.async_flush() == V.OK
+    assert g.async_stop() == V.OK
+    assert g.add_data_to_accumulate("test", pts2, org2) == V.NOT_CONFIGURED
+    assert g.async_flush() == V.OK  # no engine: nothing to wait for
+    o.insert("test", pts, org)
+    same_map(g, o)

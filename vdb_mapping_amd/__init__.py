@@ -25,7 +25,7 @@ NOALLOC_MAX_QUERIES = 393216  # vdbgpu_raytrace_noalloc: queries per call its st
 # Every symbol include/vdbgpu.h declares.
 ABI_SYMBOLS = [
     "vdbgpu_version", "vdbgpu_status_string", "vdbgpu_device_count", "vdbgpu_create",
-    "vdbgpu_remove_source",
+    "vdbgpu_remove_source", "vdbgpu_has_source",
     "vdbgpu_destroy", "vdbgpu_set_config", "vdbgpu_logodds", "vdbgpu_add_source",
     "vdbgpu_accumulate", "vdbgpu_integrate", "vdbgpu_insert", "vdbgpu_insert_batch", "vdbgpu_reset",
     "vdbgpu_empty",
@@ -121,6 +121,8 @@ def lib():
         L.vdbgpu_add_source.argtypes = [P, ctypes.c_char_p, D, D]
         L.vdbgpu_remove_source.restype = I32
         L.vdbgpu_remove_source.argtypes = [P, ctypes.c_char_p]
+        L.vdbgpu_has_source.restype = I32
+        L.vdbgpu_has_source.argtypes = [P, ctypes.c_char_p]
         for f in (L.vdbgpu_accumulate, L.vdbgpu_insert):
             f.restype = I32
             f.argtypes = [P, ctypes.c_char_p, P, I64, I64, P, I32]
@@ -289,6 +291,9 @@ class GpuMap:
 
     def remove_input_source(self, source_id: str) -> int:
         return lib().vdbgpu_remove_source(self.h, source_id.encode())
+
+    def has_source(self, source_id: str) -> bool:
+        return lib().vdbgpu_has_source(self.h, source_id.encode()) == OK
 
     @staticmethod
     def _pts(points):

@@ -11,7 +11,9 @@
 // every accumulation_period. Scans accumulated within one period OR-merge
 // into the source's update set; replaced scans are dropped, as in the
 // reference. The per-scan exact path (vdbgpu_insert, vdbgpu_server_*) is
-// unaffected; the engine only calls the public ABI.
+// unaffected; the engine only calls the public ABI. Engines are held by
+// shared pointers: a call that found an engine keeps it alive while
+// vdbgpu_async_stop runs, and a stopped engine takes no more clouds.
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -57,7 +59,7 @@ struct Engine {
 };
 
 std::mutex g_mu;
-std::map<vdbgpu_map*, std::unique_ptr<Engine>> g_engines;
+std::map<vdbgpu_map*, std::shared_ptr<Engine>> g_engines;
 
 void note_error(Engine* e, int32_t rc) {
   if (rc != VDBGPU_OK && e->first_error == VDBGPU_OK) e->first_error = rc;
@@ -107,10 +109,10 @@ void integration_loop(Engine* e) {
   }
 }
 
-Engine* find_engine(vdbgpu_map* m) {
+std::shared_ptr<Engine> find_engine(vdbgpu_map* m) {
   std::lock_guard<std::mutex> lk(g_mu);
   auto it = g_engines.find(m);
-  return it == g_engines.end() ? nullptr : it->second.get();
+  return it == g_engines.end() ? nullptr : it->second;
 }
 
 }  // namespace
@@ -121,7 +123,7 @@ int32_t vdbgpu_async_start(vdbgpu_map* m, double accumulation_period_s) {
   if (!m || !(accumulation_period_s >= 0)) return VDBGPU_BAD_ARGUMENT;
   std::lock_guard<std::mutex> lk(g_mu);
   if (g_engines.count(m)) return VDBGPU_OK;
-  auto e = std::make_unique<Engine>();
+  auto e = std::make_shared<Engine>();
   e->map = m;
   e->period_ms = accumulation_period_s * 1000.0;
   Engine* ep = e.get();
@@ -134,12 +136,10 @@ int32_t vdbgpu_async_submit(vdbgpu_map* m, const char* source_id, const float* x
                             int64_t stride_bytes, const double origin[3], double max_rate) {
   if (!m || !source_id || n < 0 || (n && !xyz) || stride_bytes < 12 || (stride_bytes & 3) || !origin)
     return VDBGPU_BAD_ARGUMENT;
-  Engine* e = find_engine(m);
+  const std::shared_ptr<Engine> e = find_engine(m);
   if (!e) return VDBGPU_NOT_CONFIGURED;
   // addDataToAccumulate on an unknown source is rejected at once (VDBMapping.hpp:359-364)
-  const double probe[3] = {0, 0, 0};
-  if (vdbgpu_accumulate(m, source_id, nullptr, 0, 16, probe, 0) == VDBGPU_UNKNOWN_SOURCE)
-    return VDBGPU_UNKNOWN_SOURCE;
+  if (vdbgpu_has_source(m, source_id) == VDBGPU_UNKNOWN_SOURCE) return VDBGPU_UNKNOWN_SOURCE;
   Slot s;
   s.xyz.resize((size_t)n * 4);
   const char* src = reinterpret_cast<const char*>(xyz);
@@ -152,6 +152,7 @@ int32_t vdbgpu_async_submit(vdbgpu_map* m, const char* source_id, const float* x
   Worker* w;
   {
     std::lock_guard<std::mutex> lk(e->mu);
+    if (e->stop) return VDBGPU_NOT_CONFIGURED;  // stopped while this call ran
     auto it = e->workers.find(source_id);
     if (it == e->workers.end()) {
       auto nw = std::make_unique<Worker>();
@@ -159,7 +160,7 @@ int32_t vdbgpu_async_submit(vdbgpu_map* m, const char* source_id, const float* x
       nw->period_ms = max_rate > 0 ? 1000.0 / max_rate : 0.0;
       w = nw.get();
       e->workers[source_id] = std::move(nw);
-      w->th = std::thread(worker_loop, e, w);
+      w->th = std::thread(worker_loop, e.get(), w);
     } else {
       w = it->second.get();
     }
@@ -173,25 +174,27 @@ int32_t vdbgpu_async_submit(vdbgpu_map* m, const char* source_id, const float* x
 
 int32_t vdbgpu_async_flush(vdbgpu_map* m) {
   if (!m) return VDBGPU_BAD_ARGUMENT;
-  Engine* e = find_engine(m);
+  const std::shared_ptr<Engine> e = find_engine(m);
   if (!e) return VDBGPU_OK;
   {
     std::unique_lock<std::mutex> lk(e->mu);
     e->cv_idle.wait(lk, [&] {
+      if (e->stop) return true;  // stopped: its workers take nothing more
       for (auto& kv : e->workers)
         if (kv.second->slot.full || kv.second->busy) return false;
       return true;
     });
+    if (e->stop) return e->first_error;
   }
   const int32_t rc = vdbgpu_integrate(m);
   std::lock_guard<std::mutex> lk(e->mu);
-  note_error(e, rc);
+  note_error(e.get(), rc);
   return e->first_error;
 }
 
 int32_t vdbgpu_async_counters(vdbgpu_map* m, int64_t out[4]) {
   if (!m || !out) return VDBGPU_BAD_ARGUMENT;
-  Engine* e = find_engine(m);
+  const std::shared_ptr<Engine> e = find_engine(m);
   if (!e) {
     out[0] = out[1] = out[2] = out[3] = 0;
     return VDBGPU_OK;
@@ -206,7 +209,7 @@ int32_t vdbgpu_async_counters(vdbgpu_map* m, int64_t out[4]) {
 
 int32_t vdbgpu_async_stop(vdbgpu_map* m) {
   if (!m) return VDBGPU_BAD_ARGUMENT;
-  std::unique_ptr<Engine> e;
+  std::shared_ptr<Engine> e;
   {
     std::lock_guard<std::mutex> lk(g_mu);
     auto it = g_engines.find(m);
@@ -220,6 +223,7 @@ int32_t vdbgpu_async_stop(vdbgpu_map* m) {
   }
   for (auto& kv : e->workers) kv.second->cv.notify_all();
   e->cv_integ.notify_all();
+  e->cv_idle.notify_all();  // a flush waiting on the workers returns
   for (auto& kv : e->workers)
     if (kv.second->th.joinable()) kv.second->th.join();
   if (e->integ.joinable()) e->integ.join();

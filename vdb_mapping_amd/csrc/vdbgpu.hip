@@ -1891,6 +1891,12 @@ int32_t vdbgpu_remove_source(vdbgpu_map* m, const char* id) {
   return VDBGPU_OK;
 }
 
+int32_t vdbgpu_has_source(vdbgpu_map* m, const char* id) {
+  if (!m || !id) return VDBGPU_BAD_ARGUMENT;
+  Guard g(m, true);  // a lookup: waiting scans stay waiting
+  return m->sources.count(id) ? VDBGPU_OK : VDBGPU_UNKNOWN_SOURCE;
+}
+
 int32_t vdbgpu_accumulate(vdbgpu_map* m, const char* id, const float* xyz, int64_t n, int64_t stride,
                           const double origin[3], int32_t on_device) {
   if (!m || !id) return VDBGPU_BAD_ARGUMENT;
