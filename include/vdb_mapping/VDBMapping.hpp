@@ -547,8 +547,8 @@ public:
   /*!
    * \brief Builds the map from points (reference OccupancyVDBMapping.hpp:136-174):
    * every point's voxel occupied (max log-odds, active), optionally the
-   * inactive voxels of the active bounding box set to min log-odds. The final
-   * pruneGrid of the reference is not applied (uniform leaves stay leaves).
+   * inactive voxels of the active bounding box set to min log-odds, then
+   * pruneGrid: uniform leaves become 8^3 tiles (FloatGrid::forEachTile).
    */
   virtual void createMapFromPointCloud(const PointCloudT::Ptr& cloud,
                                        const bool set_background,
@@ -561,7 +561,7 @@ public:
                                      static_cast<int64_t>(sizeof(PointT)),
                                      set_background ? 1 : 0,
                                      clear_map ? 1 : 0));
-    m_vdb_grid->markStale();
+    m_vdb_grid->markRebuild();  // leaves may have become tiles
   }
 
   /*!
@@ -892,6 +892,13 @@ protected:
       std::memcpy(a.values, l.values, sizeof a.values);
       std::memset(a.bits, 0, sizeof a.bits);
       g.leaves.push_back(a);
+    });
+    grid.forEachTile([&](const HostTile& t) {
+      vdbio::Tile a;
+      std::memcpy(a.origin, t.origin, sizeof a.origin);
+      a.value  = t.value;
+      a.active = t.active;
+      g.tiles.push_back(a);
     });
     for (const char* k : {"bb_min", "bb_max"})
     {

@@ -7,6 +7,9 @@
 // read, pulls only the changed leaves (vdbgpu_export_dirty_leaves). An
 // accessor taken before an insert therefore sees the new values, as the
 // reference's shared grid pointer does (tests/mapping.cpp:13 -> :28-29).
+// The leaf-level tiles of createMapFromPointCloud's pruneGrid are kept beside
+// the leaves (value and state per 8^3 tile); a changed tile comes back as a
+// dirty leaf and leaves the tile list.
 //
 // BoolGrid is UpdateGridT: per leaf, visited (active) and hit (value true)
 // masks, the content of the reference's Tree4<bool,1,4,3> update grid.
@@ -51,6 +54,14 @@ struct HostLeaf
   uint64_t mask[8];
 };
 
+// An 8^3 tile of a pruned grid: one value and one state for all its voxels.
+struct HostTile
+{
+  int32_t origin[3];
+  float value;
+  bool active;
+};
+
 class FloatGrid
 {
 public:
@@ -72,12 +83,22 @@ public:
     }
     float getValue(const openvdb::Coord& c) const
     {
-      const HostLeaf* l = m_grid->leaf(c);
+      const HostTile* t = nullptr;
+      const HostLeaf* l = m_grid->leaf(c, &t);
+      if (t)
+      {
+        return t->value;
+      }
       return l ? l->values[host_leaf_offset(c.x(), c.y(), c.z())] : 0.0f;
     }
     bool isValueOn(const openvdb::Coord& c) const
     {
-      const HostLeaf* l = m_grid->leaf(c);
+      const HostTile* t = nullptr;
+      const HostLeaf* l = m_grid->leaf(c, &t);
+      if (t)
+      {
+        return t->active;
+      }
       if (!l)
       {
         return false;
@@ -98,7 +119,7 @@ public:
   bool empty() const
   {
     sync();
-    return m_leaves.empty();
+    return m_leaves.empty() && m_tiles.empty();
   }
 
   uint64_t leafCount() const
@@ -118,6 +139,10 @@ public:
         n += (openvdb::Index64)__builtin_popcountll(kv.second.mask[w]);
       }
     }
+    for (const auto& kv : m_tiles)
+    {
+      n += kv.second.active ? 512 : 0;  // a tile counts all its voxels
+    }
     return n;
   }
 
@@ -125,7 +150,25 @@ public:
   {
     std::lock_guard<std::mutex> lk(m_mu);
     m_leaves.clear();
-    m_stale = false;
+    m_tiles.clear();
+    m_stale   = false;
+    m_rebuild = false;
+  }
+
+  // f(const HostTile&) for every leaf-level tile (unordered).
+  template <typename F>
+  void forEachTile(F&& f) const
+  {
+    sync();
+    for (const auto& kv : m_tiles)
+    {
+      f(kv.second);
+    }
+  }
+  uint64_t tileCount() const
+  {
+    sync();
+    return m_tiles.size();
   }
 
   // f(const HostLeaf&) for every leaf (unordered).
@@ -156,6 +199,11 @@ public:
         leaf->setValueOnly(n, l.values[n]);
         leaf->setActiveState(n, (l.mask[n >> 6] >> (n & 63)) & 1);
       }
+    }
+    for (const auto& kv : m_tiles)
+    {
+      const HostTile& t = kv.second;
+      tree.addTile(1, openvdb::Coord(t.origin[0], t.origin[1], t.origin[2]), t.value, t.active);
     }
     return g;
   }
@@ -214,6 +262,13 @@ public:
     std::lock_guard<std::mutex> lk(m_mu);
     m_stale = true;
   }
+  // The next read pulls every leaf and tile (after a prune turned leaves into tiles).
+  void markRebuild()
+  {
+    std::lock_guard<std::mutex> lk(m_mu);
+    m_stale   = true;
+    m_rebuild = true;
+  }
   bool attached() const
   {
     std::lock_guard<std::mutex> lk(m_mu);
@@ -221,10 +276,16 @@ public:
   }
 
 private:
-  const HostLeaf* leaf(const openvdb::Coord& c) const
+  const HostLeaf* leaf(const openvdb::Coord& c, const HostTile** tile = nullptr) const
   {
     sync();
-    auto it = m_leaves.find(host_leaf_key(c.x(), c.y(), c.z()));
+    const uint64_t k = host_leaf_key(c.x(), c.y(), c.z());
+    if (tile)
+    {
+      auto t = m_tiles.find(k);
+      *tile  = t == m_tiles.end() ? nullptr : &t->second;
+    }
+    auto it = m_leaves.find(k);
     return it == m_leaves.end() ? nullptr : &it->second;
   }
 
@@ -236,29 +297,55 @@ private:
       return;
     }
     const int64_t cap = vdbgpu_leaf_count(m_map);
+    if (m_rebuild)
+    {
+      m_leaves.clear();
+      m_tiles.clear();
+      const int64_t nt = vdbgpu_export_tiles(m_map, nullptr, nullptr, nullptr, 0);
+      if (nt > 0)
+      {
+        std::vector<int32_t> o(3 * nt);
+        std::vector<float> v(nt);
+        std::vector<uint8_t> a(nt);
+        const int64_t got = vdbgpu_export_tiles(m_map, o.data(), v.data(), a.data(), nt);
+        for (int64_t i = 0; i < got; ++i)
+        {
+          HostTile& t = m_tiles[host_leaf_key(o[3 * i], o[3 * i + 1], o[3 * i + 2])];
+          std::memcpy(t.origin, &o[3 * i], sizeof t.origin);
+          t.value  = v[i];
+          t.active = a[i] != 0;
+        }
+      }
+    }
     if (cap > 0)
     {
       std::vector<int32_t> o(3 * cap);
       std::vector<float> v(512 * cap);
       std::vector<uint64_t> msk(8 * cap);
-      const int64_t n = vdbgpu_export_dirty_leaves(m_map, o.data(), v.data(), msk.data(), cap);
+      const int64_t n = m_rebuild ? vdbgpu_export_leaves(m_map, o.data(), v.data(), msk.data(), cap)
+                                  : vdbgpu_export_dirty_leaves(m_map, o.data(), v.data(), msk.data(), cap);
       for (int64_t i = 0; i < n; ++i)
       {
-        HostLeaf& l = m_leaves[host_leaf_key(o[3 * i], o[3 * i + 1], o[3 * i + 2])];
+        const uint64_t k = host_leaf_key(o[3 * i], o[3 * i + 1], o[3 * i + 2]);
+        HostLeaf& l      = m_leaves[k];
         std::memcpy(l.origin, &o[3 * i], sizeof l.origin);
         std::memcpy(l.values, &v[512 * i], sizeof l.values);
         std::memcpy(l.mask, &msk[8 * i], sizeof l.mask);
+        m_tiles.erase(k);  // a changed tile is a leaf again
       }
     }
-    m_stale = false;
+    m_stale   = false;
+    m_rebuild = false;
   }
 
   double m_voxel_size;
   std::map<std::string, openvdb::Vec3d> m_meta;
   vdbgpu_map* m_map = nullptr;
   mutable std::mutex m_mu;
-  mutable bool m_stale = false;
+  mutable bool m_stale   = false;
+  mutable bool m_rebuild = false;
   mutable std::unordered_map<uint64_t, HostLeaf> m_leaves;
+  mutable std::unordered_map<uint64_t, HostTile> m_tiles;
 };
 
 class BoolGrid

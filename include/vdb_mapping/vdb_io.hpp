@@ -10,7 +10,8 @@
 // children, 32^3 and 16^3 internal nodes with child and value masks and their
 // tile values, leaf value masks) and buffers (leaf values in tree order).
 // Writing uses active-mask compression (a level-set map takes no zlib), zlib
-// on request, never blosc. Reading accepts none / active-mask / zlib
+// on request, never blosc; the 8^3 tiles of a pruned float grid go into the
+// value tables of the 16^3 nodes. Reading accepts none / active-mask / zlib
 // compression, voxelises tiles that are active or hold a value other than the
 // background (a root-level tile of that kind is refused), and refuses blosc and
 // half-float grids.
@@ -66,6 +67,14 @@ struct Leaf
   uint64_t bits[8];
 };
 
+// One 8^3 tile of a pruned float grid (a 16^3 node table entry without a child).
+struct Tile
+{
+  int32_t origin[3];
+  float value;
+  bool active;
+};
+
 struct Grid
 {
   bool is_float   = true;
@@ -75,6 +84,7 @@ struct Grid
   float background  = 0.0f;
   std::map<std::string, std::array<double, 3>> vec3d_meta;
   std::vector<Leaf> leaves;
+  std::vector<Tile> tiles;  // written only for float grids; read back as leaves
 };
 
 namespace detail {
@@ -370,6 +380,7 @@ bool get_values(In& in, V* dst, uint32_t n, const uint64_t* active, uint32_t com
 struct Node16
 {
   std::map<uint32_t, size_t> leaves;  // slot -> leaf index
+  std::map<uint32_t, size_t> tiles;   // slot -> tile index
 };
 struct Node32
 {
@@ -457,6 +468,12 @@ inline std::string write_archive(const Grid& g, bool seekable, uint32_t comp = k
     const std::array<int32_t, 3> ro = {lo[0] & ~4095, lo[1] & ~4095, lo[2] & ~4095};
     roots[ro].kids[slot32(lo)].leaves[slot16(lo)] = i;
   }
+  for (size_t i = 0; g.is_float && i < g.tiles.size(); ++i)
+  {
+    const int32_t* to = g.tiles[i].origin;
+    const std::array<int32_t, 3> ro = {to[0] & ~4095, to[1] & ~4095, to[2] & ~4095};
+    roots[ro].kids[slot32(to)].tiles[slot16(to)] = i;
+  }
   o.put<int32_t>(1);  // buffer count
   if (g.is_float)
   {
@@ -488,9 +505,24 @@ inline std::string write_archive(const Grid& g, bool seekable, uint32_t comp = k
     {
       std::vector<uint64_t> c16(64, 0), v16(64, 0);
       for (const auto& l : k.second.leaves) c16[l.first >> 6] |= 1ull << (l.first & 63);
+      for (const auto& t : k.second.tiles)
+      {
+        if (g.tiles[t.second].active)
+        {
+          v16[t.first >> 6] |= 1ull << (t.first & 63);
+        }
+      }
       o.bytes(c16.data(), 512);
       o.bytes(v16.data(), 512);
-      if (g.is_float)
+      if (g.is_float && !k.second.tiles.empty())
+      {
+        // the table: zero at child slots, the tile values, background elsewhere
+        std::unique_ptr<float[]> vals(new float[4096]);
+        for (uint32_t i = 0; i < 4096; ++i) vals[i] = on(c16.data(), i) ? 0.0f : g.background;
+        for (const auto& t : k.second.tiles) vals[t.first] = g.tiles[t.second].value;
+        put_values<float>(o, vals.get(), 4096, v16.data(), c16.data(), g.background, comp);
+      }
+      else if (g.is_float)
       {
         put_internal_values<float>(o, 4096, c16, g.background, comp);
       }

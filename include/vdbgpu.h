@@ -395,13 +395,24 @@ int64_t vdbgpu_export_active_points(vdbgpu_map* map, float* xyz, int64_t cap);
 /* createMapFromPointCloud (OCC:136-174), the map half of loadMapFromPCD: with
  * clear_map the grid is emptied (update sets kept); every point's voxel
  * floor(p / resolution) becomes (max log-odds, active); with set_background
- * every inactive voxel of the active-voxel bounding box gets min log-odds.
- * The reference's final pruneGrid (uniform leaves collapsed into tiles) is
- * not applied: such leaves stay leaves with the same voxel values and states.
+ * every inactive voxel of the active-voxel bounding box gets min log-odds;
+ * then pruneGrid (OCC:173) at the leaf level: every leaf of 512 equal values,
+ * all active or all inactive, becomes a tile. Tiles are not leaves: leaf
+ * counts, leaf and section exports skip them, vdbgpu_export_tiles lists them,
+ * the raytrace topology skips inactive ones, saveMapToPCD's point list holds
+ * an active tile once (at its origin). Voxel reads and writes see the tile's
+ * value; the first write that changes a voxel of a tile makes it a leaf again.
  * A NaN or out-of-range point rejects the call before any change; a
  * background box of more than 2^27 leaves returns VDBGPU_UNSUPPORTED. */
 int32_t vdbgpu_create_from_points(vdbgpu_map* map, const float* xyz, int64_t n, int64_t stride_bytes,
                                   int32_t set_background, int32_t clear_map);
+/* The leaf-level tiles of the last createMapFromPointCloud that are still
+ * tiles, sorted by origin: origins (multiples of 8), tile values, active
+ * flags (1 = active). A null output returns the count, a short cap -count,
+ * an error -1. Replaces reading the tiles of the 16^3 internal nodes of the
+ * reference's pruned grid (tree::InternalNode tile table, OCC:173). */
+int64_t vdbgpu_export_tiles(vdbgpu_map* map, int32_t* origins, float* values, uint8_t* active,
+                            int64_t cap);
 
 /* createIndexBoundingBox (VM:811-874): tf is the row-major 4x4 map-to-reference
  * transform; out = {min x, y, z, max x, y, z}. */

@@ -98,6 +98,8 @@ def _bind(L):
     L.orc_export_active_points.restype = I64
     L.orc_export_active_points.argtypes = [P, P, I64]
     L.orc_create_from_points.argtypes = [P, P, I64, I64, I32, I32]
+    L.orc_export_tiles.restype = I64
+    L.orc_export_tiles.argtypes = [P, P, P, P, I64]
     L.orc_morphology.restype = I64
     L.orc_morphology.argtypes = [I32, I32, I64, P, P, P, P, I64]
     return L
@@ -256,6 +258,15 @@ class OracleMap:
         got = self.L.orc_export_leaves(self.h, _ptr(o), _ptr(v), _ptr(m), n)
         assert got == n
         return o, v, m
+
+    def export_tiles(self):
+        """Leaf-level tiles of pruneGrid: (origins (k, 3) int32, values (k,) float32, active (k,) bool)."""
+        n = self.L.orc_export_tiles(self.h, None, None, None, 0)
+        o = np.zeros((n, 3), np.int32)
+        v = np.zeros(n, np.float32)
+        a = np.zeros(n, np.uint8)
+        assert self.L.orc_export_tiles(self.h, _ptr(o), _ptr(v), _ptr(a), n) == n
+        return o, v, a.astype(bool)
 
     def export_update(self, source_id: str):
         n = self.L.orc_update_leaf_count(self.h, source_id.encode())

@@ -32,6 +32,13 @@
 //      EXPAND_TILES, no pruning): a voxel outside every leaf is inactive; a
 //      dilation step creates a leaf exactly where it activates a voxel in none;
 //      erosion keeps every leaf, emptied or not.
+//   U15 pruneGrid (tools::prune, tolerance 0) is restated at the leaf level: a
+//      leaf of 512 equal values, all on or all off, becomes a tile; coarser
+//      internal-node tiles are not formed (they change no leaf set and no
+//      raytrace run, only the saveMapToPCD list of a fully occupied 128^3 block).
+//      A tile becomes a leaf again at the first write that changes a voxel's
+//      value or state. cbeginValueOn yields an active tile once, at its origin
+//      (saveMapToPCD, applyMapSectionUpdateGrid), placed where its leaf would be.
 //
 // Compile with -ffp-contract=off (no FMA contraction): the reference is built
 // for baseline x86-64 (Release, no -march flags, VM CMakeLists.txt:8-11),
@@ -200,6 +207,11 @@ struct UpdateSet {
 struct Leaf {
   float v[512];
   uint64_t act[8];
+  // pruneGrid (OCC:173) turned this leaf into a tile of value tv and state ts:
+  // the tree holds no leaf here, but every voxel reads (tv, ts) [U15]
+  bool tile;
+  float tv;
+  bool ts;
 };
 
 struct Source {
@@ -230,6 +242,7 @@ struct Map {
   std::unique_ptr<Intersector> isect;
   UpdateSet art;             // m_artificial_area_grid (VM:132): visited = artificial voxel
   bool art_present = false;  // m_artificial_areas_present (VM:1529)
+  std::unordered_set<uint64_t> tiles;  // keys of leaves that stand for pruned tiles
 };
 
 // Keys of the 4096^3 (root child) and 128^3 internal nodes holding voxel (x,y,z).
@@ -262,6 +275,48 @@ Leaf& touch_leaf(Map& m, uint64_t k) {
   Leaf z;
   std::memset(&z, 0, sizeof z);
   return m.leaves.emplace(k, z).first->second;
+}
+
+// A voxel write into a tile replaces the tile by a leaf exactly when it changes
+// the voxel's value or state (OpenVDB's InternalNode set / modify calls compare
+// with the tile first); the leaf then holds the tile value everywhere else, as
+// here. So after each operation that writes voxels once, a tile whose voxels are
+// no longer all (tv, ts) has become a leaf [U15].
+void tile_check(Map& m) {
+  for (auto it = m.tiles.begin(); it != m.tiles.end();) {
+    Leaf& l = m.leaves.at(*it);
+    bool same = true;
+    const uint64_t word = l.ts ? ~0ull : 0ull;
+    for (int w = 0; w < 8 && same; ++w) same = l.act[w] == word;
+    for (int n = 0; n < 512 && same; ++n) same = l.v[n] == l.tv;
+    if (same) {
+      ++it;
+    } else {
+      l.tile = false;
+      it = m.tiles.erase(it);
+    }
+  }
+}
+
+// pruneGrid (OCC:173: tools::prune, tolerance 0) at the leaf level: every leaf
+// whose 512 values compare equal and whose voxels are all on or all off becomes
+// a tile of that value and state [U15].
+void prune_leaves(Map& m) {
+  for (auto& kv : m.leaves) {
+    Leaf& l = kv.second;
+    const bool on = l.act[0] == ~0ull, off = l.act[0] == 0;
+    bool uni = on || off;
+    for (int w = 1; w < 8 && uni; ++w) uni = l.act[w] == l.act[0];
+    for (int n = 1; n < 512 && uni; ++n) uni = l.v[n] == l.v[0];
+    l.tile = uni;
+    if (uni) {
+      l.tv = l.v[0];
+      l.ts = on;
+      m.tiles.insert(kv.first);
+    } else {
+      m.tiles.erase(kv.first);
+    }
+  }
 }
 
 // setActiveState(xyz, true) over every artificial voxel (VM:785-789): a
@@ -329,7 +384,9 @@ void apply_update(Map& m, UpdateSet& u) {
       }
     }
   }
+  tile_check(m);
   activate_artificial(m);
+  tile_check(m);
 }
 
 // updateVolumeRayIntersectors (VM:1436-1449) for the state that raytrace needs.
@@ -339,7 +396,9 @@ void build_intersector(Map& m) {
   for (auto& kv : m.leaves) {
     int32_t o[3];
     key_origin(kv.first, o);
-    is->leaves.insert(kv.first);
+    // an inactive tile is neither a leaf nor an active value for VolumeHDDA; an
+    // active tile is a hit at the leaf level, like a leaf [U15]
+    if (!kv.second.tile || kv.second.ts) is->leaves.insert(kv.first);
     is->n128.insert(key128(o[0], o[1], o[2]));
     bool active = false;
     for (int w = 0; w < 8; ++w) active |= kv.second.act[w] != 0;
@@ -709,6 +768,7 @@ int orc_insert(void* h, const char* id, const float* xyz, int64_t n, int64_t str
 void orc_reset(void* h) {
   Map& m = *(Map*)h;
   m.leaves.clear();
+  m.tiles.clear();
   m.n4096.clear();
   for (auto& kv : m.sources) kv.second.upd.clear();
 }
@@ -725,13 +785,18 @@ void orc_logodds(void* h, float out[6]) {
   out[5] = m.c_min;
 }
 
-int64_t orc_leaf_count(void* h) { return (int64_t)((Map*)h)->leaves.size(); }
+int64_t orc_leaf_count(void* h) {
+  const Map& m = *(Map*)h;
+  return (int64_t)(m.leaves.size() - m.tiles.size());
+}
 
+// Leaves only: tiles are not leaves of the tree.
 int64_t orc_export_leaves(void* h, int32_t* origins, float* values, uint64_t* masks,
                           int64_t cap) {
   Map& m = *(Map*)h;
   std::vector<uint64_t> keys, sorted;
-  for (auto& kv : m.leaves) keys.push_back(kv.first);
+  for (auto& kv : m.leaves)
+    if (!kv.second.tile) keys.push_back(kv.first);
   const int64_t n = export_sorted_keys(keys, sorted);
   if (n > cap) return -n;
   for (int64_t i = 0; i < n; ++i) {
@@ -739,6 +804,22 @@ int64_t orc_export_leaves(void* h, int32_t* origins, float* values, uint64_t* ma
     key_origin(sorted[i], origins + 3 * i);
     std::memcpy(values + 512 * i, l.v, sizeof l.v);
     std::memcpy(masks + 8 * i, l.act, sizeof l.act);
+  }
+  return n;
+}
+
+// The leaf-level tiles pruneGrid made, sorted by origin: origin, value, state.
+int64_t orc_export_tiles(void* h, int32_t* origins, float* values, uint8_t* active, int64_t cap) {
+  Map& m = *(Map*)h;
+  std::vector<uint64_t> keys(m.tiles.begin(), m.tiles.end()), sorted;
+  const int64_t n = export_sorted_keys(keys, sorted);
+  if (!origins && !values && !active) return n;
+  if (n > cap) return -n;
+  for (int64_t i = 0; i < n; ++i) {
+    const Leaf& l = m.leaves.at(sorted[i]);
+    key_origin(sorted[i], origins + 3 * i);
+    values[i] = l.tv;
+    active[i] = l.ts ? 1 : 0;
   }
   return n;
 }
@@ -902,6 +983,7 @@ int orc_set_points(void* h, const float* xyz, int64_t n, int64_t stride, int32_t
     else
       leaf.act[off >> 6] &= ~(1ull << (off & 63));
   }
+  tile_check(m);
   return ORC_OK;
 }
 
@@ -930,6 +1012,7 @@ void restore_integrity(Map& m) {
       }
     }
   }
+  tile_check(m);
   m.art.clear();
   m.art_present = false;
 }
@@ -1079,6 +1162,7 @@ int64_t orc_export_section(void* h, const int32_t bbox[6], int32_t full, int32_t
   std::vector<Leaf> outs;
   std::unordered_map<uint64_t, size_t> at;
   for (auto& kv : m.leaves) {
+    if (kv.second.tile) continue;  // cbeginLeaf (VM:937) does not visit tiles
     int32_t o[3];
     key_origin(kv.first, o);
     uint64_t in[8];
@@ -1190,6 +1274,7 @@ int orc_apply_section(void* h, int64_t n, const int32_t* origins, const float* v
       if (!s128.count(key128(x, y, z))) clobber(x, y, z);
     }
   }
+  tile_check(m);
   return ORC_OK;
 }
 
@@ -1211,6 +1296,16 @@ int orc_apply_section_update(void* h, const int32_t bbox[6], int64_t n, const in
     key_origin(kv.first, o);
     uint64_t in[8];
     inside_mask(o, bbox, in);
+    if (kv.second.tile) {
+      // cbeginValueOn visits an active tile once, at its origin: only that voxel
+      // is switched off, which turns the tile into a leaf [U15]
+      if (kv.second.ts && (in[0] & 1)) {
+        kv.second.act[0] &= ~1ull;
+        kv.second.tile = false;
+        m.tiles.erase(kv.first);
+      }
+      continue;
+    }
     for (int w = 0; w < 8; ++w) kv.second.act[w] &= ~in[w];
   }
   for (int64_t i = 0; i < n; ++i) {
@@ -1223,6 +1318,7 @@ int orc_apply_section_update(void* h, const int32_t bbox[6], int64_t n, const in
     Leaf& leaf = touch_leaf(m, leaf_key(o[0], o[1], o[2]));
     for (int w = 0; w < 8; ++w) leaf.act[w] |= a[w];
   }
+  tile_check(m);
   return ORC_OK;
 }
 
@@ -1274,7 +1370,10 @@ int64_t orc_export_active_points(void* h, float* xyz, int64_t cap) {
   int64_t n = 0;
   for (auto& kv : m.leaves) {
     order.emplace_back(tree_order(kv.first), kv.first);
-    for (int w = 0; w < 8; ++w) n += __builtin_popcountll(kv.second.act[w]);
+    if (kv.second.tile)
+      n += kv.second.ts ? 1 : 0;  // cbeginValueOn yields an active tile once [U15]
+    else
+      for (int w = 0; w < 8; ++w) n += __builtin_popcountll(kv.second.act[w]);
   }
   if (!xyz) return n;
   if (n > cap) return -n;
@@ -1284,7 +1383,7 @@ int64_t orc_export_active_points(void* h, float* xyz, int64_t cap) {
     const Leaf& l = m.leaves.at(ok.second);
     int32_t o[3];
     key_origin(ok.second, o);
-    for (int v = 0; v < 512; ++v) {
+    for (int v = 0; v < (l.tile ? 1 : 512); ++v) {
       if (!((l.act[v >> 6] >> (v & 63)) & 1)) continue;
       const int32_t c[3] = {o[0] + (v >> 6), o[1] + ((v >> 3) & 7), o[2] + (v & 7)};
       for (int a = 0; a < 3; ++a) {
@@ -1298,11 +1397,11 @@ int64_t orc_export_active_points(void* h, float* xyz, int64_t cap) {
   return n;
 }
 
-// createMapFromPointCloud (OCC:136-174) without its final pruneGrid: clear
-// the grid (not the update grids) if asked, setValueOn(floor(worldToIndex(p)),
-// max log-odds) per point [U1], then with set_background every voxel of the
-// active-voxel bounding box that is not active gets setValueOff(min log-odds)
-// (a missing leaf is created when that value differs from the background 0).
+// createMapFromPointCloud (OCC:136-174): clear the grid (not the update grids)
+// if asked, setValueOn(floor(worldToIndex(p)), max log-odds) per point [U1],
+// then with set_background every voxel of the active-voxel bounding box that is
+// not active gets setValueOff(min log-odds) (a missing leaf is created when that
+// value differs from the background 0), then pruneGrid at the leaf level [U15].
 // NaN or out-of-range points reject the call before any change.
 int orc_create_from_points(void* h, const float* xyz, int64_t n, int64_t stride, int32_t set_background,
                            int32_t clear_map) {
@@ -1318,6 +1417,7 @@ int orc_create_from_points(void* h, const float* xyz, int64_t n, int64_t stride,
   }
   if (clear_map) {
     m.leaves.clear();
+    m.tiles.clear();
     m.n4096.clear();
   }
   for (int64_t i = 0; i < n; ++i) {
@@ -1330,7 +1430,10 @@ int orc_create_from_points(void* h, const float* xyz, int64_t n, int64_t stride,
     leaf.v[off] = m.c_max;
     leaf.act[off >> 6] |= 1ull << (off & 63);
   }
-  if (!set_background) return ORC_OK;
+  if (!set_background) {
+    prune_leaves(m);
+    return ORC_OK;
+  }
   int32_t lo[3] = {INT32_MAX, INT32_MAX, INT32_MAX}, hi[3] = {INT32_MIN, INT32_MIN, INT32_MIN};
   for (auto& kv : m.leaves) {
     int32_t o[3];
@@ -1344,7 +1447,10 @@ int orc_create_from_points(void* h, const float* xyz, int64_t n, int64_t stride,
       }
     }
   }
-  if (lo[0] > hi[0]) return ORC_OK;  // no active voxel: an empty box
+  if (lo[0] > hi[0]) {  // no active voxel: an empty box
+    prune_leaves(m);
+    return ORC_OK;
+  }
   for (int32_t x = lo[0]; x <= hi[0]; ++x)
     for (int32_t y = lo[1]; y <= hi[1]; ++y)
       for (int32_t z = lo[2]; z <= hi[2]; ++z) {
@@ -1359,6 +1465,7 @@ int orc_create_from_points(void* h, const float* xyz, int64_t n, int64_t stride,
           touch_leaf(m, k).v[off] = m.c_min;
         }
       }
+  prune_leaves(m);
   return ORC_OK;
 }
 

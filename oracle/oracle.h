@@ -80,6 +80,10 @@ void orc_logodds(void* h, float out[6]);
 int64_t orc_leaf_count(void* h);
 int64_t orc_export_leaves(void* h, int32_t* origins, float* values, uint64_t* active_masks,
                           int64_t cap);
+/* The leaf-level tiles pruneGrid made (createMapFromPointCloud, OCC:173), sorted by
+ * origin: origin, value, state. Tiles are not leaves: orc_leaf_count and
+ * orc_export_leaves leave them out. */
+int64_t orc_export_tiles(void* h, int32_t* origins, float* values, uint8_t* active, int64_t cap);
 /* Pending update set of one source (VM:89 UpdateGridT): visited and hit masks per leaf. */
 int64_t orc_update_leaf_count(void* h, const char* source_id);
 int64_t orc_export_update(void* h, const char* source_id, int32_t* origins, uint64_t* visited,
@@ -131,7 +135,7 @@ void orc_index_bbox(double resolution, const double min_boundary[3],
 
 /* saveMapToPCD's point list (VM:218-256): active voxel centres in tree order, as float xyz. */
 int64_t orc_export_active_points(void* h, float* xyz, int64_t cap);
-/* createMapFromPointCloud (OCC:136-174) without the final pruneGrid. */
+/* createMapFromPointCloud (OCC:136-174), pruneGrid at the leaf level included. */
 int orc_create_from_points(void* h, const float* xyz, int64_t n, int64_t stride_bytes,
                            int32_t set_background, int32_t clear_map);
 

@@ -528,6 +528,79 @@ static void vdb_persistence_test()
   rmdir(dir);
 }
 
+static std::string find_file(const char* dir, const char* suffix)
+{
+  std::string file;
+  if (DIR* d = opendir(dir))
+  {
+    while (dirent* e = readdir(d))
+    {
+      const std::string name = e->d_name;
+      if (name.find(suffix) != std::string::npos) file = std::string(dir) + "/" + name;
+    }
+    closedir(d);
+  }
+  return file;
+}
+
+static void pruned_map_test()
+{
+  // createMapFromPointCloud ends with pruneGrid (reference
+  // OccupancyVDBMapping.hpp:173): a leaf full of points becomes an active
+  // tile, a leaf of the background fill an inactive tile. The live grid reads
+  // them as tiles; a scan that changes a voxel of a tile turns it into a leaf;
+  // saveMap writes the tiles, loadMap reads them back as leaves.
+  char dir_tmpl[] = "/tmp/vdbgpu_prune_XXXXXX";
+  const char* dir = mkdtemp(dir_tmpl);
+  CHECK(dir != nullptr);
+  if (!dir) return;
+  OccupancyVDBMapping a(0.1), b(0.1);
+  Config conf             = make_config(10);
+  conf.map_directory_path = std::string(dir) + "/";
+  a.setConfig(conf);
+  b.setConfig(conf);
+  a.addInputSource("test", conf.max_range, 0);
+  OccupancyVDBMapping::PointCloudT::Ptr cloud(new OccupancyVDBMapping::PointCloudT);
+  for (int x = 0; x < 8; ++x)
+    for (int y = 0; y < 8; ++y)
+      for (int z = 0; z < 8; ++z) cloud->points.emplace_back(0.1f * x + 0.05f, 0.1f * y + 0.05f, 0.1f * z + 0.05f);
+  cloud->points.emplace_back(2.35f, 0.05f, 0.05f);
+  a.createMapFromPointCloud(cloud, true, true);
+  auto g            = a.getGrid();
+  const float c_max = logodds(0.99), c_min = logodds(0.01);
+  CHECK(g->leafCount() == 1 && g->tileCount() == 2);
+  CHECK(g->activeVoxelCount() == 513);
+  auto acc = g->getAccessor();
+  CHECK(acc.isValueOn(openvdb::Coord(3, 4, 5)) && acc.getValue(openvdb::Coord(3, 4, 5)) == c_max);
+  CHECK(!acc.isValueOn(openvdb::Coord(12, 1, 1)) && acc.getValue(openvdb::Coord(12, 1, 1)) == c_min);
+  CHECK(acc.isValueOn(openvdb::Coord(23, 0, 0)));
+  // a hit at voxel (14, 3, 3), misses at 12 and 13 (clamped: no change)
+  OccupancyVDBMapping::PointCloudT::Ptr one(new OccupancyVDBMapping::PointCloudT);
+  one->points.emplace_back(1.425f, 0.325f, 0.325f);
+  a.insertPointCloud(one, Eigen::Matrix<double, 3, 1>(1.225, 0.325, 0.325), "test");
+  CHECK(g->leafCount() == 2 && g->tileCount() == 1);
+  CHECK(acc.getValue(openvdb::Coord(14, 3, 3)) > c_min && acc.getValue(openvdb::Coord(13, 3, 3)) == c_min);
+  CHECK(acc.getValue(openvdb::Coord(3, 4, 5)) == c_max);
+  CHECK(a.saveMap());
+  const std::string file = find_file(dir, "_map.vdb");
+  CHECK(!file.empty() && b.loadMap(file));
+  auto bacc  = b.getGrid()->getAccessor();
+  int same   = 0;
+  int total  = 0;
+  for (int x = -1; x <= 24; ++x)
+    for (int y = -1; y <= 8; ++y)
+      for (int z = -1; z <= 8; ++z)
+      {
+        const openvdb::Coord q(x, y, z);
+        ++total;
+        same += bacc.getValue(q) == acc.getValue(q) && bacc.isValueOn(q) == acc.isValueOn(q);
+      }
+  CHECK(same == total);
+  CHECK(b.getGrid()->leafCount() == 3 && b.getGrid()->tileCount() == 0);
+  std::remove(file.c_str());
+  rmdir(dir);
+}
+
 int main()
 {
   struct T
@@ -549,6 +622,7 @@ int main()
     {"ArtificialAreas", artificial_areas_test},
     {"PcdPersistence", pcd_persistence_test},
     {"VdbPersistence", vdb_persistence_test},
+    {"PrunedMap", pruned_map_test},
   };
   for (const T& t : tests)
   {

@@ -2,7 +2,10 @@
 // grids written as io::File and io::Stream archives, with active-mask, zlib
 // and no compression, read back identical; every inactive-value encoding of a
 // leaf (background only, minus background, one or two other values, background
-// and minus background, more than two) occurs. No GPU. Prints one line per case and exits non-zero on failure.
+// and minus background, more than two) occurs; the 8^3 tiles of a pruned float
+// grid, written into the 16^3 node tables, read back as leaves of the tile
+// value and state (a background-valued inactive tile as nothing). No GPU.
+// Prints one line per case and exits non-zero on failure.
 #include <cstdio>
 #include <cstring>
 #include <random>
@@ -98,6 +101,39 @@ int main()
     std::printf("[%s] %s grid: truncated / foreign input refused\n", refused ? "  OK  " : "FAILED",
                 is_float ? "float" : "bool");
     g_fail += refused ? 0 : 1;
+  }
+  {
+    // tiles beside leaves: in a node of their own, next to leaves, active and inactive
+    Grid g;
+    g.grid_class = "level set";
+    g.voxel_size = 0.1;
+    g.background = 0.0f;
+    g.leaves.push_back(make_leaf(rng, 0, 0, 0, 2, true, 0.0f));
+    g.leaves.push_back(make_leaf(rng, 4096, -128, 8, 6, true, 0.0f));
+    const Tile tiles[] = {{{8, 0, 0}, 4.59512f, true}, {{16, 0, 0}, -4.59512f, false}, {{24, 8, 0}, 0.0f, false},
+                          {{-136, 64, 120}, 1.0f, true}, {{4096, -128, 0}, 0.0f, true}};
+    Grid want = g;
+    for (const Tile& t : tiles)
+    {
+      g.tiles.push_back(t);
+      if (!t.active && t.value == g.background) continue;
+      Leaf l;
+      std::memset(&l, 0, sizeof l);
+      std::memcpy(l.origin, t.origin, 12);
+      for (int w = 0; w < 8; ++w) l.active[w] = t.active ? ~0ull : 0ull;
+      for (int v = 0; v < 512; ++v) l.values[v] = t.value;
+      want.leaves.push_back(l);
+    }
+    for (uint32_t comp : {kCompressMask, kCompressZip | kCompressMask})
+    {
+      const std::string bytes = write_archive(g, true, comp);
+      std::vector<Grid> back;
+      std::string err;
+      const bool ok = read_archive(bytes, back, err) && back.size() == 1 && same(want, back[0]);
+      std::printf("[%s] float grid with 8^3 tiles, compression %u: %zu bytes %s\n", ok ? "  OK  " : "FAILED", comp,
+                  bytes.size(), err.c_str());
+      g_fail += ok ? 0 : 1;
+    }
   }
   std::printf("%d failure(s)\n", g_fail);
   return g_fail ? 1 : 0;

@@ -36,7 +36,8 @@ def mk(res=0.1, R=5.0, **probs):
 
 
 def as_dict(m):
-    """Voxel dict {(x,y,z): (value bits, active)} of every voxel of every leaf."""
+    """Voxel dict {(x,y,z): (value bits, active)} of every voxel of every leaf
+    and of every leaf-level tile (createMapFromPointCloud prunes uniform leaves)."""
     o, v, a = m.export_leaves()
     out = {}
     for k in range(o.shape[0]):
@@ -44,6 +45,11 @@ def as_dict(m):
         for n in range(512):
             c = (int(o[k, 0]) + (n >> 6), int(o[k, 1]) + ((n >> 3) & 7), int(o[k, 2]) + (n & 7))
             out[c] = (int(bits[n]), bool((int(a[k, n >> 6]) >> (n & 63)) & 1))
+    to, tv, ta = m.export_tiles()
+    for k in range(to.shape[0]):
+        st = (f32bits(tv[k]), bool(ta[k]))
+        for n in range(512):
+            out[(int(to[k, 0]) + (n >> 6), int(to[k, 1]) + ((n >> 3) & 7), int(to[k, 2]) + (n & 7))] = st
     return out
 
 

@@ -3,8 +3,9 @@
 VDBMapping.hpp:193-211, :263-284, and of gridToByteArray / byteArrayToGrid,
 :1310-1338) on the host: float and bool grids written as file and stream
 archives with active-mask, zlib and no compression read back identical, every
-inactive-value encoding of a leaf included; truncated and foreign input is
-refused. Compatibility with OpenVDB itself is not checked here (no OpenVDB)."""
+inactive-value encoding of a leaf included; the 8^3 tiles of a pruned map
+(createMapFromPointCloud's pruneGrid, OccupancyVDBMapping.hpp:173) read back as
+leaves of their value and state; truncated and foreign input is refused. Compatibility with OpenVDB itself is not checked here (no OpenVDB)."""
 import os
 import subprocess
 
@@ -22,4 +23,4 @@ def test_archive_round_trips():
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "0 failure(s)" in r.stdout
-    assert r.stdout.count("[  OK  ]") == 18
+    assert r.stdout.count("[  OK  ]") == 20

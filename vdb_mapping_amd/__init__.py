@@ -40,7 +40,7 @@ ABI_SYMBOLS = [
     "vdbgpu_server_destroy", "vdbgpu_set_points", "vdbgpu_add_artificial_areas",
     "vdbgpu_restore_map_integrity", "vdbgpu_export_artificial", "vdbgpu_add_artificial_walls", "vdbgpu_export_section",
     "vdbgpu_apply_section", "vdbgpu_apply_section_update", "vdbgpu_index_bbox",
-    "vdbgpu_export_active_points", "vdbgpu_create_from_points", "vdbgpu_morphology",
+    "vdbgpu_export_active_points", "vdbgpu_create_from_points", "vdbgpu_export_tiles", "vdbgpu_morphology",
 ]
 
 
@@ -202,6 +202,8 @@ def lib():
         L.vdbgpu_export_active_points.argtypes = [P, P, I64]
         L.vdbgpu_create_from_points.restype = I32
         L.vdbgpu_create_from_points.argtypes = [P, P, I64, I64, I32, I32]
+        L.vdbgpu_export_tiles.restype = I64
+        L.vdbgpu_export_tiles.argtypes = [P, P, P, P, I64]
         L.vdbgpu_morphology.restype = I64
         L.vdbgpu_morphology.argtypes = [P, I32, I32, I64, P, P, P, P, I64]
         L.vdbgpu_index_bbox.restype = I32
@@ -382,6 +384,20 @@ class GpuMap:
         if got != n:
             raise VdbGpuError(f"export_leaves returned {got}, expected {n}: {self.last_error()}")
         return o, v, m
+
+    def export_tiles(self):
+        """Leaf-level tiles of the last create_from_points (pruneGrid), sorted by
+        origin: origins (n,3) i32, values (n,) f32, active (n,) bool."""
+        n = lib().vdbgpu_export_tiles(self.h, None, None, None, 0)
+        if n < 0:
+            raise VdbGpuError(f"export_tiles failed: {self.last_error()}")
+        o = np.zeros((n, 3), np.int32)
+        v = np.zeros(n, np.float32)
+        a = np.zeros(n, np.uint8)
+        got = lib().vdbgpu_export_tiles(self.h, _ptr(o), _ptr(v), _ptr(a), n)
+        if got != n:
+            raise VdbGpuError(f"export_tiles returned {got}, expected {n}: {self.last_error()}")
+        return o, v, a.astype(bool)
 
     def export_dirty_leaves(self):
         cap = max(self.leaf_count(), 1)

@@ -16,7 +16,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 
 HIP_SOURCES = ["integrate_kernels.hip", "brick_kernels.hip", "batch_kernels.hip", "raytrace_kernels.hip",
-               "edit_kernels.hip", "morph_kernels.hip", "vdbgpu.hip", "server.hip", "ingest.hip"]
+               "edit_kernels.hip", "morph_kernels.hip", "tile_kernels.hip", "vdbgpu.hip", "server.hip", "ingest.hip"]
 # -ffp-contract=off: no FMA contraction, every double op rounds on its own (bit-exact DDA).
 # -packed-fp32-ops off: no v_pk_*_f32 generation.
 HIPCC_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-ffp-contract=off", "-fPIC",

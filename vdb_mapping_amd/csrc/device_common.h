@@ -130,6 +130,23 @@ struct MapView {
   uint32_t lcap;  // leaf pool capacity
 };
 
+// Leaf-level tiles of pruneGrid (createMapFromPointCloud): leaf i of the pool
+// stands for a tile of value val[i] when bit i of `bits` is set, active when
+// bit i of `on` is set. bits == nullptr: the map holds no tile.
+struct TileView {
+  uint32_t* bits;
+  uint32_t* on;
+  float* val;
+  uint32_t cap;  // leaves the arrays cover; a later leaf is never a tile
+};
+
+__host__ __device__ inline bool tile_at(const TileView& t, uint32_t i) {
+  return t.bits && i < t.cap && ((t.bits[i >> 5] >> (i & 31)) & 1u);
+}
+__host__ __device__ inline bool tile_active(const TileView& t, uint32_t i) {
+  return (t.on[i >> 5] >> (i & 31)) & 1u;
+}
+
 // Topology snapshot used by raytrace: hash sets of leaf, 128^3-node and
 // 4096^3-node keys, plus the padded bbox of leaves with an active voxel.
 struct RtView {

@@ -190,13 +190,14 @@ __device__ __forceinline__ uint64_t box_word(const int o[3], const int* b, int j
 // pool index is appended to `out` (order arbitrary; the host sorts).
 __global__ __launch_bounds__(256) void k_section_pick(MapView mp, uint32_t leaf_count, const int* bbox,
                                                       int full, uint32_t* __restrict__ out,
-                                                      unsigned int* counter) {
+                                                      unsigned int* counter, TileView tv) {
   const uint32_t waves = gridDim.x * (blockDim.x >> 6);
   const int lane = lane_id();
   __shared__ int sb[6];
   if (threadIdx.x < 6) sb[threadIdx.x] = bbox[threadIdx.x];
   __syncthreads();
   for (uint32_t L = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); L < leaf_count; L += waves) {
+    if (tile_at(tv, L)) continue;  // the leaf iteration of getMapSection (VM:937) skips tiles
     int o[3];
     key_origin(mp.leaf_keys[L], o);
     if (o[0] + 7 < sb[0] || o[0] > sb[3] || o[1] + 7 < sb[1] || o[1] > sb[4] || o[2] + 7 < sb[2] ||
@@ -397,7 +398,8 @@ __global__ __launch_bounds__(256) void k_section_tiles(const unsigned long long*
 
 // applyMapSectionUpdateGrid, first loop: every active map voxel inside the box
 // set inactive. One lane per (leaf, word).
-__global__ __launch_bounds__(256) void k_section_clear(MapView mp, uint32_t leaf_count, const int* bbox) {
+__global__ __launch_bounds__(256) void k_section_clear(MapView mp, uint32_t leaf_count, const int* bbox,
+                                                       TileView tv) {
   const uint64_t G = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < (uint64_t)leaf_count * 8; t += G) {
     const uint32_t L = (uint32_t)(t >> 3);
@@ -405,6 +407,16 @@ __global__ __launch_bounds__(256) void k_section_clear(MapView mp, uint32_t leaf
     int o[3];
     key_origin(mp.leaf_keys[L], o);
     const uint64_t in = box_word(o, bbox, j);
+    if (tile_at(tv, L)) {
+      // cbeginValueOn (VM:1072) yields an active tile once, at its origin: only that
+      // voxel is switched off, which turns the tile into a leaf
+      if (j == 0 && (in & 1u) && tile_active(tv, L)) {
+        mp.active[(size_t)L * 8] &= ~1ull;
+        mp.dirty[L] = 1;
+        atomicAnd(&tv.bits[L >> 5], ~(1u << (L & 31)));
+      }
+      continue;
+    }
     if (!in) continue;
     unsigned long long* pa = mp.active + (size_t)L * 8 + j;
     const uint64_t aw = *pa;
@@ -581,10 +593,10 @@ hipError_t launch_set_voxels(hipStream_t s, UpdView u, const uint32_t* list, con
 }
 
 hipError_t launch_section_pick(hipStream_t s, MapView mp, uint32_t leaf_count, const int* bbox, bool full,
-                               uint32_t* out, unsigned int* counter) {
+                               uint32_t* out, unsigned int* counter, TileView tv) {
   const int grid = (int)std::min<uint32_t>(8192, (leaf_count + 3) / 4);
   hipLaunchKernelGGL(k_section_pick, dim3(grid), dim3(256), 0, s, mp, leaf_count, bbox, full ? 1 : 0, out,
-                     counter);
+                     counter, tv);
   return hipGetLastError();
 }
 
@@ -623,10 +635,10 @@ hipError_t launch_section_tiles(hipStream_t s, const unsigned long long* n128, u
   return hipGetLastError();
 }
 
-hipError_t launch_section_clear(hipStream_t s, MapView mp, uint32_t leaf_count, const int* bbox) {
+hipError_t launch_section_clear(hipStream_t s, MapView mp, uint32_t leaf_count, const int* bbox, TileView tv) {
   if (leaf_count == 0) return hipSuccess;
   const int grid = (int)std::min<uint64_t>(8192, ((uint64_t)leaf_count * 8 + 255) / 256);
-  hipLaunchKernelGGL(k_section_clear, dim3(grid), dim3(256), 0, s, mp, leaf_count, bbox);
+  hipLaunchKernelGGL(k_section_clear, dim3(grid), dim3(256), 0, s, mp, leaf_count, bbox, tv);
   return hipGetLastError();
 }
 

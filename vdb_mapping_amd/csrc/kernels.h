@@ -104,7 +104,7 @@ hipError_t launch_split_emit(hipStream_t s, int grid, const uint32_t* split_slot
                              unsigned long long* shared_bits, BrickView bv, UpdView rec,
                              unsigned long long* leaf_mask, uint32_t rec_cap, Status* st);
 
-hipError_t launch_snapshot(hipStream_t s, MapView mp, uint32_t leaf_count, RtView rt, int* bbox);
+hipError_t launch_snapshot(hipStream_t s, MapView mp, uint32_t leaf_count, RtView rt, int* bbox, TileView tv);
 hipError_t launch_copy_set(hipStream_t s, const unsigned long long* src, uint32_t src_cap,
                            unsigned long long* dst, uint32_t dst_mask);
 hipError_t launch_raytrace(hipStream_t s, int grid, RtView rt, MapView mp, double inv, double res,
@@ -119,7 +119,7 @@ hipError_t launch_mark_walls(hipStream_t s, const double* verts, const int4* wal
 hipError_t launch_set_voxels(hipStream_t s, UpdView u, const uint32_t* list, const uint32_t* eidx,
                              uint32_t count, MapView mp, int op, float val, float t_max, bool clear);
 hipError_t launch_section_pick(hipStream_t s, MapView mp, uint32_t leaf_count, const int* bbox, bool full,
-                               uint32_t* out, unsigned int* counter);
+                               uint32_t* out, unsigned int* counter, TileView tv);
 hipError_t launch_section_gather(hipStream_t s, MapView mp, const uint32_t* list, uint32_t count,
                                  const int* bbox, bool full, unsigned long long* keys_out, float* vals_out,
                                  unsigned long long* act_out);
@@ -131,7 +131,13 @@ hipError_t launch_section_tiles(hipStream_t s, const unsigned long long* n128, u
                                 const unsigned long long* n4096, uint32_t c4096,
                                 const unsigned long long* sleaf, uint32_t sleaf_mask,
                                 const unsigned long long* s128, uint32_t s128_mask, MapView mp);
-hipError_t launch_section_clear(hipStream_t s, MapView mp, uint32_t leaf_count, const int* bbox);
+hipError_t launch_section_clear(hipStream_t s, MapView mp, uint32_t leaf_count, const int* bbox, TileView tv);
+// pruneGrid at the leaf level over leaves [0, leaf_count): tiles set in tv (bits and on
+// zeroed before), their indices appended to list at *counter.
+hipError_t launch_prune(hipStream_t s, MapView mp, uint32_t leaf_count, TileView tv, uint32_t* list,
+                        unsigned int* counter);
+// Tiles of `list` whose voxels are no longer all (val, state) become leaves again.
+hipError_t launch_tile_check(hipStream_t s, MapView mp, const uint32_t* list, uint32_t n, TileView tv);
 hipError_t launch_section_prep(hipStream_t s, const int* origins, const float* vals, const unsigned long long* act,
                                uint32_t n, unsigned long long* keys, uint8_t* need, uint16_t* first,
                                unsigned long long* sleaf, uint32_t sleaf_mask, unsigned long long* s128,

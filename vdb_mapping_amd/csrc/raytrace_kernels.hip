@@ -24,7 +24,7 @@ __device__ __forceinline__ int rt_lane() { return (int)__lane_id(); }
 // the leaf-granular bounding box of active voxels (evalActiveBoundingBox with
 // visitVoxels = false). One block-level min/max reduction per workgroup.
 __global__ __launch_bounds__(256) void k_snapshot(MapView mp, uint32_t leaf_count,
-                                                  RtView rt, int* __restrict__ bbox) {
+                                                  RtView rt, int* __restrict__ bbox, TileView tv) {
   __shared__ int smin[3][4], smax[3][4];
   int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
   const uint32_t G = gridDim.x * blockDim.x;
@@ -32,7 +32,9 @@ __global__ __launch_bounds__(256) void k_snapshot(MapView mp, uint32_t leaf_coun
     const uint64_t key = mp.leaf_keys[i];
     int o[3];
     key_origin(key, o);
-    set_insert(rt.leaf, rt.leaf_mask, key);
+    // an inactive tile is neither a leaf nor an active value for the HDDA; an active
+    // tile is a hit at the leaf level, as a leaf is
+    if (!tile_at(tv, i) || tile_active(tv, i)) set_insert(rt.leaf, rt.leaf_mask, key);
     set_insert(rt.n128, rt.n128_mask, node_key(o[0], o[1], o[2], 7));
     set_insert(rt.n4096, rt.n4096_mask, node_key(o[0], o[1], o[2], 12));
     uint64_t any = 0;
@@ -216,9 +218,9 @@ __global__ __launch_bounds__(256) void k_raytrace(RtView rt, MapView mp, double 
   }
 }
 
-hipError_t launch_snapshot(hipStream_t s, MapView mp, uint32_t leaf_count, RtView rt, int* bbox) {
+hipError_t launch_snapshot(hipStream_t s, MapView mp, uint32_t leaf_count, RtView rt, int* bbox, TileView tv) {
   const int grid = (int)std::min<uint32_t>(4096, (leaf_count + 255) / 256 + 1);
-  hipLaunchKernelGGL(k_snapshot, dim3(grid), dim3(256), 0, s, mp, leaf_count, rt, bbox);
+  hipLaunchKernelGGL(k_snapshot, dim3(grid), dim3(256), 0, s, mp, leaf_count, rt, bbox, tv);
   return hipGetLastError();
 }
 

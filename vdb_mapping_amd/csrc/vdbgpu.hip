@@ -108,6 +108,14 @@ struct vdbgpu_map {
   std::deque<std::pair<uint64_t, uint64_t>> ledger;
   uint64_t ledger_seq = 0;
   bool touched = false;        // any voxel modify since creation/reset
+  // leaf-level tiles of the last pruneGrid (tile_kernels.hip): tile bit, state
+  // bit and value per pool leaf below tcap, and the list of the leaves pruned
+  uint32_t* tbits = nullptr;
+  uint32_t* ton = nullptr;
+  float* tval = nullptr;
+  uint32_t* tlist = nullptr;
+  uint32_t tcap = 0;    // leaves covered, a multiple of 32 (0: no tile arrays)
+  uint32_t ntiles = 0;  // entries of tlist; 0 when the map holds no tile
   bool snap_dirty = true;      // the map changed since the last topology snapshot
   unsigned long long* noop4096 = nullptr;  // root children of updates that created no leaf
   static constexpr uint32_t kNoopCap = 4096;
@@ -394,6 +402,88 @@ UpdView upd_view(const Source& s) { return UpdView{s.keys, s.masks, s.cap - 1}; 
 
 MapView map_view(vdbgpu_map* m) {
   return MapView{m->mkeys, m->midx, m->mcap - 1, m->values, m->active, m->leaf_keys, m->dirty, m->lcap};
+}
+
+TileView tile_view(const vdbgpu_map* m) {
+  return TileView{m->ntiles ? m->tbits : nullptr, m->ton, m->tval, m->tcap};
+}
+
+void free_tiles(vdbgpu_map* m) {
+  dfree(m, &m->tbits, m->tcap / 32);
+  dfree(m, &m->ton, m->tcap / 32);
+  dfree(m, &m->tval, m->tcap);
+  dfree(m, &m->tlist, m->tcap);
+  m->tcap = 0;
+  m->ntiles = 0;
+}
+
+// After an operation that wrote voxels: a tile whose voxels are no longer all
+// (value, state) of the tile is a leaf again (pruneGrid tiles, OCC:173).
+int check_tiles(vdbgpu_map* m) {
+  if (!m->ntiles) return VDBGPU_OK;
+  CK(launch_tile_check(m->stream, map_view(m), m->tlist, m->ntiles, tile_view(m)));
+  return VDBGPU_OK;
+}
+
+// Host copies of the tile bits and state bits (empty when the map holds no
+// tile); bit i stands for pool leaf i.
+int tile_bits(vdbgpu_map* m, std::vector<uint32_t>& bits, std::vector<uint32_t>* on) {
+  bits.clear();
+  if (on) on->clear();
+  if (!m->ntiles) return VDBGPU_OK;
+  bits.resize(m->tcap / 32);
+  CK(hipMemcpyAsync(bits.data(), m->tbits, m->tcap / 8, hipMemcpyDeviceToHost, m->stream));
+  if (on) {
+    on->resize(m->tcap / 32);
+    CK(hipMemcpyAsync(on->data(), m->ton, m->tcap / 8, hipMemcpyDeviceToHost, m->stream));
+  }
+  CK(hipStreamSynchronize(m->stream));
+  return VDBGPU_OK;
+}
+
+inline bool bit_at(const std::vector<uint32_t>& b, uint32_t i) {
+  return (size_t)(i >> 5) < b.size() && ((b[i >> 5] >> (i & 31)) & 1u);
+}
+
+uint32_t count_bits(const std::vector<uint32_t>& b) {
+  uint32_t n = 0;
+  for (uint32_t w : b) n += (uint32_t)__builtin_popcount(w);
+  return n;
+}
+
+// Leaves of the map (pool leaves that are not tiles); -1 on a device error.
+int64_t live_leaves(vdbgpu_map* m) {
+  std::vector<uint32_t> tb;
+  if (read_status(m) || tile_bits(m, tb, nullptr)) return -1;
+  return (int64_t)m->h_st[1].leaf_count - count_bits(tb);
+}
+
+// pruneGrid (OCC:173: tools::prune, tolerance 0) at the leaf level over the
+// whole pool: every leaf of 512 equal values, all on or all off, is a tile.
+int prune_tiles(vdbgpu_map* m) {
+  int r;
+  if ((r = read_status(m))) return r;
+  const uint32_t L = m->h_st[1].leaf_count;
+  const uint32_t cap = (uint32_t)(((uint64_t)L + 31) & ~31ull);
+  if (cap != m->tcap) {
+    free_tiles(m);
+    if (cap == 0) return VDBGPU_OK;
+    CK(dalloc(m, &m->tbits, cap / 32));
+    CK(dalloc(m, &m->ton, cap / 32));
+    CK(dalloc(m, &m->tval, cap));
+    CK(dalloc(m, &m->tlist, cap));
+    m->tcap = cap;
+  }
+  CK(hipMemsetAsync(m->tbits, 0, cap / 8, m->stream));
+  CK(hipMemsetAsync(m->ton, 0, cap / 8, m->stream));
+  CK(hipMemsetAsync(&m->d_st[2], 0, sizeof(Status), m->stream));
+  CK(launch_prune(m->stream, map_view(m), L, TileView{m->tbits, m->ton, m->tval, cap}, m->tlist,
+                  &m->d_st[2].entries));
+  if ((r = read_status(m))) return r;
+  m->ntiles = m->h_st[2].entries;
+  if (!m->ntiles) free_tiles(m);
+  m->snap_dirty = true;
+  return VDBGPU_OK;
 }
 
 int alloc_update(vdbgpu_map* m, Source& s, uint32_t cap) {
@@ -1019,7 +1109,7 @@ int flush_pending(vdbgpu_map* m) {
     }
     m->npend = 0;
     int r;
-    if ((r = activate_artificial(m))) return r;
+    if ((r = check_tiles(m)) || (r = activate_artificial(m))) return r;
     CK(hipEventRecord(m->rec_free[X], m->stream));
     m->rec_seq[X] = m->ledger_seq;
     return VDBGPU_OK;
@@ -1249,7 +1339,8 @@ int commit_staged(vdbgpu_map* m) {
   m->ins_fifo.push_back(X);
   m->touched = true;
   m->snap_dirty = true;
-  if (m->rec_sets > 2 && recs <= kGroupRecMask) {
+  // a tile's state is checked after every scan, so scans with tiles are not deferred
+  if (m->rec_sets > 2 && recs <= kGroupRecMask && !m->ntiles) {
     // applied with the next group pass: when defer scans are waiting (the
     // next insert enqueues it while its cast runs) or the map is read or
     // changed otherwise
@@ -1268,6 +1359,7 @@ int commit_staged(vdbgpu_map* m) {
     Timer t(m, kApply);
     CK(launch_apply(m->stream, g2, rec, m->ident, m->reidx[X], recs, map_view(m), m->lo, &m->d_st[3], false));
   }
+  if ((r = check_tiles(m))) return r;
   if ((r = activate_artificial(m))) return r;  // enters its new leaves in the ledger
   CK(hipEventRecord(m->rec_free[X], m->stream));
   m->rec_seq[X] = m->ledger_seq;
@@ -1334,7 +1426,7 @@ int build_snapshot(vdbgpu_map* m) {
   if (!m->d_bbox) CK(dalloc(m, &m->d_bbox, 6));
   int hb[6] = {INT_MAX, INT_MAX, INT_MAX, INT_MIN, INT_MIN, INT_MIN};
   CK(hipMemcpyAsync(m->d_bbox, hb, sizeof hb, hipMemcpyHostToDevice, m->stream));
-  if (L) CK(launch_snapshot(m->stream, map_view(m), L, m->rt, m->d_bbox));
+  if (L) CK(launch_snapshot(m->stream, map_view(m), L, m->rt, m->d_bbox, tile_view(m)));
   CK(launch_copy_set(m->stream, m->noop4096, vdbgpu_map::kNoopCap, m->rt.n4096, m->rt.n4096_mask));
   CK(hipMemcpyAsync(hb, m->d_bbox, sizeof hb, hipMemcpyDeviceToHost, m->stream));
   CK(hipStreamSynchronize(m->stream));
@@ -1397,11 +1489,15 @@ int do_integrate(vdbgpu_map* m) {
     m->snap_dirty = true;
     s.pending = 0;
     applied = true;
+    if ((r = check_tiles(m))) return r;
+    // with tiles the activation runs per source: a tile checked in between
+    // may turn into a leaf that the activation would have left uniform
+    if (m->ntiles && (r = activate_artificial(m))) return r;
   }
   // every non-empty updateMap ends by activating the artificial voxels
   // (VM:785-789); values do not depend on active states, so once after the
   // last source gives the same map as after each one
-  if (applied) {
+  if (applied && !m->ntiles) {
     const int r = activate_artificial(m);
     if (r) return r;
   }
@@ -1470,7 +1566,7 @@ int apply_edit(vdbgpu_map* m, Source& s, uint32_t count, uint32_t* eidx, int op,
   CK(launch_set_voxels(m->stream, upd_view(s), s.list, eidx, count, map_view(m), op, val, m->lo[3], clear));
   m->touched = true;
   m->snap_dirty = true;
-  return VDBGPU_OK;
+  return check_tiles(m);
 }
 
 int ensure_eidx(vdbgpu_map* m, uint32_t n) {
@@ -1765,6 +1861,7 @@ void vdbgpu_destroy(vdbgpu_map* m) {
     dfree(m, &m->leaf_keys, m->lcap);
     dfree(m, &m->dirty, m->lcap);
   }
+  free_tiles(m);
   dfree(m, &m->d_points, m->points_bytes / 4);
   const uint32_t nb = (m->ray_cap + kPrepBlock - 1) / kPrepBlock;
   dfree(m, &m->rayinfo, m->ray_cap);
@@ -1976,10 +2073,11 @@ int32_t vdbgpu_insert_batch(vdbgpu_map* m, const char* id, int32_t nscans, const
   int i0 = 0;
   bool pending = false;
   for (auto& kv : m->sources) pending = pending || kv.second.pending > 0;
-  if ((m->tuning.traversal == 0 && !pending) || m->fast_mode) {
+  if ((m->tuning.traversal == 0 && !pending) || m->fast_mode || m->ntiles) {
     // Streaming every scan through the brick path (the next scan's cast
     // overlaps this one's map update) is faster than the (leaf, scan) fold;
-    // a fast-mode cast depends on the map state, so it runs scan by scan.
+    // a fast-mode cast depends on the map state, and the tiles of a pruned
+    // map are checked after every scan, so both run scan by scan.
     for (int i = 0; i < nscans; ++i) {
       const int r = plain_insert(i);
       if (r) return r;
@@ -2054,8 +2152,7 @@ int64_t vdbgpu_leaf_count(vdbgpu_map* m) {
   if (!m) return -1;
   Guard g(m);
   if (g.rc) return -1;
-  if (read_status(m)) return -1;
-  return m->h_st[1].leaf_count;
+  return live_leaves(m);  // tiles are not leaves
 }
 
 int64_t vdbgpu_export_leaves(vdbgpu_map* m, int32_t* origins, float* values, uint64_t* masks,
@@ -2065,8 +2162,12 @@ int64_t vdbgpu_export_leaves(vdbgpu_map* m, int32_t* origins, float* values, uin
   if (g.rc) return -1;
   if (read_status(m)) return -1;
   const uint32_t n = m->h_st[1].leaf_count;
-  if (!origins && !values && !masks) return n;
-  if ((int64_t)n > cap) return -(int64_t)n;
+  if (!m->ntiles) {
+    if (!origins && !values && !masks) return n;
+    if ((int64_t)n > cap) return -(int64_t)n;
+  } else if (!origins && !values && !masks) {
+    return live_leaves(m);
+  }
   std::vector<unsigned long long> keys(n), act((size_t)n * 8);
   std::vector<float> vals(values ? (size_t)n * 512 : 0);
   if (n) {
@@ -2076,6 +2177,23 @@ int64_t vdbgpu_export_leaves(vdbgpu_map* m, int32_t* origins, float* values, uin
       m->last_error = "export copy failed";
       return -1;
     }
+  }
+  std::vector<uint32_t> tb;
+  if (tile_bits(m, tb, nullptr)) return -1;
+  if (!tb.empty()) {  // tiles are not leaves: compact them out
+    uint32_t k = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+      if (bit_at(tb, i)) continue;
+      keys[k] = keys[i];
+      std::memcpy(&act[(size_t)k * 8], &act[(size_t)i * 8], 64);
+      if (values) std::memcpy(&vals[(size_t)k * 512], &vals[(size_t)i * 512], 2048);
+      ++k;
+    }
+    keys.resize(k);
+    act.resize((size_t)k * 8);
+    if (values) vals.resize((size_t)k * 512);
+    if (!origins && !values && !masks) return k;
+    if ((int64_t)k > cap) return -(int64_t)k;
   }
   return export_sorted(m, keys, vals, act, origins, values, masks, cap, values != nullptr);
 }
@@ -2097,7 +2215,20 @@ int64_t vdbgpu_export_dirty_leaves(vdbgpu_map* m, int32_t* origins, float* value
   }
   const int grid = std::min<int>(4096, (int)((lc + 255) / 256));
   if (launch_dirty_list(m->stream, grid, m->dirty, lc, m->eidx, &m->d_st[2]) || read_status(m)) return -1;
-  const uint32_t nd = m->h_st[2].entries;
+  uint32_t nd = m->h_st[2].entries;
+  std::vector<uint32_t> tb;
+  if (tile_bits(m, tb, nullptr)) return -1;
+  if (!tb.empty() && nd) {
+    // tiles are not leaves: drop them from the list (a tile is unchanged since
+    // the check after the last write, so its flag may go)
+    std::vector<uint32_t> idx(nd);
+    if (hipMemcpy(idx.data(), m->eidx, nd * 4, hipMemcpyDeviceToHost)) return -1;
+    uint32_t k = 0;
+    for (uint32_t i : idx)
+      if (!bit_at(tb, i)) idx[k++] = i;
+    nd = k;
+    if (nd && hipMemcpy(m->eidx, idx.data(), nd * 4, hipMemcpyHostToDevice)) return -1;
+  }
   if ((int64_t)nd > cap) {
     // put the flags back so that a retry with a larger buffer sees the same set
     std::vector<uint32_t> idx(nd);
@@ -2369,7 +2500,9 @@ int32_t vdbgpu_get_stats(vdbgpu_map* m, vdbgpu_stats* out) {
   if (g.rc) return g.rc;
   int r = read_status(m);
   if (r) return r;
-  m->stats.leaf_count = m->h_st[1].leaf_count;
+  std::vector<uint32_t> tb;
+  if ((r = tile_bits(m, tb, nullptr))) return r;
+  m->stats.leaf_count = m->h_st[1].leaf_count - count_bits(tb);  // tiles are not leaves
   m->stats.voxel_updates = m->h_st[3].voxels;
   m->stats.device_bytes = m->device_bytes;
   m->stats.scratch_bytes = walk_scratch(m, false);
@@ -2473,6 +2606,7 @@ int clear_grid(vdbgpu_map* m) {
   m->leaf_count_ub = 0;
   m->ledger.clear();
   m->ins_fifo.clear();
+  free_tiles(m);
   m->touched = false;
   m->snap_dirty = true;
   m->art_leaves = false;
@@ -2509,10 +2643,10 @@ int32_t vdbgpu_create_from_points(vdbgpu_map* m, const float* xyz, int64_t n, in
   if (clear_map && (r = clear_grid(m))) return r;
   // setValueOn(ijk, max log-odds): the same voxel state as setNodeToOccupied
   if (count && (r = apply_points(m, count, true))) return r;
-  if (!set_background) return VDBGPU_OK;
+  if (!set_background) return prune_tiles(m);
   if ((r = read_status(m))) return r;
   const uint32_t L = m->h_st[1].leaf_count;
-  if (L == 0) return VDBGPU_OK;
+  if (L == 0) return prune_tiles(m);
   int* d_box = nullptr;
   const int init[6] = {INT_MAX, INT_MAX, INT_MAX, INT_MIN, INT_MIN, INT_MIN};
   if ((r = upload(m, &d_box, init, 6))) return r;
@@ -2522,7 +2656,7 @@ int32_t vdbgpu_create_from_points(vdbgpu_map* m, const float* xyz, int64_t n, in
   CK(hipStreamSynchronize(m->stream));
   if (box[0] > box[3]) {  // no active voxel: an empty box
     dfree(m, &d_box, 6);
-    return VDBGPU_OK;
+    return prune_tiles(m);
   }
   const int lo[3] = {box[0] >> 3, box[1] >> 3, box[2] >> 3};
   const uint32_t dims[3] = {(uint32_t)((box[3] >> 3) - lo[0] + 1), (uint32_t)((box[4] >> 3) - lo[1] + 1),
@@ -2553,7 +2687,41 @@ int32_t vdbgpu_create_from_points(vdbgpu_map* m, const float* xyz, int64_t n, in
   dfree(m, &needs, N);
   dfree(m, &eidx, N);
   dfree(m, &d_box, 6);
-  return VDBGPU_OK;
+  return prune_tiles(m);  // pruneGrid (OCC:173)
+}
+
+int64_t vdbgpu_export_tiles(vdbgpu_map* m, int32_t* origins, float* values, uint8_t* active, int64_t cap) {
+  if (!m) return -1;
+  Guard g(m);
+  if (g.rc) return -1;
+  std::vector<uint32_t> tb, on;
+  if (read_status(m) || tile_bits(m, tb, &on)) return -1;
+  const int64_t n = count_bits(tb);
+  if (!origins && !values && !active) return n;
+  if (n > cap) return -n;
+  if (n == 0) return 0;
+  const uint32_t L = std::min<uint32_t>(m->h_st[1].leaf_count, m->tcap);
+  std::vector<unsigned long long> keys(L);
+  std::vector<float> tv(L);
+  if (hipMemcpy(keys.data(), m->leaf_keys, (size_t)L * 8, hipMemcpyDeviceToHost) ||
+      hipMemcpy(tv.data(), m->tval, (size_t)L * 4, hipMemcpyDeviceToHost)) {
+    m->last_error = "export copy failed";
+    return -1;
+  }
+  std::vector<std::pair<std::array<int, 3>, uint32_t>> order;
+  for (uint32_t i = 0; i < L; ++i) {
+    if (!bit_at(tb, i)) continue;
+    std::array<int, 3> o;
+    key_origin(keys[i], o.data());
+    order.push_back({o, i});
+  }
+  std::sort(order.begin(), order.end());
+  for (size_t j = 0; j < order.size(); ++j) {
+    if (origins) std::memcpy(origins + 3 * j, order[j].first.data(), 12);
+    if (values) values[j] = tv[order[j].second];
+    if (active) active[j] = bit_at(on, order[j].second) ? 1 : 0;
+  }
+  return (int64_t)order.size();
 }
 
 int64_t vdbgpu_export_active_points(vdbgpu_map* m, float* xyz, int64_t cap) {
@@ -2568,12 +2736,19 @@ int64_t vdbgpu_export_active_points(vdbgpu_map* m, float* xyz, int64_t cap) {
     m->last_error = "export copy failed";
     return -1;
   }
+  std::vector<uint32_t> tb;
+  if (tile_bits(m, tb, nullptr)) return -1;
+  // cbeginValueOn yields an active tile once, at its origin (an inactive one
+  // not at all): only voxel 0 of a tile's leaf is listed
+  for (uint32_t i = 0; i < n; ++i)
+    if (bit_at(tb, i))
+      for (int w = 0; w < 8; ++w) act[(size_t)i * 8 + w] = w == 0 ? act[(size_t)i * 8] & 1u : 0;
   int64_t total = 0;
   for (uint64_t w : act) total += __builtin_popcountll(w);
   if (!xyz) return total;
   if (total > cap) return -total;
   // cbeginValueOn order: root children by origin, then the 128^3 nodes and
-  // the leaves inside each by offset (x-major), voxels by offset
+  // the leaves (and tiles) inside each by offset (x-major), voxels by offset
   std::vector<std::pair<std::array<int, 9>, uint32_t>> order(n);
   for (uint32_t i = 0; i < n; ++i) {
     int o[3];
@@ -2790,7 +2965,8 @@ int64_t vdbgpu_export_section(vdbgpu_map* m, const int32_t bbox[6], int32_t full
     if (upload(m, &d_box, bbox, 6)) return -1;
     CKN(dalloc(m, &list, L));
     CKN(hipMemsetAsync(&m->d_st[2], 0, sizeof(Status), m->stream));
-    CKN(launch_section_pick(m->stream, map_view(m), L, d_box, full_grid != 0, list, &m->d_st[2].entries));
+    CKN(launch_section_pick(m->stream, map_view(m), L, d_box, full_grid != 0, list, &m->d_st[2].entries,
+                            tile_view(m)));
     if (read_status(m)) return -1;
     const uint32_t n = m->h_st[2].entries;
     const bool want = origins || values || masks;
@@ -2877,7 +3053,7 @@ int32_t vdbgpu_apply_section(vdbgpu_map* m, int64_t n, const int32_t* origins, c
   // the inactive tiles of the section's internal nodes (VM:1034 cbeginValueAll)
   CK(launch_section_tiles(m->stream, lists, c128, lists + N, c4096, sets, scap - 1, sets + scap, scap - 1,
                           map_view(m)));
-  if ((r = read_status(m))) return r;
+  if ((r = check_tiles(m)) || (r = read_status(m))) return r;
   m->touched = m->touched || m->h_st[1].leaf_count > 0;
   dfree(m, &d_o, (size_t)N * 3);
   dfree(m, &d_v, (size_t)N * 512);
@@ -2905,7 +3081,7 @@ int32_t vdbgpu_apply_section_update(vdbgpu_map* m, const int32_t bbox[6], int64_
   const uint32_t L = m->h_st[1].leaf_count;
   int* d_box = nullptr;
   if ((r = upload(m, &d_box, bbox, 6))) return r;
-  CK(launch_section_clear(m->stream, map_view(m), L, d_box));
+  CK(launch_section_clear(m->stream, map_view(m), L, d_box, tile_view(m)));
   if (n) {
     const uint32_t N = (uint32_t)n;
     int* d_o = nullptr;
@@ -2932,6 +3108,7 @@ int32_t vdbgpu_apply_section_update(vdbgpu_map* m, const int32_t bbox[6], int64_
     dfree(m, &first, N);
     dfree(m, &eidx, N);
   }
+  if ((r = check_tiles(m))) return r;
   CK(hipStreamSynchronize(m->stream));
   dfree(m, &d_box, 6);
   return VDBGPU_OK;
