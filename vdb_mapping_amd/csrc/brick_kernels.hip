@@ -30,10 +30,13 @@
 //   k_rec_gather    the listed records copied densely into the scan's record
 //                   set (read by the map stream, possibly several scans later)
 //
-// Segment record (uint4): ray index; crossing count c0 (bits 0-13) | visits
-// m (bits 16-22) | end-voxel hit flag (bit 31); c1; c2. The walk of a
-// segment starts at voxel s + sign * c with the integer DDA state of those
-// counts and visits m voxels, all inside the brick (checked).
+// Segment record (uint4), self-contained so the walk loads nothing else:
+//   x = n0 | n1 << 14, y = n2 | c0 << 14, z = c1 | c2 << 14  (|delta| per axis
+//       and the crossing counts at the segment's first visit, <= 8192 each)
+//   w = brick-local first voxel lx | ly << 5 | lz << 10 | visits m << 16 |
+//       negative-step bits x, y, z << 24 | end-voxel hit flag << 31
+// The walk of a segment starts at that voxel with the integer DDA state of
+// those counts and visits m (<= 94) voxels, all inside the brick (checked).
 #include "dda_math.h"
 #include "device_common.h"
 #include "kernels.h"
@@ -46,6 +49,12 @@ constexpr int kLdsBricks = kSegTab;  // per-workgroup brick aggregation table
 constexpr int kLdsBrickProbe = 32;
 constexpr uint32_t kMaxBrickProbe = 512;
 constexpr int kBrickWords = 1024;  // 64 leaves x (8 visited + 8 hit) words
+// The walk's LDS bitmap: [0, 512) visited words of layout X, [512, 1024) hit
+// words (leaf-major, as emitted), [1024, 1536) layout Y, [1536, 2048) layout Z.
+// A segment marks its voxels in the layout whose word axis it steps along
+// least (see walk_segment); the layouts are merged into leaf-major visited
+// words before the records are written.
+constexpr int kWalkWords = 2048;
 
 __device__ __forceinline__ int bw_lane() { return (int)__lane_id(); }
 
@@ -362,63 +371,156 @@ __global__ __launch_bounds__(256) void k_seg_fill(const int4* __restrict__ rayin
         atomicOr(&st->overflow, kOvfSegs);
         continue;
       }
-      segs[pos] = make_uint4(i, sg.c[0] | (sg.m << 16) | ((sg.last && g.hit) ? 0x80000000u : 0u),
-                             sg.c[1], sg.c[2]);
+      uint32_t n[3], neg = 0, loc = 0;
+      bool inside = true;
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        const int d = g.e[a] - g.s[a];
+        n[a] = (uint32_t)abs(d);
+        neg |= (d < 0 ? 1u : 0u) << a;
+        const int b = a == 0 ? sg.bx : (a == 1 ? sg.by : sg.bz);
+        const int l = g.s[a] + (d < 0 ? -(int)sg.c[a] : (int)sg.c[a]) - b * kBrickDim;
+        inside = inside && (unsigned)l < (unsigned)kBrickDim;
+        loc |= (uint32_t)(l & (kBrickDim - 1)) << (5 * a);
+      }
+      if (!inside) atomicOr(&st->error, kErrBrick);  // the cut and its brick disagree
+      segs[pos] = make_uint4(n[0] | n[1] << 14, n[2] | sg.c[0] << 14, sg.c[1] | sg.c[2] << 14,
+                             loc | sg.m << 16 | neg << 24 | ((sg.last && g.hit) ? 0x80000000u : 0u));
     }
   }
 }
 
-// Walk one segment of the item (record sr, its ray's rayinfo r) into the
-// brick bitmap `bits`. The path is monotone along every axis, so it stays
-// inside the brick when its first and last voxels do: both are checked, and
-// the LDS word index is masked into the brick so no access can leave it.
-__device__ __forceinline__ void walk_segment(const uint4 sr, const int4 r, const ScanInfo* __restrict__ scans,
-                                             const double* __restrict__ tbl, int ox, int oy, int oz,
-                                             unsigned long long* bits, Status* st) {
-  const uint32_t sc = ((uint32_t)r.w >> kScanShift) & kScanMask;
-  const int sx = scans[sc].s[0], sy = scans[sc].s[1], sz = scans[sc].s[2];
-  const uint32_t n[3] = {(uint32_t)abs(r.x - sx), (uint32_t)abs(r.y - sy), (uint32_t)abs(r.z - sz)};
-  const int stx = r.x > sx ? 1 : (r.x < sx ? -1 : 0);
-  const int sty = r.y > sy ? 1 : (r.y < sy ? -1 : 0);
-  const int stz = r.z > sz ? 1 : (r.z < sz ? -1 : 0);
-  const uint32_t c[3] = {sr.y & 0x3FFFu, sr.z, sr.w};
-  const uint32_t m = (sr.y >> 16) & 0x7Fu;
-  int lx = sx + stx * (int)c[0] - ox, ly = sy + sty * (int)c[1] - oy, lz = sz + stz * (int)c[2] - oz;
-  const bool ok0 = (unsigned)lx < (unsigned)kBrickDim && (unsigned)ly < (unsigned)kBrickDim &&
-                   (unsigned)lz < (unsigned)kBrickDim;
-  int cw = -1;
-  uint64_t cb = 0, lastb = 0;
-  IntDda d;
-  d.init(n, c);
+// 8x8 bit-matrix transpose: bit 8r + c of x goes to bit 8c + r.
+__device__ __forceinline__ uint64_t transpose8(uint64_t x) {
+  uint64_t t = (x ^ (x >> 7)) & 0x00AA00AA00AA00AAull;
+  x ^= t ^ (t << 7);
+  t = (x ^ (x >> 14)) & 0x0000CCCC0000CCCCull;
+  x ^= t ^ (t << 14);
+  t = (x ^ (x >> 28)) & 0x00000000F0F0F0F0ull;
+  x ^= t ^ (t << 28);
+  return x;
+}
+
+// Walk one segment record into the brick bitmap `bits`.
+//
+// Word layouts: a segment's voxels go into the layout whose word axis A is
+// the axis it steps along least, so most steps stay inside one 64-bit word
+// and the word is OR-ed into LDS only when the walk leaves it. With (a, b, c)
+// the voxel's brick-local coordinates along (A, B, C) -- (x, y, z) for layout
+// X, (y, x, z) for Y, (z, x, y) for Z -- the word is a * 16 + (b >> 3) * 4 +
+// (c >> 3) and the bit (b & 7) * 8 + (c & 7). The walk state is one packed
+// code u = a << 12 | b << 6 | c (6-bit fields; bit 5 of a field is a guard: a
+// field that leaves [0, 31] sets it, and the OR of every code is checked at
+// the end), stepped by one add per visit. The path is monotone along every
+// axis, so it stays inside the brick when no field ever left it.
+//
+// The step order is the exact integer DDA of dda_math.h (signs of the three
+// pairwise differences); the rational-tie rule runs only when a lane of the
+// wave meets a tie, with the crossing counts recovered from the walk state.
+__device__ __forceinline__ uint32_t walk_word(uint32_t u) {
+  return ((u >> 8) & 0x1F0u) | ((u >> 7) & 0xCu) | ((u >> 3) & 3u);
+}
+__device__ __forceinline__ uint32_t walk_bit(uint32_t u) { return ((u >> 3) & 0x38u) | (u & 7u); }
+constexpr uint32_t kWalkFields = 0x1F7DFu;  // the valid bits of a walk code
+
+__device__ __forceinline__ void walk_segment(const uint4 sr, const double* __restrict__ tbl,
+                                             unsigned long long* bits, uint32_t& used, Status* st) {
+  const uint32_t n0 = sr.x & 0x3FFFu, n1 = (sr.x >> 14) & 0x3FFFu, n2 = sr.y & 0x3FFFu;
+  const uint32_t c00 = (sr.y >> 14) & 0x3FFFu, c10 = sr.z & 0x3FFFu, c20 = (sr.z >> 14) & 0x3FFFu;
+  const uint32_t lx0 = sr.w & 31u, ly0 = (sr.w >> 5) & 31u, lz0 = (sr.w >> 10) & 31u;
+  const uint32_t m = (sr.w >> 16) & 0x7Fu;
+  const int sgx = n0 == 0 ? 0 : (((sr.w >> 24) & 1u) ? -1 : 1);
+  const int sgy = n1 == 0 ? 0 : (((sr.w >> 25) & 1u) ? -1 : 1);
+  const int sgz = n2 == 0 ? 0 : (((sr.w >> 26) & 1u) ? -1 : 1);
+  // word axis: the one with the fewest steps
+  const int L = (n0 <= n1 && n0 <= n2) ? 0 : (n1 <= n2 ? 1 : 2);
+  used |= 1u << L;
+  const int shx = L == 0 ? 12 : 6, shy = L == 1 ? 12 : (L == 0 ? 6 : 0), shz = L == 2 ? 12 : 0;
+  uint32_t u = lx0 << shx | ly0 << shy | lz0 << shz;
+  const int ix = sgx * (1 << shx), iy = sgy * (1 << shy), iz = sgz * (1 << shz);
+  const uint32_t base = L == 0 ? 0u : (L == 1 ? 1024u : 1536u);
+  const int tn0 = 2 * (int)n0, tn1 = 2 * (int)n1, tn2 = 2 * (int)n2;
+  int dyx = (int)((2 * c10 + 1) * n0) - (int)((2 * c00 + 1) * n1);
+  int dzx = (int)((2 * c20 + 1) * n0) - (int)((2 * c00 + 1) * n2);
+  int dzy = (int)((2 * c20 + 1) * n1) - (int)((2 * c10 + 1) * n2);
+  uint32_t esc = u, cw = walk_word(u);
+  uint64_t cb = 0;
   for (uint32_t k = 0;;) {
-    const int w = (((lx >> 3) & 3) << 7) | (((ly >> 3) & 3) << 5) | (((lz >> 3) & 3) << 3) | (lx & 7);
-    lastb = 1ull << (((ly & 7) << 3) | (lz & 7));
+    const uint32_t w = walk_word(u);
     if (w != cw) {
-      if (cb) atomicOr(&bits[cw], (unsigned long long)cb);
+      atomicOr(&bits[base + cw], (unsigned long long)cb);
       cw = w;
       cb = 0;
     }
-    cb |= lastb;
+    cb |= 1ull << walk_bit(u);
     if (++k >= m) break;  // m == 0: the endpoint of a zero-length ray
-    // the step without per-lane branches; the exact-tie rule only when a lane of
-    // the wave meets a tie (one A/B run on the outdoor stream: walk 1.04 -> 1.01 ms
-    // per scan beside the group pass, 136.1 -> 139.2 M rays/s)
-    bool az, ay;
-    d.next_axis_wave(tbl, az, ay);
-    d.advance_sel(az, ay);
-    lx += (az || ay) ? 0 : stx;
-    ly += (ay && !az) ? sty : 0;
-    lz += az ? stz : 0;
+    bool y = dyx < 0;
+    const int dzm = y ? dzy : dzx;
+    bool z = dzm < 0;
+    if (__any(dyx == 0 || dzm == 0)) {
+      // exact rule for every lane: crossing counts from the walk state
+      const uint32_t fa = (u >> 12) & 31u, fb = (u >> 6) & 31u, fc = u & 31u;
+      const uint32_t lx = L == 0 ? fa : fb, ly = L == 1 ? fa : (L == 0 ? fb : fc), lz = L == 2 ? fa : fc;
+      IntDda t;
+      t.n0 = n0, t.n1 = n1, t.n2 = n2;
+      t.c0 = c00 + (uint32_t)abs((int)lx - (int)lx0);
+      t.c1 = c10 + (uint32_t)abs((int)ly - (int)ly0);
+      t.c2 = c20 + (uint32_t)abs((int)lz - (int)lz0);
+      t.dyx = dyx, t.dzx = dzx, t.dzy = dzy;
+      const int ax = t.next_axis(tbl);
+      z = ax == 2;
+      y = ax == 1;
+    }
+    y = y && !z;
+    const bool x = !z && !y;
+    dyx += z ? 0 : (y ? tn0 : -tn1);
+    dzx += y ? 0 : (z ? tn0 : -tn2);
+    dzy += x ? 0 : (z ? tn1 : -tn2);
+    u += (uint32_t)(z ? iz : (y ? iy : ix));
+    esc |= u;
   }
-  if (cb) atomicOr(&bits[cw], (unsigned long long)cb);
-  const bool ok1 = (unsigned)lx < (unsigned)kBrickDim && (unsigned)ly < (unsigned)kBrickDim &&
-                   (unsigned)lz < (unsigned)kBrickDim;
-  if (!(ok0 && ok1)) {
+  atomicOr(&bits[base + cw], (unsigned long long)cb);
+  if (esc & ~kWalkFields) {
     atomicOr(&st->error, kErrBrick);  // the scan is rejected
-  } else if (sr.y & 0x80000000u) {
+  } else if (sr.w & 0x80000000u) {
     // setValueOn(end, true) (VDBMapping.hpp:533-536): the last visit is the end voxel
-    atomicOr(&bits[512 + cw], (unsigned long long)lastb);
+    const uint32_t fa = (u >> 12) & 31u, fb = (u >> 6) & 31u, fc = u & 31u;
+    const uint32_t lx = L == 0 ? fa : fb, ly = L == 1 ? fa : (L == 0 ? fb : fc), lz = L == 2 ? fa : fc;
+    const uint32_t hw = ((lx >> 3) << 7) | ((ly >> 3) << 5) | ((lz >> 3) << 3) | (lx & 7u);
+    atomicOr(&bits[512 + hw], 1ull << (((ly & 7u) << 3) | (lz & 7u)));
   }
+}
+
+// Merge the layouts of the walk bitmap into leaf-major visited words at
+// [0, 512) (word j of leaf l at l * 8 + j, bit (y & 7) * 8 + (z & 7)); `used`
+// names the layouts any segment of the item marked. Called by the whole
+// workgroup (two words per thread); ends with a barrier.
+__device__ void merge_layouts(unsigned long long* bits, uint32_t used) {
+  uint64_t out[2];
+#pragma unroll
+  for (int r = 0; r < 2; ++r) {
+    const int q = (int)threadIdx.x + r * 256;
+    const int l = q >> 3, j = q & 7;
+    const int lx = l >> 4, ly = (l >> 2) & 3, lz = l & 3;
+    uint64_t v = (used & 1u) ? bits[(lx * 8 + j) * 16 + ly * 4 + lz] : 0ull;
+    if (used & 2u) {  // layout Y: byte j of the word at y = 8 ly + yy is byte yy here
+      const unsigned long long* p = bits + 1024 + (ly * 8) * 16 + lx * 4 + lz;
+#pragma unroll
+      for (int yy = 0; yy < 8; ++yy) v |= ((p[yy * 16] >> (8 * j)) & 0xFFull) << (8 * yy);
+    }
+    if (used & 4u) {  // layout Z: bit yy of byte j of the word at z = 8 lz + zz is bit 8 yy + zz here
+      const unsigned long long* p = bits + 1536 + (lz * 8) * 16 + lx * 4 + ly;
+      uint64_t mz = 0;
+#pragma unroll
+      for (int zz = 0; zz < 8; ++zz) mz |= ((p[zz * 16] >> (8 * j)) & 0xFFull) << (8 * zz);
+      v |= transpose8(mz);
+    }
+    out[r] = v;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < 2; ++r) bits[threadIdx.x + r * 256] = out[r];
+  __syncthreads();
 }
 
 constexpr int kWalkThreads = 256;
@@ -433,11 +535,12 @@ __global__ __launch_bounds__(kWalkThreads, 8) void k_brick_walk(const BrickItem*
                                                              unsigned long long* __restrict__ shared_bits,
                                                              UpdView rec, unsigned long long* __restrict__ leaf_mask,
                                                              uint32_t rec_cap, Status* st) {
-  __shared__ unsigned long long bits[kBrickWords];
+  __shared__ unsigned long long bits[kWalkWords];
   __shared__ uint32_t s_pos[64];
   __shared__ uint32_t s_hist[128];
   __shared__ uint16_t s_order[kItemSegs];
   __shared__ unsigned int s_stop;
+  __shared__ uint32_t s_used;
   if (stopped(st, &s_stop)) return;
   const uint32_t n_items = st->items;
   if ((uint64_t)n_items * 64 > rec_cap) {
@@ -451,8 +554,9 @@ __global__ __launch_bounds__(kWalkThreads, 8) void k_brick_walk(const BrickItem*
     int b[3];
     brick_coords(bv.keys[item.slot], b);
     const int ox = b[0] * kBrickDim, oy = b[1] * kBrickDim, oz = b[2] * kBrickDim;
-    for (int w = t; w < kBrickWords; w += kWalkThreads) bits[w] = 0;
+    for (int w = t; w < kWalkWords; w += kWalkThreads) bits[w] = 0;
     if (t < 128) s_hist[t] = 0;
+    if (t == 0) s_used = 0;
     // Counting sort of the item's segments by visit count, longest first, so
     // the lanes of a wave walk segments of about the same length. Segment k
     // belongs to thread k % 256; its length is loaded once, for both passes.
@@ -461,7 +565,7 @@ __global__ __launch_bounds__(kWalkThreads, 8) void k_brick_walk(const BrickItem*
 #pragma unroll
     for (int r = 0; r < kSegsPerThread; ++r) {
       const uint32_t k = t + r * kWalkThreads;
-      if (k < nseg) lens |= ((segs[item.seg0 + k].y >> 16) & 0x7Fu) << (8 * r);
+      if (k < nseg) lens |= ((segs[item.seg0 + k].w >> 16) & 0x7Fu) << (8 * r);
     }
     __syncthreads();
 #pragma unroll
@@ -487,12 +591,26 @@ __global__ __launch_bounds__(kWalkThreads, 8) void k_brick_walk(const BrickItem*
       if (k < nseg) s_order[atomicAdd(&s_hist[127 - ((lens >> (8 * r)) & 0xFFu)], 1u)] = (uint16_t)k;
     }
     __syncthreads();
+    // Round r of the walk covers the sorted segments [256 r, 256 r + 256); wave
+    // w takes the 64 of block (w + r) mod 4, so over the rounds every wave
+    // walks long and short ones alike. The next round's record is loaded
+    // before the current one is walked.
+    uint32_t used = 0;
+    const uint32_t lane = t & 63u, wave = t >> 6;
+    uint32_t k = (((wave + 0) & 3u) << 6) + lane;
+    uint4 sr = k < nseg ? segs[item.seg0 + s_order[k]] : make_uint4(0, 0, 0, 0);
 #pragma unroll 1
-    for (uint32_t k = t; k < nseg; k += kWalkThreads) {
-      const uint4 sr = segs[item.seg0 + s_order[k]];
-      walk_segment(sr, rayinfo[sr.x], scans, tbl, ox, oy, oz, bits, st);
+    for (uint32_t r = 0; k < nseg;) {
+      ++r;
+      const uint32_t kn = r * kWalkThreads + (((wave + r) & 3u) << 6) + lane;
+      const uint4 sn = kn < nseg ? segs[item.seg0 + s_order[kn]] : make_uint4(0, 0, 0, 0);
+      walk_segment(sr, tbl, bits, used, st);
+      k = kn;
+      sr = sn;
     }
+    if (used) atomicOr(&s_used, used);
     __syncthreads();
+    merge_layouts(bits, s_used);
     if (item.shared == kNone) {
       emit_leaves(bits, ox, oy, oz, rec, it * 64, s_pos, &leaf_mask[it]);
     } else {

@@ -792,6 +792,31 @@ __global__ __launch_bounds__(256) void k_group_fill(const uint32_t* __restrict__
 constexpr int kGroupWords = 4;
 static_assert(kDeferMax == 8, "a group's record positions are read as two uint4");
 
+// Every scan's update of voxel k of a lane (bits 4s + k of vb / hb: visited /
+// hit in scan s), in scan order, visits only: one iteration per visit, the
+// clamped update without branches (the same results as update_voxel).
+__device__ __forceinline__ void apply_visits(float& x, uint32_t& ab, int k, uint32_t vb, uint32_t hb,
+                                             float l_hit, float l_miss, float t_min, float t_max,
+                                             float c_max, float c_min) {
+  uint32_t m = (vb >> k) & 0x11111111u;
+  if (!__any(m != 0)) return;
+  float v = x;
+  uint32_t on = (ab >> k) & 1u;
+  while (m) {
+    const uint32_t b = (uint32_t)__builtin_ctz(m);
+    m &= m - 1;
+    const bool h = (hb >> (b + k)) & 1u;
+    const float a = v + (h ? l_hit : l_miss);
+    // hit: above t_max -> active, clamped at c_max; miss: below t_min -> inactive, clamped at c_min
+    const bool cross = h ? (a > t_max) : (a < t_min);
+    const float cl = h ? (a > c_max ? c_max : a) : (a < c_min ? c_min : a);
+    v = cross ? cl : a;
+    on = cross ? (h ? 1u : 0u) : on;
+  }
+  x = v;
+  ab = (ab & ~(1u << k)) | (on << k);
+}
+
 struct GroupWord {
   uint32_t e, g;
   int j;
@@ -865,19 +890,13 @@ __global__ __launch_bounds__(256) void k_apply_groups(GroupScans gs, const uint3
     for (int q = 0; q < kGroupWords; ++q) {
       if (l == 0 && w[q].j == 0 && w[q].e != kNone) gslot[w[q].e] = kNone;
       if (!w[q].any) continue;  // the sixteen lanes of the word agree
-      uint32_t ab = (uint32_t)(aw[q] >> (4 * l)) & 15u, mine = 0;
-#pragma unroll
-      for (int k = 0; k < kDeferMax; ++k) {
-        // the scan has no record of this leaf (same for the whole wave: a scalar branch)
-        if (__builtin_amdgcn_readfirstlane(w[q].slot[k]) == kNone) continue;
-        const uint32_t vb = (w[q].vb >> (4 * k)) & 15u, hb = (w[q].hb >> (4 * k)) & 15u;
-        mine |= vb;
-        update_voxel(x[q].x, ab, 0, vb & 1, hb & 1, l_hit, l_miss, t_min, t_max, c_max, c_min);
-        update_voxel(x[q].y, ab, 1, vb & 2, hb & 2, l_hit, l_miss, t_min, t_max, c_max, c_min);
-        update_voxel(x[q].z, ab, 2, vb & 4, hb & 4, l_hit, l_miss, t_min, t_max, c_max, c_min);
-        update_voxel(x[q].w, ab, 3, vb & 8, hb & 8, l_hit, l_miss, t_min, t_max, c_max, c_min);
-      }
-      if (mine) reinterpret_cast<float4*>(mp.values)[(size_t)w[q].e * 128 + w[q].j * 16 + l] = x[q];
+      uint32_t ab = (uint32_t)(aw[q] >> (4 * l)) & 15u;
+      // only the visits: voxel k's scans are the bits 4s + k of vb (scan order)
+      apply_visits(x[q].x, ab, 0, w[q].vb, w[q].hb, l_hit, l_miss, t_min, t_max, c_max, c_min);
+      apply_visits(x[q].y, ab, 1, w[q].vb, w[q].hb, l_hit, l_miss, t_min, t_max, c_max, c_min);
+      apply_visits(x[q].z, ab, 2, w[q].vb, w[q].hb, l_hit, l_miss, t_min, t_max, c_max, c_min);
+      apply_visits(x[q].w, ab, 3, w[q].vb, w[q].hb, l_hit, l_miss, t_min, t_max, c_max, c_min);
+      if (w[q].vb) reinterpret_cast<float4*>(mp.values)[(size_t)w[q].e * 128 + w[q].j * 16 + l] = x[q];
       uint64_t na = (uint64_t)ab << (4 * l);
 #pragma unroll
       for (int o = 1; o < 16; o <<= 1) na |= __shfl_xor(na, o, 16);
