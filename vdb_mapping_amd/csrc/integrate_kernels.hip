@@ -11,9 +11,9 @@
 //                 source's update set at leaf-word granularity
 //   k_compact     list of occupied update-set slots + visited-voxel count
 //   k_map_insert  one lane per update leaf: find or create the map leaf
-//   k_apply       sixteen lanes per 64-voxel mask word of an update leaf:
-//                 clamped log-odds update of the visited voxels
-//                 (OccupancyVDBMapping.hpp:92-117), one float4 per lane
+//   k_apply       one wave per update leaf, eight voxels per lane: clamped
+//                 log-odds update of the visited voxels
+//                 (OccupancyVDBMapping.hpp:92-117), two float4 per lane
 //
 // Build with -ffp-contract=off: every double operation must round on its own,
 // exactly like the reference's x86-64 build.
@@ -587,145 +587,98 @@ __global__ __launch_bounds__(256) void k_map_insert(UpdView u, const uint32_t* _
 // Clamped log-odds update of one voxel (updateOccupiedNode / updateFreeNode,
 // OccupancyVDBMapping.hpp:92-117): hit -> x += l_hit, above t_max: active,
 // clamp at c_max; miss -> x += l_miss, below t_min: inactive, clamp at c_min.
+// Written with selects, no branches (the same values and states).
 __device__ __forceinline__ void update_voxel(float& x, uint32_t& ab, int k, bool visited, bool hit,
                                              float l_hit, float l_miss, float t_min, float t_max,
                                              float c_max, float c_min) {
-  if (!visited) return;
-  float v = x;
-  if (hit) {
-    v += l_hit;
-    if (v > t_max) {
-      ab |= 1u << k;
-      if (v > c_max) v = c_max;
-    }
-  } else {
-    v += l_miss;
-    if (v < t_min) {
-      ab &= ~(1u << k);
-      if (v < c_min) v = c_min;
-    }
-  }
-  x = v;
+  const float a = x + (hit ? l_hit : l_miss);
+  const bool cross = hit ? (a > t_max) : (a < t_min);  // the threshold branch is taken
+  const float cl = hit ? (a > c_max ? c_max : a) : (a < c_min ? c_min : a);
+  x = visited ? (cross ? cl : a) : x;
+  const uint32_t bit = 1u << k;
+  ab = (visited && cross) ? (hit ? (ab | bit) : (ab & ~bit)) : ab;
 }
 
 // updateMap (VDBMapping.hpp:764-783): every visited voxel of every listed
-// update leaf gets one clamped log-odds update. Sixteen lanes own one 64-voxel
-// mask word (one x-slab of the leaf, 256 contiguous bytes of values): lane l
-// updates voxels 4l .. 4l+3 with one float4 load and store and takes its
-// four visited / hit / active bits straight from the word (no cross-lane
-// mask transposes); the new active word is OR-reduced over the sixteen
-// lanes. Words with no visited voxel are skipped. Each thread carries
-// kApplyWords words per iteration, their loads issued together (list / index,
-// masks, values: three dependent steps each): the kernel is bound by the
-// bytes in flight, not by instruction issue. Optionally clears the update-set slots afterwards; counts the
+// update leaf gets one clamped log-odds update. One wave per update leaf:
+// lane l owns voxels 8l .. 8l+7 (two float4, so the wave reads and writes the
+// leaf's 2 KiB of values as one contiguous run), takes its eight visited /
+// hit / active bits from byte l & 7 of mask word l >> 3, and the eight lanes
+// of a word OR-reduce its new active word. Every word with a visited voxel is
+// written back whole (its 256 bytes have one owner, so no line is written in
+// part); nearly all of an update leaf's words have a visit, so the values load
+// beside the masks, not after them. Each wave carries kApplyLeaves leaves per
+// iteration: with few registers per lane, many waves keep many bytes in
+// flight. Optionally clears the update-set slots afterwards; counts the
 // updated voxels.
-struct ApplyWord {
-  uint32_t e, idx, slot;
-  int j;
-  uint64_t vis, hit;
-};
-
-__device__ __forceinline__ void apply_stage1(ApplyWord& w, uint64_t t, uint64_t total,
-                                             const uint32_t* __restrict__ list,
-                                             const uint32_t* __restrict__ eidx) {
-  w.vis = w.hit = 0;
-  w.slot = kNone;
-  w.idx = kNone;
-  w.e = 0;
-  w.j = 0;
-  if (t >= total) return;
-  const uint32_t ws = (uint32_t)(t >> 4);
-  w.e = ws >> 3;
-  w.j = (int)(ws & 7);
-  w.slot = list[w.e];
-  w.idx = eidx[w.e];
-}
-
-__device__ __forceinline__ void apply_stage2(ApplyWord& w, UpdView u) {
-  if (w.slot == kNone) return;
-  const unsigned long long* m = &u.masks[(size_t)w.slot * 16];
-  w.vis = m[w.j];
-  w.hit = m[8 + w.j];
-}
-
-// Clear the word's update-set slot (after every load of the iteration).
-__device__ __forceinline__ void apply_clear(const ApplyWord& w, UpdView u, int l) {
-  if (w.slot == kNone || l != 0) return;
-  unsigned long long* m = &u.masks[(size_t)w.slot * 16];
-  if (w.vis) {
-    m[w.j] = 0;
-    m[8 + w.j] = 0;
-  }
-  if (w.j == 0) u.keys[w.slot] = kEmpty;
-}
-
-// Update the lane's four voxels of word w (values x, active word aw) and store
-// them; `live`: the word has visited voxels and a map leaf.
-__device__ __forceinline__ void apply_finish(const ApplyWord& w, float4 x, uint64_t aw, bool live, MapView mp,
-                                             int l, float l_hit, float l_miss, float t_min, float t_max,
-                                             float c_max, float c_min) {
-  const uint32_t vb = (uint32_t)(w.vis >> (4 * l)) & 15u, hb = (uint32_t)(w.hit >> (4 * l)) & 15u;
-  uint32_t ab = (uint32_t)(aw >> (4 * l)) & 15u;
-  update_voxel(x.x, ab, 0, vb & 1, hb & 1, l_hit, l_miss, t_min, t_max, c_max, c_min);
-  update_voxel(x.y, ab, 1, vb & 2, hb & 2, l_hit, l_miss, t_min, t_max, c_max, c_min);
-  update_voxel(x.z, ab, 2, vb & 4, hb & 4, l_hit, l_miss, t_min, t_max, c_max, c_min);
-  update_voxel(x.w, ab, 3, vb & 8, hb & 8, l_hit, l_miss, t_min, t_max, c_max, c_min);
-  // the whole 256-byte row of a live word: unvisited voxels are written back
-  // unchanged (each row has one owner), so no cache line is written in part
-  if (live) reinterpret_cast<float4*>(mp.values)[(size_t)w.idx * 128 + w.j * 16 + l] = x;
-  uint64_t na = (uint64_t)ab << (4 * l);
-#pragma unroll
-  for (int o = 1; o < 16; o <<= 1) na |= __shfl_xor(na, o, 16);
-  if (live && l == 0) mp.active[(size_t)w.idx * 8 + w.j] = na;
-}
-
-constexpr int kApplyWords = 4;  // words in flight per thread
+constexpr int kApplyLeaves = 2;  // leaves in flight per wave
 
 __global__ __launch_bounds__(256) void k_apply(UpdView u, const uint32_t* __restrict__ list,
                                                const uint32_t* __restrict__ eidx, uint32_t count,
                                                MapView mp, float l_hit, float l_miss, float t_min,
                                                float t_max, float c_max, float c_min, Status* st,
                                                uint32_t clear) {
-  const uint64_t G = (uint64_t)gridDim.x * blockDim.x;
-  const uint64_t total = (uint64_t)count * 8 * 16;  // (record, word, lane) triples
-  const int l = lane_id() & 15;
+  const int lane = lane_id();
+  const int j = lane >> 3;        // mask word of the lane's voxels
+  const int sh = (lane & 7) * 8;  // their first bit in it
+  const uint32_t waves = gridDim.x * (blockDim.x >> 6);
   unsigned long long vox = 0;
-  // the sixteen lanes of a word group always hold the same word (G is a
-  // multiple of 16), so their loads and the active-word reduction agree
-  for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += kApplyWords * G) {
-    ApplyWord w[kApplyWords];
+  for (uint32_t e0 = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * kApplyLeaves; e0 < count;
+       e0 += waves * kApplyLeaves) {
+    uint32_t slot[kApplyLeaves], idx[kApplyLeaves];
+    uint64_t vis[kApplyLeaves], hit[kApplyLeaves], aw[kApplyLeaves];
+    float4 va[kApplyLeaves], vb4[kApplyLeaves];
 #pragma unroll
-    for (int k = 0; k < kApplyWords; ++k) apply_stage1(w[k], t + k * G, total, list, eidx);
-#pragma unroll
-    for (int k = 0; k < kApplyWords; ++k) apply_stage2(w[k], u);
-    // the whole row of every live word is read and written: whole lines and
-    // four words in flight per thread (against partial rows and two words:
-    // per-scan apply 2.31 -> 1.76 ms beside a cast, synchronous scan latency
-    // 3.18 -> 2.85 ms on the 262,144-point outdoor scans)
-    float4 x[kApplyWords];
-    uint64_t aw[kApplyWords];
-    bool live[kApplyWords];
-#pragma unroll
-    for (int k = 0; k < kApplyWords; ++k) {
-      live[k] = w[k].vis != 0 && w[k].idx != kNone;
-      x[k] = make_float4(0.f, 0.f, 0.f, 0.f);
-      aw[k] = 0;
+    for (int q = 0; q < kApplyLeaves; ++q) {
+      const uint32_t e = e0 + q;
+      slot[q] = e < count ? list[e] : kNone;
+      idx[q] = e < count ? eidx[e] : kNone;
     }
 #pragma unroll
-    for (int k = 0; k < kApplyWords; ++k)
-      if (live[k])
-        x[k] = reinterpret_cast<const float4*>(mp.values)[(size_t)w[k].idx * 128 + w[k].j * 16 + l];
-#pragma unroll
-    for (int k = 0; k < kApplyWords; ++k)
-      if (live[k]) aw[k] = mp.active[(size_t)w[k].idx * 8 + w[k].j];
-    if (clear) {
-#pragma unroll
-      for (int k = 0; k < kApplyWords; ++k) apply_clear(w[k], u, l);
+    for (int q = 0; q < kApplyLeaves; ++q) {
+      vis[q] = hit[q] = aw[q] = 0;
+      va[q] = vb4[q] = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (slot[q] != kNone) {
+        vis[q] = u.masks[(size_t)slot[q] * 16 + j];
+        hit[q] = u.masks[(size_t)slot[q] * 16 + 8 + j];
+      }
+      if (idx[q] != kNone) {
+        const float4* pv = reinterpret_cast<const float4*>(mp.values) + (size_t)idx[q] * 128 + 2 * lane;
+        va[q] = pv[0];
+        vb4[q] = pv[1];
+        aw[q] = mp.active[(size_t)idx[q] * 8 + j];
+      }
     }
 #pragma unroll
-    for (int k = 0; k < kApplyWords; ++k) {
-      if (l == 0) vox += __popcll(w[k].vis);
-      apply_finish(w[k], x[k], aw[k], live[k], mp, l, l_hit, l_miss, t_min, t_max, c_max, c_min);
+    for (int q = 0; q < kApplyLeaves; ++q) {
+      if (clear && slot[q] != kNone) {  // after the loads of the iteration
+        if ((lane & 7) == 0 && vis[q]) {
+          u.masks[(size_t)slot[q] * 16 + j] = 0;
+          u.masks[(size_t)slot[q] * 16 + 8 + j] = 0;
+        }
+        if (lane == 0) u.keys[slot[q]] = kEmpty;
+      }
+      const uint32_t v8 = (uint32_t)(vis[q] >> sh) & 0xFFu, h8 = (uint32_t)(hit[q] >> sh) & 0xFFu;
+      uint32_t ab = (uint32_t)(aw[q] >> sh) & 0xFFu;
+      vox += (unsigned long long)__popc(v8);
+      update_voxel(va[q].x, ab, 0, v8 & 1u, h8 & 1u, l_hit, l_miss, t_min, t_max, c_max, c_min);
+      update_voxel(va[q].y, ab, 1, v8 & 2u, h8 & 2u, l_hit, l_miss, t_min, t_max, c_max, c_min);
+      update_voxel(va[q].z, ab, 2, v8 & 4u, h8 & 4u, l_hit, l_miss, t_min, t_max, c_max, c_min);
+      update_voxel(va[q].w, ab, 3, v8 & 8u, h8 & 8u, l_hit, l_miss, t_min, t_max, c_max, c_min);
+      update_voxel(vb4[q].x, ab, 4, v8 & 16u, h8 & 16u, l_hit, l_miss, t_min, t_max, c_max, c_min);
+      update_voxel(vb4[q].y, ab, 5, v8 & 32u, h8 & 32u, l_hit, l_miss, t_min, t_max, c_max, c_min);
+      update_voxel(vb4[q].z, ab, 6, v8 & 64u, h8 & 64u, l_hit, l_miss, t_min, t_max, c_max, c_min);
+      update_voxel(vb4[q].w, ab, 7, v8 & 128u, h8 & 128u, l_hit, l_miss, t_min, t_max, c_max, c_min);
+      const bool live = vis[q] != 0 && idx[q] != kNone;  // the same for the word's eight lanes
+      if (live) {
+        float4* pv = reinterpret_cast<float4*>(mp.values) + (size_t)idx[q] * 128 + 2 * lane;
+        pv[0] = va[q];
+        pv[1] = vb4[q];
+      }
+      uint64_t na = (uint64_t)ab << sh;
+#pragma unroll
+      for (int o = 1; o < 8; o <<= 1) na |= __shfl_xor(na, o, 64);
+      if (live && (lane & 7) == 0) mp.active[(size_t)idx[q] * 8 + j] = na;
     }
   }
   __shared__ unsigned long long s_vox[4];
@@ -782,30 +735,30 @@ __global__ __launch_bounds__(256) void k_group_fill(const uint32_t* __restrict__
   grec[(size_t)g * kDeferMax + t] = list[i];
 }
 
-// Group apply: sixteen lanes own one 64-voxel word of a group's leaf, as in
-// k_apply; the word's values and active bits are loaded once, every scan's
-// visited / hit bits of the word are applied in scan order, and the word is
-// stored once. A lane keeps only its own four bits per scan (4 x kDeferMax
-// bits of visited and of hit flags), so each thread carries kGroupWords words
-// per iteration with their three dependent load steps (group, masks, values)
-// issued together. Frees the group's gslot entry.
-constexpr int kGroupWords = 4;
-static_assert(kDeferMax == 8, "a group's record positions are read as two uint4");
+// Group apply: one wave per group (map leaf), lane l owning voxels 8l ..
+// 8l+7 as in k_apply. The leaf's values and active words are loaded once
+// (beside the masks), every scan of the group with a record of the leaf gives
+// the lane's visited / hit byte (byte l & 7 of mask word l >> 3), the updates
+// are applied in scan order, visits only, and the leaf is stored once. Each
+// wave carries kGroupLeaves groups per iteration. Frees the group's gslot
+// entry.
+constexpr int kGroupLeaves = 2;
+static_assert(kDeferMax == 8, "a group's record positions are read as two uint4; one byte per scan");
 
-// Every scan's update of voxel k of a lane (bits 4s + k of vb / hb: visited /
+// Every scan's update of voxel i of a lane (bit 8s + i of vb / hb: visited /
 // hit in scan s), in scan order, visits only: one iteration per visit, the
-// clamped update without branches (the same results as update_voxel).
-__device__ __forceinline__ void apply_visits(float& x, uint32_t& ab, int k, uint32_t vb, uint32_t hb,
+// clamped update of update_voxel.
+__device__ __forceinline__ void apply_visits(float& x, uint32_t& ab, int i, uint64_t vb, uint64_t hb,
                                              float l_hit, float l_miss, float t_min, float t_max,
                                              float c_max, float c_min) {
-  uint32_t m = (vb >> k) & 0x11111111u;
+  uint64_t m = (vb >> i) & 0x0101010101010101ull;
   if (!__any(m != 0)) return;
   float v = x;
-  uint32_t on = (ab >> k) & 1u;
+  uint32_t on = (ab >> i) & 1u;
   while (m) {
-    const uint32_t b = (uint32_t)__builtin_ctz(m);
+    const uint32_t b = (uint32_t)__builtin_ctzll(m);
     m &= m - 1;
-    const bool h = (hb >> (b + k)) & 1u;
+    const bool h = (hb >> (b + i)) & 1u;
     const float a = v + (h ? l_hit : l_miss);
     // hit: above t_max -> active, clamped at c_max; miss: below t_min -> inactive, clamped at c_min
     const bool cross = h ? (a > t_max) : (a < t_min);
@@ -814,93 +767,93 @@ __device__ __forceinline__ void apply_visits(float& x, uint32_t& ab, int k, uint
     on = cross ? (h ? 1u : 0u) : on;
   }
   x = v;
-  ab = (ab & ~(1u << k)) | (on << k);
+  ab = (ab & ~(1u << i)) | (on << i);
 }
-
-struct GroupWord {
-  uint32_t e, g;
-  int j;
-  uint32_t slot[kDeferMax];
-  uint32_t vb, hb;  // this lane's 4 bits of scan k at bits 4k .. 4k+3
-  uint64_t any;     // visited voxels of the word over all scans
-};
 
 __global__ __launch_bounds__(256) void k_apply_groups(GroupScans gs, const uint32_t* __restrict__ glist,
                                                       const uint32_t* __restrict__ grec,
                                                       const unsigned int* __restrict__ ngroups, uint32_t* gslot,
                                                       MapView mp, float l_hit, float l_miss, float t_min,
                                                       float t_max, float c_max, float c_min, Status* st) {
-  const uint64_t G = (uint64_t)gridDim.x * blockDim.x;
-  const uint64_t total = (uint64_t)(*ngroups) * 8 * 16;  // (group, word, lane) triples
-  const int l = lane_id() & 15;
+  const uint32_t n = *ngroups;
+  const int lane = lane_id();
+  const int j = lane >> 3;        // mask word of the lane's voxels
+  const int sh = (lane & 7) * 8;  // their first bit in it
+  const uint32_t waves = gridDim.x * (blockDim.x >> 6);
   unsigned long long vox = 0;
-  // the sixteen lanes of a word group always hold the same word (G is a
-  // multiple of 16), so their loads and the active-word reduction agree
-  for (uint64_t t0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t0 < total; t0 += kGroupWords * G) {
-    GroupWord w[kGroupWords];
+  for (uint32_t g0 = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * kGroupLeaves; g0 < n;
+       g0 += waves * kGroupLeaves) {
+    uint32_t e[kGroupLeaves], slot[kGroupLeaves][kDeferMax];
 #pragma unroll
-    for (int q = 0; q < kGroupWords; ++q) {  // step 1: the group
-      const uint64_t t = t0 + q * G;
-      w[q].e = kNone;
-      w[q].vb = w[q].hb = 0;
-      w[q].any = 0;
-      w[q].j = (int)((t >> 4) & 7);
-      w[q].g = (uint32_t)(t >> 7);
-      if (t < total) {
-        w[q].e = glist[w[q].g];
-        const uint4* r = reinterpret_cast<const uint4*>(grec + (size_t)w[q].g * kDeferMax);
+    for (int q = 0; q < kGroupLeaves; ++q) {  // the group: leaf and record positions
+      const uint32_t g = g0 + q;
+      e[q] = kNone;
+#pragma unroll
+      for (int k = 0; k < kDeferMax; ++k) slot[q][k] = kNone;
+      if (g < n) {  // the same for every lane: kept in scalar registers
+        e[q] = __builtin_amdgcn_readfirstlane(glist[g]);
+        const uint4* r = reinterpret_cast<const uint4*>(grec + (size_t)g * kDeferMax);
         const uint4 a = r[0], b = r[1];
-        w[q].slot[0] = a.x, w[q].slot[1] = a.y, w[q].slot[2] = a.z, w[q].slot[3] = a.w;
-        w[q].slot[4] = b.x, w[q].slot[5] = b.y, w[q].slot[6] = b.z, w[q].slot[7] = b.w;
-      } else {
-#pragma unroll
-        for (int k = 0; k < kDeferMax; ++k) w[q].slot[k] = kNone;
+        slot[q][0] = __builtin_amdgcn_readfirstlane(a.x), slot[q][1] = __builtin_amdgcn_readfirstlane(a.y);
+        slot[q][2] = __builtin_amdgcn_readfirstlane(a.z), slot[q][3] = __builtin_amdgcn_readfirstlane(a.w);
+        slot[q][4] = __builtin_amdgcn_readfirstlane(b.x), slot[q][5] = __builtin_amdgcn_readfirstlane(b.y);
+        slot[q][6] = __builtin_amdgcn_readfirstlane(b.z), slot[q][7] = __builtin_amdgcn_readfirstlane(b.w);
       }
     }
+    uint64_t vb[kGroupLeaves], hb[kGroupLeaves], aw[kGroupLeaves];
+    float4 va[kGroupLeaves], vc[kGroupLeaves];
 #pragma unroll
-    for (int q = 0; q < kGroupWords; ++q) {  // step 2: every scan's masks of the word
+    for (int q = 0; q < kGroupLeaves; ++q) {  // every scan's masks, and the leaf beside them
+      vb[q] = hb[q] = 0;
 #pragma unroll
       for (int k = 0; k < kDeferMax; ++k) {
-        // a wave holds four words of one group: the test is uniform (a scalar branch)
-        if (k < gs.nscan && __builtin_amdgcn_readfirstlane(w[q].slot[k]) != kNone) {
-          const unsigned long long* m = gs.masks[k] + (size_t)w[q].slot[k] * 16;
-          const uint64_t v = m[w[q].j], h = m[8 + w[q].j];
-          w[q].any |= v;
-          w[q].vb |= ((uint32_t)(v >> (4 * l)) & 15u) << (4 * k);
-          w[q].hb |= ((uint32_t)(h >> (4 * l)) & 15u) << (4 * k);
-          if (l == 0) vox += __popcll(v);
+        // the wave holds one group: a scalar branch. The lane's byte is in one
+        // 32-bit half of the mask word: only that half is loaded.
+        if (k < gs.nscan && slot[q][k] != kNone) {
+          const uint32_t* m = reinterpret_cast<const uint32_t*>(gs.masks[k] + (size_t)slot[q][k] * 16);
+          const uint32_t v = m[2 * j + (sh >> 5)], h = m[16 + 2 * j + (sh >> 5)];
+          vb[q] |= (uint64_t)((v >> (sh & 31)) & 0xFFu) << (8 * k);
+          hb[q] |= (uint64_t)((h >> (sh & 31)) & 0xFFu) << (8 * k);
         }
       }
-    }
-    float4 x[kGroupWords];
-    uint64_t aw[kGroupWords];
-#pragma unroll
-    for (int q = 0; q < kGroupWords; ++q) {  // step 3: the leaf word, once
-      x[q] = make_float4(0.f, 0.f, 0.f, 0.f);
+      va[q] = vc[q] = make_float4(0.f, 0.f, 0.f, 0.f);
       aw[q] = 0;
-      if (w[q].any) {
-        uint32_t mine = 0;
-#pragma unroll
-        for (int k = 0; k < kDeferMax; ++k) mine |= (w[q].vb >> (4 * k)) & 15u;
-        if (mine) x[q] = reinterpret_cast<const float4*>(mp.values)[(size_t)w[q].e * 128 + w[q].j * 16 + l];
-        aw[q] = mp.active[(size_t)w[q].e * 8 + w[q].j];
+      if (e[q] != kNone) {
+        const float4* pv = reinterpret_cast<const float4*>(mp.values) + (size_t)e[q] * 128 + 2 * lane;
+        va[q] = pv[0];
+        vc[q] = pv[1];
+        aw[q] = mp.active[(size_t)e[q] * 8 + j];
       }
     }
 #pragma unroll
-    for (int q = 0; q < kGroupWords; ++q) {
-      if (l == 0 && w[q].j == 0 && w[q].e != kNone) gslot[w[q].e] = kNone;
-      if (!w[q].any) continue;  // the sixteen lanes of the word agree
-      uint32_t ab = (uint32_t)(aw[q] >> (4 * l)) & 15u;
-      // only the visits: voxel k's scans are the bits 4s + k of vb (scan order)
-      apply_visits(x[q].x, ab, 0, w[q].vb, w[q].hb, l_hit, l_miss, t_min, t_max, c_max, c_min);
-      apply_visits(x[q].y, ab, 1, w[q].vb, w[q].hb, l_hit, l_miss, t_min, t_max, c_max, c_min);
-      apply_visits(x[q].z, ab, 2, w[q].vb, w[q].hb, l_hit, l_miss, t_min, t_max, c_max, c_min);
-      apply_visits(x[q].w, ab, 3, w[q].vb, w[q].hb, l_hit, l_miss, t_min, t_max, c_max, c_min);
-      if (w[q].vb) reinterpret_cast<float4*>(mp.values)[(size_t)w[q].e * 128 + w[q].j * 16 + l] = x[q];
-      uint64_t na = (uint64_t)ab << (4 * l);
+    for (int q = 0; q < kGroupLeaves; ++q) {
+      if (e[q] == kNone) continue;  // the whole wave
+      if (lane == 0) gslot[e[q]] = kNone;
+      vox += (unsigned long long)__popcll(vb[q]);
+      uint32_t ab = (uint32_t)(aw[q] >> sh) & 0xFFu;
+      apply_visits(va[q].x, ab, 0, vb[q], hb[q], l_hit, l_miss, t_min, t_max, c_max, c_min);
+      apply_visits(va[q].y, ab, 1, vb[q], hb[q], l_hit, l_miss, t_min, t_max, c_max, c_min);
+      apply_visits(va[q].z, ab, 2, vb[q], hb[q], l_hit, l_miss, t_min, t_max, c_max, c_min);
+      apply_visits(va[q].w, ab, 3, vb[q], hb[q], l_hit, l_miss, t_min, t_max, c_max, c_min);
+      apply_visits(vc[q].x, ab, 4, vb[q], hb[q], l_hit, l_miss, t_min, t_max, c_max, c_min);
+      apply_visits(vc[q].y, ab, 5, vb[q], hb[q], l_hit, l_miss, t_min, t_max, c_max, c_min);
+      apply_visits(vc[q].z, ab, 6, vb[q], hb[q], l_hit, l_miss, t_min, t_max, c_max, c_min);
+      apply_visits(vc[q].w, ab, 7, vb[q], hb[q], l_hit, l_miss, t_min, t_max, c_max, c_min);
+      // every word with a visit in some scan is written back whole (one owner per
+      // row): the eight lanes of the word agree on it through the reduction
+      uint64_t na = (uint64_t)ab << sh;
+      uint32_t wany = vb[q] != 0 ? 1u : 0u;
 #pragma unroll
-      for (int o = 1; o < 16; o <<= 1) na |= __shfl_xor(na, o, 16);
-      if (l == 0) mp.active[(size_t)w[q].e * 8 + w[q].j] = na;
+      for (int o = 1; o < 8; o <<= 1) {
+        na |= __shfl_xor(na, o, 64);
+        wany |= __shfl_xor(wany, o, 64);
+      }
+      if (wany) {
+        float4* pv = reinterpret_cast<float4*>(mp.values) + (size_t)e[q] * 128 + 2 * lane;
+        pv[0] = va[q];
+        pv[1] = vc[q];
+        if ((lane & 7) == 0) mp.active[(size_t)e[q] * 8 + j] = na;
+      }
     }
   }
   __shared__ unsigned long long s_vox[4];
