@@ -12,15 +12,18 @@
   digests.json        SHA-256 digests of the oracle's map export after five
                       32,768-ray room scans (res 0.1, R 15), of the update
                       set of the first scan's first 4,096 rays, of the 16
-                      maps of the multi-robot serving test (0.05 m) and of
+                      maps of the multi-robot serving test (0.05 m), of
                       the map after the first 24 outdoor-stream scans
-                      (262,144 rays each, 0.05 m).
+                      (262,144 rays each, 0.05 m), of the map after all 64
+                      mixed-size scans, and of the RGB-D workload end to
+                      end (200 frames at 0.02 m, fast-mode map, 1,000,000
+                      raytrace queries).
 
 The fixtures are committed; the tests compare both the oracle (drift check)
 and the GPU library (parity without running the oracle) with them. Re-run
 this script only when the restated semantics change on purpose:
     python tests/golden/make_golden.py               # everything (minutes)
-    python tests/golden/make_golden.py serve outdoor # only those digests
+    python tests/golden/make_golden.py serve outdoor mixed rgbd  # only those
 """
 from __future__ import annotations
 
@@ -178,6 +181,69 @@ def outdoor_digest():
             "sha256": digest(o, v.view(np.uint32), a)}
 
 
+MIXED_SCANS = 64
+
+
+def bitcount(words) -> int:
+    return int(np.unpackbits(np.ascontiguousarray(words).view(np.uint8)).sum())
+
+
+def mixed_digest():
+    """Map digest after all MIXED_SCANS scans of the mixed-size batch workload
+    inserted in order, each with its own max range (one batched insert on the
+    device applies them in the same order)."""
+    w = W.mixed_batch_workload(MIXED_SCANS)
+    m = O.OracleMap(w.resolution)
+    m.set_config(O.make_config(max_range=w.max_range, **W.PROBS))
+    for i in range(MIXED_SCANS):
+        s = w.scan(i)
+        m.add_input_source("stream", s.max_range)
+        assert m.insert("stream", s.points, s.origin) == O.OK
+    print("mixed scans done", flush=True)
+    o, v, a = m.export_leaves()
+    return {"workload": "mixed-size batch, scans 0-63 (10k-300k points, ranges U[20, 50] m) inserted in "
+                        "order, res 0.05, p 0.7/0.4/0.12/0.97",
+            "scans": MIXED_SCANS, "rays": int(w.counts.sum()), "leaves": int(o.shape[0]),
+            "active_voxels": bitcount(a), "sha256": digest(o, v.view(np.uint32), a)}
+
+
+RGBD_FRAMES = 200
+RGBD_QUERIES = 1_000_000
+
+
+def rgbd_digest():
+    """The RGB-D workload end to end: all RGBD_FRAMES depth frames inserted in
+    order (map digest), then fast mode switched on and integrated (map digest)
+    and RGBD_QUERIES raytrace queries (digest of the success flags and the end
+    points' float64 bit patterns)."""
+    w = W.rgbd_workload(RGBD_FRAMES)
+    m = O.OracleMap(w.resolution)
+    m.set_config(O.make_config(max_range=w.max_range, **W.PROBS))
+    m.add_input_source("stream", w.max_range)
+    for i in range(RGBD_FRAMES):
+        s = w.scan(i)
+        assert m.insert("stream", s.points, s.origin) == O.OK
+        if i % 20 == 19:
+            print("rgbd frame", i, flush=True)
+    o, v, a = m.export_leaves()
+    out = {"workload": "rgbd 640 x 480 frames 0-199 inserted in order, res 0.02, R 5, p 0.7/0.4/0.12/0.97; "
+                       "then fast mode, integrate, 1,000,000 raytrace queries of 5 m",
+           "frames": RGBD_FRAMES, "queries": RGBD_QUERIES,
+           "leaves": int(o.shape[0]), "active_voxels": bitcount(a),
+           "sha256": digest(o, v.view(np.uint32), a)}
+    assert m.set_config(O.make_config(max_range=w.max_range, fast_mode=True, **W.PROBS)) == O.OK
+    assert m.integrate() == O.OK
+    o, v, a = m.export_leaves()
+    out.update(fast_leaves=int(o.shape[0]), fast_active_voxels=bitcount(a),
+               fast_sha256=digest(o, v.view(np.uint32), a))
+    q_o, q_d, q_l = W.raytrace_queries(RGBD_QUERIES)
+    st, succ, ends = m.raytrace(q_o, q_d, q_l)
+    assert st == O.OK
+    out.update(raytrace_hits=int(succ.sum()),
+               raytrace_sha256=digest(succ.astype(np.uint8), np.ascontiguousarray(ends, np.float64).view(np.uint64)))
+    return out
+
+
 def update_digests(**entries):
     path = os.path.join(HERE, "digests.json")
     with open(path) as f:
@@ -188,13 +254,17 @@ def update_digests(**entries):
 
 
 def main():
-    if len(sys.argv) > 1:  # only the named large digests: "serve", "outdoor"
+    if len(sys.argv) > 1:  # only the named large digests: serve, outdoor, mixed, rgbd
         O.build()
         W.build()
         if "serve" in sys.argv[1:]:
             update_digests(multi_robot_serve=multi_robot_digests())
         if "outdoor" in sys.argv[1:]:
             update_digests(outdoor_stream_24=outdoor_digest())
+        if "mixed" in sys.argv[1:]:
+            update_digests(mixed_batch_64=mixed_digest())
+        if "rgbd" in sys.argv[1:]:
+            update_digests(rgbd_200=rgbd_digest())
         return
     O.build()
     W.build()
@@ -263,6 +333,8 @@ def main():
            "room_scan_first_4096_update": upd}
     res["multi_robot_serve"] = multi_robot_digests()
     res["outdoor_stream_24"] = outdoor_digest()
+    res["mixed_batch_64"] = mixed_digest()
+    res["rgbd_200"] = rgbd_digest()
     with open(os.path.join(HERE, "digests.json"), "w") as f:
         json.dump(res, f, indent=1)
     print("wrote", sorted(os.listdir(HERE)))

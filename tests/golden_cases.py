@@ -32,7 +32,8 @@ def digest(*arrays) -> str:
 
 
 def popcount(words) -> int:
-    return int(sum(bin(int(x)).count("1") for x in np.asarray(words).ravel()))
+    """Set bits of an integer mask array (any width)."""
+    return int(np.unpackbits(np.ascontiguousarray(words).view(np.uint8)).sum())
 
 
 def voxel_of(origins, values, masks, ijk):

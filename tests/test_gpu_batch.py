@@ -9,6 +9,7 @@ import pytest
 import oracle as O
 import vdb_mapping_amd as V
 import parity
+from golden_cases import digest, digests, popcount
 from parity import same_map
 from vdb_mapping_amd import workloads as W
 
@@ -158,6 +159,26 @@ def test_mixed_batch_workload_subset(pair):
     g.insert_batch("test", clouds, origins, ranges, check=True)
     oracle_sequential(o, "test", clouds, origins, ranges)
     assert same_map(g, o) > 1000
+
+
+def test_mixed_batch_workload_full_matches_oracle_digest(traversal):
+    """All 64 scans of the mixed-size batch workload (10k-300k points, ranges
+    U[20, 50] m, 5.8 M rays) in one batched insert: the map equals the
+    oracle's after inserting them in order (committed digest)."""
+    dg = digests()["mixed_batch_64"]
+    w = W.mixed_batch_workload(dg["scans"])
+    scans = [w.scan(i) for i in range(dg["scans"])]
+    assert sum(s.points.shape[0] for s in scans) == dg["rays"]
+    g = V.GpuMap(w.resolution)
+    assert g.set_config(V.make_config(max_range=w.max_range, **W.PROBS)) == V.OK
+    assert g.set_tuning(traversal=traversal) == V.OK
+    g.add_input_source("lidar", w.max_range)
+    g.insert_batch("lidar", [s.points for s in scans], np.array([s.origin for s in scans]),
+                   np.array([s.max_range for s in scans]), check=True)
+    o, v, m = g.export_leaves()
+    assert (o.shape[0], popcount(m)) == (dg["leaves"], dg["active_voxels"])
+    assert digest(o, v.view(np.uint32), m) == dg["sha256"]
+    g.close()
 
 
 def test_batch_device_resident(pair):

@@ -5,6 +5,7 @@ import pytest
 
 import oracle as O
 import vdb_mapping_amd as V
+from golden_cases import digest, digests, popcount
 from parity import same_update, both, pair, same_map
 from vdb_mapping_amd import workloads as W
 
@@ -84,6 +85,36 @@ def test_raytrace_rgbd_frames():
     org, dirs, lens = W.raytrace_queries(20000)
     frac = same_raytrace(g, o, org, dirs, lens)
     assert frac > 0.3
+
+
+def test_rgbd_workload_full_matches_oracle_digest():
+    """The RGB-D workload end to end at full size: 200 frames of 640 x 480
+    at 0.02 m inserted in batches of 16, fast mode switched on and integrated,
+    then 1,000,000 raytrace queries; the map before and after the switch and
+    the query results equal the oracle's (committed digests)."""
+    dg = digests()["rgbd_200"]
+    w = W.rgbd_workload(dg["frames"])
+    g = V.GpuMap(w.resolution)
+    assert g.set_config(V.make_config(max_range=w.max_range, **W.PROBS)) == V.OK
+    g.add_input_source("lidar", w.max_range)
+    for j in range(0, dg["frames"], 16):
+        scans = [w.scan(i) for i in range(j, min(j + 16, dg["frames"]))]
+        g.insert_batch("lidar", [s.points for s in scans], np.array([s.origin for s in scans]), check=True)
+    o, v, m = g.export_leaves()
+    assert (o.shape[0], popcount(m)) == (dg["leaves"], dg["active_voxels"])
+    assert digest(o, v.view(np.uint32), m) == dg["sha256"]
+    assert g.set_config(V.make_config(max_range=w.max_range, fast_mode=True, **W.PROBS)) == V.OK
+    g.integrate(check=True)
+    o, v, m = g.export_leaves()
+    assert (o.shape[0], popcount(m)) == (dg["fast_leaves"], dg["fast_active_voxels"])
+    assert digest(o, v.view(np.uint32), m) == dg["fast_sha256"]
+    org, dirs, lens = W.raytrace_queries(dg["queries"])
+    st, ok, ends = g.raytrace(org, dirs, lens)
+    assert st == V.OK
+    assert int(ok.sum()) == dg["raytrace_hits"]
+    assert digest(ok.astype(np.uint8), np.ascontiguousarray(ends, np.float64).view(np.uint64)) == \
+        dg["raytrace_sha256"]
+    g.close()
 
 
 def test_raytrace_device_arrays():
