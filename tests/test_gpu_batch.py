@@ -161,6 +161,26 @@ def test_mixed_batch_workload_subset(pair):
     assert same_map(g, o) > 1000
 
 
+def test_batch_tables_grow_between_scans(pair):
+    """Small scans, then scans of long rays in the same batch: on the streamed
+    path a later scan's cast is already enqueued when an earlier scan's brick
+    tables or record set overflow, so the earlier scan is cast again from its
+    points; the map equals sequential inserts."""
+    rng = np.random.default_rng(11)
+    clouds, origins = [], []
+    for n, r in ((40, 0.8), (20000, 40.0), (30, 0.5), (30000, 45.0), (25000, 45.0)):
+        org = rng.uniform(-1, 1, 3)
+        d = rng.normal(size=(n, 3))
+        d /= np.linalg.norm(d, axis=1, keepdims=True)
+        pts = (org + d * rng.uniform(0.2, 1.0, (n, 1)) * r).astype(np.float32)
+        clouds.append(np.concatenate([pts, np.ones((n, 1), np.float32)], 1))
+        origins.append(org)
+    g, o = pair(0.05, max_range=50.0)
+    g.insert_batch("test", clouds, np.array(origins), None, check=True)
+    oracle_sequential(o, "test", clouds, origins, [50.0] * len(clouds))
+    assert same_map(g, o) > 10000
+
+
 def test_mixed_batch_workload_full_matches_oracle_digest(traversal):
     """All 64 scans of the mixed-size batch workload (10k-300k points, ranges
     U[20, 50] m, 5.8 M rays) in one batched insert: the map equals the

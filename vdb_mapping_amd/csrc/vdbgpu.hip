@@ -242,6 +242,12 @@ struct vdbgpu_map {
   // status: [0] scan, [1] map (leaf_count), [2] auxiliary, [3] cumulative counters
   Status* d_st = nullptr;
   Status* h_st = nullptr;
+  // brick-path casts whose status the host has not read: two host slots
+  Status* h_cst = nullptr;
+  hipEvent_t cst_done[2] = {};
+  int cst_next = 0;
+  uint64_t cast_seq = 0;    // brick casts enqueued (re-runs included)
+  uint64_t cast_epoch = 0;  // table overflows found (each clears the split-brick bitmaps)
 
   vdbgpu_tuning tuning{256, 32, 16, 0.5, 0, kDeferMax};
   bool profiling = false;
@@ -1179,21 +1185,26 @@ int flush_pending(vdbgpu_map* m) {
   return VDBGPU_OK;
 }
 
-// insertPointCloud of one scan when no source holds pending updates: the
-// brick path casts the scan into leaf records and applies them at once
-// (equal to accumulateUpdate + integrateUpdate). Table overflows grow the
-// table and re-run the brick stages (nothing is applied before the check).
-int run_brick(vdbgpu_map* m, Source& src, const float* xyz, int64_t n, int64_t stride,
-              const double origin[3], bool on_device) {
+// A brick-path scan whose cast is enqueued on the cast stream: what a re-run
+// needs, its record set and the host slot its status is copied to.
+struct BrickCast {
+  const float* xyz = nullptr;
+  int64_t n = 0, stride = 0;
+  bool on_device = false;
   ScanInfo sc{};
-  for (int a = 0; a < 3; ++a) sc.o[a] = origin[a];
-  sc.R = src.max_range;
-  sc.ray0 = 0;
-  sc.nrays = (uint32_t)n;
+  int X = 0;     // record set
+  int slot = 0;  // host status slot: m->h_cst[2 slot] (scan), [2 slot + 1] (map snapshot)
+  uint32_t cap = 0;    // capacity of set X when the records were copied into it
+  uint64_t seq = 0;    // m->cast_seq of its latest enqueue
+  uint64_t epoch = 0;  // m->cast_epoch then
+};
+
+// Tables of the brick path sized for a scan of n rays of range R (the first
+// scan: from its visit bound; the per-block segment tables: for its blocks).
+int brick_tables(vdbgpu_map* m, int64_t n, double R) {
   int r;
-  hipStream_t cs = m->cast;
   if (m->bcap == 0) {  // first brick-path scan: tables sized from the scan's visit bound
-    const double ext = std::min<double>(std::ceil(sc.R * m->inv) + 2.0, (double)kMaxTableN);
+    const double ext = std::min<double>(std::ceil(R * m->inv) + 2.0, (double)kMaxTableN);
     const uint64_t vb = (uint64_t)n * (1 + 3 * (uint64_t)ext);
     const uint64_t segs0 = std::min<uint64_t>(vb, 16ull * (uint64_t)n);
     if ((r = ensure_bricks(m, (uint32_t)std::max<uint64_t>(4096, next_pow2((uint64_t)n / 2)),
@@ -1210,96 +1221,169 @@ int run_brick(vdbgpu_map* m, Source& src, const float* xyz, int64_t n, int64_t s
     CK(dalloc(m, &m->wkey, m->wtab_cap));
     CK(dalloc(m, &m->wsc, m->wtab_cap));
   }
+  return VDBGPU_OK;
+}
+
+// Enqueues the cast of c on the cast stream: staging (prep, brick table
+// cleared), brick cut, walk, record list and the copy into record set c.X,
+// then the copy of its status (and a snapshot of the map's leaf count) into
+// host slot c.slot. A first attempt takes the next record set of the ring and
+// waits until the map stream has applied the scan that used it last; a re-run
+// (attempt > 0, tables grown) stages the scan again, since a later cast may
+// have used the shared tables meanwhile.
+int brick_enqueue(vdbgpu_map* m, BrickCast& c, int attempt) {
+  int r;
+  hipStream_t cs = m->cast;
+  const int nb = (int)((c.n + kPrepBlock - 1) / kPrepBlock);
   uint64_t vb = 0;
-  if ((r = stage_scans(m, cs, xyz, stride, on_device, {sc}, (uint32_t)n, false, &vb, true))) return r;
-  // record set X: free once the map stream has applied the scan that last used it
-  const int X = m->rec_cur;
-  CK(hipStreamWaitEvent(cs, m->rec_free[X], 0));
-  if (m->dcap[X] < m->dense_hint) {  // a first use of set X: size it like the others
-    CK(hipStreamSynchronize(cs));    // (the map stream is done with it)
-    if ((r = grow_set(m, X, m->dense_hint))) return r;
-    // the ring's sets never used yet are sized now as well, so the next
-    // scans of the ring do not each stop the cast stream for an allocation
-    for (int y = 0; y < m->rec_sets; ++y)
-      if (m->dcap[y] == 0 && (r = grow_set(m, y, m->dense_hint))) return r;
+  if ((r = stage_scans(m, cs, c.xyz, c.stride, c.on_device, {c.sc}, (uint32_t)c.n, false, &vb, true))) return r;
+  if (attempt == 0) {
+    // record set X: free once the map stream has applied the scan that last used it
+    c.X = m->rec_cur;
+    m->rec_cur = (c.X + 1) % m->rec_sets;
+    c.slot = m->cst_next;
+    m->cst_next ^= 1;
+    CK(hipStreamWaitEvent(cs, m->rec_free[c.X], 0));
+    if (m->dcap[c.X] < m->dense_hint) {  // a first use of set X: size it like the others
+      CK(hipStreamSynchronize(cs));      // (the map stream is done with it)
+      if ((r = grow_set(m, c.X, m->dense_hint))) return r;
+      // the ring's sets never used yet are sized now as well, so the next
+      // scans of the ring do not each stop the cast stream for an allocation
+      for (int y = 0; y < m->rec_sets; ++y)
+        if (m->dcap[y] == 0 && (r = grow_set(m, y, m->dense_hint))) return r;
+    }
   }
-  for (int attempt = 0;; ++attempt) {
-    const BrickView bv{m->bkeys, m->bcnt, m->bcnt + m->bcap, m->bcap - 1};
-    const UpdView rec{m->rkeys, m->rmasks, 0};
-    if (attempt > 0) {  // the first attempt's table was cleared with the staging
-      CK(launch_fill_u64(cs, m->bkeys, m->bcap, kEmpty));
-      CK(launch_fill_u64(cs, reinterpret_cast<unsigned long long*>(m->bcnt), m->bcap, 0));
-    }
-    {
-      Timer t(m, kTraverse, cs);
-      CK(launch_seg_count(cs, nb, m->rayinfo, m->d_scans, m->d_ctl, m->tbl, bv, m->wkey, m->wsc, &m->d_st[0]));
-      CK(launch_brick_offsets(cs, bv, m->bbsum, m->bboff, m->items, m->seg_cap, m->item_cap,
-                              m->split_slot, m->split_cap, &m->d_st[0]));
-      CK(launch_seg_fill(cs, nb, m->rayinfo, m->d_scans, m->d_ctl, m->tbl, bv, m->wkey, m->wsc, m->segs,
-                         m->seg_cap, &m->d_st[0]));
-    }
-    {
-      Timer t(m, kWalk, cs);
-      CK(launch_brick_walk(cs, m->num_cu * 32, m->items, m->segs, m->rayinfo, m->d_scans, m->tbl, bv,
-                           m->shared_bits, rec, m->leaf_mask, m->rec_cap, &m->d_st[0]));
-    }
-    {
-      Timer t(m, kCompact, cs);
-      CK(launch_split_emit(cs, m->num_cu, m->split_slot, m->shared_bits, bv, rec, m->leaf_mask,
-                           m->rec_cap, &m->d_st[0]));
-      CK(launch_rec_list(cs, m->leaf_mask, m->lbsum, m->lboff, m->group_cap / 256, m->rlist, &m->d_st[0]));
-      CK(launch_rec_gather(cs, m->num_cu * 8, m->rlist, rec, UpdView{m->dkeys[X], m->dmasks[X], 0}, m->dcap[X],
-                           &m->d_st[0]));
-    }
-    // the previous scan's map-stream work and a full group of deferred scans:
-    // enqueued now, while this cast runs
-    if (attempt == 0 && ((r = commit_staged(m)) || (m->npend >= m->tuning.defer && (r = flush_pending(m)))))
-      return r;
-    // the scan's status and a snapshot of the leaf count: the map stream keeps
-    // applying the previous scan meanwhile
-    CK(hipMemcpyAsync(m->h_st, m->d_st, 2 * sizeof(Status), hipMemcpyDeviceToHost, cs));
-    CK(hipStreamSynchronize(cs));
+  const BrickView bv{m->bkeys, m->bcnt, m->bcnt + m->bcap, m->bcap - 1};
+  const UpdView rec{m->rkeys, m->rmasks, 0};
+  {
+    Timer t(m, kTraverse, cs);
+    CK(launch_seg_count(cs, nb, m->rayinfo, m->d_scans, m->d_ctl, m->tbl, bv, m->wkey, m->wsc, &m->d_st[0]));
+    CK(launch_brick_offsets(cs, bv, m->bbsum, m->bboff, m->items, m->seg_cap, m->item_cap,
+                            m->split_slot, m->split_cap, &m->d_st[0]));
+    CK(launch_seg_fill(cs, nb, m->rayinfo, m->d_scans, m->d_ctl, m->tbl, bv, m->wkey, m->wsc, m->segs,
+                       m->seg_cap, &m->d_st[0]));
+  }
+  {
+    Timer t(m, kWalk, cs);
+    CK(launch_brick_walk(cs, m->num_cu * 32, m->items, m->segs, m->rayinfo, m->d_scans, m->tbl, bv,
+                         m->shared_bits, rec, m->leaf_mask, m->rec_cap, &m->d_st[0]));
+  }
+  {
+    Timer t(m, kCompact, cs);
+    CK(launch_split_emit(cs, m->num_cu, m->split_slot, m->shared_bits, bv, rec, m->leaf_mask,
+                         m->rec_cap, &m->d_st[0]));
+    CK(launch_rec_list(cs, m->leaf_mask, m->lbsum, m->lboff, m->group_cap / 256, m->rlist, &m->d_st[0]));
+    c.cap = m->dcap[c.X];  // (a later first use may size an empty set X before this is checked)
+    CK(launch_rec_gather(cs, m->num_cu * 8, m->rlist, rec, UpdView{m->dkeys[c.X], m->dmasks[c.X], 0},
+                         c.cap, &m->d_st[0]));
+  }
+  // the scan's status and a snapshot of the leaf count: the map stream keeps
+  // applying earlier scans meanwhile
+  CK(hipMemcpyAsync(m->h_cst + 2 * c.slot, m->d_st, 2 * sizeof(Status), hipMemcpyDeviceToHost, cs));
+  CK(hipEventRecord(m->cst_done[c.slot], cs));
+  c.seq = ++m->cast_seq;
+  c.epoch = m->cast_epoch;
+  return VDBGPU_OK;
+}
+
+// Waits for the status of the cast of c and checks it: errors fail the scan,
+// table overflows grow the tables and cast the scan again (nothing of it has
+// been applied), a record set too small grows and takes the records again.
+// A cast enqueued before an overflow of an earlier cast was found started
+// from split-brick bitmaps holding that cast's partial ORs: it runs again.
+// When another cast has run since c's, the shared tables no longer hold c's
+// records, so a set too small also means casting again. A scan with records
+// is left staged for the map stream (commit_staged).
+int brick_check(vdbgpu_map* m, BrickCast& c) {
+  int r;
+  const int X = c.X;
+  for (int attempt = 1;; ++attempt) {
+    CK(hipEventSynchronize(m->cst_done[c.slot]));
     collect_timing(m);
-    m->leaf_count_ub = leaf_bound(m, m->h_st[1].leaf_count, m->rec_seq[X]);
-    if (m->h_st[1].map_full)
+    if (attempt > 10) return fail(m, VDBGPU_DEVICE_ERROR, "brick tables cannot grow");
+    if (c.epoch != m->cast_epoch) {
+      if ((r = brick_enqueue(m, c, attempt))) return r;
+      continue;
+    }
+    const bool later = c.seq != m->cast_seq;
+    const Status* h = m->h_cst + 2 * c.slot;
+    m->leaf_count_ub = leaf_bound(m, h[1].leaf_count, m->rec_seq[X]);
+    if (h[1].map_full)
       return fail(m, VDBGPU_DEVICE_ERROR, "leaf pool overflow in a map insert (internal bound violated)");
-    const Status st = m->h_st[0];
+    const Status st = h[0];
     if ((r = scan_error(m, st, false))) return r;
-    if (st.overflow == 0) break;
-    if (attempt > 8) return fail(m, VDBGPU_DEVICE_ERROR, "brick tables cannot grow");
-    const uint32_t bcap = (st.overflow & kOvfBricks) ? (uint32_t)std::min<uint64_t>(1u << 30, (uint64_t)m->bcap * 4) : m->bcap;
-    const uint32_t seg_cap = st.segs > m->seg_cap ? grow_cap(st.segs, m->seg_cap) : m->seg_cap;
-    const uint32_t split_cap = st.splits > m->split_cap ? grow_cap(st.splits, m->split_cap) : m->split_cap;
-    const uint64_t positions = (uint64_t)(st.items + st.splits) * 64;
-    const uint32_t rec_cap = positions > m->rec_cap ? grow_cap(positions, m->rec_cap) : m->rec_cap;
-    if ((st.overflow & kOvfBricks) && bcap == m->bcap) return fail(m, VDBGPU_DEVICE_ERROR, "brick table cannot grow");
-    if ((r = ensure_bricks(m, bcap, seg_cap, split_cap, rec_cap))) return r;
-    // split-brick bitmaps may hold partial ORs of the stopped run
-    CK(launch_fill_u64(cs, m->shared_bits, (uint64_t)m->split_cap * 1024, 0));
-    CK(hipMemsetAsync(&m->d_st[0].overflow, 0, sizeof(unsigned int), cs));
-    CK(hipMemsetAsync(&m->d_st[0].segs, 0, 6 * sizeof(unsigned int), cs));  // segs .. pad
+    if (st.overflow == 0 && (st.records <= c.cap || !later)) break;
+    if (st.overflow) {
+      const uint32_t bcap = (st.overflow & kOvfBricks) ? (uint32_t)std::min<uint64_t>(1u << 30, (uint64_t)m->bcap * 4) : m->bcap;
+      const uint32_t seg_cap = st.segs > m->seg_cap ? grow_cap(st.segs, m->seg_cap) : m->seg_cap;
+      const uint32_t split_cap = st.splits > m->split_cap ? grow_cap(st.splits, m->split_cap) : m->split_cap;
+      const uint64_t positions = (uint64_t)(st.items + st.splits) * 64;
+      const uint32_t rec_cap = positions > m->rec_cap ? grow_cap(positions, m->rec_cap) : m->rec_cap;
+      if ((st.overflow & kOvfBricks) && bcap == m->bcap) return fail(m, VDBGPU_DEVICE_ERROR, "brick table cannot grow");
+      if ((r = ensure_bricks(m, bcap, seg_cap, split_cap, rec_cap))) return r;
+      // split-brick bitmaps may hold partial ORs of the stopped run: cleared,
+      // and any cast enqueued since runs again
+      CK(launch_fill_u64(m->cast, m->shared_bits, (uint64_t)m->split_cap * 1024, 0));
+      ++m->cast_epoch;
+    } else if (st.records > m->dcap[X] &&
+               (r = grow_set(m, X, (uint32_t)std::min<uint64_t>(st.records + (uint64_t)st.records / 4,
+                                                                0xC0000000ull)))) {
+      return r;
+    }
+    if ((r = brick_enqueue(m, c, attempt))) return r;
   }
-  const Status& st = m->h_st[0];
+  const Status& st = m->h_cst[2 * c.slot];
   const uint32_t recs = st.records;  // listed in m->rlist, copied to set X when it had room
   m->stats.scans += 1;
   m->stats.rays += st.rays;
   m->stats.visits += st.total;
-  m->rec_cur = (X + 1) % m->rec_sets;
   if (recs == 0) {
     CK(hipEventRecord(m->rec_free[X], m->stream));  // orders the next scan's snapshot as usual
     m->rec_seq[X] = m->ledger_seq;
     return VDBGPU_OK;
   }
-  if (recs > m->dcap[X]) {  // set X (no pending scan holds it) grows; the copy runs again
-    if ((r = grow_set(m, X, (uint32_t)std::min<uint64_t>(recs + (uint64_t)recs / 4, 0xC0000000ull)))) return r;
-    CK(launch_rec_gather(cs, m->num_cu * 8, m->rlist, UpdView{m->rkeys, m->rmasks, 0},
-                         UpdView{m->dkeys[X], m->dmasks[X], 0}, m->dcap[X], &m->d_st[0]));
-    CK(hipStreamSynchronize(cs));
+  if (recs > c.cap) {  // set X (no pending scan holds it) grows; the copy runs again
+    if (recs > m->dcap[X] &&
+        (r = grow_set(m, X, (uint32_t)std::min<uint64_t>(recs + (uint64_t)recs / 4, 0xC0000000ull))))
+      return r;
+    c.cap = m->dcap[X];
+    CK(launch_rec_gather(m->cast, m->num_cu * 8, m->rlist, UpdView{m->rkeys, m->rmasks, 0},
+                         UpdView{m->dkeys[X], m->dmasks[X], 0}, c.cap, &m->d_st[0]));
+    CK(hipStreamSynchronize(m->cast));
   }
   m->staged = true;
   m->staged_set = X;
   m->staged_recs = recs;
   return VDBGPU_OK;
+}
+
+BrickCast brick_cast(const Source& src, const float* xyz, int64_t n, int64_t stride, const double origin[3],
+                     bool on_device) {
+  BrickCast c;
+  c.xyz = xyz;
+  c.n = n;
+  c.stride = stride;
+  c.on_device = on_device;
+  for (int a = 0; a < 3; ++a) c.sc.o[a] = origin[a];
+  c.sc.R = src.max_range;
+  c.sc.ray0 = 0;
+  c.sc.nrays = (uint32_t)n;
+  return c;
+}
+
+// insertPointCloud of one scan when no source holds pending updates: the
+// brick path casts the scan into leaf records and applies them at once
+// (equal to accumulateUpdate + integrateUpdate). While the cast runs, the
+// previous scan's map-stream work (and a full group of deferred scans) is
+// enqueued; the call returns once the scan's status is checked.
+int run_brick(vdbgpu_map* m, Source& src, const float* xyz, int64_t n, int64_t stride,
+              const double origin[3], bool on_device) {
+  int r;
+  if ((r = brick_tables(m, n, src.max_range))) return r;
+  BrickCast c = brick_cast(src, xyz, n, stride, origin, on_device);
+  if ((r = brick_enqueue(m, c, 0))) return r;
+  if ((r = commit_staged(m)) || (m->npend >= m->tuning.defer && (r = flush_pending(m)))) return r;
+  return brick_check(m, c);
 }
 
 // The map-stream work of the staged scan (set X, recs records): map insert
@@ -1821,11 +1905,15 @@ int32_t vdbgpu_create(double resolution, vdbgpu_map** out) {
       !create_rec_events(m) ||
       hipMalloc((void**)&m->d_st, 4 * sizeof(Status)) != hipSuccess ||
       hipHostMalloc((void**)&m->h_st, 4 * sizeof(Status), hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void**)&m->h_cst, 4 * sizeof(Status), hipHostMallocDefault) != hipSuccess ||
+      hipEventCreateWithFlags(&m->cst_done[0], hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&m->cst_done[1], hipEventDisableTiming) != hipSuccess ||
       hipMemsetAsync(m->d_st, 0, 4 * sizeof(Status), m->stream) != hipSuccess) {
     vdbgpu_destroy(m);
     return VDBGPU_DEVICE_ERROR;
   }
   memset(m->h_st, 0, 4 * sizeof(Status));
+  memset(m->h_cst, 0, 4 * sizeof(Status));
   if (hipMalloc((void**)&m->d_scans, kMaxBatchScans * sizeof(ScanInfo)) != hipSuccess ||
       hipMalloc((void**)&m->d_ctl, sizeof(BatchCtl)) != hipSuccess ||
       hipHostMalloc((void**)&m->h_scans, kMaxBatchScans * sizeof(ScanInfo), hipHostMallocDefault) != hipSuccess ||
@@ -1921,6 +2009,9 @@ void vdbgpu_destroy(vdbgpu_map* m) {
   dfree(m, &m->gidx, m->gcap);
   if (m->d_st) (void)hipFree(m->d_st);
   if (m->h_st) (void)hipHostFree(m->h_st);
+  if (m->h_cst) (void)hipHostFree(m->h_cst);
+  for (hipEvent_t e : m->cst_done)
+    if (e) (void)hipEventDestroy(e);
   for (auto& t : m->timed) {
     (void)hipEventDestroy(t.a);
     (void)hipEventDestroy(t.b);
@@ -2075,6 +2166,47 @@ int32_t vdbgpu_insert_batch(vdbgpu_map* m, const char* id, int32_t nscans, const
   int i0 = 0;
   bool pending = false;
   for (auto& kv : m->sources) pending = pending || kv.second.pending > 0;
+  if (m->tuning.traversal == 0 && !pending && !m->fast_mode && !m->ntiles) {
+    // Every scan through the brick path, one scan of lookahead: the cast of
+    // scan i is enqueued before the status of scan i - 1 is read, so the cast
+    // stream does not wait for the host between scans; scan i - 1's map-stream
+    // work is enqueued once its status is checked (overlapping scan i's cast),
+    // in scan order. A scan with nothing to cast takes the plain path after
+    // the scan before it is through.
+    BrickCast prev;
+    bool have_prev = false;
+    auto finish_prev = [&]() -> int {
+      if (!have_prev) return VDBGPU_OK;
+      have_prev = false;
+      int r = brick_check(m, prev);
+      if (!r) r = commit_staged(m);
+      if (!r && m->npend >= m->tuning.defer) r = flush_pending(m);
+      return r;
+    };
+    for (int i = 0; i < nscans; ++i) {
+      const double R = scan_range(i);
+      if (!(R > 0) || counts[i] <= 0 || counts[i] > (int64_t)kMaxRays) {
+        int r = finish_prev();
+        if (!r) r = plain_insert(i);
+        if (r) return r;
+        continue;
+      }
+      BrickCast c = brick_cast(src, scan_ptr(i), counts[i], stride, origins + 3 * i, on_device != 0);
+      c.sc.R = R;
+      int r = brick_tables(m, counts[i], R);
+      if (!r) r = brick_enqueue(m, c, 0);
+      const int rp = finish_prev();  // the scans before this one are applied as in sequential inserts
+      if (r || rp) return rp ? rp : r;
+      prev = c;
+      have_prev = true;
+    }
+    if (have_prev) {  // the last scan stays staged, as after a plain insert
+      have_prev = false;
+      const int r = brick_check(m, prev);
+      if (r) return r;
+    }
+    return refresh_snapshot(m);
+  }
   if ((m->tuning.traversal == 0 && !pending) || m->fast_mode || m->ntiles) {
     // Streaming every scan through the brick path (the next scan's cast
     // overlaps this one's map update) is faster than the (leaf, scan) fold;
