@@ -15,6 +15,8 @@ Other modes (--mode, not the default line) measure the north-star layers:
   mixed  64 scans of 10k-300k points with random poses and ranges: one batch vs per-scan
   rgbd   200 RGB-D frames at 0.02 m, then 1M raytrace queries (device arrays)
   serve  16 robot streams, Poisson arrivals at 10 Hz each, one map per stream
+         (--serve-scene room: scans from a shared room pool; outdoor: each robot
+         drives its own outdoor world, so its map grows by GBs)
   fast   the outdoor stream in fast mode (castRayIntoGridFast): only voxels already
          active in the map are re-marked along each ray, so the update follows the map
 """
@@ -55,6 +57,11 @@ def parse():
     p.add_argument("--autotune", action="store_true", help="mixed mode: autotune first")
     p.add_argument("--seconds", type=float, default=10.0, help="serve mode: arrival window")
     p.add_argument("--rate", type=float, default=10.0, help="serve mode: Poisson arrivals per stream (Hz)")
+    p.add_argument("--serve-scene", choices=("room", "outdoor"), default="room",
+                   help="serve mode: room pool scans, or one outdoor world per robot (large maps)")
+    p.add_argument("--serve-budget-gb", type=float, default=200.0, help="serve mode: HBM budget (GB)")
+    p.add_argument("--serve-check", type=int, default=-1,
+                   help="serve mode: maps rebuilt alone and compared (-1: all room maps, 4 outdoor maps)")
     p.add_argument("--frames", type=int, default=200, help="rgbd mode: depth frames")
     p.add_argument("--queries", type=int, default=1_000_000, help="rgbd mode: raytrace queries")
     return p.parse_args()
@@ -270,53 +277,84 @@ def mode_rgbd(a, V, W):
 
 
 def mode_serve(a, V, W):
-    """16 streams, Poisson arrivals (10 Hz each) of 20k-262k-point room scans, one map each.
-    Afterwards every map is rebuilt by inserting its requests one by one on a fresh map
-    (alone) and compared bit for bit with the served map (in the mix)."""
-    w = W.multi_robot_pool(48)
-    pool = [w.scan(i) for i in range(w.n_scans)]
-    rng = np.random.default_rng(1005)
+    """16 streams, Poisson arrivals (10 Hz each) of 20k-262k-point scans, one map each:
+    room scans drawn from a shared pool, or (--serve-scene outdoor) each robot's own
+    scans in path order through its own outdoor world, so that its map grows by GBs.
+    Afterwards maps are rebuilt by inserting their requests one by one on a fresh map
+    (alone) and compared bit for bit with the served maps (in the mix)."""
     streams, rate = 16, a.rate
-    arrivals = []
+    rng = np.random.default_rng(1005)
+    times = []
     for sidx in range(streams):
-        t = rng.exponential(1 / rate)
+        t, ts = rng.exponential(1 / rate), []
         while t < a.seconds:
-            arrivals.append((t, sidx, int(rng.integers(0, len(pool)))))
+            ts.append(t)
             t += rng.exponential(1 / rate)
-    arrivals.sort()
+        times.append(ts)
+    if a.serve_scene == "room":
+        w = W.multi_robot_pool(48)
+        pool = [w.scan(i) for i in range(w.n_scans)]
+        scans = [[pool[sidx]] + [pool[int(rng.integers(0, len(pool)))] for _ in times[sidx]]
+                 for sidx in range(streams)]
+        n_check = streams if a.serve_check < 0 else a.serve_check
+    else:
+        robots = [W.multi_robot_outdoor(sidx, len(times[sidx]) + 1) for sidx in range(streams)]
+        w = robots[0]
+        scans = [[r.scan(i) for i in range(r.n_scans)] for r in robots]
+        n_check = 4 if a.serve_check < 0 else a.serve_check
+    print("# serve: %s scans, %d requests generated" % (a.serve_scene, sum(len(x) for x in scans)), flush=True)
+    arrivals = sorted((t, sidx, j + 1) for sidx in range(streams) for j, t in enumerate(times[sidx]))
     cfg = V.make_config(max_range=w.max_range, **W.PROBS)
-    srv = V.Server(streams, w.resolution, cfg, w.max_range, hbm_budget_bytes=200 << 30, max_batch=32)
-    order = [[] for _ in range(streams)]
+    budget = int(a.serve_budget_gb * 1e9)
+    srv = V.Server(streams, w.resolution, cfg, w.max_range, hbm_budget_bytes=budget, max_batch=32)
     # warm every map once (table sizing), then start the clock
+    req = {}  # request id -> (stream, scan)
     for sidx in range(streams):
-        srv.submit(sidx, pool[sidx].points, pool[sidx].origin)
-        order[sidx].append(sidx)
+        req[srv.submit(sidx, scans[sidx][0].points, scans[sidx][0].origin)] = (sidx, 0)
     srv.drain()
     n_warm = len(srv.records())
+    peak = srv.device_bytes()
     t0 = time.perf_counter()
-    for t, sidx, k in arrivals:
+    for t, sidx, j in arrivals:
         delay = t0 + t - time.perf_counter()
         if delay > 0:
             time.sleep(delay)
-        srv.submit(sidx, pool[k].points, pool[k].origin)
-        order[sidx].append(k)
+        req[srv.submit(sidx, scans[sidx][j].points, scans[sidx][j].origin)] = (sidx, j)
+        peak = max(peak, srv.device_bytes())
     srv.drain()
     elapsed = time.perf_counter() - t0
-    recs = srv.records()[n_warm:]
+    peak = max(peak, srv.device_bytes())
+    all_recs = srv.records()
+    applied = [[] for _ in range(streams)]  # scans of each map whose request succeeded, in order
+    for r in sorted(all_recs, key=lambda r: r["request_id"]):
+        if r["status"] == 0:
+            sidx, j = req[r["request_id"]]
+            applied[sidx].append(j)
+    status_counts = {}
+    for r in all_recs:
+        name = V.status_string(r["status"])
+        status_counts[name] = status_counts.get(name, 0) + 1
+    recs = all_recs[n_warm:]
     lat = np.array([r["latency_ms"] for r in recs])
     rays = sum(r["points"] for r in recs)
-    # alone vs in the mix
-    same = True
+    map_gb = []
     for sidx in range(streams):
+        st = srv.map(sidx).stats()
+        map_gb.append((st["device_bytes"] - st["scratch_bytes"]) / 1e9)
+    # alone vs in the mix: a fresh map gets the requests of the served map that
+    # succeeded (a refused request leaves its map unchanged), one by one in order
+    same = True
+    for sidx in range(n_check):
         ref = V.GpuMap(w.resolution)
         ref.set_config(cfg)
         ref.add_input_source("s", w.max_range)
-        for k in order[sidx]:
-            ref.insert("s", pool[k].points, pool[k].origin, check=True)
+        for j in applied[sidx]:
+            ref.insert("s", scans[sidx][j].points, scans[sidx][j].origin, check=True)
         a_o, a_v, a_m = srv.map(sidx).export_leaves()
         b_o, b_v, b_m = ref.export_leaves()
         same &= bool(np.array_equal(a_o, b_o) and np.array_equal(a_v.view(np.uint32), b_v.view(np.uint32))
                      and np.array_equal(a_m, b_m))
+        del a_o, a_v, a_m, b_o, b_v, b_m
         ref.close()
     print(json.dumps({"metric": "multi-robot serving: p50/p99 scan latency", "value": round(float(np.percentile(lat, 99)), 3),
                       "unit": "ms (p99)", "higher_is_better": False,
@@ -325,9 +363,14 @@ def mode_serve(a, V, W):
                       "offered_hz": rate * streams, "rays_per_s": round(rays / elapsed, 1),
                       "mean_batch": round(float(np.mean([r["batch_size"] for r in recs])), 3),
                       "max_batch": int(max(r["batch_size"] for r in recs)),
-                      "all_ok": all(r["status"] == 0 for r in recs), "device_gb": round(srv.device_bytes() / 1e9, 2),
-                      "maps_equal_sequential": same,
-                      "config": {"workload": "multi_robot_16_streams_0.05m", "seconds": a.seconds,
+                      "all_ok": all(r["status"] == 0 for r in recs), "status_counts": status_counts,
+                      "mem_wait_requests": int(sum(1 for r in recs if r["mem_waits"] > 0)),
+                      "device_gb": round(srv.device_bytes() / 1e9, 2), "peak_device_gb": round(peak / 1e9, 2),
+                      "budget_gb": a.serve_budget_gb,
+                      "map_gb": {"min": round(min(map_gb), 2), "mean": round(float(np.mean(map_gb)), 2),
+                                 "max": round(max(map_gb), 2)},
+                      "maps_checked": n_check, "maps_equal_sequential": same,
+                      "config": {"workload": "multi_robot_16_streams_0.05m_" + a.serve_scene, "seconds": a.seconds,
                                  "rate_hz_per_stream": rate}}), flush=True)
     srv.close()
 

@@ -578,6 +578,14 @@ void ga_release(vdbgpu_map* m, GrowArray& g) {
   g = GrowArray{};
 }
 
+// A refused call whose failure is handled (a fallback follows) leaves its error
+// as the thread's last error, which the next launch check would report: it is
+// cleared here.
+bool refused() {
+  (void)hipGetLastError();
+  return false;
+}
+
 // Map physical memory behind g up to at least `bytes` (a multiple of gran),
 // zero-filled on the map stream. Access is granted over the whole mapped
 // range (the driver refuses some ranges that start inside a reservation).
@@ -592,10 +600,10 @@ bool ga_grow(vdbgpu_map* m, GrowArray& g, size_t bytes, size_t gran) {
   prop.location.type = hipMemLocationTypeDevice;
   prop.location.id = m->device;
   hipMemGenericAllocationHandle_t h;
-  if (hipMemCreate(&h, sz, &prop, 0) != hipSuccess) return false;
+  if (hipMemCreate(&h, sz, &prop, 0) != hipSuccess) return refused();
   if (hipMemMap(g.base + g.mapped, sz, 0, h, 0) != hipSuccess) {
     (void)hipMemRelease(h);
-    return false;
+    return refused();
   }
   hipMemAccessDesc acc{};
   acc.location = prop.location;
@@ -604,7 +612,7 @@ bool ga_grow(vdbgpu_map* m, GrowArray& g, size_t bytes, size_t gran) {
       hipMemsetAsync(g.base + g.mapped, 0, sz, m->stream) != hipSuccess) {
     (void)hipMemUnmap(g.base + g.mapped, sz);
     (void)hipMemRelease(h);
-    return false;
+    return refused();
   }
   g.chunks.push_back({h, sz});
   g.mapped = bytes;
@@ -619,16 +627,20 @@ void vmm_init(vdbgpu_map* m) {
   const char* plain = std::getenv("VDBGPU_PLAIN_POOL");  // "1": plain allocations (the fallback)
   if (plain && plain[0] == '1') return;
   int ok = 0;
-  if (hipDeviceGetAttribute(&ok, hipDeviceAttributeVirtualMemoryManagementSupported, m->device) != hipSuccess || !ok)
+  if (hipDeviceGetAttribute(&ok, hipDeviceAttributeVirtualMemoryManagementSupported, m->device) != hipSuccess || !ok) {
+    refused();
     return;
+  }
   hipMemAllocationProp prop{};
   prop.type = hipMemAllocationTypePinned;
   prop.location.type = hipMemLocationTypeDevice;
   prop.location.id = m->device;
   size_t gran = 0, free_b = 0, total_b = 0;
   if (hipMemGetAllocationGranularity(&gran, &prop, hipMemAllocationGranularityMinimum) != hipSuccess || !gran ||
-      hipMemGetInfo(&free_b, &total_b) != hipSuccess)
+      hipMemGetInfo(&free_b, &total_b) != hipSuccess) {
+    refused();
     return;
+  }
   const uint64_t leaves = std::min<uint64_t>(0xF0000000ull, total_b / 2048 + 1);
   GrowArray* arrs[4] = {&m->g_values, &m->g_active, &m->g_keys, &m->g_dirty};
   const size_t elem[4] = {2048, 64, 8, 1};
@@ -636,6 +648,7 @@ void vmm_init(vdbgpu_map* m) {
     const size_t bytes = (leaves * elem[i] + gran - 1) / gran * gran;
     void* p = nullptr;
     if (hipMemAddressReserve(&p, bytes, 0, nullptr, 0) != hipSuccess) {
+      refused();
       for (int j = 0; j < i; ++j) ga_release(m, *arrs[j]);
       return;
     }

@@ -259,6 +259,32 @@ def multi_robot_pool(n_scans: int = 48) -> Workload:
     return wl
 
 
+def multi_robot_outdoor(robot: int, n_scans: int) -> Workload:
+    """Scans of robot `robot` in the large-map serving workload: its own outdoor
+    world (ground plane, 100 boxes and 100 cylinders in 200 x 200 m around its
+    path) and a 1 m-step trajectory with <= 5 deg yaw drift; point counts
+    log-uniform in [20k, 262,144] (64 rings, elevation [-22.5, 22.5] deg),
+    range noise 0.03 m; resolution 0.05 m, max range 50 m. Its map grows along
+    the path, by several GB over a hundred scans."""
+    k = 5
+    seed = (1000 + k) * 1000 + robot
+    poses = trajectory(seed, n_scans, 1.0, 5.0, 1.8)
+    scene = outdoor_scene(seed, poses)
+    rng = np.random.default_rng(seed)
+    counts = np.exp(rng.uniform(math.log(20_000), math.log(262_144), n_scans)).astype(np.int64)
+    wl = Workload("multi_robot_outdoor_0.05m", 0.05, 50.0, n_scans=n_scans)
+    wl.counts, wl.poses = counts, poses
+
+    def make(i):
+        n = int(counts[i])
+        pts = lidar_scan(scene, poses[i], 64, (n + 63) // 64, n, -22.5, 22.5, 0.03,
+                         seed * 1_000_003 + i)
+        return Scan(pts, poses[i, :3].copy(), 50.0)
+
+    wl._make = make
+    return wl
+
+
 def multi_robot_bursts(n_maps: int = 16, repeats: int = 5):
     """Request order of the serving test: map i receives pool scan 3i `repeats`
     times (a robot standing still: voxels hit that often turn active at the
