@@ -117,7 +117,10 @@ def mode_batch(a, V, W):
     n_pts = scans[0].points.shape[0]
     rays = sum(int((~np.isnan(s.points[:, :3]).any(1)).sum()) for s in scans)
     out = {}
-    for path in ("plain", "batch", "fold"):
+    for path in ("plain", "batch", "graph", "fold"):
+        # batch: the streamed path (casts launched one by one); graph: the same
+        # path with its casts replayed from captured HIP graphs
+        os.environ["VDBGPU_GRAPH"] = "1" if path == "graph" else "0"
         g = new_map(V, w, a.tuning)
         if path == "fold":  # the (leaf, scan)-keyed traversal and scan-ordered fold
             assert g.set_tuning(traversal=1) == V.OK
@@ -144,9 +147,12 @@ def mode_batch(a, V, W):
     print(json.dumps({"metric": "batched integration of the outdoor stream", "value": round(rays / out["batch"], 1),
                       "unit": "rays/s", "plain_rays_per_s": round(rays / out["plain"], 1),
                       "fold_rays_per_s": round(rays / out["fold"], 1),
-                      "speedup_vs_plain": round(out["plain"] / out["batch"], 3), "batch_scans": a.batch,
+                      "graph_rays_per_s": round(rays / out["graph"], 1),
+                      "speedup_vs_plain": round(out["plain"] / out["batch"], 3),
+                      "graph_speedup_vs_plain": round(out["plain"] / out["graph"], 3), "batch_scans": a.batch,
                       "scans": a.scans,
-                      "same_leaf_count": out["plain_leaves"] == out["batch_leaves"] == out["fold_leaves"],
+                      "same_leaf_count": out["plain_leaves"] == out["batch_leaves"] == out["graph_leaves"]
+                      == out["fold_leaves"],
                       "config": {"workload": "outdoor_stream_262k_0.05m", "input": "device-resident"}}),
           flush=True)
 
@@ -209,7 +215,9 @@ def mode_mixed(a, V, W):
         tuning = best
         tune_table = [(t, round(s * 1e3, 3)) for t, s in sorted(table, key=lambda r: r[1])[:5]]
     times = {}
-    for path in ("plain", "batch", "fold"):
+    for path in ("plain", "batch", "graph", "fold"):
+        # batch: casts launched one by one; graph: replayed from captured graphs
+        os.environ["VDBGPU_GRAPH"] = "1" if path == "graph" else "0"
         g = new_map(V, w, tuning)
         if path == "fold":  # the (leaf, scan)-keyed traversal and scan-ordered fold
             assert g.set_tuning(traversal=1) == V.OK
@@ -233,8 +241,11 @@ def mode_mixed(a, V, W):
                       "value": round(rays / times["batch"], 1), "unit": "rays/s",
                       "plain_rays_per_s": round(rays / times["plain"], 1),
                       "fold_rays_per_s": round(rays / times["fold"], 1),
+                      "graph_rays_per_s": round(rays / times["graph"], 1),
                       "speedup_vs_plain": round(times["plain"] / times["batch"], 3),
+                      "graph_speedup_vs_plain": round(times["plain"] / times["graph"], 3),
                       "batch_ms": round(times["batch"] * 1e3, 3), "plain_ms": round(times["plain"] * 1e3, 3),
+                      "graph_ms": round(times["graph"] * 1e3, 3),
                       "rays": rays, "tuning": tuning, "autotune_top5": tune_table,
                       "config": {"workload": "mixed_batch_64_0.05m", "input": "device-resident"}}), flush=True)
 

@@ -143,10 +143,13 @@ int32_t vdbgpu_insert(vdbgpu_map* map, const char* source_id, const float* xyz, 
  * `xyz` (counts[i] points each, same stride), integrated with a result
  * identical to vdbgpu_insert on each scan in order (VM:399-406 per scan).
  * origins: nscans x 3 doubles. max_ranges: per-scan max range, or NULL for
- * the source's range (a range <= 0 casts nothing for that scan). Up to 128
- * scans share one (leaf, scan)-keyed update set and one scan-ordered fold;
- * longer lists are processed 128 at a time. No reference counterpart for the
- * per-scan ranges. */
+ * the source's range (a range <= 0 casts nothing for that scan). With
+ * tuning.traversal = 0 (the default) the scans stream through the brick path,
+ * each scan's cast enqueued before the previous scan's status is read (with
+ * the environment variable VDBGPU_GRAPH=1 the casts are replayed from captured
+ * HIP graphs, one per set of buffers); with traversal = 1 up to 128 scans
+ * share one (leaf, scan)-keyed update set and one scan-ordered fold. No
+ * reference counterpart for the per-scan ranges. */
 int32_t vdbgpu_insert_batch(vdbgpu_map* map, const char* source_id, int32_t nscans,
                             const float* xyz, const int64_t* counts, int64_t stride_bytes,
                             const double* origins, const double* max_ranges,

@@ -10,7 +10,7 @@ import oracle as O
 import vdb_mapping_amd as V
 import parity
 from golden_cases import digest, digests, popcount
-from parity import same_map
+from parity import both, same_map
 from vdb_mapping_amd import workloads as W
 
 pytestmark = pytest.mark.gpu
@@ -179,6 +179,46 @@ def test_batch_tables_grow_between_scans(pair):
     g.insert_batch("test", clouds, np.array(origins), None, check=True)
     oracle_sequential(o, "test", clouds, origins, [50.0] * len(clouds))
     assert same_map(g, o) > 10000
+
+
+def test_batch_graph_replays_interleaved(pair, monkeypatch):
+    """Batched inserts on one map, with plain inserts and exports between them:
+    with VDBGPU_GRAPH=1 the streamed path replays its captured cast graphs
+    (different scan sizes, every record set of the ring); each state equals the
+    oracle's."""
+    monkeypatch.setenv("VDBGPU_GRAPH", "1")
+    rng = np.random.default_rng(21)
+    g, o = pair(0.05, max_range=8.0)
+    for _ in range(4):
+        clouds, origins, ranges = random_scans(rng, 7, 8.0, sizes=(500, 6000))
+        assert g.insert_batch("test", clouds, origins, ranges, check=True) == V.OK
+        oracle_sequential(o, "test", clouds, origins, ranges)
+        same_map(g, o)
+        for m in (g, o):
+            assert m.add_input_source("test", 8.0) == V.OK
+        both(g, o, "insert", "test", clouds[1], origins[1])
+        same_map(g, o)
+
+
+def test_batch_graph_equals_launches(monkeypatch):
+    """The streamed batch path with its casts replayed from graphs
+    (VDBGPU_GRAPH=1) and with the same casts launched one by one gives the
+    same map."""
+    rng = np.random.default_rng(22)
+    clouds, origins, ranges = random_scans(rng, 20, 10.0, sizes=(100, 8000))
+    maps = []
+    for on in ("1", "0"):
+        monkeypatch.setenv("VDBGPU_GRAPH", on)
+        m = V.GpuMap(0.05)
+        m.set_config(V.make_config(max_range=10.0, **W.PROBS))
+        m.add_input_source("s", 10.0)
+        assert m.insert_batch("s", clouds, origins, ranges, check=True) == V.OK
+        maps.append(m.export_leaves())
+        m.close()
+    (ao, av, am), (bo, bv, bm) = maps
+    assert ao.shape[0] > 1000
+    assert np.array_equal(ao, bo) and np.array_equal(av.view(np.uint32), bv.view(np.uint32))
+    assert np.array_equal(am, bm)
 
 
 def test_mixed_batch_workload_full_matches_oracle_digest(traversal):

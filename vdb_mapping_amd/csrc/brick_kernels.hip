@@ -28,7 +28,8 @@
 //   k_mask_sums, k_mask_list  dense list of the filled record positions from
 //                   the per-item leaf masks (scan, no shared counters)
 //   k_rec_gather    the listed records copied densely into the scan's record
-//                   set (read by the map stream, possibly several scans later)
+//                   set, named in device memory (read by the map stream,
+//                   possibly several scans later)
 //
 // Segment record (uint4), self-contained so the walk loads nothing else:
 //   x = n0 | n1 << 14, y = n2 | c0 << 14, z = c1 | c2 << 14  (|delta| per axis
@@ -698,11 +699,13 @@ __global__ __launch_bounds__(256) void k_mask_list(const unsigned long long* __r
 // per record) into the record set the map stream reads; the positional set is
 // then free for the next scan's walk. Copies nothing when the count exceeds
 // dcap (the host grows the set and runs it again).
-__global__ __launch_bounds__(256) void k_rec_gather(const uint32_t* __restrict__ list, UpdView sparse, UpdView dense,
-                                                    uint32_t dcap, const Status* st) {
+__global__ __launch_bounds__(256) void k_rec_gather(const uint32_t* __restrict__ list, UpdView sparse,
+                                                    const RecDest* __restrict__ dest, const Status* st) {
   if (st->error | st->overflow) return;
   const uint32_t n = st->records;
-  if (n > dcap) return;
+  const RecDest d = *dest;
+  if (n > d.cap) return;
+  const UpdView dense{d.keys, d.masks, 0};
   // eight lanes per record, 16 bytes each (the 128 mask bytes); each lane
   // group keeps kGatherRecs records in flight (list, then record loads, then
   // stores, issued together)
@@ -733,6 +736,11 @@ __global__ __launch_bounds__(256) void k_rec_gather(const uint32_t* __restrict__
   }
 }
 
+// The record set of the next cast (one lane).
+__global__ void k_put_dest(RecDest* dst, RecDest v) {
+  if (threadIdx.x == 0) *dst = v;
+}
+
 // The map's leaf count into host-visible memory (one lane; no copy engine).
 __global__ void k_snap_count(const Status* st, uint32_t* host_dst) {
   if (threadIdx.x == 0) __hip_atomic_store(host_dst, st->leaf_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
@@ -745,9 +753,14 @@ __global__ __launch_bounds__(256) void k_iota_u32(uint32_t* p, uint32_t n) {
 
 // ---------------------------------------------------------------- launchers --
 
-hipError_t launch_rec_gather(hipStream_t s, int grid, const uint32_t* list, UpdView sparse, UpdView dense,
-                             uint32_t dcap, const Status* st) {
-  hipLaunchKernelGGL(k_rec_gather, dim3(grid), dim3(256), 0, s, list, sparse, dense, dcap, st);
+hipError_t launch_rec_gather(hipStream_t s, int grid, const uint32_t* list, UpdView sparse, const RecDest* dest,
+                             const Status* st) {
+  hipLaunchKernelGGL(k_rec_gather, dim3(grid), dim3(256), 0, s, list, sparse, dest, st);
+  return hipGetLastError();
+}
+
+hipError_t launch_put_dest(hipStream_t s, RecDest* dst, RecDest v) {
+  hipLaunchKernelGGL(k_put_dest, dim3(1), dim3(64), 0, s, dst, v);
   return hipGetLastError();
 }
 

@@ -98,6 +98,16 @@ struct UpdView {
   uint32_t cap_mask;
 };
 
+// Where the brick path copies a scan's records (one record set of the ring):
+// kept in device memory, so a replayed cast graph copies into the set of the
+// scan it runs for.
+struct RecDest {
+  unsigned long long* keys;
+  unsigned long long* masks;  // 16 words per record
+  uint32_t cap;
+  uint32_t pad;
+};
+
 // Batch fold groups: leaf (57-bit batched key) -> 128-bit scan mask + create flag.
 struct GroupView {
   unsigned long long* keys;

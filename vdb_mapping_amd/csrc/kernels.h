@@ -43,8 +43,9 @@ hipError_t launch_map_insert(hipStream_t s, int grid, UpdView u, const uint32_t*
                              uint32_t count, MapView mp, uint32_t* eidx, bool create_hit,
                              bool create_miss, unsigned long long* noop_set, uint32_t noop_mask,
                              Status* st, Status* stc);
-hipError_t launch_rec_gather(hipStream_t s, int grid, const uint32_t* list, UpdView sparse, UpdView dense,
-                             uint32_t dcap, const Status* st);
+hipError_t launch_rec_gather(hipStream_t s, int grid, const uint32_t* list, UpdView sparse, const RecDest* dest,
+                             const Status* st);
+hipError_t launch_put_dest(hipStream_t s, RecDest* dst, RecDest v);
 hipError_t launch_iota_u32(hipStream_t s, uint32_t* p, uint32_t n);
 hipError_t launch_snap_count(hipStream_t s, const Status* st, uint32_t* host_dst);
 // Deferred brick-path scans applied as one group pass (integrate_kernels.hip).

@@ -244,9 +244,21 @@ struct vdbgpu_map {
   Status* h_st = nullptr;
   // brick-path casts whose status the host has not read: two host slots
   Status* h_cst = nullptr;
+  RecDest* d_dest = nullptr;  // the record set the next cast copies into
   hipEvent_t cst_done[2] = {};
   int cst_next = 0;
   uint64_t cast_seq = 0;    // brick casts enqueued (re-runs included)
+  // brick casts of batched inserts replayed from captured graphs, one per
+  // exact set of launch arguments (least recently used dropped first)
+  struct CastGraph {
+    std::vector<uint64_t> key;
+    hipGraphExec_t exec;
+    uint64_t used;
+  };
+  static constexpr size_t kCastGraphs = 48;
+  std::vector<CastGraph> cast_graphs;
+  uint64_t cast_graph_clock = 0;
+  bool graph_casts = false;  // set while vdbgpu_insert_batch streams scans (VDBGPU_GRAPH=1)
   uint64_t cast_epoch = 0;  // table overflows found (each clears the split-brick bitmaps)
 
   vdbgpu_tuning tuning{256, 32, 16, 0.5, 0, kDeferMax};
@@ -1238,6 +1250,91 @@ int brick_tables(vdbgpu_map* m, int64_t n, double R) {
   return VDBGPU_OK;
 }
 
+// The brick stages of a staged scan on the cast stream: cut, walk, record
+// list and the copy into the record set named in m->d_dest. `nb` = workgroups
+// of the cut (lanes past the scan's rays do nothing).
+int cast_chain(vdbgpu_map* m, int nb) {
+  hipStream_t cs = m->cast;
+  const BrickView bv{m->bkeys, m->bcnt, m->bcnt + m->bcap, m->bcap - 1};
+  const UpdView rec{m->rkeys, m->rmasks, 0};
+  {
+    Timer t(m, kTraverse, cs);
+    CK(launch_seg_count(cs, nb, m->rayinfo, m->d_scans, m->d_ctl, m->tbl, bv, m->wkey, m->wsc, &m->d_st[0]));
+    CK(launch_brick_offsets(cs, bv, m->bbsum, m->bboff, m->items, m->seg_cap, m->item_cap,
+                            m->split_slot, m->split_cap, &m->d_st[0]));
+    CK(launch_seg_fill(cs, nb, m->rayinfo, m->d_scans, m->d_ctl, m->tbl, bv, m->wkey, m->wsc, m->segs,
+                       m->seg_cap, &m->d_st[0]));
+  }
+  {
+    Timer t(m, kWalk, cs);
+    CK(launch_brick_walk(cs, m->num_cu * 32, m->items, m->segs, m->rayinfo, m->d_scans, m->tbl, bv,
+                         m->shared_bits, rec, m->leaf_mask, m->rec_cap, &m->d_st[0]));
+  }
+  {
+    Timer t(m, kCompact, cs);
+    CK(launch_split_emit(cs, m->num_cu, m->split_slot, m->shared_bits, bv, rec, m->leaf_mask,
+                         m->rec_cap, &m->d_st[0]));
+    CK(launch_rec_list(cs, m->leaf_mask, m->lbsum, m->lboff, m->group_cap / 256, m->rlist, &m->d_st[0]));
+    CK(launch_rec_gather(cs, m->num_cu * 8, m->rlist, rec, m->d_dest, &m->d_st[0]));
+  }
+  return VDBGPU_OK;
+}
+
+void drop_cast_graphs(vdbgpu_map* m) {
+  for (auto& g : m->cast_graphs) (void)hipGraphExecDestroy(g.exec);
+  m->cast_graphs.clear();
+}
+
+// cast_chain replayed from a captured HIP graph: one instantiated graph per
+// exact set of its launch arguments (every pointer and size the chain passes),
+// so a graph is only replayed with the buffers it was captured with, and a
+// grown table simply yields another graph. The record set comes from device
+// memory and the cut's grid is rounded up to a power of two, so that every
+// set of the ring and scans of different sizes share graphs.
+int launch_cast_graph(vdbgpu_map* m, int nb) {
+  hipStream_t cs = m->cast;
+  const int grid = (int)next_pow2((uint64_t)nb);
+  auto u = [](const void* p) { return (uint64_t)(uintptr_t)p; };
+  const std::vector<uint64_t> key = {
+      (uint64_t)grid, (uint64_t)m->num_cu, u(m->rayinfo), u(m->d_scans), u(m->d_ctl), u(m->tbl), u(m->bkeys),
+      u(m->bcnt), m->bcap, u(m->wkey), u(m->wsc), u(m->segs), m->seg_cap, u(m->bbsum), u(m->bboff),
+      u(m->items), m->item_cap, u(m->split_slot), m->split_cap, u(m->shared_bits), u(m->rkeys), u(m->rmasks),
+      u(m->leaf_mask), m->rec_cap, u(m->lbsum), u(m->lboff), m->group_cap, u(m->rlist), u(m->d_dest),
+      u(m->d_st)};
+  vdbgpu_map::CastGraph* hit = nullptr;
+  for (auto& g : m->cast_graphs)
+    if (g.key == key) hit = &g;
+  if (!hit) {
+    if (m->cast_graphs.size() >= vdbgpu_map::kCastGraphs) {  // drop the least recently used
+      size_t lru = 0;
+      for (size_t i = 1; i < m->cast_graphs.size(); ++i)
+        if (m->cast_graphs[i].used < m->cast_graphs[lru].used) lru = i;
+      CK(hipStreamSynchronize(cs));  // (no launch of it is still running)
+      (void)hipGraphExecDestroy(m->cast_graphs[lru].exec);
+      m->cast_graphs.erase(m->cast_graphs.begin() + (long)lru);
+    }
+    CK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+    const int rc = cast_chain(m, grid);
+    hipGraph_t graph = nullptr;
+    const hipError_t ec = hipStreamEndCapture(cs, &graph);
+    if (rc || ec != hipSuccess || !graph) {
+      if (graph) (void)hipGraphDestroy(graph);
+      if (rc) return rc;
+      CK(ec);
+      return fail(m, VDBGPU_DEVICE_ERROR, "stream capture of a brick cast returned no graph");
+    }
+    hipGraphExec_t exec = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    CK(ei);
+    m->cast_graphs.push_back({key, exec, 0});
+    hit = &m->cast_graphs.back();
+  }
+  hit->used = ++m->cast_graph_clock;
+  CK(hipGraphLaunch(hit->exec, cs));
+  return VDBGPU_OK;
+}
+
 // Enqueues the cast of c on the cast stream: staging (prep, brick table
 // cleared), brick cut, walk, record list and the copy into record set c.X,
 // then the copy of its status (and a snapshot of the map's leaf count) into
@@ -1267,29 +1364,12 @@ int brick_enqueue(vdbgpu_map* m, BrickCast& c, int attempt) {
         if (m->dcap[y] == 0 && (r = grow_set(m, y, m->dense_hint))) return r;
     }
   }
-  const BrickView bv{m->bkeys, m->bcnt, m->bcnt + m->bcap, m->bcap - 1};
-  const UpdView rec{m->rkeys, m->rmasks, 0};
-  {
-    Timer t(m, kTraverse, cs);
-    CK(launch_seg_count(cs, nb, m->rayinfo, m->d_scans, m->d_ctl, m->tbl, bv, m->wkey, m->wsc, &m->d_st[0]));
-    CK(launch_brick_offsets(cs, bv, m->bbsum, m->bboff, m->items, m->seg_cap, m->item_cap,
-                            m->split_slot, m->split_cap, &m->d_st[0]));
-    CK(launch_seg_fill(cs, nb, m->rayinfo, m->d_scans, m->d_ctl, m->tbl, bv, m->wkey, m->wsc, m->segs,
-                       m->seg_cap, &m->d_st[0]));
-  }
-  {
-    Timer t(m, kWalk, cs);
-    CK(launch_brick_walk(cs, m->num_cu * 32, m->items, m->segs, m->rayinfo, m->d_scans, m->tbl, bv,
-                         m->shared_bits, rec, m->leaf_mask, m->rec_cap, &m->d_st[0]));
-  }
-  {
-    Timer t(m, kCompact, cs);
-    CK(launch_split_emit(cs, m->num_cu, m->split_slot, m->shared_bits, bv, rec, m->leaf_mask,
-                         m->rec_cap, &m->d_st[0]));
-    CK(launch_rec_list(cs, m->leaf_mask, m->lbsum, m->lboff, m->group_cap / 256, m->rlist, &m->d_st[0]));
-    c.cap = m->dcap[c.X];  // (a later first use may size an empty set X before this is checked)
-    CK(launch_rec_gather(cs, m->num_cu * 8, m->rlist, rec, UpdView{m->dkeys[c.X], m->dmasks[c.X], 0},
-                         c.cap, &m->d_st[0]));
+  c.cap = m->dcap[c.X];  // (a later first use may size an empty set X before this is checked)
+  CK(launch_put_dest(cs, m->d_dest, RecDest{m->dkeys[c.X], m->dmasks[c.X], c.cap, 0}));
+  if (m->graph_casts && !m->profiling) {
+    if ((r = launch_cast_graph(m, nb))) return r;
+  } else if ((r = cast_chain(m, nb))) {
+    return r;
   }
   // the scan's status and a snapshot of the leaf count: the map stream keeps
   // applying earlier scans meanwhile
@@ -1361,8 +1441,9 @@ int brick_check(vdbgpu_map* m, BrickCast& c) {
         (r = grow_set(m, X, (uint32_t)std::min<uint64_t>(recs + (uint64_t)recs / 4, 0xC0000000ull))))
       return r;
     c.cap = m->dcap[X];
-    CK(launch_rec_gather(m->cast, m->num_cu * 8, m->rlist, UpdView{m->rkeys, m->rmasks, 0},
-                         UpdView{m->dkeys[X], m->dmasks[X], 0}, c.cap, &m->d_st[0]));
+    CK(launch_put_dest(m->cast, m->d_dest, RecDest{m->dkeys[X], m->dmasks[X], c.cap, 0}));
+    CK(launch_rec_gather(m->cast, m->num_cu * 8, m->rlist, UpdView{m->rkeys, m->rmasks, 0}, m->d_dest,
+                         &m->d_st[0]));
     CK(hipStreamSynchronize(m->cast));
   }
   m->staged = true;
@@ -1920,6 +2001,7 @@ int32_t vdbgpu_create(double resolution, vdbgpu_map** out) {
       hipMalloc((void**)&m->d_st, 4 * sizeof(Status)) != hipSuccess ||
       hipHostMalloc((void**)&m->h_st, 4 * sizeof(Status), hipHostMallocDefault) != hipSuccess ||
       hipHostMalloc((void**)&m->h_cst, 4 * sizeof(Status), hipHostMallocDefault) != hipSuccess ||
+      hipMalloc((void**)&m->d_dest, sizeof(RecDest)) != hipSuccess ||
       hipEventCreateWithFlags(&m->cst_done[0], hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&m->cst_done[1], hipEventDisableTiming) != hipSuccess ||
       hipMemsetAsync(m->d_st, 0, 4 * sizeof(Status), m->stream) != hipSuccess) {
@@ -2023,6 +2105,8 @@ void vdbgpu_destroy(vdbgpu_map* m) {
   dfree(m, &m->gidx, m->gcap);
   if (m->d_st) (void)hipFree(m->d_st);
   if (m->h_st) (void)hipHostFree(m->h_st);
+  drop_cast_graphs(m);
+  if (m->d_dest) (void)hipFree(m->d_dest);
   if (m->h_cst) (void)hipHostFree(m->h_cst);
   for (hipEvent_t e : m->cst_done)
     if (e) (void)hipEventDestroy(e);
@@ -2187,6 +2271,17 @@ int32_t vdbgpu_insert_batch(vdbgpu_map* m, const char* id, int32_t nscans, const
     // work is enqueued once its status is checked (overlapping scan i's cast),
     // in scan order. A scan with nothing to cast takes the plain path after
     // the scan before it is through.
+    // VDBGPU_GRAPH=1: the casts are replayed from captured graphs instead of
+    // launched one by one (the same results; measured no faster: the GPU, not
+    // the host, sets the pace); the flag ends with the call
+    struct GraphCasts {
+      vdbgpu_map* m;
+      explicit GraphCasts(vdbgpu_map* mm) : m(mm) {
+        const char* on = std::getenv("VDBGPU_GRAPH");
+        m->graph_casts = on && on[0] == '1';
+      }
+      ~GraphCasts() { m->graph_casts = false; }
+    } graph_casts(m);
     BrickCast prev;
     bool have_prev = false;
     auto finish_prev = [&]() -> int {
@@ -2630,6 +2725,7 @@ int32_t vdbgpu_release_scratch(vdbgpu_map* m) {
   if (g.rc) return g.rc;
   CK(hipStreamSynchronize(m->cast));
   CK(hipStreamSynchronize(m->stream));
+  drop_cast_graphs(m);  // their buffers go
   walk_scratch(m, true);
   return VDBGPU_OK;
 }
