@@ -1297,7 +1297,7 @@ int launch_cast_graph(vdbgpu_map* m, int nb) {
   auto u = [](const void* p) { return (uint64_t)(uintptr_t)p; };
   const std::vector<uint64_t> key = {
       (uint64_t)grid, (uint64_t)m->num_cu, u(m->rayinfo), u(m->d_scans), u(m->d_ctl), u(m->tbl), u(m->bkeys),
-      u(m->bcnt), m->bcap, u(m->wkey), u(m->wsc), u(m->segs), m->seg_cap, u(m->bbsum), u(m->bboff),
+      u(m->bcnt), m->bcap, u(m->wkey), u(m->wsc), m->wtab_cap, u(m->segs), m->seg_cap, u(m->bbsum), u(m->bboff),
       u(m->items), m->item_cap, u(m->split_slot), m->split_cap, u(m->shared_bits), u(m->rkeys), u(m->rmasks),
       u(m->leaf_mask), m->rec_cap, u(m->lbsum), u(m->lboff), m->group_cap, u(m->rlist), u(m->d_dest),
       u(m->d_st)};
@@ -1336,9 +1336,9 @@ int launch_cast_graph(vdbgpu_map* m, int nb) {
 }
 
 // Enqueues the cast of c on the cast stream: staging (prep, brick table
-// cleared), brick cut, walk, record list and the copy into record set c.X,
-// then the copy of its status (and a snapshot of the map's leaf count) into
-// host slot c.slot. A first attempt takes the next record set of the ring and
+// cleared), record set c.X named for the copy, brick cut, walk, record list
+// and the copy (launched, or replayed from a graph), then the copy of its
+// status (and a snapshot of the map's leaf count) into host slot c.slot. A first attempt takes the next record set of the ring and
 // waits until the map stream has applied the scan that used it last; a re-run
 // (attempt > 0, tables grown) stages the scan again, since a later cast may
 // have used the shared tables meanwhile.
