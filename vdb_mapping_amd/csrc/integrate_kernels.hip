@@ -608,7 +608,7 @@ __device__ __forceinline__ void update_voxel(float& x, uint32_t& ab, int k, bool
 // lanes) are read and written only when one of its 32 voxels is visited: a
 // line is never written in part, and the 8% of an update leaf's lines with no
 // visit (outdoor stream) are skipped (synchronous latency after 100 scans
-// 2.76 -> 2.47 ms at p50; 64-byte blocks were slower than whole lines). Each wave carries kApplyLeaves leaves per
+// 2.76-2.77 -> 2.71-2.72 ms at p50; 64-byte blocks were slower). Each wave carries kApplyLeaves leaves per
 // iteration: with few registers per lane, many waves keep many bytes in
 // flight. Optionally clears the update-set slots afterwards; counts the
 // updated voxels.
