@@ -391,17 +391,6 @@ __global__ __launch_bounds__(256) void k_seg_fill(const int4* __restrict__ rayin
   }
 }
 
-// 8x8 bit-matrix transpose: bit 8r + c of x goes to bit 8c + r.
-__device__ __forceinline__ uint64_t transpose8(uint64_t x) {
-  uint64_t t = (x ^ (x >> 7)) & 0x00AA00AA00AA00AAull;
-  x ^= t ^ (t << 7);
-  t = (x ^ (x >> 14)) & 0x0000CCCC0000CCCCull;
-  x ^= t ^ (t << 14);
-  t = (x ^ (x >> 28)) & 0x00000000F0F0F0F0ull;
-  x ^= t ^ (t << 28);
-  return x;
-}
-
 // Walk one segment record into the brick bitmap `bits`.
 //
 // Word layouts: a segment's voxels go into the layout whose word axis A is

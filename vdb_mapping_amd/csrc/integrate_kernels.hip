@@ -753,20 +753,21 @@ __global__ __launch_bounds__(256) void k_group_fill(const uint32_t* __restrict__
 constexpr int kGroupLeaves = 2;
 static_assert(kDeferMax == 8, "a group's record positions are read as two uint4; one byte per scan");
 
-// Every scan's update of voxel i of a lane (bit 8s + i of vb / hb: visited /
-// hit in scan s), in scan order, visits only: one iteration per visit, the
-// clamped update of update_voxel.
-__device__ __forceinline__ void apply_visits(float& x, uint32_t& ab, int i, uint64_t vb, uint64_t hb,
+// Every scan's update of voxel i of a lane (bit 8i + s of vt / ht: visited /
+// hit in scan s, the lane's masks transposed once per group), in scan order,
+// visits only: one iteration per visit, the clamped update of update_voxel.
+__device__ __forceinline__ void apply_visits(float& x, uint32_t& ab, int i, uint64_t vt, uint64_t ht,
                                              float l_hit, float l_miss, float t_min, float t_max,
                                              float c_max, float c_min) {
-  uint64_t m = (vb >> i) & 0x0101010101010101ull;
+  uint32_t m = (uint32_t)(vt >> (8 * i)) & 0xFFu;
   if (!__any(m != 0)) return;
+  const uint32_t hs = (uint32_t)(ht >> (8 * i)) & 0xFFu;
   float v = x;
   uint32_t on = (ab >> i) & 1u;
   while (m) {
-    const uint32_t b = (uint32_t)__builtin_ctzll(m);
+    const uint32_t b = (uint32_t)__builtin_ctz(m);
     m &= m - 1;
-    const bool h = (hb >> (b + i)) & 1u;
+    const bool h = (hs >> b) & 1u;
     const float a = v + (h ? l_hit : l_miss);
     // hit: above t_max -> active, clamped at c_max; miss: below t_min -> inactive, clamped at c_min
     const bool cross = h ? (a > t_max) : (a < t_min);
@@ -839,14 +840,16 @@ __global__ __launch_bounds__(256) void k_apply_groups(GroupScans gs, const uint3
       if (lane == 0) gslot[e[q]] = kNone;
       vox += (unsigned long long)__popcll(vb[q]);
       uint32_t ab = (uint32_t)(aw[q] >> sh) & 0xFFu;
-      apply_visits(va[q].x, ab, 0, vb[q], hb[q], l_hit, l_miss, t_min, t_max, c_max, c_min);
-      apply_visits(va[q].y, ab, 1, vb[q], hb[q], l_hit, l_miss, t_min, t_max, c_max, c_min);
-      apply_visits(va[q].z, ab, 2, vb[q], hb[q], l_hit, l_miss, t_min, t_max, c_max, c_min);
-      apply_visits(va[q].w, ab, 3, vb[q], hb[q], l_hit, l_miss, t_min, t_max, c_max, c_min);
-      apply_visits(vc[q].x, ab, 4, vb[q], hb[q], l_hit, l_miss, t_min, t_max, c_max, c_min);
-      apply_visits(vc[q].y, ab, 5, vb[q], hb[q], l_hit, l_miss, t_min, t_max, c_max, c_min);
-      apply_visits(vc[q].z, ab, 6, vb[q], hb[q], l_hit, l_miss, t_min, t_max, c_max, c_min);
-      apply_visits(vc[q].w, ab, 7, vb[q], hb[q], l_hit, l_miss, t_min, t_max, c_max, c_min);
+      // byte i: voxel i's visits / hits, bit s per scan
+      const uint64_t vt = transpose8(vb[q]), ht = transpose8(hb[q]);
+      apply_visits(va[q].x, ab, 0, vt, ht, l_hit, l_miss, t_min, t_max, c_max, c_min);
+      apply_visits(va[q].y, ab, 1, vt, ht, l_hit, l_miss, t_min, t_max, c_max, c_min);
+      apply_visits(va[q].z, ab, 2, vt, ht, l_hit, l_miss, t_min, t_max, c_max, c_min);
+      apply_visits(va[q].w, ab, 3, vt, ht, l_hit, l_miss, t_min, t_max, c_max, c_min);
+      apply_visits(vc[q].x, ab, 4, vt, ht, l_hit, l_miss, t_min, t_max, c_max, c_min);
+      apply_visits(vc[q].y, ab, 5, vt, ht, l_hit, l_miss, t_min, t_max, c_max, c_min);
+      apply_visits(vc[q].z, ab, 6, vt, ht, l_hit, l_miss, t_min, t_max, c_max, c_min);
+      apply_visits(vc[q].w, ab, 7, vt, ht, l_hit, l_miss, t_min, t_max, c_max, c_min);
       // every word with a visit in some scan is written back whole (one owner per
       // row): the eight lanes of the word agree on it through the reduction
       uint64_t na = (uint64_t)ab << sh;
