@@ -523,14 +523,19 @@ __global__ __launch_bounds__(256) void k_map_insert(UpdView u, const uint32_t* _
       ++listed;
       const uint32_t slot = list ? list[e] : e;
       key = u.keys[slot];
-      const unsigned long long* w = &u.masks[(size_t)slot * 16];
-      uint64_t anyhit = 0, anymiss = 0;
+      // every listed leaf has a visited voxel: with both ops creating, it needs
+      // its leaf whatever its masks hold (the usual case: p_hit, p_miss != 0.5)
+      bool need = true;
+      if (!(create_hit && create_miss)) {
+        const unsigned long long* w = &u.masks[(size_t)slot * 16];
+        uint64_t anyhit = 0, anymiss = 0;
 #pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        anyhit |= w[j] & w[8 + j];
-        anymiss |= w[j] & ~w[8 + j];
+        for (int j = 0; j < 8; ++j) {
+          anyhit |= w[j] & w[8 + j];
+          anymiss |= w[j] & ~w[8 + j];
+        }
+        need = (anyhit && create_hit) || (anymiss && create_miss);
       }
-      const bool need = (anyhit && create_hit) || (anymiss && create_miss);
       uint32_t p = (uint32_t)fmix64(key) & mp.cap_mask;
       for (uint32_t probe = 0; probe <= mp.cap_mask; ++probe) {
         const unsigned long long k = mp.keys[p];
