@@ -1267,7 +1267,9 @@ int cast_chain(vdbgpu_map* m, int nb) {
   }
   {
     Timer t(m, kWalk, cs);
-    CK(launch_brick_walk(cs, m->num_cu * 32, m->items, m->segs, m->rayinfo, m->d_scans, m->tbl, bv,
+    // 64 work-groups per CU over the grid-stride item loop: bricks cost very
+    // different amounts, and the finer split shortens the tail.
+    CK(launch_brick_walk(cs, m->num_cu * 64, m->items, m->segs, m->rayinfo, m->d_scans, m->tbl, bv,
                          m->shared_bits, rec, m->leaf_mask, m->rec_cap, &m->d_st[0]));
   }
   {
