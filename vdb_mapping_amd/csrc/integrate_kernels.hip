@@ -769,6 +769,20 @@ __device__ __forceinline__ void apply_visits(float& x, uint32_t& ab, int i, uint
   const uint32_t hs = (uint32_t)(ht >> (8 * i)) & 0xFFu;
   float v = x;
   uint32_t on = (ab >> i) & 1u;
+  if (!__any(hs != 0)) {
+    // misses only (most voxels of most waves): the same clamped add once per
+    // visit, so which scans made them does not matter
+    while (m) {
+      m &= m - 1;
+      const float a = v + l_miss;
+      const bool cross = a < t_min;
+      v = cross ? (a < c_min ? c_min : a) : a;
+      on = cross ? 0u : on;
+    }
+    x = v;
+    ab = (ab & ~(1u << i)) | (on << i);
+    return;
+  }
   while (m) {
     const uint32_t b = (uint32_t)__builtin_ctz(m);
     m &= m - 1;
