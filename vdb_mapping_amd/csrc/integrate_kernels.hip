@@ -614,10 +614,11 @@ __device__ __forceinline__ void update_voxel(float& x, uint32_t& ab, int k, bool
 // line is never written in part, and the 8% of an update leaf's lines with no
 // visit (outdoor stream) are skipped (synchronous latency after 100 scans
 // 2.76-2.77 -> 2.71-2.72 ms at p50; 64-byte blocks were slower). Each wave carries kApplyLeaves leaves per
-// iteration: with few registers per lane, many waves keep many bytes in
-// flight. Optionally clears the update-set slots afterwards; counts the
+// iteration: four (103 VGPRs, four waves per SIMD) gave p50 2.64-2.65 ms
+// against 2.70-2.71 with two (64 VGPRs, eight waves), 2.72 with six and 2.87
+// with eight. Optionally clears the update-set slots afterwards; counts the
 // updated voxels.
-constexpr int kApplyLeaves = 2;  // leaves in flight per wave
+constexpr int kApplyLeaves = 4;  // leaves in flight per wave
 
 __global__ __launch_bounds__(256) void k_apply(UpdView u, const uint32_t* __restrict__ list,
                                                const uint32_t* __restrict__ eidx, uint32_t count,
