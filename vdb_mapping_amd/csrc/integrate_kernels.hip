@@ -715,38 +715,32 @@ __global__ __launch_bounds__(256) void k_apply(UpdView u, const uint32_t* __rest
 // updateMap, VDBMapping.hpp:764-783, scan after scan). gslot (one entry per
 // map leaf, kNone outside a grouping) names the record that opened the group.
 
-// Lane per record of scan t: the first record of each map leaf opens a group.
-__global__ __launch_bounds__(256) void k_group_open(const uint32_t* __restrict__ eidx, uint32_t count, uint32_t t,
-                                                    uint32_t* gslot, uint32_t* rgrp, uint32_t rgrp_stride,
-                                                    uint32_t* glist, uint32_t* grec, unsigned int* ngroups) {
+// Lane per record of scan t (launched for t = 0, 1, ... in order): the first
+// record of a map leaf opens its group (the group's record positions cleared,
+// its own entered); a later one enters its position in the group that an
+// earlier scan's record opened (a map leaf has one record per scan).
+__global__ __launch_bounds__(256) void k_group_add(const uint32_t* __restrict__ eidx,
+                                                   const uint32_t* __restrict__ list, uint32_t count, uint32_t t,
+                                                   uint32_t* gslot, uint32_t* rgrp, uint32_t rgrp_stride,
+                                                   uint32_t* glist, uint32_t* grec, unsigned int* ngroups) {
   __shared__ uint32_t s_wave[4];
   __shared__ uint32_t s_base;
   const uint32_t i = blockIdx.x * 256u + threadIdx.x;
   const uint32_t e = i < count ? eidx[i] : kNone;
-  bool opener = false;
-  if (e != kNone) opener = atomicCAS(&gslot[e], kNone, (t << kGroupScanShift) | i) == kNone;
+  uint32_t code = kNone;
+  if (e != kNone) code = atomicCAS(&gslot[e], kNone, (t << kGroupScanShift) | i);
+  const bool opener = e != kNone && code == kNone;
   const uint32_t g = block_append(opener, ngroups, s_wave, &s_base);
   if (opener) {
     glist[g] = e;
     rgrp[(size_t)t * rgrp_stride + i] = g;
 #pragma unroll
-    for (int k = 0; k < kDeferMax; ++k) grec[(size_t)g * kDeferMax + k] = kNone;
+    for (int k = 0; k < kDeferMax; ++k) grec[(size_t)g * kDeferMax + k] = k == (int)t ? list[i] : kNone;
+  } else if (e != kNone && (code >> kGroupScanShift) < t) {
+    // opened by an earlier scan's launch (a scan's records name distinct leaves)
+    const uint32_t go = rgrp[(size_t)(code >> kGroupScanShift) * rgrp_stride + (code & kGroupRecMask)];
+    grec[(size_t)go * kDeferMax + t] = list[i];
   }
-}
-
-// Lane per record of scan t: enter the record's position in its group.
-__global__ __launch_bounds__(256) void k_group_fill(const uint32_t* __restrict__ eidx,
-                                                    const uint32_t* __restrict__ list, uint32_t count, uint32_t t,
-                                                    const uint32_t* __restrict__ gslot,
-                                                    const uint32_t* __restrict__ rgrp, uint32_t rgrp_stride,
-                                                    uint32_t* grec) {
-  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-  if (i >= count) return;
-  const uint32_t e = eidx[i];
-  if (e == kNone) return;
-  const uint32_t code = gslot[e];
-  const uint32_t g = rgrp[(size_t)(code >> kGroupScanShift) * rgrp_stride + (code & kGroupRecMask)];
-  grec[(size_t)g * kDeferMax + t] = list[i];
 }
 
 // Group apply: one wave per group (map leaf), lane l owning voxels 8l ..
@@ -1069,20 +1063,12 @@ hipError_t launch_apply(hipStream_t s, int grid, UpdView u, const uint32_t* list
   return hipGetLastError();
 }
 
-hipError_t launch_group_open(hipStream_t s, const uint32_t* eidx, uint32_t count, uint32_t t, uint32_t* gslot,
-                             uint32_t* rgrp, uint32_t rgrp_stride, uint32_t* glist, uint32_t* grec,
-                             unsigned int* ngroups) {
+hipError_t launch_group_add(hipStream_t s, const uint32_t* eidx, const uint32_t* list, uint32_t count, uint32_t t,
+                            uint32_t* gslot, uint32_t* rgrp, uint32_t rgrp_stride, uint32_t* glist, uint32_t* grec,
+                            unsigned int* ngroups) {
   if (!count) return hipSuccess;
-  hipLaunchKernelGGL(k_group_open, dim3((count + 255) / 256), dim3(256), 0, s, eidx, count, t, gslot, rgrp,
+  hipLaunchKernelGGL(k_group_add, dim3((count + 255) / 256), dim3(256), 0, s, eidx, list, count, t, gslot, rgrp,
                      rgrp_stride, glist, grec, ngroups);
-  return hipGetLastError();
-}
-
-hipError_t launch_group_fill(hipStream_t s, const uint32_t* eidx, const uint32_t* list, uint32_t count, uint32_t t,
-                             const uint32_t* gslot, const uint32_t* rgrp, uint32_t rgrp_stride, uint32_t* grec) {
-  if (!count) return hipSuccess;
-  hipLaunchKernelGGL(k_group_fill, dim3((count + 255) / 256), dim3(256), 0, s, eidx, list, count, t, gslot, rgrp,
-                     rgrp_stride, grec);
   return hipGetLastError();
 }
 

@@ -56,11 +56,9 @@ struct GroupScans {
   const unsigned long long* masks[kDeferMax];  // record masks of scan t (16 words per position)
   int nscan;
 };
-hipError_t launch_group_open(hipStream_t s, const uint32_t* eidx, uint32_t count, uint32_t t, uint32_t* gslot,
-                             uint32_t* rgrp, uint32_t rgrp_stride, uint32_t* glist, uint32_t* grec,
-                             unsigned int* ngroups);
-hipError_t launch_group_fill(hipStream_t s, const uint32_t* eidx, const uint32_t* list, uint32_t count, uint32_t t,
-                             const uint32_t* gslot, const uint32_t* rgrp, uint32_t rgrp_stride, uint32_t* grec);
+hipError_t launch_group_add(hipStream_t s, const uint32_t* eidx, const uint32_t* list, uint32_t count, uint32_t t,
+                            uint32_t* gslot, uint32_t* rgrp, uint32_t rgrp_stride, uint32_t* glist, uint32_t* grec,
+                            unsigned int* ngroups);
 hipError_t launch_apply_groups(hipStream_t s, int grid, const GroupScans& gs, const uint32_t* glist,
                                const uint32_t* grec, const unsigned int* ngroups, uint32_t* gslot, MapView mp,
                                const float lo[6], Status* st);

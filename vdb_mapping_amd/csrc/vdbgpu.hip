@@ -1181,13 +1181,10 @@ int flush_pending(vdbgpu_map* m) {
   gs.nscan = P;
   {
     Timer t(m, kCompact);
-    for (int p = 0; p < P; ++p)
-      CK(launch_group_open(m->stream, m->reidx[m->pend_set[p]], m->pend_recs[p], (uint32_t)p, m->gslot, m->rgrp,
-                           rs, m->gl, m->grec, m->ngroups));
-    for (int p = 0; p < P; ++p) {
+    for (int p = 0; p < P; ++p) {  // in scan order: a scan's records join the groups earlier ones opened
       const int X = m->pend_set[p];
-      CK(launch_group_fill(m->stream, m->reidx[X], m->ident, m->pend_recs[p], (uint32_t)p, m->gslot, m->rgrp,
-                           rs, m->grec));
+      CK(launch_group_add(m->stream, m->reidx[X], m->ident, m->pend_recs[p], (uint32_t)p, m->gslot, m->rgrp, rs,
+                          m->gl, m->grec, m->ngroups));
       gs.masks[p] = m->dmasks[X];
     }
   }
