@@ -1190,10 +1190,10 @@ int flush_pending(vdbgpu_map* m) {
   }
   {
     Timer t(m, kApply);
-    // six workgroups per CU (one wave per leaf, 63 VGPRs): the outdoor stream
-    // runs at the same rate with 4, 6 or 8 (the GPU is busy either way), and
-    // the pass beside the casts takes 4.1 ms per 8 scans at 6 against 4.4 at 8
-    // and 4.8-5.0 at 4
+    // six workgroups per CU (one wave per leaf, 64 VGPRs): on the outdoor
+    // stream 2, 3, 4 and 8 per CU were slower (166.9-168.3, 169.9-170.5,
+    // 167.7-170.3, 169.3-170.1 M rays/s against 170.8-171.0), the pass beside
+    // the casts 7.5, 5.4, 4.2 and 4.0 ms against 3.3-3.5
     CK(launch_apply_groups(m->stream, m->num_cu * 6, gs, m->gl, m->grec, m->ngroups, m->gslot, map_view(m), m->lo,
                            &m->d_st[3]));
   }
