@@ -12,8 +12,15 @@ __device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
 
 // Rounded world->index of one component (VDBMapping.hpp:612-631): shift by
 // res/2 unless fmod(c, res) == 0, multiply by the stored 1/res, floor.
+// fmod(c, res) == 0 exactly when c = k res for an integer k. Then c * inv is
+// within k * 2^-51 of k, so k = rint(c * inv) and fma(-k, res, c) is exactly
+// 0; for any other c, c - k res is a nonzero multiple of 2^-1074 and its
+// rounding is not 0. (For |c / res| >= 2^51 the index is outside int either
+// way; NaN and infinities compare unequal to 0 in both forms.) Same result as
+// the fmod test, without fmod's division and reduction loop.
 __device__ __forceinline__ int world_to_index1(double c, double res, double inv) {
-  if (fmod(c, res) != 0.0) c = c + res / 2.0;
+  const double k = rint(c * inv);
+  if (fma(-k, res, c) != 0.0) c = c + res / 2.0;
   const double f = floor(c * inv);
   return (f >= -2147483648.0 && f < 2147483648.0) ? (int)f : INT_MIN;
 }
