@@ -10,7 +10,7 @@ same map (scans are reused cyclically when warmup + steps > 100).
 
 Prints one JSON line (see --help); `value` is rays/s of the whole job.
 
-Other modes (--mode, not the default line) measure the north-star layers:
+Other modes (--mode, not the default line) measure batching, queries and serving:
   batch  the outdoor stream integrated in batches of --batch scans vs the plain path
   mixed  64 scans of 10k-300k points with random poses and ranges: one batch vs per-scan
   rgbd   200 RGB-D frames at 0.02 m, then 1M raytrace queries (device arrays)
